@@ -1,0 +1,495 @@
+/* mmsp_oracle.c -- TEST INFRASTRUCTURE: CPU restatement of the reference's
+ * explicit finite-difference update path, used only as the checker by the
+ * tests, smoke() and the CPU-baseline leg of bench.py. Product code never
+ * links or calls it.
+ *
+ * Every function follows the reference's evaluation order so that, with the
+ * same cube routine, results are bitwise those of the reference build
+ * (g++ -O3 without -march: no FMA contraction, libm pow for cubes).
+ *
+ *   check_boundary        include/MMSP.utility.cpp:42-57 (serial branch)
+ *   node <-> coordinates  include/MMSP.grid.cpp:306-317, 375-411
+ *   scalar laplacian      include/MMSP.grid.cpp:2049-2066
+ *   vector laplacian      include/MMSP.grid.cpp:2068-2088
+ *   Allen-Cahn update     examples/phase_transitions/allen-cahn/allen-cahn.cpp:58-73
+ *   Cahn-Hilliard update  examples/phase_transitions/cahn-hilliard/explicit/cahn-hilliard.cpp:59-80
+ *   grain growth update   examples/coarsening/grain_growth/isotropic/phase_field/graingrowth.cpp:84-110
+ *
+ * Cube: the reference evaluates pow(phi,3) with glibc pow (a call, checked in
+ * the disassembly of the reference build). ORC_CUBE_POW reproduces that;
+ * ORC_CUBE_CR is the correctly rounded cube (double-double product, one final
+ * rounding) that the device kernels use. The two differ in ~1e-3 of inputs by
+ * one ulp; the tests measure what that does to trajectories.
+ *
+ * Precision forms:
+ *   *_f64      double storage and arithmetic (the reference with T = double)
+ *   *_f32ref   float storage with the reference's mixed semantics for T = float
+ *              (laplacian accumulates a double term into a float per axis;
+ *              pow, dt and 2.0 promote the update expression to double)
+ *   *_f32      float storage and float arithmetic with the same operation
+ *              order (the fast FP32 device path; not a reference form)
+ *
+ * Reductions (mass, free energy) are summed with Neumaier compensation in
+ * long double.
+ *
+ * Pinning: each function is checked by tests/test_oracle_*.py against golden
+ * vectors written by the reference itself (oracle/_ref, tests/golden/) and
+ * against closed forms (constant and Fourier-mode laplacians, the uniform-state
+ * scalar map, fixed points, symmetry, mass invariance).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+enum { ORC_MIRROR = 0, ORC_NEUMANN = 1, ORC_PERIODIC = 2, ORC_PARALLEL = 3, ORC_DIRICHLET = 4 };
+enum { ORC_CUBE_POW = 0, ORC_CUBE_CR = 1 };
+enum { ORC_MU_POLY = 0, ORC_MU_DW = 1 };
+
+typedef struct {
+	int dim;
+	int n[3];
+	int b0[3];
+	int b1[3];
+	double dx[3];
+} orc_grid;
+
+/* ---- seeded input generator (workload definition) ---------------------- */
+/* Element i of a seeded stream is the (i+1)-th output of SplitMix64 started
+ * from state `seed` (Steele, Lea, Flood 2014; Vigna's reference constants). */
+uint64_t orc_splitmix64(uint64_t seed, uint64_t i)
+{
+	uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	return z ^ (z >> 31);
+}
+
+void orc_fill_uniform(double* out, int64_t n, uint64_t seed, double lo, double hi)
+{
+	for (int64_t i = 0; i < n; i++) {
+		double u = (double)(orc_splitmix64(seed, (uint64_t)i) >> 11) * 0x1.0p-53;
+		out[i] = lo + (hi - lo) * u;
+	}
+}
+
+/* ---- boundary mapping and indexing -------------------------------------- */
+int orc_check_boundary(int x, int x0, int x1, int b0, int b1)
+{
+	if (x < x0) {
+		if (b0 == ORC_NEUMANN || b0 == ORC_DIRICHLET) x = x0;
+		else if (b0 == ORC_PERIODIC) x = x1 - (x0 - x);
+		else if (b0 == ORC_MIRROR) x = 2 * x0 - x;
+	} else if (x >= x1) {
+		if (b1 == ORC_NEUMANN || b1 == ORC_DIRICHLET) x = x1 - 1;
+		else if (b1 == ORC_PERIODIC) x = x0 + (x - x1);
+		else if (b1 == ORC_MIRROR) x = 2 * (x1 - 1) - x;
+	}
+	return x;
+}
+
+static int64_t nodes_of(const orc_grid* g)
+{
+	int64_t n = 1;
+	for (int d = 0; d < g->dim; d++) n *= g->n[d];
+	return n;
+}
+
+/* stride of axis d: product of the extents of the faster axes */
+static int64_t stride_of(const orc_grid* g, int d)
+{
+	int64_t s = 1;
+	for (int k = d + 1; k < g->dim; k++) s *= g->n[k];
+	return s;
+}
+
+static void position(const orc_grid* g, int64_t i, int x[3])
+{
+	for (int d = 0; d < g->dim; d++) {
+		int64_t s = stride_of(g, d);
+		x[d] = (int)(i / s);
+		i -= (int64_t)x[d] * s;
+	}
+}
+
+/* node index of the neighbour of x shifted by `off` along axis d */
+static int64_t neighbour(const orc_grid* g, const int x[3], int d, int off)
+{
+	int64_t idx = 0;
+	for (int k = 0; k < g->dim; k++) {
+		int xk = x[k];
+		if (k == d) xk = orc_check_boundary(xk + off, 0, g->n[k], g->b0[k], g->b1[k]);
+		idx += (int64_t)xk * stride_of(g, k);
+	}
+	return idx;
+}
+
+/* ---- cubes -------------------------------------------------------------- */
+double orc_cube_cr(double x)
+{
+	double p = x * x;
+	double e = fma(x, x, -p);     /* x*x = p + e exactly */
+	double hi = p * x;
+	double lo = fma(p, x, -hi);   /* p*x = hi + lo exactly */
+	lo = lo + e * x;
+	return hi + lo;
+}
+
+static double cube64(double x, int mode)
+{
+	return mode == ORC_CUBE_POW ? pow(x, 3.0) : orc_cube_cr(x);
+}
+
+/* ---- laplacians ---------------------------------------------------------- */
+static double weight_of(const orc_grid* g, int d)
+{
+	return 1.0 / (g->dx[d] * g->dx[d]);
+}
+
+double orc_laplacian_f64(const orc_grid* g, const double* a, int64_t i)
+{
+	int x[3] = {0, 0, 0};
+	position(g, i, x);
+	double y = a[i];
+	double lap = 0.0;
+	for (int d = 0; d < g->dim; d++) {
+		double yh = a[neighbour(g, x, d, +1)];
+		double yl = a[neighbour(g, x, d, -1)];
+		lap += weight_of(g, d) * (yh - 2.0 * y + yl);
+	}
+	return lap;
+}
+
+/* T = float in the reference: each axis adds a double term into a float. */
+float orc_laplacian_f32ref(const orc_grid* g, const float* a, int64_t i)
+{
+	int x[3] = {0, 0, 0};
+	position(g, i, x);
+	float y = a[i];
+	float lap = 0.0f;
+	for (int d = 0; d < g->dim; d++) {
+		float yh = a[neighbour(g, x, d, +1)];
+		float yl = a[neighbour(g, x, d, -1)];
+		lap = (float)((double)lap + weight_of(g, d) * ((double)yh - 2.0 * (double)y + (double)yl));
+	}
+	return lap;
+}
+
+float orc_laplacian_f32(const orc_grid* g, const float* a, int64_t i)
+{
+	int x[3] = {0, 0, 0};
+	position(g, i, x);
+	float y = a[i];
+	float lap = 0.0f;
+	for (int d = 0; d < g->dim; d++) {
+		float yh = a[neighbour(g, x, d, +1)];
+		float yl = a[neighbour(g, x, d, -1)];
+		float w = (float)weight_of(g, d);
+		lap += w * (yh - 2.0f * y + yl);
+	}
+	return lap;
+}
+
+/* ---- Allen-Cahn: p = {r, u, K, M, dt} ----------------------------------- */
+/* new = phi - dt*M*(-r*phi + u*phi^3 - K*lap)   (allen-cahn.cpp:70) */
+void orc_ac_steps_f64(const orc_grid* g, double* a, double* b, const double* p, int steps, int cube)
+{
+	const double r = p[0], u = p[1], K = p[2], M = p[3], dt = p[4];
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			double phi = a[i];
+			b[i] = phi - dt * M * (-r * phi + u * cube64(phi, cube) - K * orc_laplacian_f64(g, a, i));
+		}
+		memcpy(a, b, (size_t)n * sizeof(double));
+	}
+}
+
+void orc_ac_steps_f32ref(const orc_grid* g, float* a, float* b, const double* p, int steps, int cube)
+{
+	const double r = p[0], u = p[1], K = p[2], M = p[3], dt = p[4];
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			float phi = a[i];
+			float lap = orc_laplacian_f32ref(g, a, i);
+			b[i] = (float)((double)phi - dt * M * (-r * (double)phi + u * cube64((double)phi, cube) - K * (double)lap));
+		}
+		memcpy(a, b, (size_t)n * sizeof(float));
+	}
+}
+
+void orc_ac_steps_f32(const orc_grid* g, float* a, float* b, const double* p, int steps)
+{
+	const float r = (float)p[0], u = (float)p[1], K = (float)p[2], M = (float)p[3], dt = (float)p[4];
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			float phi = a[i];
+			float c3 = phi * phi * phi;
+			b[i] = phi - dt * M * (-r * phi + u * c3 - K * orc_laplacian_f32(g, a, i));
+		}
+		memcpy(a, b, (size_t)n * sizeof(float));
+	}
+}
+
+/* ---- Cahn-Hilliard ------------------------------------------------------- */
+/* ORC_MU_POLY: p = {r, u, K, M, dt}, mu = -r*phi + u*phi^3 - K*lap  (cahn-hilliard.cpp:71)
+ * ORC_MU_DW:   p = {kappa, M, dt},  mu = 2c(1-c)(1-2c) - kappa*lap
+ * then new = old + dt*M*lap(mu)                                      (cahn-hilliard.cpp:76) */
+void orc_ch_steps_f64(const orc_grid* g, double* a, double* b, double* mu, const double* p, int form,
+                      int steps, int cube)
+{
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			double c = a[i];
+			double lap = orc_laplacian_f64(g, a, i);
+			if (form == ORC_MU_POLY)
+				mu[i] = -p[0] * c + p[1] * cube64(c, cube) - p[2] * lap;
+			else
+				mu[i] = 2.0 * c * (1.0 - c) * (1.0 - 2.0 * c) - p[0] * lap;
+		}
+		const double M = form == ORC_MU_POLY ? p[3] : p[1];
+		const double dt = form == ORC_MU_POLY ? p[4] : p[2];
+		for (int64_t i = 0; i < n; i++)
+			b[i] = a[i] + dt * M * orc_laplacian_f64(g, mu, i);
+		memcpy(a, b, (size_t)n * sizeof(double));
+	}
+}
+
+void orc_ch_steps_f32ref(const orc_grid* g, float* a, float* b, float* mu, const double* p, int form,
+                         int steps, int cube)
+{
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			double c = a[i];
+			double lap = orc_laplacian_f32ref(g, a, i);
+			if (form == ORC_MU_POLY)
+				mu[i] = (float)(-p[0] * c + p[1] * cube64(c, cube) - p[2] * lap);
+			else
+				mu[i] = (float)(2.0 * c * (1.0 - c) * (1.0 - 2.0 * c) - p[0] * lap);
+		}
+		const double M = form == ORC_MU_POLY ? p[3] : p[1];
+		const double dt = form == ORC_MU_POLY ? p[4] : p[2];
+		for (int64_t i = 0; i < n; i++)
+			b[i] = (float)((double)a[i] + dt * M * (double)orc_laplacian_f32ref(g, mu, i));
+		memcpy(a, b, (size_t)n * sizeof(float));
+	}
+}
+
+void orc_ch_steps_f32(const orc_grid* g, float* a, float* b, float* mu, const double* p, int form, int steps)
+{
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			float c = a[i];
+			float lap = orc_laplacian_f32(g, a, i);
+			if (form == ORC_MU_POLY)
+				mu[i] = -(float)p[0] * c + (float)p[1] * (c * c * c) - (float)p[2] * lap;
+			else
+				mu[i] = 2.0f * c * (1.0f - c) * (1.0f - 2.0f * c) - (float)p[0] * lap;
+		}
+		const float M = (float)(form == ORC_MU_POLY ? p[3] : p[1]);
+		const float dt = (float)(form == ORC_MU_POLY ? p[4] : p[2]);
+		for (int64_t i = 0; i < n; i++)
+			b[i] = a[i] + dt * M * orc_laplacian_f32(g, mu, i);
+		memcpy(a, b, (size_t)n * sizeof(float));
+	}
+}
+
+/* ---- grain growth: fields node-major (Q values per node), p = {dt} ------- */
+/* lap_j as MMSP.grid.cpp:2068-2088; sum = sum_j phi_j^2 in T;
+ * new_j = phi_j - dt*(-phi_j - phi_j^3 + 2.0*(phi_j*sum - lap_j))   (graingrowth.cpp:98-105) */
+static void gg_lap_f64(const orc_grid* g, int Q, const double* a, int64_t i, double* lap)
+{
+	int x[3] = {0, 0, 0};
+	position(g, i, x);
+	const double* y = a + i * Q;
+	for (int j = 0; j < Q; j++) lap[j] = 0.0;
+	for (int d = 0; d < g->dim; d++) {
+		const double* yh = a + neighbour(g, x, d, +1) * Q;
+		const double* yl = a + neighbour(g, x, d, -1) * Q;
+		double w = weight_of(g, d);
+		for (int j = 0; j < Q; j++) lap[j] += w * (yh[j] - 2.0 * y[j] + yl[j]);
+	}
+}
+
+void orc_gg_steps_f64(const orc_grid* g, int Q, double* a, double* b, double dt, int steps, int cube)
+{
+	const int64_t n = nodes_of(g);
+	double* lap = (double*)malloc(sizeof(double) * (size_t)Q);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			const double* phi = a + i * Q;
+			gg_lap_f64(g, Q, a, i, lap);
+			double sum = 0.0;
+			for (int j = 0; j < Q; j++) sum += phi[j] * phi[j];
+			for (int j = 0; j < Q; j++)
+				b[i * Q + j] = phi[j] - dt * (-phi[j] - cube64(phi[j], cube) + 2.0 * (phi[j] * sum - lap[j]));
+		}
+		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(double));
+	}
+	free(lap);
+}
+
+void orc_gg_steps_f32ref(const orc_grid* g, int Q, float* a, float* b, double dt, int steps, int cube)
+{
+	const int64_t n = nodes_of(g);
+	float* lap = (float*)malloc(sizeof(float) * (size_t)Q);
+	int x[3] = {0, 0, 0};
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			const float* phi = a + i * Q;
+			position(g, i, x);
+			for (int j = 0; j < Q; j++) lap[j] = 0.0f;
+			for (int d = 0; d < g->dim; d++) {
+				const float* yh = a + neighbour(g, x, d, +1) * Q;
+				const float* yl = a + neighbour(g, x, d, -1) * Q;
+				double w = weight_of(g, d);
+				for (int j = 0; j < Q; j++)
+					lap[j] = (float)((double)lap[j] + w * ((double)yh[j] - 2.0 * (double)phi[j] + (double)yl[j]));
+			}
+			float sum = 0.0f;
+			for (int j = 0; j < Q; j++) sum += phi[j] * phi[j];
+			for (int j = 0; j < Q; j++) {
+				double pj = phi[j];
+				float ps = phi[j] * sum;
+				float inner = ps - lap[j];
+				b[i * Q + j] = (float)(pj - dt * (-pj - cube64(pj, cube) + 2.0 * (double)inner));
+			}
+		}
+		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(float));
+	}
+	free(lap);
+}
+
+void orc_gg_steps_f32(const orc_grid* g, int Q, float* a, float* b, double dt64, int steps)
+{
+	const int64_t n = nodes_of(g);
+	const float dt = (float)dt64;
+	float* lap = (float*)malloc(sizeof(float) * (size_t)Q);
+	int x[3] = {0, 0, 0};
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			const float* phi = a + i * Q;
+			position(g, i, x);
+			for (int j = 0; j < Q; j++) lap[j] = 0.0f;
+			for (int d = 0; d < g->dim; d++) {
+				const float* yh = a + neighbour(g, x, d, +1) * Q;
+				const float* yl = a + neighbour(g, x, d, -1) * Q;
+				float w = (float)weight_of(g, d);
+				for (int j = 0; j < Q; j++) lap[j] += w * (yh[j] - 2.0f * phi[j] + yl[j]);
+			}
+			float sum = 0.0f;
+			for (int j = 0; j < Q; j++) sum += phi[j] * phi[j];
+			for (int j = 0; j < Q; j++) {
+				float pj = phi[j];
+				b[i * Q + j] = pj - dt * (-pj - pj * pj * pj + 2.0f * (pj * sum - lap[j]));
+			}
+		}
+		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(float));
+	}
+	free(lap);
+}
+
+/* ---- reductions ----------------------------------------------------------- */
+typedef struct {
+	long double s, c;
+} nsum;
+
+static void nadd(nsum* a, long double v)
+{
+	long double t = a->s + v;
+	if (fabsl(a->s) >= fabsl(v)) a->c += (a->s - t) + v;
+	else a->c += (v - t) + a->s;
+	a->s = t;
+}
+
+/* per-field mass sum_i a[i*Q+j] (node-major); float inputs via the _f32 entry */
+void orc_mass_f64(const double* a, int64_t n, int Q, double* out)
+{
+	for (int j = 0; j < Q; j++) {
+		nsum acc = {0.0L, 0.0L};
+		for (int64_t i = 0; i < n; i++) nadd(&acc, a[i * Q + j]);
+		out[j] = (double)(acc.s + acc.c);
+	}
+}
+
+void orc_mass_f32(const float* a, int64_t n, int Q, double* out)
+{
+	for (int j = 0; j < Q; j++) {
+		nsum acc = {0.0L, 0.0L};
+		for (int64_t i = 0; i < n; i++) nadd(&acc, a[i * Q + j]);
+		out[j] = (double)(acc.s + acc.c);
+	}
+}
+
+/* Free energies, node terms evaluated in double, summed compensated.
+ * Forward differences (the discrete gradient whose adjoint gives the
+ * 5/7-point laplacian), neighbours through check_boundary.
+ *   kind 0 (AC / CH poly, p = {r, u, K}):  -r/2 c^2 + u/4 c^4 + K/2 |grad+ c|^2
+ *   kind 1 (CH double well, p = {kappa}):   c^2 (1-c)^2 + kappa/2 |grad+ c|^2
+ *   kind 2 (grain growth, p = {kappa}):     sum_j (-phi_j^2/2 + phi_j^4/4)
+ *                                           + sum_{j<k} phi_j^2 phi_k^2 + kappa/2 sum_j |grad+ phi_j|^2 */
+static double bulk_term(int kind, const double* p, const double* v, int Q)
+{
+	if (kind == 0) {
+		double c = v[0], c2 = c * c;
+		return -0.5 * p[0] * c2 + 0.25 * p[1] * (c2 * c2);
+	}
+	if (kind == 1) {
+		double c = v[0], m = 1.0 - c;
+		return (c * c) * (m * m);
+	}
+	double e = 0.0, s2 = 0.0;
+	for (int j = 0; j < Q; j++) {
+		double q = v[j] * v[j];
+		e += -0.5 * q + 0.25 * (q * q);
+		e += q * s2; /* pairs j<k counted once: phi_j^2 * sum_{k<j} phi_k^2 */
+		s2 += q;
+	}
+	return e;
+}
+
+static double energy_impl(const orc_grid* g, int Q, const double* a, int kind, const double* p)
+{
+	const int64_t n = nodes_of(g);
+	const double kap = kind == 0 ? p[2] : p[0];
+	nsum acc = {0.0L, 0.0L};
+	int x[3] = {0, 0, 0};
+	for (int64_t i = 0; i < n; i++) {
+		position(g, i, x);
+		double e = bulk_term(kind, p, a + i * Q, Q);
+		double gsum = 0.0;
+		for (int d = 0; d < g->dim; d++) {
+			const double* yh = a + neighbour(g, x, d, +1) * Q;
+			const double* y = a + i * Q;
+			for (int j = 0; j < Q; j++) {
+				double df = (yh[j] - y[j]) / g->dx[d];
+				gsum += df * df;
+			}
+		}
+		e += 0.5 * kap * gsum;
+		nadd(&acc, e);
+	}
+	double vol = 1.0;
+	for (int d = 0; d < g->dim; d++) vol *= g->dx[d];
+	return (double)((acc.s + acc.c) * vol);
+}
+
+double orc_energy_f64(const orc_grid* g, int Q, const double* a, int kind, const double* p)
+{
+	return energy_impl(g, Q, a, kind, p);
+}
+
+double orc_energy_f32(const orc_grid* g, int Q, const float* a, int kind, const double* p)
+{
+	int64_t n = nodes_of(g) * Q;
+	double* tmp = (double*)malloc(sizeof(double) * (size_t)n);
+	for (int64_t i = 0; i < n; i++) tmp[i] = a[i];
+	double e = energy_impl(g, Q, tmp, kind, p);
+	free(tmp);
+	return e;
+}
