@@ -1,0 +1,211 @@
+/* mmsp_amd.h -- C ABI of the MI355X (gfx950) explicit finite-difference
+ * update library.
+ *
+ * The library replaces the per-node loops of the reference's example
+ * update(grid&, int steps) plug-ins and the MMSP::laplacian they call:
+ *
+ *   MMSP::update(grid<dim,T>&, int)            allen-cahn.cpp:47-75
+ *   MMSP::update(grid<dim,T>&, int)            cahn-hilliard/explicit/cahn-hilliard.cpp:47-81
+ *   MMSP::update(grid<dim,vector<T> >&, int)   grain_growth/isotropic/phase_field/graingrowth.cpp:73-111
+ *   MMSP::laplacian(grid, x)                   include/MMSP.grid.cpp:2049-2088
+ *   check_boundary / ghostswap                 include/MMSP.utility.cpp:42-57, MMSP.grid.cpp:414-515
+ *   grid::swap                                 include/MMSP.grid.cpp:1913-1991
+ *   grid::output / grid::input (.dat layout)   include/MMSP.grid.cpp:1183-1402, 1818-1909
+ *
+ * (paths relative to the reference tree). The reference has no C ABI: its
+ * plug-in boundary is the C++ template update(grid&, steps) called by
+ * MMSP.main.hpp:279,305,331. include/mmsp_gpu.hpp maps that template onto
+ * these calls.
+ *
+ * Conventions: every function returns 0 on success or a negative MMSPX_E*
+ * code (never throws, never exits); mmspx_strerror() names the code and
+ * mmspx_last_error() gives the message of the calling thread's last failure.
+ * Host pointers are borrowed for the duration of the call. Device state is
+ * owned by the library and persists across calls. Work is enqueued on the
+ * state's own HIP stream; calls that return data to the host synchronise it.
+ * One state must not be used from two host threads at once; distinct states
+ * may be.
+ */
+#ifndef MMSP_AMD_H
+#define MMSP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMSPX_ABI_VERSION 1
+
+/* ---- error codes ---- */
+enum {
+	MMSPX_OK = 0,
+	MMSPX_EINVAL = -1,    /* bad argument (shape, model, parameter count ...) */
+	MMSPX_ENOMEM = -2,    /* device or host allocation failed */
+	MMSPX_EDEVICE = -3,   /* a HIP runtime call failed */
+	MMSPX_EIO = -4,       /* .dat read/write failure */
+	MMSPX_EFORMAT = -5,   /* .dat content does not match the expected grid */
+	MMSPX_EBUSY = -6,     /* admission refused (serving budget) */
+	MMSPX_ENOTFOUND = -7, /* unknown request id */
+	MMSPX_EUNSUPPORTED = -8
+};
+
+/* ---- boundary conditions: the reference's enum (MMSP.utility.h:22-29) ---- */
+enum { MMSPX_MIRROR = 0, MMSPX_NEUMANN = 1, MMSPX_PERIODIC = 2, MMSPX_PARALLEL = 3, MMSPX_DIRICHLET = 4 };
+
+/* ---- storage / arithmetic forms ---- */
+enum {
+	MMSPX_F64 = 0,    /* double storage and arithmetic, the reference's order (grid<dim,double>) */
+	MMSPX_F32 = 1,    /* float storage and float arithmetic, same operation order */
+	MMSPX_F32REF = 2  /* float storage, the reference's mixed float/double semantics (grid<dim,float>) */
+};
+
+/* ---- models and their parameter vectors ---- */
+enum {
+	MMSPX_AC = 0,     /* Allen-Cahn, allen-cahn.cpp:58-71.    p = {r, u, K, M, dt}
+	                     new = phi - dt*M*(-r*phi + u*phi^3 - K*lap(phi)) */
+	MMSPX_CH = 1,     /* Cahn-Hilliard, cahn-hilliard.cpp:59-79. p = {r, u, K, M, dt}
+	                     mu = -r*c + u*c^3 - K*lap(c);  new = c + dt*M*lap(mu) */
+	MMSPX_CH_DW = 2,  /* Cahn-Hilliard, double-well mu.      p = {kappa, M, dt}
+	                     mu = 2c(1-c)(1-2c) - kappa*lap(c); new = c + dt*M*lap(mu) */
+	MMSPX_GG = 3      /* grain growth, graingrowth.cpp:84-105. p = {dt}
+	                     new_j = phi_j - dt*(-phi_j - phi_j^3 + 2*(phi_j*sum_k phi_k^2 - lap phi_j)) */
+};
+
+/* ---- host array layouts for upload/download ---- */
+enum {
+	MMSPX_NODE_MAJOR = 0,  /* node i's fields contiguous (the reference's node order; scalar grids: plain array) */
+	MMSPX_FIELD_MAJOR = 1  /* one contiguous plane per field (the device layout) */
+};
+
+/* ---- reductions ---- */
+enum {
+	MMSPX_MASS = 0,    /* per field: sum over nodes (FP64, compensated) -> out[fields] */
+	MMSPX_ENERGY = 1   /* total free energy of the model (FP64, compensated) -> out[0] */
+};
+
+typedef struct mmspx_desc {
+	int dim;          /* 1, 2 or 3 */
+	int g0[3];        /* lower global coordinate per axis (written to .dat) */
+	int n[3];         /* extent per axis; axis 0 slowest, axis dim-1 contiguous (MMSP.grid.cpp:306-313) */
+	int fields;       /* 1 for scalar grids, Q for grid<dim,vector<T> > */
+	int vector;       /* 1 when the grid's node type is vector<T> (affects .dat type name and node layout) */
+	int precision;    /* MMSPX_F64 / MMSPX_F32 / MMSPX_F32REF */
+	int b0[3], b1[3]; /* boundary conditions per axis */
+	double dx[3];     /* grid spacing per axis */
+} mmspx_desc;
+
+typedef struct mmspx_state mmspx_state;
+
+const char* mmspx_strerror(int code);
+const char* mmspx_last_error(void);
+int mmspx_abi_version(void);
+
+/* Grid state: creation allocates the device buffers (two state buffers; CH
+ * adds one chemical-potential buffer on first use). Replaces grid::setup
+ * (MMSP.grid.cpp:141-327) and the copy-constructed newGrid/temp of update(). */
+int mmspx_create(const mmspx_desc* desc, mmspx_state** out);
+void mmspx_destroy(mmspx_state* s);
+int mmspx_get_desc(const mmspx_state* s, mmspx_desc* out);
+/* Bytes of device memory the state holds for `model` (admission accounting). */
+size_t mmspx_footprint(const mmspx_desc* desc, int model);
+
+int mmspx_upload(mmspx_state* s, const void* host, int layout);
+int mmspx_download(mmspx_state* s, void* host, int layout);
+
+/* Advance `steps` explicit steps. Enqueued; returns without waiting. */
+int mmspx_step(mmspx_state* s, int model, const double* params, int nparams, int steps);
+int mmspx_sync(mmspx_state* s);
+
+/* Same as mmspx_step, bracketed by HIP events on the state's stream; waits and
+ * returns the elapsed seconds and the kernel launches enqueued. */
+int mmspx_step_timed(mmspx_state* s, int model, const double* params, int nparams, int steps,
+                     double* seconds, int64_t* launches);
+
+/* On-device reductions (FP64 with Neumaier compensation). `params` as for
+ * mmspx_step (ENERGY uses the model's gradient coefficient and bulk terms). */
+int mmspx_reduce(mmspx_state* s, int what, int model, const double* params, int nparams, double* out, int nout);
+
+/* .dat output/input in the reference's serial layout. compress=1 writes a
+ * zlib level-9 block (the reference's default), 0 a raw block (readable by the
+ * reference: MMSP.grid.cpp:1306-1330). */
+int mmspx_write_dat(mmspx_state* s, const char* filename, int compress);
+/* Reads a .dat file into a new state with the given precision (-1: from the
+ * file's type). Multi-block files are assembled by their block limits. */
+int mmspx_read_dat(const char* filename, int precision, mmspx_state** out);
+/* Reads just the header of a .dat file into *desc (no device work). */
+int mmspx_peek_dat(const char* filename, mmspx_desc* desc);
+
+/* Seeded inputs: element i = lo + (hi-lo) * u_i, u_i = top 53 bits of the
+ * (i+1)-th SplitMix64 output from state `seed`, over [0,1). */
+void mmspx_fill_uniform(double* out, int64_t n, uint64_t seed, double lo, double hi);
+
+/* Kernel variant selection ("auto" picks per shape via the tuner cache). */
+int mmspx_set_variant(mmspx_state* s, int variant); /* 0 auto, 1 generic, 2 tiled/fused */
+
+/* ---- batches: many independent grids advanced together ---------------------
+ * All members share model and precision; extents, parameters and step counts
+ * may differ. One kernel launch per step covers every member's tiles through
+ * a device work list; multi-step runs replay a captured hipGraph. */
+typedef struct mmspx_batch mmspx_batch;
+
+int mmspx_batch_create(const mmspx_desc* descs, int count, int model, mmspx_batch** out);
+void mmspx_batch_destroy(mmspx_batch* b);
+int mmspx_batch_upload(mmspx_batch* b, int member, const void* host, int layout);
+int mmspx_batch_download(mmspx_batch* b, int member, void* host, int layout);
+/* params: count * nparams doubles, member-major. */
+int mmspx_batch_set_params(mmspx_batch* b, const double* params, int nparams);
+/* Advance every member by `steps`; graph_steps > 0 captures that many steps
+ * per hipGraph and replays it. */
+int mmspx_batch_step(mmspx_batch* b, int steps, int graph_steps);
+int mmspx_batch_step_timed(mmspx_batch* b, int steps, int graph_steps, double* seconds);
+int mmspx_batch_reduce(mmspx_batch* b, int member, int what, double* out, int nout);
+/* Tile configuration: autotune over the candidates on the batch's shapes. */
+int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t report_len);
+int mmspx_batch_get_tile(const mmspx_batch* b, int* tile_y, int* tile_x, int* cells_per_thread);
+
+/* ---- serving: continuous batching of solves ------------------------------ */
+typedef struct mmspx_server mmspx_server;
+
+typedef struct mmspx_server_config {
+	size_t hbm_budget;   /* bytes of device memory the server may hold for request state */
+	int max_slots;       /* most requests resident at once */
+	int chunk_steps;     /* steps per scheduling round before admitting/retiring */
+	int model;           /* MMSPX_CH_DW, MMSPX_CH or MMSPX_AC */
+	int precision;       /* MMSPX_F32 or MMSPX_F64 */
+} mmspx_server_config;
+
+typedef struct mmspx_request_info {
+	int64_t id;
+	int status;           /* 0 queued, 1 running, 2 done, 3 rejected, 4 cancelled */
+	int steps_done;
+	double t_submit;      /* seconds on the server clock */
+	double t_admit;
+	double t_done;
+	double mass0, mass1;  /* FP64 compensated mass at admission and at completion */
+} mmspx_request_info;
+
+int mmspx_server_create(const mmspx_server_config* cfg, mmspx_server** out);
+/* Stops the worker after the current round; pending requests are cancelled. */
+void mmspx_server_destroy(mmspx_server* sv);
+/* Queue a solve. `init` (node-major doubles, converted to the server's
+ * precision) is copied before return. params as for mmspx_step. */
+int mmspx_server_submit(mmspx_server* sv, const mmspx_desc* desc, const double* init, const double* params,
+                        int nparams, int steps, int64_t* id);
+int mmspx_server_cancel(mmspx_server* sv, int64_t id);
+/* Wait up to timeout_s (negative: forever) for a request to finish. */
+int mmspx_server_wait(mmspx_server* sv, int64_t id, double timeout_s, mmspx_request_info* info);
+/* Copy a finished request's final state (node-major doubles) and release it. */
+int mmspx_server_fetch(mmspx_server* sv, int64_t id, double* out, size_t count);
+int mmspx_server_info(mmspx_server* sv, int64_t id, mmspx_request_info* info);
+double mmspx_server_clock(mmspx_server* sv);
+/* Counters: [0] admitted, [1] completed, [2] rejected, [3] rounds, [4] peak bytes,
+ * [5] peak concurrent requests, [6] kernel launches. */
+int mmspx_server_stats(mmspx_server* sv, int64_t* out, int nout);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MMSP_AMD_H */

@@ -1,0 +1,291 @@
+"""mmsp_amd -- MI355X-native explicit finite-difference update path of MMSP.
+
+Python bindings (ctypes) over the C ABI in include/mmsp_amd.h, implemented by
+libmmsp_amd.so (HIP kernels for gfx950 + host C++). There is no CPU fallback:
+importing this package raises if the library has not been built, and every
+compute call goes to the GPU.
+
+    import mmsp_amd as mx
+    g = mx.Grid((512, 512), precision="f64")
+    g.upload(phi0)
+    g.step("ac", [1, 1, 1, 1, 0.01], 1000)   # allen-cahn.cpp:58-71
+    phi = g.download()
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmmsp_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "mmsp_amd.h")
+
+MIRROR, NEUMANN, PERIODIC, PARALLEL, DIRICHLET = 0, 1, 2, 3, 4
+F64, F32, F32REF = 0, 1, 2
+AC, CH, CH_DW, GG = 0, 1, 2, 3
+NODE_MAJOR, FIELD_MAJOR = 0, 1
+MASS, ENERGY = 0, 1
+
+PRECISIONS = {"f64": F64, "f32": F32, "f32ref": F32REF}
+MODELS = {"ac": AC, "ch": CH, "ch_dw": CH_DW, "gg": GG}
+NPARAMS = {AC: 5, CH: 5, CH_DW: 3, GG: 1}
+
+
+class MmspxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (_strerror(code), code, msg))
+        self.code = code
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("g0", C.c_int * 3),
+        ("n", C.c_int * 3),
+        ("fields", C.c_int),
+        ("vector", C.c_int),
+        ("precision", C.c_int),
+        ("b0", C.c_int * 3),
+        ("b1", C.c_int * 3),
+        ("dx", C.c_double * 3),
+    ]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_size_t),
+        ("max_slots", C.c_int),
+        ("chunk_steps", C.c_int),
+        ("model", C.c_int),
+        ("precision", C.c_int),
+    ]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [
+        ("id", C.c_int64),
+        ("status", C.c_int),
+        ("steps_done", C.c_int),
+        ("t_submit", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
+        ("mass0", C.c_double),
+        ("mass1", C.c_double),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "mmsp_amd: %s is missing; build it with `python -m mmsp_amd.build` "
+            "(there is no CPU fallback)" % LIB_PATH
+        )
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp, dp, i64p = C.c_void_p, P(C.c_double), P(C.c_int64)
+    sig = {
+        "mmspx_strerror": (C.c_char_p, [C.c_int]),
+        "mmspx_last_error": (C.c_char_p, []),
+        "mmspx_abi_version": (C.c_int, []),
+        "mmspx_create": (C.c_int, [P(Desc), P(vp)]),
+        "mmspx_destroy": (None, [vp]),
+        "mmspx_get_desc": (C.c_int, [vp, P(Desc)]),
+        "mmspx_footprint": (C.c_size_t, [P(Desc), C.c_int]),
+        "mmspx_upload": (C.c_int, [vp, vp, C.c_int]),
+        "mmspx_download": (C.c_int, [vp, vp, C.c_int]),
+        "mmspx_step": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int]),
+        "mmspx_sync": (C.c_int, [vp]),
+        "mmspx_step_timed": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, dp, i64p]),
+        "mmspx_reduce": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, dp, C.c_int]),
+        "mmspx_write_dat": (C.c_int, [vp, C.c_char_p, C.c_int]),
+        "mmspx_read_dat": (C.c_int, [C.c_char_p, C.c_int, P(vp)]),
+        "mmspx_peek_dat": (C.c_int, [C.c_char_p, P(Desc)]),
+        "mmspx_fill_uniform": (None, [dp, C.c_int64, C.c_uint64, C.c_double, C.c_double]),
+        "mmspx_set_variant": (C.c_int, [vp, C.c_int]),
+        "mmspx_batch_create": (C.c_int, [P(Desc), C.c_int, C.c_int, P(vp)]),
+        "mmspx_batch_destroy": (None, [vp]),
+        "mmspx_batch_upload": (C.c_int, [vp, C.c_int, vp, C.c_int]),
+        "mmspx_batch_download": (C.c_int, [vp, C.c_int, vp, C.c_int]),
+        "mmspx_batch_set_params": (C.c_int, [vp, dp, C.c_int]),
+        "mmspx_batch_step": (C.c_int, [vp, C.c_int, C.c_int]),
+        "mmspx_batch_step_timed": (C.c_int, [vp, C.c_int, C.c_int, dp]),
+        "mmspx_batch_reduce": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int]),
+        "mmspx_batch_autotune": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_size_t]),
+        "mmspx_batch_get_tile": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "mmspx_server_create": (C.c_int, [P(ServerConfig), P(vp)]),
+        "mmspx_server_destroy": (None, [vp]),
+        "mmspx_server_submit": (C.c_int, [vp, P(Desc), dp, dp, C.c_int, C.c_int, i64p]),
+        "mmspx_server_cancel": (C.c_int, [vp, C.c_int64]),
+        "mmspx_server_wait": (C.c_int, [vp, C.c_int64, C.c_double, P(RequestInfo)]),
+        "mmspx_server_fetch": (C.c_int, [vp, C.c_int64, dp, C.c_size_t]),
+        "mmspx_server_info": (C.c_int, [vp, C.c_int64, P(RequestInfo)]),
+        "mmspx_server_clock": (C.c_double, [vp]),
+        "mmspx_server_stats": (C.c_int, [vp, i64p, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def _strerror(code):
+    return lib.mmspx_strerror(code).decode()
+
+
+def check(rc):
+    if rc != 0:
+        raise MmspxError(rc, lib.mmspx_last_error().decode())
+    return rc
+
+
+def declared_symbols(header: str = HEADER):
+    """Function names declared in include/mmsp_amd.h."""
+    with open(header) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(mmspx_[a-z_0-9]+)\s*\(", text)))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_desc(shape, fields=1, vector=False, precision="f64", bc=None, dx=None, g0=None) -> Desc:
+    d = Desc()
+    dim = len(shape)
+    d.dim = dim
+    for k in range(3):
+        d.n[k] = int(shape[k]) if k < dim else 1
+        d.g0[k] = int(g0[k]) if g0 is not None and k < dim else 0
+        lo, hi = bc[k] if bc is not None and k < dim else (PERIODIC, PERIODIC)
+        d.b0[k], d.b1[k] = lo, hi
+        d.dx[k] = float(dx[k]) if dx is not None and k < dim else 1.0
+    d.fields = int(fields)
+    d.vector = 1 if vector else 0
+    d.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    return d
+
+
+def _model(m):
+    return MODELS[m] if isinstance(m, str) else int(m)
+
+
+def _params(model, params):
+    p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1))
+    return p
+
+
+def fill_uniform(n, seed, lo, hi):
+    out = np.empty(int(n), dtype=np.float64)
+    lib.mmspx_fill_uniform(_dp(out), int(n), seed, lo, hi)
+    return out
+
+
+def footprint(shape, model, fields=1, vector=False, precision="f64"):
+    d = make_desc(shape, fields, vector, precision)
+    return lib.mmspx_footprint(C.byref(d), _model(model))
+
+
+class Grid:
+    """A device-resident grid<dim,T> (scalar) or grid<dim,vector<T> > (vector=True)."""
+
+    def __init__(self, shape=None, fields=1, vector=False, precision="f64", bc=None, dx=None, g0=None, _handle=None):
+        if _handle is not None:
+            self._h = _handle
+        else:
+            d = make_desc(shape, fields, vector, precision, bc, dx, g0)
+            h = C.c_void_p()
+            check(lib.mmspx_create(C.byref(d), C.byref(h)))
+            self._h = h
+        d = Desc()
+        check(lib.mmspx_get_desc(self._h, C.byref(d)))
+        self.desc = d
+        self.shape = tuple(d.n[k] for k in range(d.dim))
+        self.vector = bool(d.vector)
+        self.fields = d.fields if self.vector else 1
+        self.dtype = np.float64 if d.precision == F64 else np.float32
+
+    @classmethod
+    def from_dat(cls, path, precision=None):
+        h = C.c_void_p()
+        p = -1 if precision is None else PRECISIONS[precision]
+        check(lib.mmspx_read_dat(path.encode(), p, C.byref(h)))
+        return cls(_handle=h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.mmspx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def host_shape(self):
+        return self.shape + ((self.fields,) if self.vector else ())
+
+    def upload(self, a, layout=NODE_MAJOR):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != int(np.prod(self.host_shape)):
+            raise ValueError("array size %d does not match grid %s" % (a.size, self.host_shape))
+        check(lib.mmspx_upload(self._h, a.ctypes.data_as(C.c_void_p), layout))
+
+    def download(self, layout=NODE_MAJOR):
+        if layout == NODE_MAJOR:
+            out = np.empty(self.host_shape, dtype=self.dtype)
+        else:
+            out = np.empty(((self.fields,) if self.vector else ()) + self.shape, dtype=self.dtype)
+        check(lib.mmspx_download(self._h, out.ctypes.data_as(C.c_void_p), layout))
+        return out
+
+    def step(self, model, params, steps):
+        m = _model(model)
+        p = _params(m, params)
+        check(lib.mmspx_step(self._h, m, _dp(p), p.size, int(steps)))
+
+    def step_timed(self, model, params, steps):
+        m = _model(model)
+        p = _params(m, params)
+        sec = C.c_double()
+        nl = C.c_int64()
+        check(lib.mmspx_step_timed(self._h, m, _dp(p), p.size, int(steps), C.byref(sec), C.byref(nl)))
+        return sec.value, nl.value
+
+    def sync(self):
+        check(lib.mmspx_sync(self._h))
+
+    def set_variant(self, v):
+        check(lib.mmspx_set_variant(self._h, int(v)))
+
+    def mass(self):
+        out = np.zeros(self.fields, dtype=np.float64)
+        check(lib.mmspx_reduce(self._h, MASS, 0, None, 0, _dp(out), out.size))
+        return out
+
+    def energy(self, model, params):
+        m = _model(model)
+        p = _params(m, params)
+        out = np.zeros(1, dtype=np.float64)
+        check(lib.mmspx_reduce(self._h, ENERGY, m, _dp(p), p.size, _dp(out), 1))
+        return float(out[0])
+
+    def write_dat(self, path, compress=True):
+        check(lib.mmspx_write_dat(self._h, path.encode(), 1 if compress else 0))
+
+
+def peek_dat(path) -> Desc:
+    d = Desc()
+    check(lib.mmspx_peek_dat(path.encode(), C.byref(d)))
+    return d

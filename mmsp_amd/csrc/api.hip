@@ -1,0 +1,411 @@
+// api.hip -- the grid-state half of the C ABI (include/mmsp_amd.h).
+//
+// A state is the device-resident replacement of grid<dim,T> storage
+// (MMSP.grid.cpp:141-327) together with the newGrid/temp grids that each
+// reference update() copy-constructs (allen-cahn.cpp:56, cahn-hilliard.cpp:56-57):
+// two state buffers swapped by pointer after every step (grid::swap,
+// MMSP.grid.cpp:1913-1991) plus, for Cahn-Hilliard, a chemical-potential
+// buffer. Vector grids are stored field-major (one plane per field).
+#include <cmath>
+#include <cstdio>
+#include <mutex>
+#include <vector>
+
+#include "core.hpp"
+
+namespace mmspx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg)
+{
+	set_error(msg);
+	return code;
+}
+
+int validate_desc(const mmspx_desc* d)
+{
+	if (!d) return fail(MMSPX_EINVAL, "null descriptor");
+	if (d->dim < 1 || d->dim > 3) return fail(MMSPX_EINVAL, "dim must be 1, 2 or 3");
+	if (d->precision != MMSPX_F64 && d->precision != MMSPX_F32 && d->precision != MMSPX_F32REF)
+		return fail(MMSPX_EINVAL, "unknown precision");
+	const int q = d->vector ? d->fields : 1;
+	if (q < 1 || q > 4096) return fail(MMSPX_EINVAL, "fields must be in [1, 4096] for vector grids");
+	int64_t nodes = 1;
+	for (int k = 0; k < d->dim; k++) {
+		if (d->n[k] < 1) return fail(MMSPX_EINVAL, "every extent must be >= 1");
+		for (int side = 0; side < 2; side++) {
+			int b = side ? d->b1[k] : d->b0[k];
+			if (b != MMSPX_MIRROR && b != MMSPX_NEUMANN && b != MMSPX_PERIODIC && b != MMSPX_DIRICHLET)
+				return fail(MMSPX_EINVAL, "boundary condition must be mirror, Neumann, periodic or Dirichlet "
+				                          "(the serial reference has no 'parallel' faces)");
+			if (b == MMSPX_MIRROR && d->n[k] < 2) return fail(MMSPX_EINVAL, "mirror boundary needs extent >= 2");
+		}
+		if (!(d->dx[k] > 0.0) || !std::isfinite(d->dx[k])) return fail(MMSPX_EINVAL, "dx must be positive");
+		nodes *= d->n[k];
+	}
+	if (nodes * q > 0x7fffffffLL) return fail(MMSPX_EINVAL, "nodes * fields must stay below 2^31");
+	return 0;
+}
+
+Geom make_geom(const mmspx_desc& d)
+{
+	Geom g;
+	std::memset(&g, 0, sizeof(g));
+	g.dim = d.dim;
+	g.nodes = 1;
+	g.vol = 1.0;
+	for (int k = 0; k < 3; k++) {
+		g.n[k] = k < d.dim ? d.n[k] : 1;
+		g.b0[k] = k < d.dim ? d.b0[k] : MMSPX_PERIODIC;
+		g.b1[k] = k < d.dim ? d.b1[k] : MMSPX_PERIODIC;
+		g.dx[k] = k < d.dim ? d.dx[k] : 1.0;
+		g.w[k] = 1.0 / (g.dx[k] * g.dx[k]);
+		g.wf[k] = (float)g.w[k];
+	}
+	for (int k = 0; k < d.dim; k++) {
+		g.nodes *= g.n[k];
+		g.vol *= g.dx[k];
+		int s = 1;
+		for (int j = k + 1; j < d.dim; j++) s *= g.n[j];
+		g.s[k] = s;
+	}
+	for (int k = d.dim; k < 3; k++) g.s[k] = 0;
+	g.fields = d.vector ? d.fields : 1;
+	return g;
+}
+
+int model_nparams(int model)
+{
+	switch (model) {
+	case MMSPX_AC: return 5;
+	case MMSPX_CH: return 5;
+	case MMSPX_CH_DW: return 3;
+	case MMSPX_GG: return 1;
+	}
+	return -1;
+}
+
+int make_params(int model, const double* p, int np, ModelParams* out)
+{
+	const int need = model_nparams(model);
+	if (need < 0) return fail(MMSPX_EINVAL, "unknown model");
+	if (!p || np != need) return fail(MMSPX_EINVAL, "wrong number of model parameters");
+	for (int k = 0; k < np; k++)
+		if (!std::isfinite(p[k])) return fail(MMSPX_EINVAL, "model parameters must be finite");
+	ModelParams m;
+	std::memset(&m, 0, sizeof(m));
+	m.model = model;
+	if (model == MMSPX_AC || model == MMSPX_CH) {
+		// p = {r, u, K, M, dt}; the reference writes dt*M*(...) and -r*phi
+		m.nr = -p[0];
+		m.u = p[1];
+		m.K = p[2];
+		m.dtM = p[4] * p[3];
+		m.nrf = -(float)p[0];
+		m.uf = (float)p[1];
+		m.Kf = (float)p[2];
+		m.dtMf = (float)p[4] * (float)p[3];
+	} else if (model == MMSPX_CH_DW) {
+		// p = {kappa, M, dt}
+		m.kappa = p[0];
+		m.dtM = p[2] * p[1];
+		m.kappaf = (float)p[0];
+		m.dtMf = (float)p[2] * (float)p[1];
+	} else {
+		m.dt = p[0];
+		m.dtf = (float)p[0];
+	}
+	*out = m;
+	return 0;
+}
+
+}  // namespace mmspx
+
+using namespace mmspx;
+
+extern "C" {
+
+const char* mmspx_strerror(int code)
+{
+	switch (code) {
+	case MMSPX_OK: return "ok";
+	case MMSPX_EINVAL: return "invalid argument";
+	case MMSPX_ENOMEM: return "out of memory";
+	case MMSPX_EDEVICE: return "HIP runtime error";
+	case MMSPX_EIO: return "file I/O error";
+	case MMSPX_EFORMAT: return "bad .dat content";
+	case MMSPX_EBUSY: return "admission refused";
+	case MMSPX_ENOTFOUND: return "unknown request";
+	case MMSPX_EUNSUPPORTED: return "unsupported";
+	}
+	return "unknown error";
+}
+
+const char* mmspx_last_error(void) { return g_last_error.c_str(); }
+
+int mmspx_abi_version(void) { return MMSPX_ABI_VERSION; }
+
+size_t mmspx_footprint(const mmspx_desc* d, int model)
+{
+	if (validate_desc(d)) return 0;
+	const size_t elem = d->precision == MMSPX_F64 ? 8 : 4;
+	size_t nodes = 1;
+	for (int k = 0; k < d->dim; k++) nodes *= (size_t)d->n[k];
+	const size_t q = d->vector ? (size_t)d->fields : 1;
+	size_t bufs = (model == MMSPX_CH || model == MMSPX_CH_DW) ? 3 : 2;
+	return bufs * nodes * q * elem + q * 2 * kRedBlocks * sizeof(double);
+}
+
+void mmspx_destroy(mmspx_state* s)
+{
+	if (!s) return;
+	// teardown: errors here cannot be reported to anyone, so they are dropped
+	if (s->stream) (void)hipStreamSynchronize(s->stream);
+	if (s->cur) (void)hipFree(s->cur);
+	if (s->nxt) (void)hipFree(s->nxt);
+	if (s->aux) (void)hipFree(s->aux);
+	if (s->part) (void)hipFree(s->part);
+	if (s->part_host) (void)hipHostFree(s->part_host);
+	if (s->ev0) (void)hipEventDestroy(s->ev0);
+	if (s->ev1) (void)hipEventDestroy(s->ev1);
+	if (s->stream) (void)hipStreamDestroy(s->stream);
+	delete s;
+}
+
+int mmspx_create(const mmspx_desc* d, mmspx_state** out)
+{
+	if (!out) return fail(MMSPX_EINVAL, "null output pointer");
+	*out = nullptr;
+	int rc = validate_desc(d);
+	if (rc) return rc;
+	mmspx_state* s = new (std::nothrow) mmspx_state();
+	if (!s) return fail(MMSPX_ENOMEM, "host allocation failed");
+	s->desc = *d;
+	s->geom = make_geom(*d);
+	s->elem = d->precision == MMSPX_F64 ? 8 : 4;
+	s->bytes = (size_t)s->geom.nodes * s->geom.fields * s->elem;
+	hipError_t e = hipGetDevice(&s->device);
+	if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+	if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+	if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+	if (e != hipSuccess) {
+		mmspx_destroy(s);
+		return fail(MMSPX_EDEVICE, std::string("HIP setup failed: ") + hipGetErrorString(e));
+	}
+	const size_t red = (size_t)s->geom.fields * 2 * kRedBlocks * sizeof(double);
+	if (hipMalloc(&s->cur, s->bytes) != hipSuccess || hipMalloc(&s->nxt, s->bytes) != hipSuccess ||
+	    hipMalloc((void**)&s->part, red) != hipSuccess || hipHostMalloc((void**)&s->part_host, red, 0) != hipSuccess) {
+		mmspx_destroy(s);
+		return fail(MMSPX_ENOMEM, "device allocation failed");
+	}
+	if (hipMemsetAsync(s->cur, 0, s->bytes, s->stream) != hipSuccess) {
+		mmspx_destroy(s);
+		return fail(MMSPX_EDEVICE, "hipMemsetAsync failed");
+	}
+	*out = s;
+	return 0;
+}
+
+int mmspx_get_desc(const mmspx_state* s, mmspx_desc* out)
+{
+	if (!s || !out) return fail(MMSPX_EINVAL, "null argument");
+	*out = s->desc;
+	return 0;
+}
+
+int mmspx_upload(mmspx_state* s, const void* host, int layout)
+{
+	if (!s || !host) return fail(MMSPX_EINVAL, "null argument");
+	if (layout != MMSPX_NODE_MAJOR && layout != MMSPX_FIELD_MAJOR) return fail(MMSPX_EINVAL, "unknown layout");
+	if (s->geom.fields == 1 || layout == MMSPX_FIELD_MAJOR) {
+		MMSPX_HIP(hipMemcpyAsync(s->cur, host, s->bytes, hipMemcpyHostToDevice, s->stream));
+	} else {
+		// stage node-major in the spare buffer, transpose into the state
+		MMSPX_HIP(hipMemcpyAsync(s->nxt, host, s->bytes, hipMemcpyHostToDevice, s->stream));
+		int rc = launch_transpose(s->nxt, s->cur, s->geom.nodes, s->geom.fields, s->elem, true, s->stream);
+		if (rc) return rc;
+	}
+	MMSPX_HIP(hipStreamSynchronize(s->stream));
+	return 0;
+}
+
+int mmspx_download(mmspx_state* s, void* host, int layout)
+{
+	if (!s || !host) return fail(MMSPX_EINVAL, "null argument");
+	if (layout != MMSPX_NODE_MAJOR && layout != MMSPX_FIELD_MAJOR) return fail(MMSPX_EINVAL, "unknown layout");
+	if (s->geom.fields == 1 || layout == MMSPX_FIELD_MAJOR) {
+		MMSPX_HIP(hipMemcpyAsync(host, s->cur, s->bytes, hipMemcpyDeviceToHost, s->stream));
+	} else {
+		int rc = launch_transpose(s->cur, s->nxt, s->geom.nodes, s->geom.fields, s->elem, false, s->stream);
+		if (rc) return rc;
+		MMSPX_HIP(hipMemcpyAsync(host, s->nxt, s->bytes, hipMemcpyDeviceToHost, s->stream));
+	}
+	MMSPX_HIP(hipStreamSynchronize(s->stream));
+	return 0;
+}
+
+int mmspx_set_variant(mmspx_state* s, int variant)
+{
+	if (!s || variant < 0 || variant > 2) return fail(MMSPX_EINVAL, "variant must be 0, 1 or 2");
+	s->variant = variant;
+	return 0;
+}
+
+static int step_impl(mmspx_state* s, int model, const double* params, int np, int steps)
+{
+	if (!s) return fail(MMSPX_EINVAL, "null state");
+	if (steps < 0) return fail(MMSPX_EINVAL, "steps must be >= 0");
+	ModelParams p;
+	int rc = make_params(model, params, np, &p);
+	if (rc) return rc;
+	if (model == MMSPX_GG) {
+		if (!s->desc.vector) return fail(MMSPX_EINVAL, "grain growth needs a vector grid");
+	} else if (s->desc.vector) {
+		return fail(MMSPX_EINVAL, "Allen-Cahn and Cahn-Hilliard need a scalar grid");
+	}
+	const Geom& g = s->geom;
+	bool fused = false;
+	TuneChoice tc{};
+	if (s->variant != 1 && fused_supported(g, s->desc.precision, model)) {
+		tc = default_choice(g, s->desc.precision, model);
+		fused = s->variant == 2 || tc.variant == 2;
+	} else if (s->variant == 2) {
+		return fail(MMSPX_EUNSUPPORTED, "no tiled/fused kernel for this model, shape and boundary condition");
+	}
+	if ((model == MMSPX_CH || model == MMSPX_CH_DW) && !fused && !s->aux) {
+		if (hipMalloc(&s->aux, s->bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");
+	}
+	for (int k = 0; k < steps; k++) {
+		rc = fused ? launch_fused_step(g, s->desc.precision, p, s->cur, s->nxt, s->stream, tc, &s->launches)
+		           : launch_generic_step(g, s->desc.precision, p, s->cur, s->nxt, s->aux, s->stream, &s->launches);
+		if (rc) return rc;
+		std::swap(s->cur, s->nxt);
+	}
+	return 0;
+}
+
+int mmspx_step(mmspx_state* s, int model, const double* params, int np, int steps)
+{
+	return step_impl(s, model, params, np, steps);
+}
+
+int mmspx_sync(mmspx_state* s)
+{
+	if (!s) return fail(MMSPX_EINVAL, "null state");
+	MMSPX_HIP(hipStreamSynchronize(s->stream));
+	return 0;
+}
+
+int mmspx_step_timed(mmspx_state* s, int model, const double* params, int np, int steps, double* seconds,
+                     int64_t* launches)
+{
+	if (!s) return fail(MMSPX_EINVAL, "null state");
+	const int64_t l0 = s->launches;
+	MMSPX_HIP(hipEventRecord(s->ev0, s->stream));
+	int rc = step_impl(s, model, params, np, steps);
+	if (rc) return rc;
+	MMSPX_HIP(hipEventRecord(s->ev1, s->stream));
+	MMSPX_HIP(hipEventSynchronize(s->ev1));
+	float ms = 0.0f;
+	MMSPX_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+	if (seconds) *seconds = ms * 1e-3;
+	if (launches) *launches = s->launches - l0;
+	return 0;
+}
+
+int mmspx_reduce(mmspx_state* s, int what, int model, const double* params, int np, double* out, int nout)
+{
+	if (!s || !out) return fail(MMSPX_EINVAL, "null argument");
+	ModelParams p;
+	std::memset(&p, 0, sizeof(p));
+	if (what == MMSPX_ENERGY) {
+		int rc = make_params(model, params, np, &p);
+		if (rc) return rc;
+		if (nout < 1) return fail(MMSPX_EINVAL, "energy needs one output");
+	} else if (what == MMSPX_MASS) {
+		if (nout < s->geom.fields) return fail(MMSPX_EINVAL, "mass needs one output per field");
+	} else {
+		return fail(MMSPX_EINVAL, "unknown reduction");
+	}
+	int rc = launch_reduce(s->geom, s->desc.precision, what, p, s->cur, s->part, s->stream);
+	if (rc) return rc;
+	const int sets = what == MMSPX_MASS ? s->geom.fields : 1;
+	const size_t bytes = (size_t)sets * 2 * kRedBlocks * sizeof(double);
+	MMSPX_HIP(hipMemcpyAsync(s->part_host, s->part, bytes, hipMemcpyDeviceToHost, s->stream));
+	MMSPX_HIP(hipStreamSynchronize(s->stream));
+	for (int j = 0; j < sets; j++) out[j] = sum_partials(s->part_host + (size_t)j * 2 * kRedBlocks, kRedBlocks);
+	if (what == MMSPX_ENERGY) out[0] *= s->geom.vol;
+	return 0;
+}
+
+int mmspx_write_dat(mmspx_state* s, const char* filename, int compress)
+{
+	if (!s || !filename) return fail(MMSPX_EINVAL, "null argument");
+	std::vector<char> host(s->bytes);
+	int rc = mmspx_download(s, host.data(), MMSPX_NODE_MAJOR);
+	if (rc) return rc;
+	return dat_write(filename, s->desc, host.data(), s->elem, compress);
+}
+
+int mmspx_peek_dat(const char* filename, mmspx_desc* d)
+{
+	if (!filename || !d) return fail(MMSPX_EINVAL, "null argument");
+	int f64 = 0;
+	return dat_read_header(filename, d, &f64);
+}
+
+int mmspx_read_dat(const char* filename, int precision, mmspx_state** out)
+{
+	if (!filename || !out) return fail(MMSPX_EINVAL, "null argument");
+	mmspx_desc d;
+	int f64 = 0;
+	int rc = dat_read_header(filename, &d, &f64);
+	if (rc) return rc;
+	if (precision >= 0) d.precision = precision;
+	rc = validate_desc(&d);
+	if (rc) return rc;
+	const Geom g = make_geom(d);
+	const size_t count = (size_t)g.nodes * g.fields;
+	std::vector<char> host(count * (f64 ? 8 : 4));
+	rc = dat_read(filename, d, f64, host.data());
+	if (rc) return rc;
+	const bool want64 = d.precision == MMSPX_F64;
+	std::vector<char> conv;
+	const void* src = host.data();
+	if (want64 != (bool)f64) {
+		conv.resize(count * (want64 ? 8 : 4));
+		for (size_t i = 0; i < count; i++) {
+			if (want64)
+				((double*)conv.data())[i] = ((const float*)host.data())[i];
+			else
+				((float*)conv.data())[i] = (float)((const double*)host.data())[i];
+		}
+		src = conv.data();
+	}
+	mmspx_state* s = nullptr;
+	rc = mmspx_create(&d, &s);
+	if (rc) return rc;
+	rc = mmspx_upload(s, src, MMSPX_NODE_MAJOR);
+	if (rc) {
+		mmspx_destroy(s);
+		return rc;
+	}
+	*out = s;
+	return 0;
+}
+
+void mmspx_fill_uniform(double* out, int64_t n, uint64_t seed, double lo, double hi)
+{
+	for (int64_t i = 0; i < n; i++) {
+		uint64_t z = seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+		z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+		z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+		z ^= z >> 31;
+		out[i] = lo + (hi - lo) * ((double)(z >> 11) * 0x1.0p-53);
+	}
+}
+
+}  // extern "C"

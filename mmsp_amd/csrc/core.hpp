@@ -1,0 +1,104 @@
+// core.hpp -- internal declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "../../include/mmsp_amd.h"
+#include "dat.hpp"
+
+namespace mmspx {
+
+// ---- error plumbing ---------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define MMSPX_HIP(call)                                                                        \
+	do {                                                                                       \
+		hipError_t e_ = (call);                                                                \
+		if (e_ != hipSuccess)                                                                  \
+			return ::mmspx::fail(MMSPX_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+	} while (0)
+
+// ---- geometry passed to kernels by value -----------------------------------
+// Axis 0 is the slowest, axis dim-1 the contiguous one (MMSP.grid.cpp:306-313).
+struct Geom {
+	int dim;
+	int n[3];      // extents (unused axes 1)
+	int b0[3];
+	int b1[3];
+	int s[3];      // node strides per axis
+	int nodes;     // product of extents
+	int fields;
+	double w[3];   // 1/(dx*dx), as MMSP.grid.cpp:2062
+	float wf[3];   // the same weight rounded to float (FP32 form)
+	double vol;    // product of dx (energy density -> energy)
+	double dx[3];
+};
+
+Geom make_geom(const mmspx_desc& d);
+int validate_desc(const mmspx_desc* d);
+
+// ---- model parameters, pre-folded the way the reference's expressions fold
+struct ModelParams {
+	int model;
+	double nr, u, K, dtM, kappa; // nr = -r; dtM = dt*M (left-to-right as in the reference)
+	double dt;                   // grain growth
+	float nrf, uf, Kf, dtMf, kappaf, dtf;
+};
+int make_params(int model, const double* p, int np, ModelParams* out);
+int model_nparams(int model);
+
+// ---- the grid state ------------------------------------------------------------
+struct TuneChoice {
+	int variant;  // 1 generic, 2 tiled/fused
+	int ty, tz, chunk;
+};
+
+struct mmspx_state_impl;
+
+}  // namespace mmspx
+
+struct mmspx_state {
+	mmspx_desc desc;
+	mmspx::Geom geom;
+	size_t elem;           // 4 or 8
+	size_t bytes;          // one state buffer
+	void* cur = nullptr;   // current state (field-major)
+	void* nxt = nullptr;   // next state
+	void* aux = nullptr;   // chemical potential / staging
+	double* part = nullptr;  // reduction partials (device)
+	double* part_host = nullptr;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	int device = 0;
+	int variant = 0;
+	int64_t launches = 0;
+};
+
+namespace mmspx {
+
+constexpr int kRedBlocks = 512;
+constexpr int kRedThreads = 256;
+
+// ---- kernel launchers (kernels_generic.hip, kernels_fused.hip, reduce.hip) ----
+// Each returns 0 or a negative code; all enqueue on `st`.
+int launch_generic_step(const Geom& g, int prec, const ModelParams& p, const void* a, void* b, void* aux,
+                        hipStream_t st, int64_t* launches);
+bool fused_supported(const Geom& g, int prec, int model);
+int launch_fused_step(const Geom& g, int prec, const ModelParams& p, const void* a, void* b, hipStream_t st,
+                      const TuneChoice& t, int64_t* launches);
+TuneChoice default_choice(const Geom& g, int prec, int model);
+int launch_reduce(const Geom& g, int prec, int what, const ModelParams& p, const void* a, double* part,
+                  hipStream_t st);
+int launch_transpose(const void* src, void* dst, int nodes, int fields, size_t elem, bool to_field_major,
+                     hipStream_t st);
+int launch_convert(const void* src, int src_f64, void* dst, int dst_f64, int64_t n, hipStream_t st);
+
+// host-side compensated sum of the (sum, comp) partial pairs
+double sum_partials(const double* part, int count);
+
+}  // namespace mmspx

@@ -42,6 +42,7 @@ class DatGrid:
     b0: list = field(default_factory=list)
     b1: list = field(default_factory=list)
     dx: list = field(default_factory=list)
+    hdr_fields: int = -1  # header "fields" value; scalar grids made by generate() carry 0
 
     @property
     def dim(self) -> int:
@@ -86,7 +87,8 @@ def payload(g: DatGrid) -> bytes:
 
 
 def write_dat(path: str, g: DatGrid, compress: bool = True) -> bytes:
-    hdr = [g.type_name, str(g.dim), str(g.fields)]
+    nf = g.hdr_fields if g.hdr_fields >= 0 else (g.fields if g.vector else 0)
+    hdr = [g.type_name, str(g.dim), str(nf)]
     for d in range(g.dim):
         hdr.append("%d %d" % (g.g0[d], g.g0[d] + g.extents[d]))
     for d in range(g.dim):
@@ -165,4 +167,4 @@ def read_dat(path_or_bytes) -> DatGrid:
                 b0[d] = bcs[2 * d]
             if hi[d] == g1[d]:
                 b1[d] = bcs[2 * d + 1]
-    return DatGrid(data=data, vector=vector, g0=g0, b0=b0, b1=b1, dx=dx)
+    return DatGrid(data=data, vector=vector, g0=g0, b0=b0, b1=b1, dx=dx, hdr_fields=fields)

@@ -1,0 +1,196 @@
+"""GPU parity: the HIP path through the C ABI against the oracle and the
+reference's golden vectors.
+
+Bars (the tolerance of each comparison is written next to it):
+  * FP64 form vs the oracle with the correctly rounded cube: bitwise.
+  * FP64 form vs the reference's own output (libm pow cube): normwise
+    max|gpu-ref|/max|ref| <= 1e-12 per field.
+  * F32REF form (the reference's grid<dim,float> semantics) vs oracle: bitwise;
+    vs the reference: <= 1e-6 normwise.
+  * F32 form (float arithmetic) vs the oracle's F32 restatement: bitwise.
+  * .dat written from device state: byte-identical to the reference's file.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import dat
+from conftest import golden_cases, golden_path, model_of, params_of
+
+pytestmark = pytest.mark.gpu
+
+mx = pytest.importorskip("mmsp_amd")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def normwise(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def oracle_run(model, a, params, steps, form, bc=None, dx=None, cube=O.CUBE_CR):
+    if model == "ac":
+        return O.ac(a, params, steps, form=form, cube=cube, bc=bc, dx=dx)
+    if model == "ch":
+        return O.ch(a, params, steps, mu=O.MU_POLY, form=form, cube=cube, bc=bc, dx=dx)
+    if model == "ch_dw":
+        return O.ch(a, params, steps, mu=O.MU_DW, form=form, cube=cube, bc=bc, dx=dx)
+    return O.gg(a, params[0], steps, form=form, cube=cube, bc=bc, dx=dx)
+
+
+CASES = golden_cases()
+
+
+@pytest.mark.parametrize("variant", [1, 0])
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_golden_case(case, variant):
+    name = case["name"]
+    prec = "f64" if case["dtype"] == "f64" else "f32ref"
+    say("[gpu] golden %s variant=%d precision=%s" % (name, variant, prec))
+    gin = dat.read_dat(golden_path(name, "in"))
+    gout = dat.read_dat(golden_path(name, "out"))
+    with mx.Grid.from_dat(golden_path(name, "in"), precision=prec) as g:
+        g.set_variant(variant)
+        g.step(model_of(case), params_of(case), case["steps"])
+        got = g.download()
+    exp = oracle_run(model_of(case), gin.data, params_of(case), case["steps"], prec,
+                     bc=list(zip(gin.b0, gin.b1)), dx=gin.dx)
+    assert got.dtype == exp.dtype
+    assert np.array_equal(got, exp), (name, normwise(got, exp))
+    tol = 1e-12 if prec == "f64" else 1e-6
+    if gout.vector:
+        for j in range(gout.fields):
+            assert normwise(got[..., j], gout.data[..., j]) <= tol
+    else:
+        assert normwise(got, gout.data) <= tol
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_dat_written_from_device_is_byte_identical(case, tmp_path):
+    name = case["name"]
+    say("[gpu] dat round trip %s" % name)
+    src = golden_path(name, "out")
+    out = str(tmp_path / "w.dat")
+    with mx.Grid.from_dat(src) as g:
+        g.write_dat(out, compress=True)
+    assert open(out, "rb").read() == open(src, "rb").read()
+    with mx.Grid.from_dat(src) as g:
+        g.write_dat(out, compress=False)
+    r = dat.read_dat(out)
+    assert np.array_equal(r.data, dat.read_dat(src).data)
+
+
+P, N, M, D = O.PERIODIC, O.NEUMANN, O.MIRROR, O.DIRICHLET
+F32_CASES = [
+    ("ac", (200,), None, [1.0, 1.0, 1.3, 1.0, 0.01], 40, (-1.0, 1.0)),
+    ("ac", (37, 29), [(N, M), (P, D)], [1.0, 1.0, 1.0, 1.0, 0.01], 40, (-1.0, 1.0)),
+    ("ac", (64, 64), None, [1.0, 1.0, 1.0, 1.0, 0.01], 100, (-0.01, 0.01)),
+    ("ac", (11, 9, 13), [(M, N), (D, P), (P, P)], [1.0, 1.0, 1.0, 1.0, 0.01], 30, (-1.0, 1.0)),
+    ("ch", (33, 31), None, [1.0, 1.0, 1.0, 1.0, 0.01], 30, (-1.0, 1.0)),
+    ("ch_dw", (40, 36), [(N, N), (M, M)], [1.0, 1.0, 0.005], 60, (0.45, 0.55)),
+    ("ch_dw", (16, 18, 20), None, [1.0, 1.0, 0.005], 60, (0.45, 0.55)),
+    ("ch_dw", (32, 32, 64), None, [1.0, 1.0, 0.005], 40, (0.45, 0.55)),
+    ("gg", (30, 26), None, [0.05], 30, (0.0, 0.01)),
+    ("gg", (9, 10, 11), [(N, M), (P, P), (D, N)], [0.01], 20, (0.0, 1.0)),
+]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64", "f32ref"])
+@pytest.mark.parametrize("idx", range(len(F32_CASES)))
+def test_seeded_forms_bitwise(idx, prec):
+    model, shape, bc, params, steps, (lo, hi) = F32_CASES[idx]
+    q = 5 if model == "gg" else 0
+    say("[gpu] seeded %s %s %s bc=%s steps=%d" % (model, shape, prec, bc, steps))
+    n = int(np.prod(shape)) * max(q, 1)
+    a = O.uniform(n, 1000 + idx, lo, hi).reshape(tuple(shape) + ((q,) if q else ()))
+    if prec != "f64":
+        a = a.astype(np.float32)
+    for variant in (1, 0):
+        with mx.Grid(shape, fields=max(q, 1), vector=q > 0, precision=prec, bc=bc) as g:
+            g.set_variant(variant)
+            g.upload(a)
+            g.step(model, params, steps)
+            got = g.download()
+        exp = oracle_run(model, a, params, steps, prec, bc=bc)
+        assert np.array_equal(got, exp), (model, shape, prec, variant, normwise(got, exp))
+
+
+def test_upload_download_layouts():
+    say("[gpu] layouts")
+    a = O.uniform(7 * 6 * 5 * 3, 4, 0, 1).reshape(7, 6, 5, 3)
+    with mx.Grid((7, 6, 5), fields=3, vector=True) as g:
+        g.upload(a)
+        assert np.array_equal(g.download(), a)
+        fm = g.download(layout=mx.FIELD_MAJOR)
+        assert np.array_equal(fm, np.moveaxis(a, -1, 0))
+        g.upload(fm, layout=mx.FIELD_MAJOR)
+        assert np.array_equal(g.download(), a)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_reductions_match_oracle(prec):
+    say("[gpu] reductions %s" % prec)
+    rng_a = O.uniform(48 * 40, 3, 0.3, 0.7).reshape(48, 40)
+    v = O.uniform(20 * 22 * 4, 5, 0.0, 1.0).reshape(20, 22, 4)
+    bc = [(N, M), (P, D)]
+    if prec == "f32":
+        rng_a, v = rng_a.astype(np.float32), v.astype(np.float32)
+    with mx.Grid((48, 40), precision=prec, bc=bc, dx=[0.5, 2.0]) as g:
+        g.upload(rng_a)
+        assert abs(g.mass()[0] - O.mass(rng_a)[0]) <= 1e-15 * abs(O.mass(rng_a)[0])
+        for model, params, kind, ep in [("ac", [1.0, 1.0, 1.7, 1.0, 0.01], O.E_POLY, [1.0, 1.0, 1.7]),
+                                        ("ch_dw", [0.8, 1.0, 0.005], O.E_DW, [0.8])]:
+            e = g.energy(model, params)
+            r = O.energy(rng_a, kind, ep, bc=bc, dx=[0.5, 2.0])
+            assert abs(e - r) <= 1e-13 * abs(r), (model, e, r)
+    with mx.Grid((20, 22), fields=4, vector=True, precision=prec) as g:
+        g.upload(v)
+        assert np.allclose(g.mass(), O.mass(v, vector=True), rtol=1e-15, atol=0)
+        e = g.energy("gg", [0.05])
+        r = O.energy(v, O.E_GG, [2.0], vector=True)
+        assert abs(e - r) <= 1e-13 * abs(r)
+
+
+def test_c1_allen_cahn_full_size_bitwise():
+    """512^2 FP64, 1000 steps, phi0 ~ U(-0.01, 0.01) from seed 20260101."""
+    say("[gpu] C1 512^2 AC 1000 steps")
+    a = O.uniform(512 * 512, 20260101, -0.01, 0.01).reshape(512, 512)
+    with mx.Grid((512, 512)) as g:
+        g.upload(a)
+        g.step("ac", [1.0, 1.0, 1.0, 1.0, 0.01], 1000)
+        got = g.download()
+    exp = O.ac(a, [1.0, 1.0, 1.0, 1.0, 0.01], 1000)
+    assert np.array_equal(got, exp)
+
+
+def test_c2_cahn_hilliard_full_size_mass():
+    """256^3 FP64, 2000 steps: mass conserved to 1e-12, state finite and bounded,
+    generic two-pass and fused paths agree bitwise after 50 steps."""
+    say("[gpu] C2 256^3 CH mass")
+    n = 256
+    a = O.uniform(n ** 3, 20260102, 0.45, 0.55).reshape(n, n, n)
+    p = [1.0, 1.0, 0.005]
+    with mx.Grid((n, n, n)) as g:
+        g.upload(a)
+        m0 = g.mass()[0]
+        g.step("ch_dw", p, 2000)
+        m1 = g.mass()[0]
+        got = g.download()
+    assert abs(m1 - m0) / abs(m0) <= 1e-12, (m0, m1)
+    assert np.all(np.isfinite(got)) and got.min() > -0.5 and got.max() < 1.5
+    outs = []
+    for variant in (1, 2):
+        with mx.Grid((n, n, n)) as g:
+            g.set_variant(variant)
+            g.upload(a)
+            g.step("ch_dw", p, 50)
+            outs.append(g.download())
+    assert np.array_equal(outs[0], outs[1])
