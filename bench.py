@@ -1,0 +1,325 @@
+"""Benchmark of the explicit update path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ch3d|ac2d|gg2d]
+                    [--no-cpu-baseline] [--no-tune]
+
+Default workload: 3-D Cahn-Hilliard, grid<3,double> 256^3 periodic,
+mu = 2c(1-c)(1-2c) - kappa*lap(c), dc/dt = M*lap(mu), kappa = M = dx = 1,
+dt = 0.005, c0 = 0.5 + 0.05*U(-1,1) (SplitMix64 seed 20260102), FP64,
+2000 steps (one step = one explicit time step over the whole grid, the
+state resident in HBM). Prints ONE JSON line.
+
+value = cell-updates/s over the timed steps (one node advanced one step).
+roofline.achieved = algorithmic bytes per launch / mean device time of one
+launch, timed live with a HIP event pair around every step on the stream the
+kernel runs on; algorithmic bytes per cell-update: 16 for FP64 CH/AC (read c,
+write c'), 8 per field for FP32 forms.
+cpu_baseline = the reference's own update() (oracle/_ref, compiled from the
+reference sources) timed serially on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+WORKLOADS = {
+    "ch3d": dict(model="ch_dw", shape=(256, 256, 256), prec="f64", params=[1.0, 1.0, 0.005], steps=2000,
+                 seed=20260102, lo=0.45, hi=0.55, fields=1, bytes_per_cell=16,
+                 name="cahn-hilliard-3d", cpu_steps=2),
+    "ac2d": dict(model="ac", shape=(512, 512), prec="f64", params=[1.0, 1.0, 1.0, 1.0, 0.01], steps=1000,
+                 seed=20260101, lo=-0.01, hi=0.01, fields=1, bytes_per_cell=16,
+                 name="allen-cahn-2d", cpu_steps=100),
+    "gg2d": dict(model="gg", shape=(2048, 2048), prec="f32", params=[0.05], steps=500, seed=20260103,
+                 lo=0.0, hi=0.01, fields=32, bytes_per_cell=256, name="grain-growth-2d-q32", cpu_steps=2),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, init):
+    """Time the reference's update() serially on a bounded sample of the workload."""
+    import oracle as O
+    from oracle import dat
+
+    shape, steps = w["shape"], w["cpu_steps"]
+    nodes = int(np.prod(shape))
+    if O.ref_available():
+        with tempfile.TemporaryDirectory() as td:
+            fin = os.path.join(td, "in.dat")
+            vec = w["fields"] > 1
+            data = init if not vec else init.reshape(shape + (w["fields"],))
+            if w["prec"] != "f64":
+                data = data.astype(np.float32)
+            dat.write_dat(fin, dat.make_grid(data, vector=vec), compress=False)
+            model = {"ch_dw": "ch", "ac": "ac", "gg": "gg"}[w["model"]]
+            variant = "config"
+            sec = O.ref_run(model, fin, "-", steps, variant, w["params"], timeout=600)
+        kind, what = "reference", "oracle/_ref reference update() compiled from the reference sources"
+    else:
+        t0 = time.perf_counter()
+        if w["model"] == "ch_dw":
+            O.ch(init.reshape(shape), w["params"], steps, mu=O.MU_DW)
+        elif w["model"] == "ac":
+            O.ac(init.reshape(shape), w["params"], steps)
+        else:
+            O.gg(init.reshape(shape + (w["fields"],)), w["params"][0], steps)
+        sec = time.perf_counter() - t0
+        kind, what = "port", "oracle/liboracle C restatement"
+    return {
+        "value": nodes * steps / sec,
+        "unit": "cell-updates/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": "%s, %s, %d steps of the full %s grid, serial, update() only (I/O excluded)"
+        % (what, w["name"], steps, "x".join(map(str, shape))),
+    }
+
+
+def c4_members(seed=20260104, count=64):
+    """64 Allen-Cahn instances: sides from {128..2048} and kappa ~ U(0.5, 2) drawn from the seed."""
+    import mmsp_amd as mx
+
+    pick = mx.fill_uniform(count, seed, 0.0, 5.0)
+    kap = mx.fill_uniform(count, seed + 1, 0.5, 2.0)
+    sides = [128 << int(v) for v in pick]
+    return sides, kap
+
+
+def bench_batch_ac(args):
+    """Batched Allen-Cahn: 64 uneven FP32 instances in one persistent launch per
+    step, hipGraph of 100 steps, autotuned strips; vs the same 64 solves run one
+    after another on the plain FP64 single-grid path."""
+    import mmsp_amd as mx
+
+    sides, kap = c4_members()
+    K = args.steps if args.steps is not None else 1000
+    G = 100
+    shapes = [(s, s) for s in sides]
+    params = np.array([[1.0, 1.0, k, 1.0, 0.01] for k in kap])
+    inits = [mx.fill_uniform(s * s, 20260105 + m, -0.01, 0.01).reshape(s, s) for m, s in enumerate(sides)]
+    cells = sum(s * s for s in sides)
+    b = mx.Batch(shapes, "ac", precision="f32")
+    b.set_params(params)
+    for m, a in enumerate(inits):
+        b.upload(m, a.astype(np.float32))
+    report = b.autotune(trial_steps=5)
+    log(report)
+    b.step(args.warmup, graph_steps=0)
+    for m, a in enumerate(inits):
+        b.upload(m, a.astype(np.float32))
+    # the batch's step counter parity is where the warm-up left it; the solves
+    # start from the uploaded state either way
+    t0 = time.perf_counter()
+    dev_s = b.step_timed(K, graph_steps=G)
+    wall_s = time.perf_counter() - t0
+    out32 = [b.download(m) for m in range(len(sides))]
+    tile = b.get_tile()
+    b.close()
+    # the same solves, sequentially, plain FP64 single grids
+    seq_s = 0.0
+    err = 0.0
+    for m, s in enumerate(sides):
+        with mx.Grid((s, s), precision="f64") as g:
+            g.upload(inits[m])
+            g.step("ac", params[m], 2)
+            g.upload(inits[m])
+            sec, _ = g.step_timed("ac", params[m], K)
+            seq_s += sec
+            r64 = g.download()
+        err = max(err, float(np.abs(out32[m] - r64).max() / np.abs(r64).max()))
+    value = cells * K / dev_s
+    return {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": dev_s / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched-allen-cahn-64",
+            "members": len(sides),
+            "sides": sides,
+            "cells": cells,
+            "graph_steps": G,
+            "tile": tile,
+            "solves_per_s": len(sides) / dev_s,
+            "fp64_sequential_solves_per_s": len(sides) / seq_s,
+            "speedup_vs_fp64_sequential": seq_s / dev_s,
+            "max_normwise_err_vs_fp64": err,
+            "wall_s_timed": wall_s,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": value * 8 / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": value * 8 / 1e9 / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles("ac_batch_batch_ac"),
+            "kernel": "ac_batch",
+            "algorithmic_bytes_per_launch": 8 * cells,
+        },
+    }
+
+
+def traffic_from_profiles(kernel_tag):
+    """HBM bytes per launch from committed rocprofv3 --pmc summaries, if any."""
+    p = os.path.join(ROOT, "profiles", kernel_tag + ".traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="ch3d", choices=sorted(WORKLOADS) + ["batch_ac"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner first instead of the tuned default tile")
+    ap.add_argument("--tile", default=None, help="ty,tz,chunk for the fused kernel")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the path is the per-GPU kernel; "
+              "multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import mmsp_amd as mx
+
+    if args.workload == "batch_ac":
+        print(json.dumps(bench_batch_ac(args)), flush=True)
+        return
+    w = WORKLOADS[args.workload]
+    K = args.steps if args.steps is not None else w["steps"]
+    W = args.warmup
+    shape = w["shape"]
+    nodes = int(np.prod(shape))
+    q = w["fields"]
+    init = mx.fill_uniform(nodes * q, w["seed"], w["lo"], w["hi"])
+
+    g = mx.Grid(shape, fields=q, vector=q > 1, precision=w["prec"])
+    g.upload(init.astype(g.dtype).reshape(g.host_shape))
+    tune_report = ""
+    if args.tile:
+        g.set_tile(*[int(v) for v in args.tile.split(",")])
+    elif args.tune and w["model"] in ("ch_dw", "ch"):
+        tune_report = g.autotune(w["model"], w["params"], trial_steps=10)
+        log(tune_report)
+        g.upload(init.astype(g.dtype).reshape(g.host_shape))
+    tile = g.get_tile()
+    m0 = g.mass()
+
+    g.step(w["model"], w["params"], W)
+    g.sync()
+    t0 = time.perf_counter()
+    dev_s, launches = g.step_timed(w["model"], w["params"], K)
+    wall_s = time.perf_counter() - t0
+    m1 = g.mass()
+    state = g.download()
+    finite = bool(np.all(np.isfinite(state)))
+    err64 = None
+    if w["prec"] != "f64":
+        # the same K+W steps in FP64 (bitwise the reference's double semantics
+        # with a correctly rounded cube) from the float-rounded start
+        with mx.Grid(shape, fields=q, vector=q > 1, precision="f64") as g64:
+            g64.upload(init.astype(np.float32).astype(np.float64).reshape(g64.host_shape))
+            g64.step(w["model"], w["params"], W + K)
+            r64 = g64.download()
+            e64 = g64.energy(w["model"], w["params"])
+        d = np.abs(state.astype(np.float64) - r64)
+        if q > 1:
+            err64 = float(max(d[..., j].max() / np.abs(r64[..., j]).max() for j in range(q)))
+        else:
+            err64 = float(d.max() / np.abs(r64).max())
+
+    # Mean device time per launch over the timed region (HIP events on the
+    # state's stream bracket the K back-to-back steps); with the fused path
+    # one step is one launch of the dominant kernel. A second measurement
+    # brackets every step with its own event pair (adds launch gaps).
+    lps = launches / K
+    mean_launch_s = dev_s / launches
+    reps = min(K, 200)
+    mean_s, min_s = g.kernel_time(w["model"], w["params"], reps)
+    g.close()
+
+    alg_bytes = w["bytes_per_cell"] * nodes
+    per_launch = alg_bytes / lps
+    achieved = per_launch / mean_launch_s / 1e9
+    value = nodes * K / dev_s
+    kernel_tag = "%s_%s" % ("ch_fused" if tile["variant"] == 2 and w["model"] in ("ch", "ch_dw") else w["model"],
+                             args.workload)
+    out = {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": dev_s / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": w["prec"],
+        "data": "synthetic",
+        "config": {
+            "workload": w["name"],
+            "grid": list(shape),
+            "fields": q,
+            "model": w["model"],
+            "params": w["params"],
+            "precision": w["prec"],
+            "boundary": "periodic",
+            "seed": w["seed"],
+            "tile": tile,
+            "launches_per_step": launches / K,
+            "wall_s_timed": wall_s,
+            "hbm_frac_of_value": value * w["bytes_per_cell"] / (HBM_PEAK_GBS * 1e9),
+            "mass_rel_change": float(np.max(np.abs(m1 - m0) / np.abs(m0))),
+            "max_normwise_err_vs_fp64": err64,
+            "finite": finite,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(kernel_tag),
+            "kernel": kernel_tag,
+            "kernel_mean_us": mean_launch_s * 1e6,
+            "kernel_isolated_mean_us": mean_s / lps * 1e6,
+            "kernel_isolated_min_us": min_s / lps * 1e6,
+            "algorithmic_bytes_per_launch": per_launch,
+        },
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(w, init)
+        except Exception as e:  # the baseline is reported, never the measured value
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -141,13 +141,34 @@ int mmspx_peek_dat(const char* filename, mmspx_desc* desc);
  * (i+1)-th SplitMix64 output from state `seed`, over [0,1). */
 void mmspx_fill_uniform(double* out, int64_t n, uint64_t seed, double lo, double hi);
 
-/* Kernel variant selection ("auto" picks per shape via the tuner cache). */
-int mmspx_set_variant(mmspx_state* s, int variant); /* 0 auto, 1 generic, 2 tiled/fused */
+/* Kernel variant selection: 0 auto (tuned tile if any, else the default),
+ * 1 the generic one-thread-per-node kernels, 2 the tiled/fused kernels
+ * (MMSPX_EUNSUPPORTED where none exists for the model/shape/boundaries). */
+int mmspx_set_variant(mmspx_state* s, int variant);
+/* Tile of the fused kernels: ty x tz cells of the (axis 1, last axis) plane,
+ * marching `chunk` planes along axis 0 (2-D grids: ty = 1). */
+int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
+int mmspx_get_tile(const mmspx_state* s, int* ty, int* tz, int* chunk);
+/* Project autotuner: times every candidate tile for `model` on this state's
+ * shape (trial_steps each, state restored afterwards), keeps the fastest and
+ * writes a one-line-per-candidate report. */
+int mmspx_autotune(mmspx_state* s, int model, const double* params, int nparams, int trial_steps, char* report,
+                   size_t report_len);
+/* Per-step device time: `reps` steps, each bracketed by its own HIP event
+ * pair on the state's stream; mean and minimum seconds per step. */
+int mmspx_kernel_time(mmspx_state* s, int model, const double* params, int nparams, int reps, double* mean_s,
+                      double* min_s);
+/* Device-to-device copy kernel over `bytes` (read + write), best of `reps`;
+ * gives the attainable HBM rate and the counter calibration kernel. */
+int mmspx_copy_bandwidth(size_t bytes, int reps, double* bytes_per_s);
 
 /* ---- batches: many independent grids advanced together ---------------------
- * All members share model and precision; extents, parameters and step counts
- * may differ. One kernel launch per step covers every member's tiles through
- * a device work list; multi-step runs replay a captured hipGraph. */
+ * All members share model, precision and field count and advance by the same
+ * number of steps; extents and parameters may differ. One persistent kernel
+ * launch per step walks a device work list holding every member's tiles
+ * (2-D Allen-Cahn / grain growth, periodic 2-D and 3-D Cahn-Hilliard; 3-D
+ * members take a second launch); multi-step runs replay a captured hipGraph.
+ * No counterpart in the reference, which advances one grid per process. */
 typedef struct mmspx_batch mmspx_batch;
 
 int mmspx_batch_create(const mmspx_desc* descs, int count, int model, mmspx_batch** out);
@@ -161,9 +182,12 @@ int mmspx_batch_set_params(mmspx_batch* b, const double* params, int nparams);
 int mmspx_batch_step(mmspx_batch* b, int steps, int graph_steps);
 int mmspx_batch_step_timed(mmspx_batch* b, int steps, int graph_steps, double* seconds);
 int mmspx_batch_reduce(mmspx_batch* b, int member, int what, double* out, int nout);
-/* Tile configuration: autotune over the candidates on the batch's shapes. */
+/* Project autotuner: times every candidate (strip width, rows per work item,
+ * persistent workgroups per CU) on the batch itself, keeps the fastest and
+ * restores the members' state. */
 int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t report_len);
-int mmspx_batch_get_tile(const mmspx_batch* b, int* tile_y, int* tile_x, int* cells_per_thread);
+/* Returns 1 when the tile came from the autotuner, 0 for the default. */
+int mmspx_batch_get_tile(const mmspx_batch* b, int* strip_width, int* rows_per_item, int* blocks_per_cu);
 
 /* ---- serving: continuous batching of solves ------------------------------ */
 typedef struct mmspx_server mmspx_server;

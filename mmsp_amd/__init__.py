@@ -105,6 +105,11 @@ def _load():
         "mmspx_peek_dat": (C.c_int, [C.c_char_p, P(Desc)]),
         "mmspx_fill_uniform": (None, [dp, C.c_int64, C.c_uint64, C.c_double, C.c_double]),
         "mmspx_set_variant": (C.c_int, [vp, C.c_int]),
+        "mmspx_set_tile": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "mmspx_get_tile": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "mmspx_autotune": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
+        "mmspx_kernel_time": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, dp, dp]),
+        "mmspx_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, dp]),
         "mmspx_batch_create": (C.c_int, [P(Desc), C.c_int, C.c_int, P(vp)]),
         "mmspx_batch_destroy": (None, [vp]),
         "mmspx_batch_upload": (C.c_int, [vp, C.c_int, vp, C.c_int]),
@@ -269,6 +274,29 @@ class Grid:
     def set_variant(self, v):
         check(lib.mmspx_set_variant(self._h, int(v)))
 
+    def set_tile(self, ty, tz, chunk):
+        check(lib.mmspx_set_tile(self._h, ty, tz, chunk))
+
+    def get_tile(self):
+        ty, tz, ch = C.c_int(), C.c_int(), C.c_int()
+        v = lib.mmspx_get_tile(self._h, C.byref(ty), C.byref(tz), C.byref(ch))
+        return dict(variant=v, ty=ty.value, tz=tz.value, chunk=ch.value)
+
+    def autotune(self, model, params, trial_steps=20):
+        m = _model(model)
+        p = _params(m, params)
+        buf = C.create_string_buffer(8192)
+        check(lib.mmspx_autotune(self._h, m, _dp(p), p.size, int(trial_steps), buf, 8192))
+        return buf.value.decode()
+
+    def kernel_time(self, model, params, reps):
+        """(mean, min) device seconds of one step, each step bracketed by its own events."""
+        m = _model(model)
+        p = _params(m, params)
+        mean, mn = C.c_double(), C.c_double()
+        check(lib.mmspx_kernel_time(self._h, m, _dp(p), p.size, int(reps), C.byref(mean), C.byref(mn)))
+        return mean.value, mn.value
+
     def mass(self):
         out = np.zeros(self.fields, dtype=np.float64)
         check(lib.mmspx_reduce(self._h, MASS, 0, None, 0, _dp(out), out.size))
@@ -283,6 +311,88 @@ class Grid:
 
     def write_dat(self, path, compress=True):
         check(lib.mmspx_write_dat(self._h, path.encode(), 1 if compress else 0))
+
+
+class Batch:
+    """Independent grids advanced together: one persistent launch per step over
+    every member's tiles, multi-step runs replayed from a captured hipGraph."""
+
+    def __init__(self, shapes, model, precision="f32", fields=1, vector=False, bcs=None, dxs=None):
+        self.model = _model(model)
+        descs = (Desc * len(shapes))()
+        for m, sh in enumerate(shapes):
+            descs[m] = make_desc(sh, fields, vector, precision, bcs[m] if bcs else None, dxs[m] if dxs else None)
+        h = C.c_void_p()
+        check(lib.mmspx_batch_create(descs, len(shapes), self.model, C.byref(h)))
+        self._h = h
+        self.shapes = [tuple(s) for s in shapes]
+        self.fields = fields if vector else 1
+        self.vector = vector
+        self.dtype = np.float64 if make_desc((1,), precision=precision).precision == F64 else np.float32
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.mmspx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def host_shape(self, m):
+        return self.shapes[m] + ((self.fields,) if self.vector else ())
+
+    def upload(self, m, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != int(np.prod(self.host_shape(m))):
+            raise ValueError("array does not match member %d" % m)
+        check(lib.mmspx_batch_upload(self._h, m, a.ctypes.data_as(C.c_void_p), NODE_MAJOR))
+
+    def download(self, m):
+        out = np.empty(self.host_shape(m), dtype=self.dtype)
+        check(lib.mmspx_batch_download(self._h, m, out.ctypes.data_as(C.c_void_p), NODE_MAJOR))
+        return out
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        if p.ndim != 2 or p.shape[0] != len(self.shapes):
+            raise ValueError("params must have one row per member")
+        check(lib.mmspx_batch_set_params(self._h, _dp(p), p.shape[1]))
+
+    def step(self, steps, graph_steps=0):
+        check(lib.mmspx_batch_step(self._h, int(steps), int(graph_steps)))
+
+    def step_timed(self, steps, graph_steps=0):
+        sec = C.c_double()
+        check(lib.mmspx_batch_step_timed(self._h, int(steps), int(graph_steps), C.byref(sec)))
+        return sec.value
+
+    def mass(self, m):
+        out = np.zeros(self.fields, dtype=np.float64)
+        check(lib.mmspx_batch_reduce(self._h, m, MASS, _dp(out), out.size))
+        return out
+
+    def autotune(self, trial_steps=10):
+        buf = C.create_string_buffer(16384)
+        check(lib.mmspx_batch_autotune(self._h, int(trial_steps), buf, 16384))
+        return buf.value.decode()
+
+    def get_tile(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        tuned = lib.mmspx_batch_get_tile(self._h, C.byref(a), C.byref(b), C.byref(c))
+        return dict(strip_width=a.value, rows_per_item=b.value, blocks_per_cu=c.value, tuned=bool(tuned))
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=5):
+    """Attainable device copy rate (bytes read + written per second)."""
+    out = C.c_double()
+    check(lib.mmspx_copy_bandwidth(int(nbytes), int(reps), C.byref(out)))
+    return out.value
 
 
 def peek_dat(path) -> Desc:

@@ -6,6 +6,7 @@
 // two state buffers swapped by pointer after every step (grid::swap,
 // MMSP.grid.cpp:1913-1991) plus, for Cahn-Hilliard, a chemical-potential
 // buffer. Vector grids are stored field-major (one plane per field).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <mutex>
@@ -270,7 +271,8 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	bool fused = false;
 	TuneChoice tc{};
 	if (s->variant != 1 && fused_supported(g, s->desc.precision, model)) {
-		tc = default_choice(g, s->desc.precision, model);
+		tc = s->tuned ? s->tile : default_choice(g, s->desc.precision, model);
+		if (s->variant == 2 && tc.variant != 2) tc = default_choice(g, s->desc.precision, model);
 		fused = s->variant == 2 || tc.variant == 2;
 	} else if (s->variant == 2) {
 		return fail(MMSPX_EUNSUPPORTED, "no tiled/fused kernel for this model, shape and boundary condition");
@@ -290,6 +292,154 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 int mmspx_step(mmspx_state* s, int model, const double* params, int np, int steps)
 {
 	return step_impl(s, model, params, np, steps);
+}
+
+int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
+{
+	if (!s) return fail(MMSPX_EINVAL, "null state");
+	TuneChoice t{2, ty, tz, chunk};
+	if (!tile_known(s->geom, t)) return fail(MMSPX_EINVAL, "no compiled tile of that shape for this grid");
+	s->tile = t;
+	s->tuned = true;
+	return 0;
+}
+
+int mmspx_get_tile(const mmspx_state* s, int* ty, int* tz, int* chunk)
+{
+	if (!s) return fail(MMSPX_EINVAL, "null state");
+	TuneChoice t = s->tuned ? s->tile : default_choice(s->geom, s->desc.precision, MMSPX_CH_DW);
+	if (ty) *ty = t.ty;
+	if (tz) *tz = t.tz;
+	if (chunk) *chunk = t.chunk;
+	return t.variant;
+}
+
+int mmspx_kernel_time(mmspx_state* s, int model, const double* params, int np, int reps, double* mean_s,
+                      double* min_s)
+{
+	if (!s || reps < 1 || reps > 100000) return fail(MMSPX_EINVAL, "bad arguments");
+	std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+	int rc = 0;
+	for (auto& e : ev) {
+		if (hipEventCreate(&e) != hipSuccess) {
+			rc = fail(MMSPX_EDEVICE, "hipEventCreate failed");
+			break;
+		}
+	}
+	for (int k = 0; k < reps && rc == 0; k++) {
+		if (hipEventRecord(ev[2 * k], s->stream) != hipSuccess) rc = fail(MMSPX_EDEVICE, "hipEventRecord failed");
+		if (rc == 0) rc = step_impl(s, model, params, np, 1);
+		if (rc == 0 && hipEventRecord(ev[2 * k + 1], s->stream) != hipSuccess)
+			rc = fail(MMSPX_EDEVICE, "hipEventRecord failed");
+	}
+	if (rc == 0 && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(MMSPX_EDEVICE, "synchronize failed");
+	double sum = 0.0, mn = 1e30;
+	for (int k = 0; k < reps && rc == 0; k++) {
+		float ms = 0.0f;
+		if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) != hipSuccess) {
+			rc = fail(MMSPX_EDEVICE, "hipEventElapsedTime failed");
+			break;
+		}
+		sum += ms * 1e-3;
+		mn = std::min(mn, ms * 1e-3);
+	}
+	for (auto& e : ev)
+		if (e) (void)hipEventDestroy(e);
+	if (rc) return rc;
+	if (mean_s) *mean_s = sum / reps;
+	if (min_s) *min_s = mn;
+	return 0;
+}
+
+int mmspx_autotune(mmspx_state* s, int model, const double* params, int np, int trial_steps, char* report,
+                   size_t report_len)
+{
+	if (!s || trial_steps < 1) return fail(MMSPX_EINVAL, "bad arguments");
+	ModelParams p;
+	int rc = make_params(model, params, np, &p);
+	if (rc) return rc;
+	std::string rep;
+	TuneChoice cand[32];
+	int nc = fused_supported(s->geom, s->desc.precision, model) ? tile_candidates(s->geom, cand, 32) : 0;
+	// keep the state: every trial starts from it and it is restored at the end
+	void* saved = nullptr;
+	if (hipMalloc(&saved, s->bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");
+	MMSPX_HIP(hipMemcpyAsync(saved, s->cur, s->bytes, hipMemcpyDeviceToDevice, s->stream));
+	const int old_variant = s->variant;
+	const bool old_tuned = s->tuned;
+	const TuneChoice old_tile = s->tile;
+	double best = 1e30;
+	TuneChoice best_t{1, 0, 0, 0};
+	for (int k = -1; k < nc && rc == 0; k++) {
+		if (k < 0) {
+			s->variant = 1;
+		} else {
+			s->variant = 2;
+			s->tile = cand[k];
+			s->tuned = true;
+		}
+		double mean = 0.0, mn = 0.0;
+		rc = step_impl(s, model, params, np, 2);  // warm-up (and lazy buffers)
+		if (rc == 0) rc = mmspx_kernel_time(s, model, params, np, trial_steps, &mean, &mn);
+		if (rc == 0 && hipMemcpyAsync(s->cur, saved, s->bytes, hipMemcpyDeviceToDevice, s->stream) != hipSuccess)
+			rc = fail(MMSPX_EDEVICE, "restore copy failed");
+		if (rc) break;
+		char line[160];
+		const TuneChoice t = k < 0 ? TuneChoice{1, 0, 0, 0} : cand[k];
+		std::snprintf(line, sizeof line, "variant=%d ty=%d tz=%d chunk=%d mean_us=%.2f min_us=%.2f\n", t.variant, t.ty,
+		              t.tz, t.chunk, mean * 1e6, mn * 1e6);
+		rep += line;
+		if (mean < best) {
+			best = mean;
+			best_t = t;
+		}
+	}
+	(void)hipStreamSynchronize(s->stream);
+	(void)hipFree(saved);
+	s->variant = old_variant;
+	if (rc) {
+		s->tuned = old_tuned;
+		s->tile = old_tile;
+		return rc;
+	}
+	s->tile = best_t;
+	s->tuned = true;
+	if (report && report_len) {
+		std::snprintf(report, report_len, "%s", rep.c_str());
+	}
+	return 0;
+}
+
+int mmspx_copy_bandwidth(size_t bytes, int reps, double* bytes_per_s)
+{
+	if (bytes < 4096 || reps < 1) return fail(MMSPX_EINVAL, "bad arguments");
+	bytes &= ~(size_t)255;
+	void *a = nullptr, *b = nullptr;
+	hipStream_t st = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	int rc = 0;
+	if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) rc = fail(MMSPX_ENOMEM, "alloc");
+	if (rc == 0 && (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+	                hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+		rc = fail(MMSPX_EDEVICE, "stream/event setup failed");
+	if (rc == 0 && hipMemsetAsync(a, 1, bytes, st) != hipSuccess) rc = fail(MMSPX_EDEVICE, "memset failed");
+	double best = 1e30;
+	for (int k = 0; k < reps + 1 && rc == 0; k++) {
+		(void)hipEventRecord(e0, st);
+		rc = launch_copy(a, b, bytes, st);
+		(void)hipEventRecord(e1, st);
+		if (rc == 0 && hipEventSynchronize(e1) != hipSuccess) rc = fail(MMSPX_EDEVICE, "sync failed");
+		float ms = 0.0f;
+		if (rc == 0) (void)hipEventElapsedTime(&ms, e0, e1);
+		if (k > 0) best = std::min(best, (double)ms * 1e-3);
+	}
+	if (rc == 0 && bytes_per_s) *bytes_per_s = 2.0 * (double)bytes / best;
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	if (st) (void)hipStreamDestroy(st);
+	if (a) (void)hipFree(a);
+	if (b) (void)hipFree(b);
+	return rc;
 }
 
 int mmspx_sync(mmspx_state* s)

@@ -1,15 +1,538 @@
-// batch.hip -- placeholder until the batched solver lands.
-#include "core.hpp"
-using namespace mmspx;
-extern "C" {
-int mmspx_batch_create(const mmspx_desc*, int, int, mmspx_batch** out) { if (out) *out = nullptr; return fail(MMSPX_EUNSUPPORTED, "batch not built"); }
-void mmspx_batch_destroy(mmspx_batch*) {}
-int mmspx_batch_upload(mmspx_batch*, int, const void*, int) { return MMSPX_EUNSUPPORTED; }
-int mmspx_batch_download(mmspx_batch*, int, void*, int) { return MMSPX_EUNSUPPORTED; }
-int mmspx_batch_set_params(mmspx_batch*, const double*, int) { return MMSPX_EUNSUPPORTED; }
-int mmspx_batch_step(mmspx_batch*, int, int) { return MMSPX_EUNSUPPORTED; }
-int mmspx_batch_step_timed(mmspx_batch*, int, int, double*) { return MMSPX_EUNSUPPORTED; }
-int mmspx_batch_reduce(mmspx_batch*, int, int, double*, int) { return MMSPX_EUNSUPPORTED; }
-int mmspx_batch_autotune(mmspx_batch*, int, char*, size_t) { return MMSPX_EUNSUPPORTED; }
-int mmspx_batch_get_tile(const mmspx_batch*, int*, int*, int*) { return MMSPX_EUNSUPPORTED; }
+// batch.hip -- many independent grids advanced together (batched solves).
+//
+// The reference advances one grid per process (MMSP.main.hpp:278-350); serving
+// many small solves that way leaves an MI355X idle. Here every member's tiles
+// go into one device work list and one persistent launch per step walks it
+// (workgroups take items blockIdx.x, blockIdx.x + gridDim.x, ...), whatever the
+// members' extents. Multi-step runs are captured once into a hipGraph (one
+// graph per starting parity, since each launch's arguments name the source
+// buffer) and replayed. The tile bodies are those of the single-grid kernels
+// (tiles.hpp), so each member's bits equal a single-grid run's.
+//
+// Members share model and precision. Supported: Allen-Cahn and grain growth
+// on 2-D grids (any boundary condition), Cahn-Hilliard on periodic 2-D and 3-D
+// grids (2-D and 3-D members go to two work lists, one launch each).
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+
+namespace mmspx {
+
+template <int PREC, int TZ>
+__global__ void __launch_bounds__(TZ) ac_batch(const BMember* __restrict__ mem, const BItem* __restrict__ items,
+                                               int nitems, int parity)
+{
+	using S = typename Form<PREC>::S;
+	for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+		const BItem w = items[it];
+		const BMember& m = mem[w.m];
+		const int src = (parity + m.phase) & 1;
+		ac_strip<PREC, TZ>((const S*)m.buf[src], (S*)m.buf[src ^ 1], m.g, m.p, w.y0, w.xb, w.xe);
+	}
 }
+
+template <int PREC, int Q, int TZ>
+__global__ void __launch_bounds__(TZ) gg_batch(const BMember* __restrict__ mem, const BItem* __restrict__ items,
+                                               int nitems, int parity)
+{
+	using S = typename Form<PREC>::S;
+	for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+		const BItem w = items[it];
+		const BMember& m = mem[w.m];
+		const int src = (parity + m.phase) & 1;
+		gg_strip<PREC, Q, TZ>((const S*)m.buf[src], (S*)m.buf[src ^ 1], m.g, m.p, w.y0, w.xb, w.xe);
+	}
+}
+
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+__global__ void __launch_bounds__(NT) ch_batch(const BMember* __restrict__ mem, const BItem* __restrict__ items,
+                                               int nitems, int parity)
+{
+	using S = typename Form<PREC>::S;
+	for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+		const BItem w = items[it];
+		const BMember& m = mem[w.m];
+		const int src = (parity + m.phase) & 1;
+		ch_tile<PREC, DW, DIM, TY, TZ, NT>((const S*)m.buf[src], (S*)m.buf[src ^ 1], m.g, m.p, w.y0, w.z0, w.xb, w.xe);
+	}
+}
+
+// tile shapes of the batched Cahn-Hilliard kernels (fixed; the strips' width
+// and chunk depth are what the tuner varies)
+constexpr int kCh3TY = 8, kCh3TZ = 64, kCh3NT = 256;
+
+bool batch_model_ok(int model, const Geom& g, std::string* why)
+{
+	auto no = [&](const char* m) {
+		if (why) *why = m;
+		return false;
+	};
+	switch (model) {
+	case MMSPX_AC:
+		if (g.dim != 2) return no("batched Allen-Cahn needs 2-D members");
+		return true;
+	case MMSPX_GG:
+		if (g.dim != 2) return no("batched grain growth needs 2-D members");
+		if (!(g.fields == 2 || g.fields == 4 || g.fields == 8 || g.fields == 16 || g.fields == 32))
+			return no("batched grain growth needs 2, 4, 8, 16 or 32 fields");
+		return true;
+	case MMSPX_CH:
+	case MMSPX_CH_DW:
+		if (g.dim < 2) return no("batched Cahn-Hilliard needs 2-D or 3-D members");
+		for (int k = 0; k < g.dim; k++)
+			if (g.b0[k] != MMSPX_PERIODIC || g.b1[k] != MMSPX_PERIODIC)
+				return no("batched Cahn-Hilliard needs periodic members");
+		return true;
+	}
+	return no("unknown model");
+}
+
+// Work items of one member: strips/tiles of the last axes times chunks of axis 0.
+void member_items(int m, const Geom& g, int model, const BatchTile& t, std::vector<BItem>& out2,
+                  std::vector<BItem>& out3)
+{
+	const int nx = g.n[0];
+	if ((model == MMSPX_CH || model == MMSPX_CH_DW) && g.dim == 3) {
+		const int chunk = t.chunk3;
+		for (int xb = 0; xb < nx; xb += chunk)
+			for (int y0 = 0; y0 < g.n[1]; y0 += kCh3TY)
+				for (int z0 = 0; z0 < g.n[2]; z0 += kCh3TZ)
+					out3.push_back(BItem{m, y0, z0, xb, std::min(xb + chunk, nx)});
+		return;
+	}
+	for (int xb = 0; xb < nx; xb += t.chunk)
+		for (int y0 = 0; y0 < g.n[1]; y0 += t.tz) out2.push_back(BItem{m, y0, 0, xb, std::min(xb + t.chunk, nx)});
+}
+
+template <int PREC, int TZ>
+static void launch_strips(int model, int fields, const BMember* mem, const BItem* items, int n, int parity, int blocks,
+                          hipStream_t st)
+{
+	if (model == MMSPX_AC) {
+		hipLaunchKernelGGL((ac_batch<PREC, TZ>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity);
+	} else if (model == MMSPX_GG) {
+		switch (fields) {
+		case 2: hipLaunchKernelGGL((gg_batch<PREC, 2, TZ>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity); break;
+		case 4: hipLaunchKernelGGL((gg_batch<PREC, 4, TZ>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity); break;
+		case 8: hipLaunchKernelGGL((gg_batch<PREC, 8, TZ>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity); break;
+		case 16: hipLaunchKernelGGL((gg_batch<PREC, 16, TZ>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity); break;
+		default: hipLaunchKernelGGL((gg_batch<PREC, 32, TZ>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity); break;
+		}
+	} else {
+		const bool dw = model == MMSPX_CH_DW;
+		if (dw)
+			hipLaunchKernelGGL((ch_batch<PREC, true, 2, 1, TZ, TZ>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity);
+		else
+			hipLaunchKernelGGL((ch_batch<PREC, false, 2, 1, TZ, TZ>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity);
+	}
+}
+
+template <int PREC>
+static void launch_ch3(bool dw, const BMember* mem, const BItem* items, int n, int parity, int blocks, hipStream_t st)
+{
+	if (dw)
+		hipLaunchKernelGGL((ch_batch<PREC, true, 3, kCh3TY, kCh3TZ, kCh3NT>), dim3(blocks), dim3(kCh3NT), 0, st, mem, items,
+		                   n, parity);
+	else
+		hipLaunchKernelGGL((ch_batch<PREC, false, 3, kCh3TY, kCh3TZ, kCh3NT>), dim3(blocks), dim3(kCh3NT), 0, st, mem,
+		                   items, n, parity);
+}
+
+template <int PREC>
+static void launch_prec(int model, int fields, const BatchTile& t, const BMember* mem, const BItem* l2, int n2,
+                        const BItem* l3, int n3, int parity, int cus, hipStream_t st)
+{
+	if (n2 > 0) {
+		const int blocks = std::max(1, std::min(n2, cus * t.bpc));
+		if (t.tz == 64)
+			launch_strips<PREC, 64>(model, fields, mem, l2, n2, parity, blocks, st);
+		else if (t.tz == 128)
+			launch_strips<PREC, 128>(model, fields, mem, l2, n2, parity, blocks, st);
+		else
+			launch_strips<PREC, 256>(model, fields, mem, l2, n2, parity, blocks, st);
+	}
+	if (n3 > 0) {
+		const int blocks = std::max(1, std::min(n3, cus * t.bpc));
+		launch_ch3<PREC>(model == MMSPX_CH_DW, mem, l3, n3, parity, blocks, st);
+	}
+}
+
+int launch_batch_step(int model, int prec, int fields, const BatchTile& t, const BMember* mem, const BItem* l2, int n2,
+                      const BItem* l3, int n3, int parity, int cus, hipStream_t st, int64_t* launches)
+{
+	switch (prec) {
+	case MMSPX_F64: launch_prec<MMSPX_F64>(model, fields, t, mem, l2, n2, l3, n3, parity, cus, st); break;
+	case MMSPX_F32: launch_prec<MMSPX_F32>(model, fields, t, mem, l2, n2, l3, n3, parity, cus, st); break;
+	case MMSPX_F32REF: launch_prec<MMSPX_F32REF>(model, fields, t, mem, l2, n2, l3, n3, parity, cus, st); break;
+	default: return fail(MMSPX_EINVAL, "unknown precision");
+	}
+	MMSPX_HIP(hipGetLastError());
+	*launches += (n2 > 0) + (n3 > 0);
+	return 0;
+}
+
+int device_cus()
+{
+	int dev = 0, cus = 0;
+	if (hipGetDevice(&dev) != hipSuccess) return 256;
+	if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 256;
+	return cus;
+}
+
+}  // namespace mmspx
+
+using namespace mmspx;
+
+struct mmspx_batch {
+	int count = 0, model = 0, prec = 0, fields = 1;
+	std::vector<mmspx_desc> desc;
+	std::vector<Geom> geom;
+	std::vector<size_t> off;   // byte offset of each member's first buffer in the arena
+	std::vector<size_t> bytes; // one buffer
+	char* arena = nullptr;
+	size_t arena_bytes = 0;
+	std::vector<ModelParams> params;
+	bool params_set = false;
+	BMember* d_mem = nullptr;
+	BItem* d_items = nullptr;
+	int n2 = 0, n3 = 0, items_cap = 0;
+	BatchTile tile{256, 32, 16, 8};
+	bool tuned = false;
+	int cus = 256;
+	int64_t steps_done = 0, launches = 0;
+	hipStream_t st = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	hipGraphExec_t graph[2] = {nullptr, nullptr};
+	int graph_steps = 0;
+	double* part = nullptr;
+	double* part_host = nullptr;
+	void* staging = nullptr;
+	size_t staging_bytes = 0;
+};
+
+static void drop_graphs(mmspx_batch* b)
+{
+	for (auto& gx : b->graph)
+		if (gx) {
+			(void)hipGraphExecDestroy(gx);
+			gx = nullptr;
+		}
+	b->graph_steps = 0;
+}
+
+// (Re)builds the device member table and work lists for the current tile.
+static int upload_plan(mmspx_batch* b)
+{
+	std::vector<BItem> l2, l3;
+	std::vector<BMember> mem(b->count);
+	for (int m = 0; m < b->count; m++) {
+		mem[m].buf[0] = b->arena + b->off[m];
+		mem[m].buf[1] = b->arena + b->off[m] + b->bytes[m];
+		mem[m].g = b->geom[m];
+		mem[m].p = b->params[m];
+		mem[m].phase = 0;
+		member_items(m, b->geom[m], b->model, b->tile, l2, l3);
+	}
+	const int need = (int)(l2.size() + l3.size());
+	if (need > b->items_cap) {
+		if (b->d_items) (void)hipFree(b->d_items);
+		b->d_items = nullptr;
+		if (hipMalloc((void**)&b->d_items, sizeof(BItem) * (size_t)need) != hipSuccess)
+			return fail(MMSPX_ENOMEM, "device allocation failed (work list)");
+		b->items_cap = need;
+	}
+	std::vector<BItem> all(l2);
+	all.insert(all.end(), l3.begin(), l3.end());
+	MMSPX_HIP(hipMemcpyAsync(b->d_items, all.data(), sizeof(BItem) * all.size(), hipMemcpyHostToDevice, b->st));
+	MMSPX_HIP(hipMemcpyAsync(b->d_mem, mem.data(), sizeof(BMember) * mem.size(), hipMemcpyHostToDevice, b->st));
+	MMSPX_HIP(hipStreamSynchronize(b->st));
+	b->n2 = (int)l2.size();
+	b->n3 = (int)l3.size();
+	drop_graphs(b);
+	return 0;
+}
+
+static int enqueue_steps(mmspx_batch* b, int steps)
+{
+	for (int k = 0; k < steps; k++) {
+		const int parity = (int)((b->steps_done + k) & 1);
+		int rc = launch_batch_step(b->model, b->prec, b->fields, b->tile, b->d_mem, b->d_items, b->n2,
+		                           b->d_items + b->n2, b->n3, parity, b->cus, b->st, &b->launches);
+		if (rc) return rc;
+	}
+	b->steps_done += steps;
+	return 0;
+}
+
+static int build_graphs(mmspx_batch* b, int gsteps)
+{
+	drop_graphs(b);
+	for (int start = 0; start < 2; start++) {
+		hipGraph_t graph = nullptr;
+		MMSPX_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+		int64_t dummy = 0;
+		int rc = 0;
+		for (int k = 0; k < gsteps && rc == 0; k++)
+			rc = launch_batch_step(b->model, b->prec, b->fields, b->tile, b->d_mem, b->d_items, b->n2, b->d_items + b->n2,
+			                       b->n3, (start + k) & 1, b->cus, b->st, &dummy);
+		hipError_t e = hipStreamEndCapture(b->st, &graph);
+		if (rc) {
+			if (graph) (void)hipGraphDestroy(graph);
+			return rc;
+		}
+		if (e != hipSuccess) return fail(MMSPX_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+		e = hipGraphInstantiate(&b->graph[start], graph, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(graph);
+		if (e != hipSuccess) return fail(MMSPX_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+	}
+	b->graph_steps = gsteps;
+	return 0;
+}
+
+static int batch_step_impl(mmspx_batch* b, int steps, int gsteps)
+{
+	if (!b || steps < 0 || gsteps < 0) return fail(MMSPX_EINVAL, "bad arguments");
+	if (!b->params_set) return fail(MMSPX_EINVAL, "mmspx_batch_set_params has not been called");
+	if (gsteps > 0 && steps >= gsteps) {
+		if (b->graph_steps != gsteps) {
+			int rc = build_graphs(b, gsteps);
+			if (rc) return rc;
+		}
+		const int reps = steps / gsteps;
+		for (int r = 0; r < reps; r++) {
+			MMSPX_HIP(hipGraphLaunch(b->graph[b->steps_done & 1], b->st));
+			b->steps_done += gsteps;
+			b->launches += (int64_t)gsteps * ((b->n2 > 0) + (b->n3 > 0));
+		}
+		steps -= reps * gsteps;
+	}
+	return enqueue_steps(b, steps);
+}
+
+extern "C" {
+
+void mmspx_batch_destroy(mmspx_batch* b)
+{
+	if (!b) return;
+	if (b->st) (void)hipStreamSynchronize(b->st);
+	drop_graphs(b);
+	if (b->arena) (void)hipFree(b->arena);
+	if (b->d_mem) (void)hipFree(b->d_mem);
+	if (b->d_items) (void)hipFree(b->d_items);
+	if (b->part) (void)hipFree(b->part);
+	if (b->part_host) (void)hipHostFree(b->part_host);
+	if (b->staging) (void)hipFree(b->staging);
+	if (b->e0) (void)hipEventDestroy(b->e0);
+	if (b->e1) (void)hipEventDestroy(b->e1);
+	if (b->st) (void)hipStreamDestroy(b->st);
+	delete b;
+}
+
+int mmspx_batch_create(const mmspx_desc* descs, int count, int model, mmspx_batch** out)
+{
+	if (!out) return fail(MMSPX_EINVAL, "null output pointer");
+	*out = nullptr;
+	if (!descs || count < 1 || count > (1 << 20)) return fail(MMSPX_EINVAL, "bad member count");
+	auto* b = new (std::nothrow) mmspx_batch();
+	if (!b) return fail(MMSPX_ENOMEM, "host allocation failed");
+	b->count = count;
+	b->model = model;
+	b->prec = descs[0].precision;
+	size_t total = 0;
+	for (int m = 0; m < count; m++) {
+		int rc = validate_desc(&descs[m]);
+		if (rc) {
+			delete b;
+			return rc;
+		}
+		if (descs[m].precision != b->prec) {
+			delete b;
+			return fail(MMSPX_EINVAL, "batch members must share one precision");
+		}
+		Geom g = make_geom(descs[m]);
+		std::string why;
+		if (!batch_model_ok(model, g, &why)) {
+			delete b;
+			return fail(MMSPX_EUNSUPPORTED, why);
+		}
+		if (m > 0 && g.fields != b->fields) {
+			delete b;
+			return fail(MMSPX_EINVAL, "batch members must share one field count");
+		}
+		b->fields = g.fields;
+		b->desc.push_back(descs[m]);
+		b->geom.push_back(g);
+		const size_t bytes = ((size_t)g.nodes * g.fields * (b->prec == MMSPX_F64 ? 8 : 4) + 255) / 256 * 256;
+		b->off.push_back(total);
+		b->bytes.push_back(bytes);
+		total += 2 * bytes;
+	}
+	b->params.assign(count, ModelParams{});
+	b->cus = device_cus();
+	b->arena_bytes = total;
+	const size_t red = (size_t)b->fields * 2 * kRedBlocks * sizeof(double);
+	int rc = 0;
+	if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
+	    hipEventCreate(&b->e1) != hipSuccess)
+		rc = fail(MMSPX_EDEVICE, "HIP stream/event setup failed");
+	if (rc == 0 && (hipMalloc((void**)&b->arena, total) != hipSuccess ||
+	                hipMalloc((void**)&b->d_mem, sizeof(BMember) * (size_t)count) != hipSuccess ||
+	                hipMalloc((void**)&b->part, red) != hipSuccess ||
+	                hipHostMalloc((void**)&b->part_host, red, 0) != hipSuccess))
+		rc = fail(MMSPX_ENOMEM, "device allocation failed");
+	if (rc == 0) rc = upload_plan(b);
+	if (rc) {
+		mmspx_batch_destroy(b);
+		return rc;
+	}
+	*out = b;
+	return 0;
+}
+
+static void* cur_buf(mmspx_batch* b, int m) { return b->arena + b->off[m] + ((b->steps_done & 1) ? b->bytes[m] : 0); }
+
+int mmspx_batch_upload(mmspx_batch* b, int m, const void* host, int layout)
+{
+	if (!b || !host || m < 0 || m >= b->count) return fail(MMSPX_EINVAL, "bad member");
+	const Geom& g = b->geom[m];
+	const size_t n = (size_t)g.nodes * g.fields * (b->prec == MMSPX_F64 ? 8 : 4);
+	void* dst = cur_buf(b, m);
+	if (g.fields == 1 || layout == MMSPX_FIELD_MAJOR) {
+		MMSPX_HIP(hipMemcpyAsync(dst, host, n, hipMemcpyHostToDevice, b->st));
+	} else {
+		void* spare = b->arena + b->off[m] + ((b->steps_done & 1) ? 0 : b->bytes[m]);
+		MMSPX_HIP(hipMemcpyAsync(spare, host, n, hipMemcpyHostToDevice, b->st));
+		int rc = launch_transpose(spare, dst, g.nodes, g.fields, b->prec == MMSPX_F64 ? 8 : 4, true, b->st);
+		if (rc) return rc;
+	}
+	MMSPX_HIP(hipStreamSynchronize(b->st));
+	return 0;
+}
+
+int mmspx_batch_download(mmspx_batch* b, int m, void* host, int layout)
+{
+	if (!b || !host || m < 0 || m >= b->count) return fail(MMSPX_EINVAL, "bad member");
+	const Geom& g = b->geom[m];
+	const size_t n = (size_t)g.nodes * g.fields * (b->prec == MMSPX_F64 ? 8 : 4);
+	void* src = cur_buf(b, m);
+	if (g.fields == 1 || layout == MMSPX_FIELD_MAJOR) {
+		MMSPX_HIP(hipMemcpyAsync(host, src, n, hipMemcpyDeviceToHost, b->st));
+	} else {
+		void* spare = b->arena + b->off[m] + ((b->steps_done & 1) ? 0 : b->bytes[m]);
+		int rc = launch_transpose(src, spare, g.nodes, g.fields, b->prec == MMSPX_F64 ? 8 : 4, false, b->st);
+		if (rc) return rc;
+		MMSPX_HIP(hipMemcpyAsync(host, spare, n, hipMemcpyDeviceToHost, b->st));
+	}
+	MMSPX_HIP(hipStreamSynchronize(b->st));
+	return 0;
+}
+
+int mmspx_batch_set_params(mmspx_batch* b, const double* params, int np)
+{
+	if (!b || !params) return fail(MMSPX_EINVAL, "null argument");
+	for (int m = 0; m < b->count; m++) {
+		int rc = make_params(b->model, params + (size_t)m * np, np, &b->params[m]);
+		if (rc) return rc;
+	}
+	b->params_set = true;
+	return upload_plan(b);
+}
+
+int mmspx_batch_step(mmspx_batch* b, int steps, int graph_steps) { return batch_step_impl(b, steps, graph_steps); }
+
+int mmspx_batch_step_timed(mmspx_batch* b, int steps, int graph_steps, double* seconds)
+{
+	if (!b) return fail(MMSPX_EINVAL, "null batch");
+	MMSPX_HIP(hipEventRecord(b->e0, b->st));
+	int rc = batch_step_impl(b, steps, graph_steps);
+	if (rc) return rc;
+	MMSPX_HIP(hipEventRecord(b->e1, b->st));
+	MMSPX_HIP(hipEventSynchronize(b->e1));
+	float ms = 0.0f;
+	MMSPX_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
+	if (seconds) *seconds = ms * 1e-3;
+	return 0;
+}
+
+int mmspx_batch_reduce(mmspx_batch* b, int m, int what, double* out, int nout)
+{
+	if (!b || !out || m < 0 || m >= b->count) return fail(MMSPX_EINVAL, "bad member");
+	const Geom& g = b->geom[m];
+	if (what == MMSPX_MASS && nout < g.fields) return fail(MMSPX_EINVAL, "mass needs one output per field");
+	if (what == MMSPX_ENERGY && nout < 1) return fail(MMSPX_EINVAL, "energy needs one output");
+	if (what != MMSPX_MASS && what != MMSPX_ENERGY) return fail(MMSPX_EINVAL, "unknown reduction");
+	int rc = launch_reduce(g, b->prec, what, b->params[m], cur_buf(b, m), b->part, b->st);
+	if (rc) return rc;
+	const int sets = what == MMSPX_MASS ? g.fields : 1;
+	MMSPX_HIP(hipMemcpyAsync(b->part_host, b->part, (size_t)sets * 2 * kRedBlocks * sizeof(double),
+	                         hipMemcpyDeviceToHost, b->st));
+	MMSPX_HIP(hipStreamSynchronize(b->st));
+	for (int j = 0; j < sets; j++) out[j] = sum_partials(b->part_host + (size_t)j * 2 * kRedBlocks, kRedBlocks);
+	if (what == MMSPX_ENERGY) out[0] *= g.vol;
+	return 0;
+}
+
+int mmspx_batch_get_tile(const mmspx_batch* b, int* tz, int* chunk, int* blocks_per_cu)
+{
+	if (!b) return fail(MMSPX_EINVAL, "null batch");
+	if (tz) *tz = b->tile.tz;
+	if (chunk) *chunk = b->tile.chunk;
+	if (blocks_per_cu) *blocks_per_cu = b->tile.bpc;
+	return b->tuned ? 1 : 0;
+}
+
+int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t report_len)
+{
+	if (!b || trial_steps < 1) return fail(MMSPX_EINVAL, "bad arguments");
+	if (!b->params_set) return fail(MMSPX_EINVAL, "mmspx_batch_set_params has not been called");
+	// every trial starts from the same state, restored afterwards
+	if (b->staging_bytes < b->arena_bytes) {
+		if (b->staging) (void)hipFree(b->staging);
+		b->staging = nullptr;
+		if (hipMalloc(&b->staging, b->arena_bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");
+		b->staging_bytes = b->arena_bytes;
+	}
+	MMSPX_HIP(hipMemcpyAsync(b->staging, b->arena, b->arena_bytes, hipMemcpyDeviceToDevice, b->st));
+	const int64_t steps0 = b->steps_done;
+	const BatchTile keep = b->tile;
+	std::vector<BatchTile> cand;
+	for (int tz : {64, 128, 256})
+		for (int chunk : {16, 32, 64, 128})
+			for (int bpc : {4, 8, 16}) cand.push_back(BatchTile{tz, chunk, 16, bpc});
+	std::string rep;
+	double best = 1e30;
+	BatchTile best_t = keep;
+	int rc = 0;
+	for (const BatchTile& t : cand) {
+		b->tile = t;
+		rc = upload_plan(b);
+		if (rc == 0) rc = enqueue_steps(b, 2);
+		double sec = 0.0;
+		if (rc == 0) rc = mmspx_batch_step_timed(b, trial_steps, 0, &sec);
+		if (rc) break;
+		char line[128];
+		std::snprintf(line, sizeof line, "tz=%d chunk=%d blocks_per_cu=%d us_per_step=%.2f\n", t.tz, t.chunk, t.bpc,
+		              sec / trial_steps * 1e6);
+		rep += line;
+		if (sec < best) {
+			best = sec;
+			best_t = t;
+		}
+	}
+	b->tile = rc ? keep : best_t;
+	b->steps_done = steps0;
+	int rc2 = upload_plan(b);
+	if (hipMemcpyAsync(b->arena, b->staging, b->arena_bytes, hipMemcpyDeviceToDevice, b->st) != hipSuccess ||
+	    hipStreamSynchronize(b->st) != hipSuccess)
+		return fail(MMSPX_EDEVICE, "restoring the batch after tuning failed");
+	if (rc) return rc;
+	if (rc2) return rc2;
+	b->tuned = true;
+	if (report && report_len) std::snprintf(report, report_len, "%s", rep.c_str());
+	return 0;
+}
+
+}  // extern "C"

@@ -76,6 +76,8 @@ struct mmspx_state {
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
 	int device = 0;
 	int variant = 0;
+	bool tuned = false;      // a tile chosen by mmspx_set_tile / mmspx_autotune
+	mmspx::TuneChoice tile{};
 	int64_t launches = 0;
 };
 
@@ -97,6 +99,9 @@ int launch_reduce(const Geom& g, int prec, int what, const ModelParams& p, const
 int launch_transpose(const void* src, void* dst, int nodes, int fields, size_t elem, bool to_field_major,
                      hipStream_t st);
 int launch_convert(const void* src, int src_f64, void* dst, int dst_f64, int64_t n, hipStream_t st);
+int launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
+bool tile_known(const Geom& g, const TuneChoice& t);
+int tile_candidates(const Geom& g, TuneChoice* out, int max);
 
 // host-side compensated sum of the (sum, comp) partial pairs
 double sum_partials(const double* part, int count);

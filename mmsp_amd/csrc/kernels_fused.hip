@@ -1,16 +1,240 @@
-// kernels_fused.hip -- tiled / fused kernels for the hot shapes.
-#include "device.hpp"
+// kernels_fused.hip -- tiled kernels for single grids: fused single-pass
+// Cahn-Hilliard (2-D/3-D, periodic) and marching-strip Allen-Cahn and grain
+// growth (2-D, any boundary condition). Tile bodies live in tiles.hpp.
+//
+// Cahn-Hilliard: the reference takes two full sweeps per step
+// (cahn-hilliard.cpp:69-76): mu into a temporary grid, then c + dt*M*lap(mu).
+// Unfused, that moves 40 bytes per FP64 cell-update through HBM (c in, mu out,
+// mu in, c in, c' out); the fused tile moves the compulsory 16 (c in, c' out)
+// and recomputes mu on a one-cell halo from the same wrapped values, so the
+// result is bitwise that of the two-pass form. Only periodic boundaries take
+// it; other boundaries keep the two-pass kernels, where mu at a mapped
+// neighbour is mu of the mapped node exactly as in the reference.
+//
+// Workgroup ids are remapped so that the workgroups one XCD receives (ids are
+// dealt round-robin over the 8 XCDs) form a contiguous run of tiles with the
+// axis-0 chunk slowest: an XCD sweeps whole planes of one chunk, so halo reads
+// of neighbouring tiles hit its own L2.
+#include "tiles.hpp"
 
 namespace mmspx {
 
-bool fused_supported(const Geom&, int, int) { return false; }
-
-TuneChoice default_choice(const Geom&, int, int) { return TuneChoice{1, 0, 0, 0}; }
-
-int launch_fused_step(const Geom&, int, const ModelParams&, const void*, void*, hipStream_t, const TuneChoice&,
-                      int64_t*)
+__device__ __forceinline__ int xcd_tile(int ntiles)
 {
-	return fail(MMSPX_EUNSUPPORTED, "no fused kernel for this shape");
+	int tile = blockIdx.x;
+	if ((gridDim.x & 7) == 0) tile = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+	return tile < ntiles ? tile : -1;
+}
+
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+__global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __restrict__ a,
+                                               typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
+                                               int chunk_len, int tiles_y, int tiles_z, int ntiles)
+{
+	const int tile = xcd_tile(ntiles);
+	if (tile < 0) return;
+	const int tz = tile % tiles_z;
+	const int t2 = tile / tiles_z;
+	const int ty = t2 % tiles_y;
+	const int chunk = t2 / tiles_y;
+	const int xb = chunk * chunk_len;
+	const int xe = min(xb + chunk_len, g.n[0]);
+	ch_tile<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
+}
+
+template <int PREC, int TZ>
+__global__ void __launch_bounds__(TZ) ac_tiled(const typename Form<PREC>::S* __restrict__ a,
+                                               typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
+                                               int chunk_len, int strips, int ntiles)
+{
+	const int tile = xcd_tile(ntiles);
+	if (tile < 0) return;
+	const int s = tile % strips, chunk = tile / strips;
+	const int xb = chunk * chunk_len;
+	ac_strip<PREC, TZ>(a, b, g, p, s * TZ, xb, min(xb + chunk_len, g.n[0]));
+}
+
+template <int PREC, int Q, int TZ>
+__global__ void __launch_bounds__(TZ) gg_tiled(const typename Form<PREC>::S* __restrict__ a,
+                                               typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
+                                               int chunk_len, int strips, int ntiles)
+{
+	const int tile = xcd_tile(ntiles);
+	if (tile < 0) return;
+	const int s = tile % strips, chunk = tile / strips;
+	const int xb = chunk * chunk_len;
+	gg_strip<PREC, Q, TZ>(a, b, g, p, s * TZ, xb, min(xb + chunk_len, g.n[0]));
+}
+
+static bool all_periodic(const Geom& g)
+{
+	for (int k = 0; k < g.dim; k++)
+		if (g.b0[k] != MMSPX_PERIODIC || g.b1[k] != MMSPX_PERIODIC) return false;
+	return true;
+}
+
+static bool gg_q_compiled(int q) { return q == 2 || q == 4 || q == 8 || q == 16 || q == 32; }
+
+bool fused_supported(const Geom& g, int prec, int model)
+{
+	(void)prec;
+	switch (model) {
+	case MMSPX_CH:
+	case MMSPX_CH_DW: return g.dim >= 2 && all_periodic(g);
+	case MMSPX_AC: return g.dim == 2;
+	case MMSPX_GG: return g.dim == 2 && gg_q_compiled(g.fields);
+	}
+	return false;
+}
+
+// Defaults come from the autotuner's sweep on MI355X: for the 256^3 periodic
+// CH grid 16x64 tiles with 32-plane chunks were fastest; grids too small to
+// fill the chip with them take narrower tiles and shorter chunks.
+TuneChoice default_choice(const Geom& g, int prec, int model)
+{
+	if (!fused_supported(g, prec, model)) return TuneChoice{1, 0, 0, 0};
+	const int64_t nodes = g.nodes;
+	if (model == MMSPX_AC) return TuneChoice{2, 1, 256, nodes >= (1 << 22) ? 64 : 16};
+	if (model == MMSPX_GG) return TuneChoice{2, 1, prec == MMSPX_F64 ? 64 : 128, 32};
+	if (g.dim == 2) return TuneChoice{2, 1, 256, 32};
+	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16};
+	return TuneChoice{2, 16, 64, 32};
+}
+
+bool tile_known(const Geom& g, const TuneChoice& t)
+{
+	if (t.chunk < 2 || t.chunk > 4096) return false;
+	if (g.dim == 3)
+		return (t.ty == 8 && t.tz == 64) || (t.ty == 16 && t.tz == 64) || (t.ty == 4 && t.tz == 64) ||
+		       (t.ty == 8 && t.tz == 32) || (t.ty == 4 && t.tz == 128);
+	if (g.dim == 2) return t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256);
+	return false;
+}
+
+int tile_candidates(const Geom& g, TuneChoice* out, int max)
+{
+	int n = 0;
+	auto add = [&](int ty, int tz, int chunk) {
+		if (n < max) out[n++] = TuneChoice{2, ty, tz, chunk};
+	};
+	if (g.dim == 3) {
+		const int tiles[5][2] = {{8, 64}, {16, 64}, {4, 64}, {8, 32}, {4, 128}};
+		const int chunks[3] = {16, 32, 64};
+		for (auto& t : tiles)
+			for (int c : chunks) add(t[0], t[1], c);
+	} else if (g.dim == 2) {
+		for (int tz : {64, 128, 256})
+			for (int c : {16, 32, 64, 128}) add(1, tz, c);
+	}
+	return n;
+}
+
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+static int launch_ch(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	using S = typename Form<PREC>::S;
+	const int nx = g.n[0];
+	const int ny = DIM == 3 ? g.n[1] : 1;
+	const int nz = DIM == 3 ? g.n[2] : g.n[1];
+	const int tiles_y = (ny + TY - 1) / TY;
+	const int tiles_z = (nz + TZ - 1) / TZ;
+	const int chunks = (nx + chunk - 1) / chunk;
+	const int ntiles = tiles_y * tiles_z * chunks;
+	// round the grid up to a multiple of 8 so the XCD remap applies; extra ids exit
+	const int blocks = (ntiles + 7) / 8 * 8;
+	hipLaunchKernelGGL((ch_fused<PREC, DW, DIM, TY, TZ, NT>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g, p,
+	                   chunk, tiles_y, tiles_z, ntiles);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+template <int PREC, bool DW>
+static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st,
+                          const TuneChoice& t)
+{
+	const int chunk = t.chunk > 0 ? t.chunk : 32;
+	if (g.dim == 3) {
+		if (t.ty == 16 && t.tz == 64) return launch_ch<PREC, DW, 3, 16, 64, 256>(g, p, a, b, st, chunk);
+		if (t.ty == 4 && t.tz == 64) return launch_ch<PREC, DW, 3, 4, 64, 256>(g, p, a, b, st, chunk);
+		if (t.ty == 8 && t.tz == 32) return launch_ch<PREC, DW, 3, 8, 32, 256>(g, p, a, b, st, chunk);
+		if (t.ty == 4 && t.tz == 128) return launch_ch<PREC, DW, 3, 4, 128, 256>(g, p, a, b, st, chunk);
+		return launch_ch<PREC, DW, 3, 8, 64, 256>(g, p, a, b, st, chunk);
+	}
+	if (t.tz == 64) return launch_ch<PREC, DW, 2, 1, 64, 64>(g, p, a, b, st, chunk);
+	if (t.tz == 128) return launch_ch<PREC, DW, 2, 1, 128, 128>(g, p, a, b, st, chunk);
+	return launch_ch<PREC, DW, 2, 1, 256, 256>(g, p, a, b, st, chunk);
+}
+
+template <int PREC, int TZ>
+static int launch_ac(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	using S = typename Form<PREC>::S;
+	const int strips = (g.n[1] + TZ - 1) / TZ;
+	const int ntiles = strips * ((g.n[0] + chunk - 1) / chunk);
+	const int blocks = (ntiles + 7) / 8 * 8;
+	hipLaunchKernelGGL((ac_tiled<PREC, TZ>), dim3(blocks), dim3(TZ), 0, st, (const S*)a, (S*)b, g, p, chunk, strips,
+	                   ntiles);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+template <int PREC, int Q, int TZ>
+static int launch_gg_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	using S = typename Form<PREC>::S;
+	const int strips = (g.n[1] + TZ - 1) / TZ;
+	const int ntiles = strips * ((g.n[0] + chunk - 1) / chunk);
+	const int blocks = (ntiles + 7) / 8 * 8;
+	hipLaunchKernelGGL((gg_tiled<PREC, Q, TZ>), dim3(blocks), dim3(TZ), 0, st, (const S*)a, (S*)b, g, p, chunk, strips,
+	                   ntiles);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+template <int PREC, int TZ>
+static int launch_gg_tz(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	switch (g.fields) {
+	case 2: return launch_gg_q<PREC, 2, TZ>(g, p, a, b, st, chunk);
+	case 4: return launch_gg_q<PREC, 4, TZ>(g, p, a, b, st, chunk);
+	case 8: return launch_gg_q<PREC, 8, TZ>(g, p, a, b, st, chunk);
+	case 16: return launch_gg_q<PREC, 16, TZ>(g, p, a, b, st, chunk);
+	case 32: return launch_gg_q<PREC, 32, TZ>(g, p, a, b, st, chunk);
+	}
+	return fail(MMSPX_EUNSUPPORTED, "no tiled grain-growth kernel for this field count");
+}
+
+template <int PREC>
+static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, const TuneChoice& t)
+{
+	const int chunk = t.chunk > 0 ? t.chunk : 32;
+	switch (p.model) {
+	case MMSPX_CH: return launch_ch_prec<PREC, false>(g, p, a, b, st, t);
+	case MMSPX_CH_DW: return launch_ch_prec<PREC, true>(g, p, a, b, st, t);
+	case MMSPX_AC:
+		if (t.tz == 64) return launch_ac<PREC, 64>(g, p, a, b, st, chunk);
+		if (t.tz == 128) return launch_ac<PREC, 128>(g, p, a, b, st, chunk);
+		return launch_ac<PREC, 256>(g, p, a, b, st, chunk);
+	case MMSPX_GG:
+		if (t.tz == 64) return launch_gg_tz<PREC, 64>(g, p, a, b, st, chunk);
+		if (t.tz == 256 && PREC != MMSPX_F64) return launch_gg_tz<PREC, 256>(g, p, a, b, st, chunk);
+		return launch_gg_tz<PREC, 128>(g, p, a, b, st, chunk);
+	}
+	return fail(MMSPX_EINVAL, "unknown model");
+}
+
+int launch_fused_step(const Geom& g, int prec, const ModelParams& p, const void* a, void* b, hipStream_t st,
+                      const TuneChoice& t, int64_t* launches)
+{
+	int rc;
+	switch (prec) {
+	case MMSPX_F64: rc = launch_prec<MMSPX_F64>(g, p, a, b, st, t); break;
+	case MMSPX_F32: rc = launch_prec<MMSPX_F32>(g, p, a, b, st, t); break;
+	case MMSPX_F32REF: rc = launch_prec<MMSPX_F32REF>(g, p, a, b, st, t); break;
+	default: return fail(MMSPX_EINVAL, "unknown precision");
+	}
+	if (rc == 0) *launches += 1;
+	return rc;
 }
 
 }  // namespace mmspx

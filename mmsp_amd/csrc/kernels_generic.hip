@@ -250,4 +250,19 @@ int launch_convert(const void* src, int src_f64, void* dst, int dst_f64, int64_t
 	return 0;
 }
 
+// Plain streaming copy (16-byte vectors): the HBM calibration kernel.
+__global__ void __launch_bounds__(256) copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n)
+{
+	for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+		dst[k] = src[k];
+}
+
+int launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st)
+{
+	const int64_t n = (int64_t)(bytes / sizeof(uint4));
+	hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, st, (const uint4*)src, (uint4*)dst, n);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
 }  // namespace mmspx

@@ -1,0 +1,235 @@
+// tiles.hpp -- per-tile device bodies shared by the single-grid kernels
+// (kernels_fused.hip) and the batched/persistent kernels (batch.hip).
+//
+// Every body reproduces the operation order of the generic kernels (and so of
+// the reference), so a grid gives the same bits whichever kernel advances it.
+#pragma once
+
+#include "device.hpp"
+
+namespace mmspx {
+
+__device__ __forceinline__ int wrap(int v, int n)
+{
+	v %= n;
+	return v < 0 ? v + n : v;
+}
+
+// ---- Cahn-Hilliard, one (TY x TZ) tile marching over planes [xb, xe) ---------
+// Periodic grids only. phi planes: global -> registers (one plane ahead) ->
+// 4-slot LDS ring with a 2-cell halo; mu: 3-slot LDS ring with a 1-cell halo;
+// c' written one plane behind. DIM 2 uses a single row (TY = 1, no axis-1 halo).
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+__device__ __forceinline__ void ch_tile(const typename Form<PREC>::S* __restrict__ a,
+                                        typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                        int y0, int z0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	constexpr int HY = DIM == 3 ? 2 : 0;
+	constexpr int PY = TY + 2 * HY, PZ = TZ + 4;
+	constexpr int MY = TY + HY, MZ = TZ + 2;
+	constexpr int PN = PY * PZ, MN = MY * MZ, ON = TY * TZ;
+	constexpr int PREG = (PN + NT - 1) / NT;
+	constexpr int MREG = (MN + NT - 1) / NT;
+	constexpr int OREG = (ON + NT - 1) / NT;
+	__shared__ S sphi[4][PN];
+	__shared__ S smu[3][MN];
+
+	const int nx = g.n[0];
+	const int ny = DIM == 3 ? g.n[1] : 1;
+	const int nz = DIM == 3 ? g.n[2] : g.n[1];
+	const int64_t plane = (int64_t)ny * nz;
+
+	int poff[PREG];
+#pragma unroll
+	for (int k = 0; k < PREG; k++) {
+		const int e = threadIdx.x + k * NT;
+		poff[k] = 0;
+		if (e < PN) {
+			const int r = e / PZ, c = e - (e / PZ) * PZ;
+			const int y = DIM == 3 ? wrap(y0 - HY + r, ny) : 0;
+			const int z = wrap(z0 - 2 + c, nz);
+			poff[k] = y * nz + z;
+		}
+	}
+	auto load_plane = [&](int x, S* regs) {
+		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
+#pragma unroll
+		for (int k = 0; k < PREG; k++) {
+			const int e = threadIdx.x + k * NT;
+			if (e < PN) regs[k] = src[poff[k]];
+		}
+	};
+	auto store_plane = [&](int slot, const S* regs) {
+#pragma unroll
+		for (int k = 0; k < PREG; k++) {
+			const int e = threadIdx.x + k * NT;
+			if (e < PN) sphi[slot][e] = regs[k];
+		}
+	};
+
+	S pre[PREG];
+	// the previous tile of a persistent loop may still read the rings
+	__syncthreads();
+	load_plane(xb - 2, pre);
+	store_plane((xb - 2) & 3, pre);
+	load_plane(xb - 1, pre);
+	store_plane((xb - 1) & 3, pre);
+	load_plane(xb, pre);
+
+	for (int P = xb - 1; P <= xe; P++) {
+		store_plane((P + 1) & 3, pre);
+		if (P + 2 <= xe + 1) load_plane(P + 2, pre);
+		__syncthreads();
+		{
+			const S* __restrict__ fm = sphi[(P - 1) & 3];
+			const S* __restrict__ fc = sphi[P & 3];
+			const S* __restrict__ fp = sphi[(P + 1) & 3];
+			S* __restrict__ mu = smu[wrap(P, 3)];
+#pragma unroll
+			for (int k = 0; k < MREG; k++) {
+				const int e = threadIdx.x + k * NT;
+				if (e < MN) {
+					const int r = e / MZ, c = e - (e / MZ) * MZ;
+					const int q = (r + HY / 2) * PZ + (c + 1);
+					const S y = fc[q];
+					typename F::L lap = F::lap_zero();
+					F::lap_add(lap, g.w[0], g.wf[0], fp[q], y, fm[q]);
+					if (DIM == 3) F::lap_add(lap, g.w[1], g.wf[1], fc[q + PZ], y, fc[q - PZ]);
+					F::lap_add(lap, g.w[DIM - 1], g.wf[DIM - 1], fc[q + 1], y, fc[q - 1]);
+					mu[e] = DW ? F::mu_dw(y, lap, p) : F::mu_poly(y, lap, p);
+				}
+			}
+		}
+		__syncthreads();
+		if (P - 1 >= xb) {
+			const S* __restrict__ mm = smu[wrap(P - 2, 3)];
+			const S* __restrict__ mc = smu[wrap(P - 1, 3)];
+			const S* __restrict__ mp = smu[wrap(P, 3)];
+			const S* __restrict__ fc = sphi[(P - 1) & 3];
+			S* __restrict__ dst = b + (int64_t)(P - 1) * plane;
+#pragma unroll
+			for (int k = 0; k < OREG; k++) {
+				const int e = threadIdx.x + k * NT;
+				if (e < ON) {
+					const int r = e / TZ, c = e - (e / TZ) * TZ;
+					const int y = y0 + r, z = z0 + c;
+					const int q = (r + HY / 2) * MZ + (c + 1);
+					const S m = mc[q];
+					typename F::L lap = F::lap_zero();
+					F::lap_add(lap, g.w[0], g.wf[0], mp[q], m, mm[q]);
+					if (DIM == 3) F::lap_add(lap, g.w[1], g.wf[1], mc[q + MZ], m, mc[q - MZ]);
+					F::lap_add(lap, g.w[DIM - 1], g.wf[DIM - 1], mc[q + 1], m, mc[q - 1]);
+					const S c0 = fc[(r + HY) * PZ + (c + 2)];
+					if (y < ny && z < nz) dst[(int64_t)y * nz + z] = F::ch(c0, lap, p);
+				}
+			}
+		}
+	}
+}
+
+// ---- 2-D strips: TZ contiguous columns [y0, y0+TZ) marching rows [xb, xe) ----
+// Each thread owns one column; the axis-0 neighbours stay in registers (a
+// three-row window), the axis-1 neighbours come from a double-buffered LDS
+// row. Any boundary condition: rows and halo columns go through check_boundary's
+// mapping. One global load and one store per cell (per field).
+template <int PREC, int TZ>
+__device__ __forceinline__ void ac_strip(const typename Form<PREC>::S* __restrict__ a,
+                                         typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                         int y0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	__shared__ S row[2][TZ + 2];
+	const int nx = g.n[0], ny = g.n[1];
+	const int t = threadIdx.x;
+	const int nact = min(TZ, ny - y0);
+	const bool act = t < nact;
+	const int y = y0 + (act ? t : nact - 1);
+	const int yl = bc_map(y0 - 1, ny, g.b0[1], g.b1[1]);
+	const int yr = bc_map(y0 + nact, ny, g.b0[1], g.b1[1]);
+	__syncthreads();
+	S prev = a[(int64_t)bc_map(xb - 1, nx, g.b0[0], g.b1[0]) * ny + y];
+	S cur = a[(int64_t)xb * ny + y];
+	for (int x = xb; x < xe; x++) {
+		const S next = a[(int64_t)bc_map(x + 1, nx, g.b0[0], g.b1[0]) * ny + y];
+		S* lr = row[x & 1];
+		const S* rc = a + (int64_t)x * ny;
+		if (act) lr[t + 1] = cur;
+		if (t == 0) lr[0] = rc[yl];
+		if (t == nact - 1) lr[nact + 1] = rc[yr];
+		__syncthreads();
+		if (act) {
+			typename F::L lap = F::lap_zero();
+			F::lap_add(lap, g.w[0], g.wf[0], next, cur, prev);
+			F::lap_add(lap, g.w[1], g.wf[1], lr[t + 2], cur, lr[t]);
+			b[(int64_t)x * ny + y] = F::ac(cur, lap, p);
+		}
+		prev = cur;
+		cur = next;
+	}
+}
+
+// Grain growth, Q fields stored field-major (plane j at offset j*nodes).
+template <int PREC, int Q, int TZ>
+__device__ __forceinline__ void gg_strip(const typename Form<PREC>::S* __restrict__ a,
+                                         typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                         int y0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	__shared__ S row[2][Q][TZ + 2];
+	const int nx = g.n[0], ny = g.n[1];
+	const int64_t N = (int64_t)nx * ny;
+	const int t = threadIdx.x;
+	const int nact = min(TZ, ny - y0);
+	const bool act = t < nact;
+	const int y = y0 + (act ? t : nact - 1);
+	const int yl = bc_map(y0 - 1, ny, g.b0[1], g.b1[1]);
+	const int yr = bc_map(y0 + nact, ny, g.b0[1], g.b1[1]);
+	S prev[Q], cur[Q], next[Q];
+	__syncthreads();
+	{
+		const int64_t rp = (int64_t)bc_map(xb - 1, nx, g.b0[0], g.b1[0]) * ny + y;
+		const int64_t rcx = (int64_t)xb * ny + y;
+#pragma unroll
+		for (int j = 0; j < Q; j++) {
+			prev[j] = a[j * N + rp];
+			cur[j] = a[j * N + rcx];
+		}
+	}
+	for (int x = xb; x < xe; x++) {
+		const int64_t rn = (int64_t)bc_map(x + 1, nx, g.b0[0], g.b1[0]) * ny + y;
+		const int64_t rc = (int64_t)x * ny;
+#pragma unroll
+		for (int j = 0; j < Q; j++) next[j] = a[j * N + rn];
+		const int s = x & 1;
+#pragma unroll
+		for (int j = 0; j < Q; j++) {
+			if (act) row[s][j][t + 1] = cur[j];
+			if (t == 0) row[s][j][0] = a[j * N + rc + yl];
+			if (t == nact - 1) row[s][j][nact + 1] = a[j * N + rc + yr];
+		}
+		__syncthreads();
+		if (act) {
+			typename F::Sum sum = 0;
+#pragma unroll
+			for (int j = 0; j < Q; j++) sum += cur[j] * cur[j];
+#pragma unroll
+			for (int j = 0; j < Q; j++) {
+				typename F::L lap = F::lap_zero();
+				F::lap_add(lap, g.w[0], g.wf[0], next[j], cur[j], prev[j]);
+				F::lap_add(lap, g.w[1], g.wf[1], row[s][j][t + 2], cur[j], row[s][j][t]);
+				b[j * N + rc + y] = F::gg(cur[j], sum, lap, p);
+			}
+		}
+#pragma unroll
+		for (int j = 0; j < Q; j++) {
+			prev[j] = cur[j];
+			cur[j] = next[j];
+		}
+	}
+}
+
+}  // namespace mmspx

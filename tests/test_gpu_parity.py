@@ -100,6 +100,9 @@ F32_CASES = [
     ("ch_dw", (32, 32, 64), None, [1.0, 1.0, 0.005], 40, (0.45, 0.55)),
     ("gg", (30, 26), None, [0.05], 30, (0.0, 0.01)),
     ("gg", (9, 10, 11), [(N, M), (P, P), (D, N)], [0.01], 20, (0.0, 1.0)),
+    ("gg4", (70, 300), [(N, M), (D, P)], [0.05], 25, (0.0, 1.0)),
+    ("gg32", (40, 150), None, [0.05], 20, (0.0, 0.01)),
+    ("ac", (150, 600), [(M, N), (P, P)], [1.0, 1.0, 1.0, 1.0, 0.01], 50, (-1.0, 1.0)),
 ]
 
 
@@ -107,7 +110,8 @@ F32_CASES = [
 @pytest.mark.parametrize("idx", range(len(F32_CASES)))
 def test_seeded_forms_bitwise(idx, prec):
     model, shape, bc, params, steps, (lo, hi) = F32_CASES[idx]
-    q = 5 if model == "gg" else 0
+    q = {"gg": 5, "gg4": 4, "gg32": 32}.get(model, 0)
+    model = "gg" if q else model
     say("[gpu] seeded %s %s %s bc=%s steps=%d" % (model, shape, prec, bc, steps))
     n = int(np.prod(shape)) * max(q, 1)
     a = O.uniform(n, 1000 + idx, lo, hi).reshape(tuple(shape) + ((q,) if q else ()))
