@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -181,6 +182,110 @@ def bench_batch_ac(args):
     }
 
 
+def serve_trace(n, seed=20260106, lam=200.0):
+    """Poisson arrivals (rate lam); half 2-D side U{64..1024}, half 3-D side
+    U{64..256}; steps log-uniform in [100, 10000]."""
+    import mmsp_amd as mx
+
+    u = mx.fill_uniform(n * 4, seed, 0.0, 1.0).reshape(n, 4)
+    gaps = -np.log1p(-u[:, 0]) / lam
+    arrivals = np.cumsum(gaps) - gaps[0]
+    out = []
+    for k in range(n):
+        if u[k, 1] < 0.5:
+            side = 64 + int(u[k, 2] * (1024 - 64 + 1))
+            shape = (side, side)
+        else:
+            side = 64 + int(u[k, 2] * (256 - 64 + 1))
+            shape = (side, side, side)
+        steps = int(round(math.exp(math.log(100) + u[k, 3] * (math.log(10000) - math.log(100)))))
+        out.append(dict(t=float(arrivals[k]), shape=shape, steps=steps, seed=seed + 1000 + k))
+    return out
+
+
+def bench_serve(args):
+    """Served Cahn-Hilliard stream through the continuous-batching server (FP32
+    state, FP64 mass check); latency is completion minus submission."""
+    import threading
+
+    import mmsp_amd as mx
+
+    n = args.requests
+    trace = serve_trace(n)
+    params = [1.0, 1.0, 0.005]
+    inits = [mx.fill_uniform(int(np.prod(r["shape"])), r["seed"], 0.45, 0.55).reshape(r["shape"]) for r in trace]
+    cells = sum(int(np.prod(r["shape"])) * r["steps"] for r in trace)
+    sv = mx.Server("ch_dw", "f32", hbm_budget=args.hbm_budget_gb << 30, max_slots=512, chunk_steps=100)
+    ids = [None] * n
+    late = [0.0] * n
+
+    def submitter():
+        t0 = time.perf_counter()
+        for k, r in enumerate(trace):
+            dt = r["t"] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            late[k] = max(0.0, (time.perf_counter() - t0) - r["t"])
+            ids[k] = sv.submit(inits[k], params, r["steps"])
+
+    th = threading.Thread(target=submitter)
+    c0 = sv.clock()
+    th.start()
+    th.join()
+    infos = [sv.wait(i, 600) for i in ids]
+    st = sv.stats()
+    sv.close()
+    done = [i for i in infos if i["status"] == mx.DONE]
+    lat = np.array([i["t_done"] - i["t_submit"] for i in done])
+    span = max(i["t_done"] for i in done) - min(i["t_submit"] for i in infos)
+    mass = [abs(i["mass1"] - i["mass0"]) / abs(i["mass0"]) for i in done]
+    value = len(done) / span
+    cu = cells / span
+    return {
+        "metric": "solves/s",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": 0,
+        "ms_per_step": span / max(1, len(done)) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "served-cahn-hilliard-stream",
+            "requests": n,
+            "arrival_rate_per_s": 200.0,
+            "completed": len(done),
+            "rejected": st["rejected"],
+            "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_latency_ms": float(lat.mean() * 1e3),
+            "makespan_s": span,
+            "cell_updates_per_s": cu,
+            "total_cell_updates": cells,
+            "max_rel_mass_drift_fp32": float(max(mass)),
+            "peak_concurrent": st["peak_concurrent"],
+            "peak_bytes": st["peak_bytes"],
+            "rounds": st["rounds"],
+            "launches": st["launches"],
+            "max_submit_lateness_ms": float(max(late) * 1e3),
+            "hbm_budget_gb": args.hbm_budget_gb,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": cu * 8 / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": cu * 8 / 1e9 / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "ch_batch",
+        },
+    }
+
+
 def traffic_from_profiles(kernel_tag):
     """HBM bytes per launch from committed rocprofv3 --pmc summaries, if any."""
     p = os.path.join(ROOT, "profiles", kernel_tag + ".traffic.json")
@@ -195,7 +300,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="ch3d", choices=sorted(WORKLOADS) + ["batch_ac"])
+    ap.add_argument("--workload", default="ch3d", choices=sorted(WORKLOADS) + ["batch_ac", "serve"])
+    ap.add_argument("--requests", type=int, default=200, help="serve: requests in the trace")
+    ap.add_argument("--hbm-budget-gb", type=int, default=128, help="serve: device memory the server may hold")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="store_true", help="run the autotuner first instead of the tuned default tile")
     ap.add_argument("--tile", default=None, help="ty,tz,chunk for the fused kernel")
@@ -209,6 +316,9 @@ def main():
 
     if args.workload == "batch_ac":
         print(json.dumps(bench_batch_ac(args)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args)), flush=True)
         return
     w = WORKLOADS[args.workload]
     K = args.steps if args.steps is not None else w["steps"]

@@ -1,0 +1,200 @@
+// mmsp_gpu.hpp -- drop-in GPU bodies for MMSP's update(grid&, steps) plug-ins.
+//
+// The reference's plug-in boundary is the C++ template each example defines in
+// namespace MMSP and MMSP.main.hpp calls (MMSP.main.hpp:279, 305, 331):
+//
+//   template <int dim, typename T> void update(grid<dim,T>& oldGrid, int steps);   // allen-cahn.cpp:47,
+//                                                                                  // cahn-hilliard.cpp:47
+//   template <int dim, typename T> void update(grid<dim,vector<T> >&, int steps);  // graingrowth.cpp:73
+//
+// This header keeps that boundary. An example replaces its per-node loop with
+// one call, and main(), output() and the .dat layout stay the reference's:
+//
+//   #include "MMSP.hpp"
+//   #include "mmsp_gpu.hpp"
+//   namespace MMSP {
+//   template <int dim, typename T> void update(grid<dim,T>& g, int steps)
+//   { mmsp_gpu::allen_cahn(g, steps, 1.0, 1.0, 1.0, 1.0, 0.01); }   // r, u, K, M, dt of allen-cahn.cpp:58-62
+//   }
+//   #include "MMSP.main.hpp"
+//
+// Link with -lmmsp_amd (include/mmsp_amd.h is the C ABI underneath).
+//
+// What a call does: the grid's node data goes to the device once, all `steps`
+// explicit steps run there (the MMSP::laplacian, check_boundary, swap and
+// ghostswap of the loop become device work), and the result comes back into
+// the same grid, so the caller sees the same grid the reference's loop leaves
+// (the reference swaps a locally constructed grid into the caller's,
+// MMSP.grid.cpp:1913-1991; values are what matter to callers). The progress
+// bar of MMSP::print_progress (MMSP.utility.cpp:138-178) is printed at the
+// same steps as the reference prints it. grid<dim,double> runs the FP64 form
+// (bitwise the reference's arithmetic except the cube, see the library);
+// grid<dim,float> and vector<float> run the reference's mixed float/double
+// semantics unless set_native_float(true) selects pure float arithmetic.
+//
+// Errors follow the reference's convention: a message on std::cerr, then
+// MMSP::Abort(-1) (MMSP.utility.cpp:30-36).
+//
+// Serial builds only: under MPI the grid holds ghost cells and one block per
+// rank (MMSP.grid.cpp:141-327), which this header does not map.
+#ifndef MMSP_GPU_HPP
+#define MMSP_GPU_HPP
+
+#ifdef MPI_VERSION
+#error "mmsp_gpu.hpp drives one serial grid per process; MPI builds of MMSP are not supported"
+#endif
+
+#include <cstring>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "mmsp_amd.h"
+
+namespace mmsp_gpu {
+
+inline bool& native_float_flag()
+{
+	static bool v = false;
+	return v;
+}
+// Pure float arithmetic for float grids (MMSPX_F32) instead of the reference's
+// float-storage/double-expression semantics (MMSPX_F32REF).
+inline void set_native_float(bool on) { native_float_flag() = on; }
+
+inline void check(int rc, const char* what)
+{
+	if (rc != 0) {
+		std::cerr << "mmsp_gpu: " << what << " failed: " << mmspx_strerror(rc) << ": " << mmspx_last_error()
+		          << std::endl;
+		MMSP::Abort(-1);
+	}
+}
+
+template <typename T> struct precision_of;
+template <> struct precision_of<double> {
+	static int get() { return MMSPX_F64; }
+};
+template <> struct precision_of<float> {
+	static int get() { return native_float_flag() ? MMSPX_F32 : MMSPX_F32REF; }
+};
+
+// Descriptor of a serial MMSP grid through its public friends
+// (MMSP.grid.h:268-337): extents x0/x1, spacing dx, boundary conditions b0/b1.
+template <int dim, typename G> mmspx_desc describe(const G& grid, int fields, int is_vector, int precision)
+{
+	mmspx_desc d;
+	std::memset(&d, 0, sizeof d);
+	d.dim = dim;
+	for (int k = 0; k < 3; k++) {
+		d.n[k] = 1;
+		d.dx[k] = 1.0;
+		d.b0[k] = d.b1[k] = MMSPX_PERIODIC;
+	}
+	// the accessors are friends defined inside grid: found by argument-dependent lookup
+	for (int k = 0; k < dim; k++) {
+		d.g0[k] = x0(grid, k);
+		d.n[k] = x1(grid, k) - x0(grid, k);
+		d.dx[k] = dx(grid, k);
+		d.b0[k] = b0(grid, k);
+		d.b1[k] = b1(grid, k);
+	}
+	d.fields = fields;
+	d.vector = is_vector;
+	d.precision = precision;
+	return d;
+}
+
+// Runs `steps` on the state, printing the reference's progress bar at the
+// steps where MMSP::print_progress prints (step 0, every steps/20, the last).
+inline void run_with_progress(mmspx_state* s, int model, const double* p, int np, int steps)
+{
+	int done = 0;
+	for (int step = 0; step < steps; step++) {
+		const bool prints = step == 0 || step == steps - 1 || (20 * step) % steps == 0;
+		if (!prints) continue;
+		check(mmspx_step(s, model, p, np, step - done), "mmspx_step");
+		check(mmspx_sync(s), "mmspx_sync");
+		done = step;
+		MMSP::print_progress(step, steps);
+	}
+	check(mmspx_step(s, model, p, np, steps - done), "mmspx_step");
+	check(mmspx_sync(s), "mmspx_sync");
+}
+
+// Scalar grids: grid<dim,double> / grid<dim,float>; in a serial build the
+// nodes are one contiguous array starting at &grid(0) (MMSP.grid.cpp:320, 385-400).
+template <int dim, typename T> void run_scalar(MMSP::grid<dim, T>& grid, int model, const double* p, int np, int steps)
+{
+	if (nodes(grid) == 0 || steps <= 0) return;
+	mmspx_desc d = describe<dim>(grid, fields(grid), 0, precision_of<T>::get());
+	mmspx_state* s = nullptr;
+	check(mmspx_create(&d, &s), "mmspx_create");
+	T* data = &grid(0);
+	check(mmspx_upload(s, data, MMSPX_NODE_MAJOR), "mmspx_upload");
+	run_with_progress(s, model, p, np, steps);
+	check(mmspx_download(s, data, MMSPX_NODE_MAJOR), "mmspx_download");
+	mmspx_destroy(s);
+}
+
+// Allen-Cahn, allen-cahn.cpp:47-75: new = phi - dt*M*(-r*phi + u*phi^3 - K*lap(phi)).
+template <int dim, typename T>
+void allen_cahn(MMSP::grid<dim, T>& grid, int steps, double r = 1.0, double u = 1.0, double K = 1.0, double M = 1.0,
+                double dt = 0.01)
+{
+	const double p[5] = {r, u, K, M, dt};
+	run_scalar(grid, MMSPX_AC, p, 5, steps);
+}
+
+// Cahn-Hilliard with the reference's chemical potential, cahn-hilliard.cpp:47-81:
+// mu = -r*phi + u*phi^3 - K*lap(phi); new = phi + dt*M*lap(mu).
+template <int dim, typename T>
+void cahn_hilliard(MMSP::grid<dim, T>& grid, int steps, double r = 1.0, double u = 1.0, double K = 1.0, double M = 1.0,
+                   double dt = 0.01)
+{
+	const double p[5] = {r, u, K, M, dt};
+	run_scalar(grid, MMSPX_CH, p, 5, steps);
+}
+
+// Cahn-Hilliard with the double-well chemical potential mu = 2c(1-c)(1-2c) - kappa*lap(c).
+template <int dim, typename T>
+void cahn_hilliard_double_well(MMSP::grid<dim, T>& grid, int steps, double kappa = 1.0, double M = 1.0,
+                               double dt = 0.005)
+{
+	const double p[3] = {kappa, M, dt};
+	run_scalar(grid, MMSPX_CH_DW, p, 3, steps);
+}
+
+// Grain growth on grid<dim,vector<T> >, graingrowth.cpp:73-111. The per-node
+// vectors are packed node-major (the reference's node order) for the upload.
+template <int dim, typename T> void grain_growth(MMSP::grid<dim, MMSP::vector<T> >& grid, int steps, double dt = 0.01)
+{
+	const int n = nodes(grid);
+	const int q = fields(grid);
+	if (n == 0 || steps <= 0) return;
+	mmspx_desc d = describe<dim>(grid, q, 1, precision_of<T>::get());
+	std::vector<T> host((size_t)n * q);
+	for (int i = 0; i < n; i++) {
+		const MMSP::vector<T>& v = grid(i);
+		if (MMSP::length(v) != q) {
+			std::cerr << "mmsp_gpu: node " << i << " has " << MMSP::length(v) << " fields, expected " << q << std::endl;
+			MMSP::Abort(-1);
+		}
+		for (int j = 0; j < q; j++) host[(size_t)i * q + j] = v[j];
+	}
+	mmspx_state* s = nullptr;
+	check(mmspx_create(&d, &s), "mmspx_create");
+	check(mmspx_upload(s, host.data(), MMSPX_NODE_MAJOR), "mmspx_upload");
+	const double p[1] = {dt};
+	run_with_progress(s, MMSPX_GG, p, 1, steps);
+	check(mmspx_download(s, host.data(), MMSPX_NODE_MAJOR), "mmspx_download");
+	mmspx_destroy(s);
+	for (int i = 0; i < n; i++) {
+		MMSP::vector<T>& v = grid(i);
+		for (int j = 0; j < q; j++) v[j] = host[(size_t)i * q + j];
+	}
+}
+
+}  // namespace mmsp_gpu
+
+#endif  // MMSP_GPU_HPP

@@ -388,6 +388,84 @@ class Batch:
         return dict(strip_width=a.value, rows_per_item=b.value, blocks_per_cu=c.value, tuned=bool(tuned))
 
 
+QUEUED, RUNNING, DONE, REJECTED, CANCELLED = 0, 1, 2, 3, 4
+
+
+class Server:
+    """Continuous-batching solve server (a worker thread in the library owns the GPU)."""
+
+    def __init__(self, model="ch_dw", precision="f32", hbm_budget=64 << 30, max_slots=256, chunk_steps=100):
+        cfg = ServerConfig()
+        cfg.hbm_budget = int(hbm_budget)
+        cfg.max_slots = int(max_slots)
+        cfg.chunk_steps = int(chunk_steps)
+        cfg.model = _model(model)
+        cfg.precision = PRECISIONS[precision]
+        h = C.c_void_p()
+        check(lib.mmspx_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.mmspx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, init, params, steps, bc=None, dx=None):
+        a = np.ascontiguousarray(init, dtype=np.float64)
+        d = make_desc(a.shape, 1, False, "f64", bc, dx)
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1))
+        rid = C.c_int64()
+        check(lib.mmspx_server_submit(self._h, C.byref(d), _dp(a.reshape(-1)), _dp(p), p.size, int(steps),
+                                      C.byref(rid)))
+        self.shapes[rid.value] = a.shape
+        return rid.value
+
+    @staticmethod
+    def _info(i):
+        return dict(id=i.id, status=i.status, steps_done=i.steps_done, t_submit=i.t_submit, t_admit=i.t_admit,
+                    t_done=i.t_done, mass0=i.mass0, mass1=i.mass1)
+
+    def wait(self, rid, timeout=-1.0):
+        i = RequestInfo()
+        check(lib.mmspx_server_wait(self._h, rid, float(timeout), C.byref(i)))
+        return self._info(i)
+
+    def info(self, rid):
+        i = RequestInfo()
+        check(lib.mmspx_server_info(self._h, rid, C.byref(i)))
+        return self._info(i)
+
+    def fetch(self, rid):
+        shape = self.shapes.pop(rid)
+        out = np.empty(shape, dtype=np.float64)
+        check(lib.mmspx_server_fetch(self._h, rid, _dp(out.reshape(-1)), out.size))
+        return out
+
+    def cancel(self, rid):
+        check(lib.mmspx_server_cancel(self._h, rid))
+
+    def clock(self):
+        return lib.mmspx_server_clock(self._h)
+
+    def stats(self):
+        out = (C.c_int64 * 7)()
+        err = lib.mmspx_server_stats(self._h, out, 7)
+        keys = ["admitted", "completed", "rejected", "rounds", "peak_bytes", "peak_concurrent", "launches"]
+        d = dict(zip(keys, list(out)))
+        d["worker_error"] = err
+        return d
+
+
 def copy_bandwidth(nbytes=1 << 30, reps=5):
     """Attainable device copy rate (bytes read + written per second)."""
     out = C.c_double()

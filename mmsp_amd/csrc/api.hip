@@ -165,6 +165,7 @@ void mmspx_destroy(mmspx_state* s)
 	if (!s) return;
 	// teardown: errors here cannot be reported to anyone, so they are dropped
 	if (s->stream) (void)hipStreamSynchronize(s->stream);
+	if (s->graph) (void)hipGraphExecDestroy(s->graph);
 	if (s->cur) (void)hipFree(s->cur);
 	if (s->nxt) (void)hipFree(s->nxt);
 	if (s->aux) (void)hipFree(s->aux);
@@ -279,6 +280,52 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	}
 	if ((model == MMSPX_CH || model == MMSPX_CH_DW) && !fused && !s->aux) {
 		if (hipMalloc(&s->aux, s->bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");
+	}
+	// Small grids are launch-bound: replay an even number of steps from a
+	// captured graph (the buffers are back in place after each replay).
+	const size_t small_bytes = (size_t)64 << 20;
+	if (steps >= 2 * kGraphSteps && s->bytes <= small_bytes) {
+		const ModelParams& q = s->graph_params;
+		const bool same_params = q.nr == p.nr && q.u == p.u && q.K == p.K && q.dtM == p.dtM && q.kappa == p.kappa &&
+		                         q.dt == p.dt && q.nrf == p.nrf && q.uf == p.uf && q.Kf == p.Kf && q.dtMf == p.dtMf &&
+		                         q.kappaf == p.kappaf && q.dtf == p.dtf;
+		const bool same_tile = s->graph_tile.variant == tc.variant && s->graph_tile.ty == tc.ty &&
+		                       s->graph_tile.tz == tc.tz && s->graph_tile.chunk == tc.chunk;
+		const bool hit = s->graph && s->graph_model == model && s->graph_fused == fused && s->graph_cur == s->cur &&
+		                 same_params && (!fused || same_tile);
+		if (!hit) {
+			if (s->graph) (void)hipGraphExecDestroy(s->graph);
+			s->graph = nullptr;
+			hipGraph_t graph = nullptr;
+			MMSPX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+			int64_t nl = 0;
+			void *a = s->cur, *b = s->nxt;
+			for (int k = 0; k < kGraphSteps && rc == 0; k++) {
+				rc = fused ? launch_fused_step(g, s->desc.precision, p, a, b, s->stream, tc, &nl)
+				           : launch_generic_step(g, s->desc.precision, p, a, b, s->aux, s->stream, &nl);
+				std::swap(a, b);
+			}
+			hipError_t e = hipStreamEndCapture(s->stream, &graph);
+			if (rc) {
+				if (graph) (void)hipGraphDestroy(graph);
+				return rc;
+			}
+			if (e != hipSuccess) return fail(MMSPX_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+			e = hipGraphInstantiate(&s->graph, graph, nullptr, nullptr, 0);
+			(void)hipGraphDestroy(graph);
+			if (e != hipSuccess) return fail(MMSPX_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+			s->graph_model = model;
+			s->graph_fused = fused;
+			s->graph_cur = s->cur;
+			s->graph_params = p;
+			s->graph_tile = tc;
+			s->graph_launches = (int)nl;
+		}
+		while (steps >= kGraphSteps) {
+			MMSPX_HIP(hipGraphLaunch(s->graph, s->stream));
+			s->launches += s->graph_launches;
+			steps -= kGraphSteps;
+		}
 	}
 	for (int k = 0; k < steps; k++) {
 		rc = fused ? launch_fused_step(g, s->desc.precision, p, s->cur, s->nxt, s->stream, tc, &s->launches)

@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(NT) ch_batch(const BMember* __restrict__ mem, 
 
 // tile shapes of the batched Cahn-Hilliard kernels (fixed; the strips' width
 // and chunk depth are what the tuner varies)
-constexpr int kCh3TY = 8, kCh3TZ = 64, kCh3NT = 256;
+constexpr int kCh3TY = 16, kCh3TZ = 64, kCh3NT = 256;
 
 bool batch_model_ok(int model, const Geom& g, std::string* why)
 {
@@ -103,8 +103,14 @@ void member_items(int m, const Geom& g, int model, const BatchTile& t, std::vect
 					out3.push_back(BItem{m, y0, z0, xb, std::min(xb + chunk, nx)});
 		return;
 	}
+	// 2-D: strips of the contiguous axis (axis 1). The strip bodies take the
+	// strip origin as y0; the 2-D Cahn-Hilliard tile names that axis z.
+	const bool ch2 = model == MMSPX_CH || model == MMSPX_CH_DW;
 	for (int xb = 0; xb < nx; xb += t.chunk)
-		for (int y0 = 0; y0 < g.n[1]; y0 += t.tz) out2.push_back(BItem{m, y0, 0, xb, std::min(xb + t.chunk, nx)});
+		for (int c0 = 0; c0 < g.n[1]; c0 += t.tz) {
+			const int xe = std::min(xb + t.chunk, nx);
+			out2.push_back(ch2 ? BItem{m, 0, c0, xb, xe} : BItem{m, c0, 0, xb, xe});
+		}
 }
 
 template <int PREC, int TZ>
@@ -172,6 +178,20 @@ int launch_batch_step(int model, int prec, int fields, const BatchTile& t, const
 	MMSPX_HIP(hipGetLastError());
 	*launches += (n2 > 0) + (n3 > 0);
 	return 0;
+}
+
+std::vector<BItem> xcd_order(const std::vector<BItem>& in)
+{
+	const size_t n = in.size();
+	std::vector<BItem> out;
+	out.reserve(n);
+	size_t start[9];
+	for (int k = 0; k <= 8; k++) start[k] = n * k / 8;
+	const size_t longest = start[1] - start[0] + 1;
+	for (size_t j = 0; j < longest; j++)
+		for (int k = 0; k < 8; k++)
+			if (start[k] + j < start[k + 1]) out.push_back(in[start[k] + j]);
+	return out;
 }
 
 int device_cus()
@@ -244,8 +264,9 @@ static int upload_plan(mmspx_batch* b)
 			return fail(MMSPX_ENOMEM, "device allocation failed (work list)");
 		b->items_cap = need;
 	}
-	std::vector<BItem> all(l2);
-	all.insert(all.end(), l3.begin(), l3.end());
+	std::vector<BItem> all = xcd_order(l2);
+	const std::vector<BItem> o3 = xcd_order(l3);
+	all.insert(all.end(), o3.begin(), o3.end());
 	MMSPX_HIP(hipMemcpyAsync(b->d_items, all.data(), sizeof(BItem) * all.size(), hipMemcpyHostToDevice, b->st));
 	MMSPX_HIP(hipMemcpyAsync(b->d_mem, mem.data(), sizeof(BMember) * mem.size(), hipMemcpyHostToDevice, b->st));
 	MMSPX_HIP(hipStreamSynchronize(b->st));

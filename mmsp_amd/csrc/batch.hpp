@@ -36,5 +36,10 @@ void member_items(int m, const Geom& g, int model, const BatchTile& t, std::vect
 int launch_batch_step(int model, int prec, int fields, const BatchTile& t, const BMember* mem, const BItem* l2, int n2,
                       const BItem* l3, int n3, int parity, int cus, hipStream_t st, int64_t* launches);
 int device_cus();
+// Reorders a spatially ordered work list so that the persistent launch (item
+// p runs on workgroup p mod gridDim, and workgroup ids are dealt round-robin
+// over the 8 XCDs) gives each XCD one contiguous run of it: neighbouring
+// tiles then share their halo reads in one L2.
+std::vector<BItem> xcd_order(const std::vector<BItem>& in);
 
 }  // namespace mmspx

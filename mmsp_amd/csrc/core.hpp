@@ -79,11 +79,20 @@ struct mmspx_state {
 	bool tuned = false;      // a tile chosen by mmspx_set_tile / mmspx_autotune
 	mmspx::TuneChoice tile{};
 	int64_t launches = 0;
+	// captured multi-step graph for small grids (launch-bound), keyed by what
+	// its launches baked in
+	hipGraphExec_t graph = nullptr;
+	int graph_steps = 0, graph_model = -1, graph_launches = 0;
+	bool graph_fused = false;
+	mmspx::TuneChoice graph_tile{};
+	mmspx::ModelParams graph_params{};
+	void* graph_cur = nullptr;
 };
 
 namespace mmspx {
 
 constexpr int kRedBlocks = 512;
+constexpr int kGraphSteps = 50;  // steps per captured graph (even: buffers return to place)
 constexpr int kRedThreads = 256;
 
 // ---- kernel launchers (kernels_generic.hip, kernels_fused.hip, reduce.hip) ----

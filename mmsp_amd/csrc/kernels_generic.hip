@@ -250,17 +250,27 @@ int launch_convert(const void* src, int src_f64, void* dst, int dst_f64, int64_t
 	return 0;
 }
 
-// Plain streaming copy (16-byte vectors): the HBM calibration kernel.
-__global__ void __launch_bounds__(256) copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n)
+// Plain streaming copy: the HBM calibration kernel. Each thread moves four
+// 16-byte vectors per iteration (loads issued before stores) over a
+// grid-stride range; n4 counts groups of four vectors.
+__global__ void __launch_bounds__(256) copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n4)
 {
-	for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-		dst[k] = src[k];
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n4; k += stride) {
+		uint4 v0 = src[k], v1 = src[k + n4], v2 = src[k + 2 * n4], v3 = src[k + 3 * n4];
+		dst[k] = v0;
+		dst[k + n4] = v1;
+		dst[k + 2 * n4] = v2;
+		dst[k + 3 * n4] = v3;
+	}
 }
 
 int launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st)
 {
-	const int64_t n = (int64_t)(bytes / sizeof(uint4));
-	hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, st, (const uint4*)src, (uint4*)dst, n);
+	const int64_t n4 = (int64_t)(bytes / (4 * sizeof(uint4)));
+	int cus = 256, dev = 0;
+	if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+	hipLaunchKernelGGL(copy_kernel, dim3(cus * 8), dim3(256), 0, st, (const uint4*)src, (uint4*)dst, n4);
 	MMSPX_HIP(hipGetLastError());
 	return 0;
 }

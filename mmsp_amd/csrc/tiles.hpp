@@ -149,16 +149,30 @@ __device__ __forceinline__ void ac_strip(const typename Form<PREC>::S* __restric
 	const int y = y0 + (act ? t : nact - 1);
 	const int yl = bc_map(y0 - 1, ny, g.b0[1], g.b1[1]);
 	const int yr = bc_map(y0 + nact, ny, g.b0[1], g.b1[1]);
+	auto at = [&](int x) { return a + (int64_t)bc_map(x, nx, g.b0[0], g.b1[0]) * ny; };
 	__syncthreads();
-	S prev = a[(int64_t)bc_map(xb - 1, nx, g.b0[0], g.b1[0]) * ny + y];
+	// three-row window prev/cur/next plus row x+2 in flight; the halo
+	// columns of the current row are loaded one row ahead as well
+	S prev = at(xb - 1)[y];
 	S cur = a[(int64_t)xb * ny + y];
+	S next = at(xb + 1)[y];
+	S hl = 0, hr = 0;
+	if (t == 0) hl = a[(int64_t)xb * ny + yl];
+	if (t == nact - 1) hr = a[(int64_t)xb * ny + yr];
 	for (int x = xb; x < xe; x++) {
-		const S next = a[(int64_t)bc_map(x + 1, nx, g.b0[0], g.b1[0]) * ny + y];
+		// row x+2 is needed only if row x+1 is computed here; then x+2 <= nx,
+		// at most one row past the edge, which check_boundary's mapping covers
+		S nn = 0;
+		if (x + 1 < xe) nn = at(x + 2)[y];
+		S nhl = 0, nhr = 0;
+		if (x + 1 < xe) {
+			if (t == 0) nhl = a[(int64_t)(x + 1) * ny + yl];
+			if (t == nact - 1) nhr = a[(int64_t)(x + 1) * ny + yr];
+		}
 		S* lr = row[x & 1];
-		const S* rc = a + (int64_t)x * ny;
 		if (act) lr[t + 1] = cur;
-		if (t == 0) lr[0] = rc[yl];
-		if (t == nact - 1) lr[nact + 1] = rc[yr];
+		if (t == 0) lr[0] = hl;
+		if (t == nact - 1) lr[nact + 1] = hr;
 		__syncthreads();
 		if (act) {
 			typename F::L lap = F::lap_zero();
@@ -168,10 +182,14 @@ __device__ __forceinline__ void ac_strip(const typename Form<PREC>::S* __restric
 		}
 		prev = cur;
 		cur = next;
+		next = nn;
+		hl = nhl;
+		hr = nhr;
 	}
 }
 
-// Grain growth, Q fields stored field-major (plane j at offset j*nodes).
+// Grain growth, Q fields stored field-major (plane j at offset j*nodes). Same
+// window as ac_strip per field; the sum over fields stays in registers.
 template <int PREC, int Q, int TZ>
 __device__ __forceinline__ void gg_strip(const typename Form<PREC>::S* __restrict__ a,
                                          typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
@@ -188,22 +206,25 @@ __device__ __forceinline__ void gg_strip(const typename Form<PREC>::S* __restric
 	const int y = y0 + (act ? t : nact - 1);
 	const int yl = bc_map(y0 - 1, ny, g.b0[1], g.b1[1]);
 	const int yr = bc_map(y0 + nact, ny, g.b0[1], g.b1[1]);
-	S prev[Q], cur[Q], next[Q];
+	auto rowoff = [&](int x) { return (int64_t)bc_map(x, nx, g.b0[0], g.b1[0]) * ny; };
+	S prev[Q], cur[Q], next[Q], nn[Q];
 	__syncthreads();
 	{
-		const int64_t rp = (int64_t)bc_map(xb - 1, nx, g.b0[0], g.b1[0]) * ny + y;
-		const int64_t rcx = (int64_t)xb * ny + y;
+		const int64_t rp = rowoff(xb - 1) + y, rc = (int64_t)xb * ny + y, rn = rowoff(xb + 1) + y;
 #pragma unroll
 		for (int j = 0; j < Q; j++) {
 			prev[j] = a[j * N + rp];
-			cur[j] = a[j * N + rcx];
+			cur[j] = a[j * N + rc];
+			next[j] = a[j * N + rn];
 		}
 	}
 	for (int x = xb; x < xe; x++) {
-		const int64_t rn = (int64_t)bc_map(x + 1, nx, g.b0[0], g.b1[0]) * ny + y;
-		const int64_t rc = (int64_t)x * ny;
+		if (x + 1 < xe) {  // row x+2 only when it will be used (x+2 <= nx)
+			const int64_t rn2 = rowoff(x + 2) + y;
 #pragma unroll
-		for (int j = 0; j < Q; j++) next[j] = a[j * N + rn];
+			for (int j = 0; j < Q; j++) nn[j] = a[j * N + rn2];
+		}
+		const int64_t rc = (int64_t)x * ny;
 		const int s = x & 1;
 #pragma unroll
 		for (int j = 0; j < Q; j++) {
@@ -228,6 +249,7 @@ __device__ __forceinline__ void gg_strip(const typename Form<PREC>::S* __restric
 		for (int j = 0; j < Q; j++) {
 			prev[j] = cur[j];
 			cur[j] = next[j];
+			next[j] = nn[j];
 		}
 	}
 }

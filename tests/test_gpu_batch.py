@@ -66,7 +66,7 @@ def test_batch_allen_cahn_boundaries_and_alone_vs_in_mix():
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_batch_cahn_hilliard_2d_and_3d_members(prec):
-    shapes = [(64, 64), (20, 36, 40), (100, 70), (16, 16, 16), (33, 65, 70)]
+    shapes = [(64, 64), (20, 36, 40), (100, 70), (16, 16, 16), (33, 65, 70), (40, 600), (9, 20, 300), (70, 257)]
     say("[gpu] batch CH %s" % prec)
     params = np.array([[1.0, 1.0, 0.005]] * len(shapes))
     init = [seeded(s, 80 + m, 0.45, 0.55, f32=prec == "f32") for m, s in enumerate(shapes)]

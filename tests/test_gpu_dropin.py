@@ -1,0 +1,100 @@
+"""GPU: the plug-in boundary. The reference's own programs (main(), command
+line, .dat reader/writer and checkpoint naming from MMSP.main.hpp and the MMSP
+headers) compiled with their update() bodies replaced by include/mmsp_gpu.hpp
+(oracle/_ref/*-gpu, built from mmsp_amd/dropin/) must write the same files as
+the stock reference programs (oracle/_ref/allen-cahn ...), run side by side:
+same names, same headers, data within 1e-12 (FP64) and bitwise equal to the
+oracle with the device's cube."""
+import os
+import re
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import dat
+from conftest import golden_path, params_of
+
+pytestmark = pytest.mark.gpu
+mx = pytest.importorskip("mmsp_amd")
+
+PROGRAMS = {"ac": ("allen-cahn", "allen-cahn-gpu"), "ch": ("cahn-hilliard", "cahn-hilliard-gpu"),
+            "gg": ("graingrowth", "graingrowth-gpu")}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def have(name):
+    return os.path.exists(os.path.join(O.REF_DIR, name))
+
+
+def run(prog, cwd, args):
+    r = subprocess.run([os.path.join(O.REF_DIR, prog)] + args, cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (prog, r.stdout[-2000:], r.stderr[-2000:])
+    return r.stdout
+
+
+def progress_shape(out):
+    # the bar's dots and the structure of the lines, without clock values
+    return [re.sub(r"[0-9]", "#", line) for line in out.splitlines()
+            if "•" in line or line.startswith("No.")]
+
+
+# stock-parameter golden cases: the drop-in programs use the examples' own parameters
+# (the stock programs' typedefs are double grids, so the float fixtures are not used here)
+CASES = [("ac2d_f64", "ac"), ("ac3d_bc_f64", "ac"), ("ch2d_stock_f64", "ch"),
+         ("gg3d_f64", "gg"), ("gg2d_bc_f64", "gg"), ("gg1d_f64", "gg")]
+
+
+@pytest.mark.parametrize("name,model", CASES)
+def test_dropin_program_matches_stock_program(name, model, tmp_path):
+    stock, gpu = PROGRAMS[model]
+    if not (have(stock) and have(gpu)):
+        pytest.skip("reference programs not built here")
+    say("[gpu] drop-in %s via %s" % (name, gpu))
+    import json
+
+    case = [c for c in json.load(open(os.path.join(os.path.dirname(golden_path(name, "in")), "manifest.json")))
+            if c["name"] == name][0]
+    steps = case["steps"]
+    inc = max(1, steps // 3)
+    while steps % inc:
+        inc += 1
+    outs = {}
+    for prog in (stock, gpu):
+        d = tmp_path / prog
+        d.mkdir()
+        shutil.copy(golden_path(name, "in"), d / "start.dat")
+        out = run(prog, str(d), ["start.dat", str(steps), str(inc)])
+        outs[prog] = (sorted(p for p in os.listdir(d) if p != "start.dat"), out, d)
+    files_s, out_s, ds = outs[stock]
+    files_g, out_g, dg = outs[gpu]
+    assert files_g == files_s and len(files_g) == steps // inc  # checkpoint names from MMSP.main.hpp
+    assert progress_shape(out_g) == progress_shape(out_s)
+    gin = dat.read_dat(golden_path(name, "in"))
+    for f in files_s:
+        a, b = dat.read_dat(str(ds / f)), dat.read_dat(str(dg / f))
+        assert (a.dim, a.hdr_fields, a.b0, a.b1, a.dx, a.g0) == (b.dim, b.hdr_fields, b.b0, b.b1, b.dx, b.g0)
+        ref = a.data.astype(np.float64)
+        err = np.abs(b.data.astype(np.float64) - ref).max() / np.abs(ref).max()
+        assert err <= (1e-12 if a.data.dtype == np.float64 else 1e-6), (f, err)
+    # the last checkpoint equals the oracle run with the device's cube, bitwise
+    form = "f64" if case["dtype"] == "f64" else "f32ref"
+    p = params_of(case)
+    bc = list(zip(gin.b0, gin.b1))
+    if model == "ac":
+        exp = O.ac(gin.data, p, steps, form=form, bc=bc, dx=gin.dx)
+    elif model == "ch":
+        exp = O.ch(gin.data, p, steps, mu=O.MU_POLY, form=form, bc=bc, dx=gin.dx)
+    else:
+        exp = O.gg(gin.data, p[0], steps, form=form, bc=bc, dx=gin.dx)
+    last = dat.read_dat(str(dg / files_g[-1])).data
+    assert np.array_equal(last, exp)
+    # and the stock program's last file equals the reference's golden output
+    assert np.array_equal(dat.read_dat(str(ds / files_s[-1])).data, dat.read_dat(golden_path(name, "out")).data)

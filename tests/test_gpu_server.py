@@ -1,0 +1,116 @@
+"""GPU: the continuous-batching server. Results must not depend on what else
+is resident: each request equals (bitwise) the oracle's run of it alone, the
+single-grid path's run of it, and its own run on an otherwise empty server."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+mx = pytest.importorskip("mmsp_amd")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+P_DW = [1.0, 1.0, 0.005]
+
+
+def req(shape, seed):
+    return O.uniform(int(np.prod(shape)), seed, 0.45, 0.55).reshape(shape)
+
+
+def oracle_f32(init, params, steps):
+    return O.ch(init.astype(np.float32), params, steps, mu=O.MU_DW, form="f32").astype(np.float64)
+
+
+def test_mixed_stream_alone_vs_in_mix():
+    specs = [((64, 64), 130, 1), ((24, 20, 28), 75, 2), ((100, 36), 310, 3), ((16, 16, 16), 40, 4),
+             ((70, 130), 95, 5), ((33, 40, 18), 200, 6), ((30, 700), 60, 7), ((20, 70, 150), 45, 8)]
+    say("[gpu] server mixed stream")
+    params = [[1.0, 1.0, 0.005], [0.8, 1.0, 0.004], [1.2, 1.0, 0.005], [1.0, 0.5, 0.005], [1.0, 1.0, 0.003],
+              [1.0, 1.0, 0.005], [1.0, 1.0, 0.005], [0.9, 1.0, 0.005]]
+    with mx.Server("ch_dw", "f32", max_slots=8, chunk_steps=16) as sv:
+        ids = []
+        for k, (shape, steps, seed) in enumerate(specs):
+            ids.append(sv.submit(req(shape, seed), params[k], steps))
+            time.sleep(0.003 * (k % 3))  # arrivals while earlier requests run
+        infos = [sv.wait(i, 120) for i in ids]
+        mix = [sv.fetch(i) for i in ids]
+        st = sv.stats()
+    assert st["completed"] == len(specs) and st["rejected"] == 0 and st["worker_error"] == 0
+    assert st["peak_concurrent"] >= 2
+    for k, (shape, steps, seed) in enumerate(specs):
+        assert infos[k]["status"] == mx.DONE and infos[k]["steps_done"] == steps
+        # FP32 state, FP64 mass check
+        assert abs(infos[k]["mass1"] - infos[k]["mass0"]) <= 1e-5 * abs(infos[k]["mass0"])
+        exp = oracle_f32(req(shape, seed), params[k], steps)
+        assert np.array_equal(mix[k], exp), (k, shape)
+        with mx.Grid(shape, precision="f32") as g:
+            g.upload(req(shape, seed).astype(np.float32))
+            g.step("ch_dw", params[k], steps)
+            assert np.array_equal(g.download().astype(np.float64), mix[k])
+    # one of them alone on a fresh server
+    with mx.Server("ch_dw", "f32", max_slots=8, chunk_steps=7) as sv:
+        shape, steps, seed = specs[2]
+        i = sv.submit(req(shape, seed), params[2], steps)
+        sv.wait(i, 60)
+        assert np.array_equal(sv.fetch(i), mix[2])
+
+
+def test_fp64_server_conserves_mass():
+    say("[gpu] server fp64 mass")
+    with mx.Server("ch_dw", "f64", chunk_steps=50) as sv:
+        a = req((48, 40, 36), 11)
+        i = sv.submit(a, P_DW, 200)
+        info = sv.wait(i, 60)
+        got = sv.fetch(i)
+    assert abs(info["mass1"] - info["mass0"]) <= 1e-12 * abs(info["mass0"])
+    assert np.array_equal(got, O.ch(a, P_DW, 200, mu=O.MU_DW))
+
+
+def test_admission_by_memory_and_rejection():
+    say("[gpu] server admission")
+    small = req((64, 64), 21)
+    one = 2 * ((64 * 64 * 4 + 255) // 256 * 256)
+    with mx.Server("ch_dw", "f32", hbm_budget=one, max_slots=16, chunk_steps=10) as sv:
+        ids = [sv.submit(small, P_DW, 30) for _ in range(4)]
+        big = sv.submit(req((128, 128), 22), P_DW, 10)  # can never fit the budget
+        infos = [sv.wait(i, 60) for i in ids]
+        binfo = sv.wait(big, 60)
+        st = sv.stats()
+        outs = [sv.fetch(i) for i in ids]
+    assert binfo["status"] == mx.REJECTED
+    assert all(i["status"] == mx.DONE for i in infos)
+    assert st["peak_concurrent"] == 1 and st["peak_bytes"] <= one and st["rejected"] == 1
+    # admitted one after the other, in submission order
+    assert all(infos[k]["t_admit"] >= infos[k - 1]["t_done"] - 1e-3 for k in range(1, 4))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+def test_cancel_queued_request():
+    say("[gpu] server cancel")
+    a = req((64, 64), 31)
+    one = 2 * ((64 * 64 * 4 + 255) // 256 * 256)
+    with mx.Server("ch_dw", "f32", hbm_budget=one, max_slots=1, chunk_steps=10) as sv:
+        i1 = sv.submit(a, P_DW, 2000)
+        i2 = sv.submit(a, P_DW, 10)
+        sv.cancel(i2)
+        assert sv.wait(i2, 60)["status"] == mx.CANCELLED
+        assert sv.wait(i1, 60)["status"] == mx.DONE
+        with pytest.raises(mx.MmspxError):
+            sv.fetch(i2)
+
+
+def test_submit_validation():
+    with mx.Server("ch_dw", "f32") as sv:
+        with pytest.raises(mx.MmspxError):
+            sv.submit(req((8, 8), 1), [1.0, 1.0], 10)  # wrong parameter count
+        with pytest.raises(mx.MmspxError):
+            sv.submit(req((8, 8), 1), P_DW, 10, bc=[(mx.NEUMANN, mx.NEUMANN), (mx.PERIODIC, mx.PERIODIC)])
