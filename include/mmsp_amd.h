@@ -148,6 +148,10 @@ int mmspx_set_variant(mmspx_state* s, int variant);
 /* Tile of the fused kernels: ty x tz cells of the (axis 1, last axis) plane,
  * marching `chunk` planes along axis 0 (2-D grids: ty = 1). */
 int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
+/* Fused Cahn-Hilliard tile body: 0 keeps every neighbour in LDS rings, 1
+ * keeps the axis-0 neighbours in per-thread register queues. Same bits. */
+int mmspx_set_tile_options(mmspx_state* s, int register_queues);
+/* Returns variant + 16 * tile body (1: register queues). */
 int mmspx_get_tile(const mmspx_state* s, int* ty, int* tz, int* chunk);
 /* Project autotuner: times every candidate tile for `model` on this state's
  * shape (trial_steps each, state restored afterwards), keeps the fastest and
@@ -186,7 +190,13 @@ int mmspx_batch_reduce(mmspx_batch* b, int member, int what, double* out, int no
  * persistent workgroups per CU) on the batch itself, keeps the fastest and
  * restores the members' state. */
 int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t report_len);
-/* Returns 1 when the tile came from the autotuner, 0 for the default. */
+/* Sets the work decomposition directly: strips of strip_width (64, 128, 256)
+ * columns by rows_per_item rows with blocks_per_cu persistent workgroups per
+ * CU, or, for Allen-Cahn with cells > 0, runs of `cells` consecutive nodes. */
+int mmspx_batch_set_tile(mmspx_batch* b, int strip_width, int rows_per_item, int blocks_per_cu, int cells);
+/* Returns 1 when the tile came from the autotuner, 0 for the default. For
+ * Allen-Cahn batches run by runs of consecutive nodes, strip_width and
+ * blocks_per_cu are 0 and rows_per_item holds the nodes per workgroup. */
 int mmspx_batch_get_tile(const mmspx_batch* b, int* strip_width, int* rows_per_item, int* blocks_per_cu);
 
 /* ---- serving: continuous batching of solves ------------------------------ */

@@ -107,6 +107,7 @@ def _load():
         "mmspx_set_variant": (C.c_int, [vp, C.c_int]),
         "mmspx_set_tile": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mmspx_get_tile": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "mmspx_set_tile_options": (C.c_int, [vp, C.c_int]),
         "mmspx_autotune": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
         "mmspx_kernel_time": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, dp, dp]),
         "mmspx_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, dp]),
@@ -120,6 +121,7 @@ def _load():
         "mmspx_batch_reduce": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int]),
         "mmspx_batch_autotune": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_size_t]),
         "mmspx_batch_get_tile": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "mmspx_batch_set_tile": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
         "mmspx_server_create": (C.c_int, [P(ServerConfig), P(vp)]),
         "mmspx_server_destroy": (None, [vp]),
         "mmspx_server_submit": (C.c_int, [vp, P(Desc), dp, dp, C.c_int, C.c_int, i64p]),
@@ -277,10 +279,13 @@ class Grid:
     def set_tile(self, ty, tz, chunk):
         check(lib.mmspx_set_tile(self._h, ty, tz, chunk))
 
+    def set_tile_options(self, register_queues=False):
+        check(lib.mmspx_set_tile_options(self._h, 1 if register_queues else 0))
+
     def get_tile(self):
         ty, tz, ch = C.c_int(), C.c_int(), C.c_int()
         v = lib.mmspx_get_tile(self._h, C.byref(ty), C.byref(tz), C.byref(ch))
-        return dict(variant=v, ty=ty.value, tz=tz.value, chunk=ch.value)
+        return dict(variant=v % 16, ty=ty.value, tz=tz.value, chunk=ch.value, opt=v // 16)
 
     def autotune(self, model, params, trial_steps=20):
         m = _model(model)
@@ -381,6 +386,9 @@ class Batch:
         buf = C.create_string_buffer(16384)
         check(lib.mmspx_batch_autotune(self._h, int(trial_steps), buf, 16384))
         return buf.value.decode()
+
+    def set_tile(self, strip_width=256, rows_per_item=32, blocks_per_cu=8, cells=0):
+        check(lib.mmspx_batch_set_tile(self._h, strip_width, rows_per_item, blocks_per_cu, cells))
 
     def get_tile(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

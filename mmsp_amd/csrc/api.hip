@@ -274,6 +274,7 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	if (s->variant != 1 && fused_supported(g, s->desc.precision, model)) {
 		tc = s->tuned ? s->tile : default_choice(g, s->desc.precision, model);
 		if (s->variant == 2 && tc.variant != 2) tc = default_choice(g, s->desc.precision, model);
+		if (s->variant == 2 && tc.variant != 2) tc = TuneChoice{2, 1, 128, 16, 0};  // strips where generic is the default
 		fused = s->variant == 2 || tc.variant == 2;
 	} else if (s->variant == 2) {
 		return fail(MMSPX_EUNSUPPORTED, "no tiled/fused kernel for this model, shape and boundary condition");
@@ -290,9 +291,10 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 		                         q.dt == p.dt && q.nrf == p.nrf && q.uf == p.uf && q.Kf == p.Kf && q.dtMf == p.dtMf &&
 		                         q.kappaf == p.kappaf && q.dtf == p.dtf;
 		const bool same_tile = s->graph_tile.variant == tc.variant && s->graph_tile.ty == tc.ty &&
-		                       s->graph_tile.tz == tc.tz && s->graph_tile.chunk == tc.chunk;
+		                       s->graph_tile.tz == tc.tz && s->graph_tile.chunk == tc.chunk &&
+		                       s->graph_tile.opt == tc.opt;
 		const bool hit = s->graph && s->graph_model == model && s->graph_fused == fused && s->graph_cur == s->cur &&
-		                 same_params && (!fused || same_tile);
+		                 s->graph_parity == (int)(s->steps_run & 1) && same_params && (!fused || same_tile);
 		if (!hit) {
 			if (s->graph) (void)hipGraphExecDestroy(s->graph);
 			s->graph = nullptr;
@@ -315,6 +317,7 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 			(void)hipGraphDestroy(graph);
 			if (e != hipSuccess) return fail(MMSPX_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
 			s->graph_model = model;
+			s->graph_parity = (int)(s->steps_run & 1);
 			s->graph_fused = fused;
 			s->graph_cur = s->cur;
 			s->graph_params = p;
@@ -324,6 +327,7 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 		while (steps >= kGraphSteps) {
 			MMSPX_HIP(hipGraphLaunch(s->graph, s->stream));
 			s->launches += s->graph_launches;
+			s->steps_run += kGraphSteps;
 			steps -= kGraphSteps;
 		}
 	}
@@ -332,6 +336,7 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 		           : launch_generic_step(g, s->desc.precision, p, s->cur, s->nxt, s->aux, s->stream, &s->launches);
 		if (rc) return rc;
 		std::swap(s->cur, s->nxt);
+		s->steps_run++;
 	}
 	return 0;
 }
@@ -344,10 +349,21 @@ int mmspx_step(mmspx_state* s, int model, const double* params, int np, int step
 int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
-	TuneChoice t{2, ty, tz, chunk};
+	TuneChoice t{2, ty, tz, chunk, s->tuned ? s->tile.opt : 0};
 	if (!tile_known(s->geom, t)) return fail(MMSPX_EINVAL, "no compiled tile of that shape for this grid");
 	s->tile = t;
 	s->tuned = true;
+	return 0;
+}
+
+int mmspx_set_tile_options(mmspx_state* s, int register_queues)
+{
+	if (!s) return fail(MMSPX_EINVAL, "null state");
+	if (!s->tuned) {
+		s->tile = default_choice(s->geom, s->desc.precision, MMSPX_CH_DW);
+		s->tuned = true;
+	}
+	s->tile.opt = register_queues ? 1 : 0;
 	return 0;
 }
 
@@ -358,7 +374,7 @@ int mmspx_get_tile(const mmspx_state* s, int* ty, int* tz, int* chunk)
 	if (ty) *ty = t.ty;
 	if (tz) *tz = t.tz;
 	if (chunk) *chunk = t.chunk;
-	return t.variant;
+	return t.variant + 16 * t.opt;
 }
 
 int mmspx_kernel_time(mmspx_state* s, int model, const double* params, int np, int reps, double* mean_s,
@@ -416,7 +432,7 @@ int mmspx_autotune(mmspx_state* s, int model, const double* params, int np, int 
 	const bool old_tuned = s->tuned;
 	const TuneChoice old_tile = s->tile;
 	double best = 1e30;
-	TuneChoice best_t{1, 0, 0, 0};
+	TuneChoice best_t{1, 0, 0, 0, 0};
 	for (int k = -1; k < nc && rc == 0; k++) {
 		if (k < 0) {
 			s->variant = 1;
@@ -432,9 +448,9 @@ int mmspx_autotune(mmspx_state* s, int model, const double* params, int np, int 
 			rc = fail(MMSPX_EDEVICE, "restore copy failed");
 		if (rc) break;
 		char line[160];
-		const TuneChoice t = k < 0 ? TuneChoice{1, 0, 0, 0} : cand[k];
-		std::snprintf(line, sizeof line, "variant=%d ty=%d tz=%d chunk=%d mean_us=%.2f min_us=%.2f\n", t.variant, t.ty,
-		              t.tz, t.chunk, mean * 1e6, mn * 1e6);
+		const TuneChoice t = k < 0 ? TuneChoice{1, 0, 0, 0, 0} : cand[k];
+		std::snprintf(line, sizeof line, "variant=%d ty=%d tz=%d chunk=%d opt=%d mean_us=%.2f min_us=%.2f\n", t.variant,
+		              t.ty, t.tz, t.chunk, t.opt, mean * 1e6, mn * 1e6);
 		rep += line;
 		if (mean < best) {
 			best = mean;

@@ -34,6 +34,37 @@ __global__ void __launch_bounds__(TZ) ac_batch(const BMember* __restrict__ mem, 
 	}
 }
 
+// Allen-Cahn by runs of consecutive nodes (item: member m, first node y0,
+// count z0), one workgroup per item, one thread per node as the generic
+// kernel: neighbours through check_boundary's mapping, same arithmetic.
+template <int PREC>
+__global__ void __launch_bounds__(256) ac_batch_cells(const BMember* __restrict__ mem, const BItem* __restrict__ items,
+                                                      int parity)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	const BItem w = items[blockIdx.x];
+	const BMember& m = mem[w.m];
+	const int src = (parity + m.phase) & 1;
+	const S* __restrict__ a = (const S*)m.buf[src];
+	S* __restrict__ b = (S*)m.buf[src ^ 1];
+	const int n0 = m.g.n[0], n1 = m.g.n[1];
+	const int end = w.y0 + w.z0;
+	for (int i = w.y0 + (int)threadIdx.x; i < end; i += 256) {
+		const int x = i / n1, y = i - x * n1;
+		int xh = x + 1, xl = x - 1, yh = y + 1, yl = y - 1;
+		if (xh >= n0) xh = bc_map(xh, n0, m.g.b0[0], m.g.b1[0]);
+		if (xl < 0) xl = bc_map(xl, n0, m.g.b0[0], m.g.b1[0]);
+		if (yh >= n1) yh = bc_map(yh, n1, m.g.b0[1], m.g.b1[1]);
+		if (yl < 0) yl = bc_map(yl, n1, m.g.b0[1], m.g.b1[1]);
+		const S c = a[i];
+		typename F::L lap = F::lap_zero();
+		F::lap_add(lap, m.g.w[0], m.g.wf[0], a[xh * n1 + y], c, a[xl * n1 + y]);
+		F::lap_add(lap, m.g.w[1], m.g.wf[1], a[x * n1 + yh], c, a[x * n1 + yl]);
+		b[i] = F::ac(c, lap, m.p);
+	}
+}
+
 template <int PREC, int Q, int TZ>
 __global__ void __launch_bounds__(TZ) gg_batch(const BMember* __restrict__ mem, const BItem* __restrict__ items,
                                                int nitems, int parity)
@@ -106,6 +137,10 @@ void member_items(int m, const Geom& g, int model, const BatchTile& t, std::vect
 	// 2-D: strips of the contiguous axis (axis 1). The strip bodies take the
 	// strip origin as y0; the 2-D Cahn-Hilliard tile names that axis z.
 	const bool ch2 = model == MMSPX_CH || model == MMSPX_CH_DW;
+	if (model == MMSPX_AC && t.cells > 0) {
+		for (int f = 0; f < g.nodes; f += t.cells) out2.push_back(BItem{m, f, std::min(t.cells, g.nodes - f), 0, 0});
+		return;
+	}
 	for (int xb = 0; xb < nx; xb += t.chunk)
 		for (int c0 = 0; c0 < g.n[1]; c0 += t.tz) {
 			const int xe = std::min(xb + t.chunk, nx);
@@ -151,7 +186,9 @@ template <int PREC>
 static void launch_prec(int model, int fields, const BatchTile& t, const BMember* mem, const BItem* l2, int n2,
                         const BItem* l3, int n3, int parity, int cus, hipStream_t st)
 {
-	if (n2 > 0) {
+	if (n2 > 0 && model == MMSPX_AC && t.cells > 0) {
+		hipLaunchKernelGGL((ac_batch_cells<PREC>), dim3(n2), dim3(256), 0, st, mem, l2, parity);
+	} else if (n2 > 0) {
 		const int blocks = std::max(1, std::min(n2, cus * t.bpc));
 		if (t.tz == 64)
 			launch_strips<PREC, 64>(model, fields, mem, l2, n2, parity, blocks, st);
@@ -219,7 +256,7 @@ struct mmspx_batch {
 	BMember* d_mem = nullptr;
 	BItem* d_items = nullptr;
 	int n2 = 0, n3 = 0, items_cap = 0;
-	BatchTile tile{256, 32, 16, 8};
+	BatchTile tile{256, 32, 16, 8, 0};
 	bool tuned = false;
 	int cus = 256;
 	int64_t steps_done = 0, launches = 0;
@@ -496,12 +533,22 @@ int mmspx_batch_reduce(mmspx_batch* b, int m, int what, double* out, int nout)
 	return 0;
 }
 
+int mmspx_batch_set_tile(mmspx_batch* b, int tz, int chunk, int blocks_per_cu, int cells)
+{
+	if (!b) return fail(MMSPX_EINVAL, "null batch");
+	if ((tz != 64 && tz != 128 && tz != 256) || chunk < 1 || blocks_per_cu < 1 || cells < 0 ||
+	    (cells > 0 && b->model != MMSPX_AC))
+		return fail(MMSPX_EINVAL, "strip width 64/128/256, positive rows and workgroups; cell runs for Allen-Cahn only");
+	b->tile = BatchTile{tz, chunk, b->tile.chunk3, blocks_per_cu, cells};
+	return upload_plan(b);
+}
+
 int mmspx_batch_get_tile(const mmspx_batch* b, int* tz, int* chunk, int* blocks_per_cu)
 {
 	if (!b) return fail(MMSPX_EINVAL, "null batch");
-	if (tz) *tz = b->tile.tz;
-	if (chunk) *chunk = b->tile.chunk;
-	if (blocks_per_cu) *blocks_per_cu = b->tile.bpc;
+	if (tz) *tz = b->tile.cells > 0 ? 0 : b->tile.tz;
+	if (chunk) *chunk = b->tile.cells > 0 ? b->tile.cells : b->tile.chunk;
+	if (blocks_per_cu) *blocks_per_cu = b->tile.cells > 0 ? 0 : b->tile.bpc;
 	return b->tuned ? 1 : 0;
 }
 
@@ -522,7 +569,9 @@ int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t r
 	std::vector<BatchTile> cand;
 	for (int tz : {64, 128, 256})
 		for (int chunk : {16, 32, 64, 128})
-			for (int bpc : {4, 8, 16}) cand.push_back(BatchTile{tz, chunk, 16, bpc});
+			for (int bpc : {4, 8, 16}) cand.push_back(BatchTile{tz, chunk, 16, bpc, 0});
+	if (b->model == MMSPX_AC)
+		for (int cells : {256, 512, 1024, 2048, 4096}) cand.push_back(BatchTile{256, 32, 16, 8, cells});
 	std::string rep;
 	double best = 1e30;
 	BatchTile best_t = keep;
@@ -535,8 +584,8 @@ int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t r
 		if (rc == 0) rc = mmspx_batch_step_timed(b, trial_steps, 0, &sec);
 		if (rc) break;
 		char line[128];
-		std::snprintf(line, sizeof line, "tz=%d chunk=%d blocks_per_cu=%d us_per_step=%.2f\n", t.tz, t.chunk, t.bpc,
-		              sec / trial_steps * 1e6);
+		std::snprintf(line, sizeof line, "tz=%d chunk=%d blocks_per_cu=%d cells=%d us_per_step=%.2f\n", t.tz, t.chunk,
+		              t.bpc, t.cells, sec / trial_steps * 1e6);
 		rep += line;
 		if (sec < best) {
 			best = sec;

@@ -27,6 +27,7 @@ struct BItem {
 // Cahn-Hilliard kernel, and the persistent grid size in workgroups per CU.
 struct BatchTile {
 	int tz, chunk, chunk3, bpc;
+	int cells;  // > 0: 2-D Allen-Cahn by runs of `cells` consecutive nodes, one workgroup each
 };
 
 bool batch_model_ok(int model, const Geom& g, std::string* why);

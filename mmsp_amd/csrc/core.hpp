@@ -56,6 +56,7 @@ int model_nparams(int model);
 struct TuneChoice {
 	int variant;  // 1 generic, 2 tiled/fused
 	int ty, tz, chunk;
+	int opt;      // fused CH body: 0 LDS rings, 1 register queues
 };
 
 struct mmspx_state_impl;
@@ -79,10 +80,11 @@ struct mmspx_state {
 	bool tuned = false;      // a tile chosen by mmspx_set_tile / mmspx_autotune
 	mmspx::TuneChoice tile{};
 	int64_t launches = 0;
+	int64_t steps_run = 0;
 	// captured multi-step graph for small grids (launch-bound), keyed by what
 	// its launches baked in
 	hipGraphExec_t graph = nullptr;
-	int graph_steps = 0, graph_model = -1, graph_launches = 0;
+	int graph_steps = 0, graph_model = -1, graph_launches = 0, graph_parity = 0;
 	bool graph_fused = false;
 	mmspx::TuneChoice graph_tile{};
 	mmspx::ModelParams graph_params{};

@@ -26,7 +26,7 @@ __device__ __forceinline__ int xcd_tile(int ntiles)
 	return tile < ntiles ? tile : -1;
 }
 
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, bool RQ>
 __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __restrict__ a,
                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
                                                int chunk_len, int tiles_y, int tiles_z, int ntiles)
@@ -39,7 +39,10 @@ __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __r
 	const int chunk = t2 / tiles_y;
 	const int xb = chunk * chunk_len;
 	const int xe = min(xb + chunk_len, g.n[0]);
-	ch_tile<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
+	if (RQ)
+		ch_tile_rq<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
+	else
+		ch_tile<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
 }
 
 template <int PREC, int TZ>
@@ -92,21 +95,23 @@ bool fused_supported(const Geom& g, int prec, int model)
 // fill the chip with them take narrower tiles and shorter chunks.
 TuneChoice default_choice(const Geom& g, int prec, int model)
 {
-	if (!fused_supported(g, prec, model)) return TuneChoice{1, 0, 0, 0};
+	if (!fused_supported(g, prec, model)) return TuneChoice{1, 0, 0, 0, 0};
 	const int64_t nodes = g.nodes;
-	if (model == MMSPX_AC) return TuneChoice{2, 1, 256, nodes >= (1 << 22) ? 64 : 16};
-	if (model == MMSPX_GG) return TuneChoice{2, 1, prec == MMSPX_F64 ? 64 : 128, 32};
-	if (g.dim == 2) return TuneChoice{2, 1, 256, 32};
-	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16};
-	return TuneChoice{2, 16, 64, 32};
+	// Allen-Cahn and grain growth: the one-thread-per-node kernels beat the
+	// strips at every size swept (512^2 FP64, 2048^2 FP32, 2048^2 x 32 FP32);
+	// the strips stay selectable (mmspx_set_variant 2) and are what batches run.
+	if (model == MMSPX_AC || model == MMSPX_GG) return TuneChoice{1, 0, 0, 0, 0};
+	if (g.dim == 2) return TuneChoice{2, 1, 64, 16, 0};
+	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16, 0};
+	return TuneChoice{2, 16, 64, prec == MMSPX_F64 ? 32 : 16, 0};
 }
 
 bool tile_known(const Geom& g, const TuneChoice& t)
 {
 	if (t.chunk < 2 || t.chunk > 4096) return false;
 	if (g.dim == 3)
-		return (t.ty == 8 && t.tz == 64) || (t.ty == 16 && t.tz == 64) || (t.ty == 4 && t.tz == 64) ||
-		       (t.ty == 8 && t.tz == 32) || (t.ty == 4 && t.tz == 128);
+		return (t.ty == 8 && t.tz == 64) || (t.ty == 16 && t.tz == 64) || (t.ty == 8 && t.tz == 32) ||
+		       (t.ty == 4 && t.tz == 128);
 	if (g.dim == 2) return t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256);
 	return false;
 }
@@ -114,23 +119,24 @@ bool tile_known(const Geom& g, const TuneChoice& t)
 int tile_candidates(const Geom& g, TuneChoice* out, int max)
 {
 	int n = 0;
-	auto add = [&](int ty, int tz, int chunk) {
-		if (n < max) out[n++] = TuneChoice{2, ty, tz, chunk};
+	auto add = [&](int ty, int tz, int chunk, int opt) {
+		if (n < max) out[n++] = TuneChoice{2, ty, tz, chunk, opt};
 	};
 	if (g.dim == 3) {
-		const int tiles[5][2] = {{8, 64}, {16, 64}, {4, 64}, {8, 32}, {4, 128}};
-		const int chunks[3] = {16, 32, 64};
+		const int tiles[4][2] = {{16, 64}, {8, 64}, {8, 32}, {4, 128}};
 		for (auto& t : tiles)
-			for (int c : chunks) add(t[0], t[1], c);
+			for (int c : {16, 32, 64})
+				for (int opt = 0; opt < 2; opt++) add(t[0], t[1], c, opt);
 	} else if (g.dim == 2) {
 		for (int tz : {64, 128, 256})
-			for (int c : {16, 32, 64, 128}) add(1, tz, c);
+			for (int c : {16, 32, 64, 128})
+				for (int opt = 0; opt < 2; opt++) add(1, tz, c, opt);
 	}
 	return n;
 }
 
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
-static int launch_ch(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, bool RQ>
+static int launch_ch_body(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
 	using S = typename Form<PREC>::S;
 	const int nx = g.n[0];
@@ -142,10 +148,18 @@ static int launch_ch(const Geom& g, const ModelParams& p, const void* a, void* b
 	const int ntiles = tiles_y * tiles_z * chunks;
 	// round the grid up to a multiple of 8 so the XCD remap applies; extra ids exit
 	const int blocks = (ntiles + 7) / 8 * 8;
-	hipLaunchKernelGGL((ch_fused<PREC, DW, DIM, TY, TZ, NT>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g, p,
-	                   chunk, tiles_y, tiles_z, ntiles);
+	hipLaunchKernelGGL((ch_fused<PREC, DW, DIM, TY, TZ, NT, RQ>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g,
+	                   p, chunk, tiles_y, tiles_z, ntiles);
 	MMSPX_HIP(hipGetLastError());
 	return 0;
+}
+
+// opt 1 selects the register-queue body
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+static int launch_ch(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk, int opt)
+{
+	if (opt & 1) return launch_ch_body<PREC, DW, DIM, TY, TZ, NT, true>(g, p, a, b, st, chunk);
+	return launch_ch_body<PREC, DW, DIM, TY, TZ, NT, false>(g, p, a, b, st, chunk);
 }
 
 template <int PREC, bool DW>
@@ -154,15 +168,14 @@ static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, vo
 {
 	const int chunk = t.chunk > 0 ? t.chunk : 32;
 	if (g.dim == 3) {
-		if (t.ty == 16 && t.tz == 64) return launch_ch<PREC, DW, 3, 16, 64, 256>(g, p, a, b, st, chunk);
-		if (t.ty == 4 && t.tz == 64) return launch_ch<PREC, DW, 3, 4, 64, 256>(g, p, a, b, st, chunk);
-		if (t.ty == 8 && t.tz == 32) return launch_ch<PREC, DW, 3, 8, 32, 256>(g, p, a, b, st, chunk);
-		if (t.ty == 4 && t.tz == 128) return launch_ch<PREC, DW, 3, 4, 128, 256>(g, p, a, b, st, chunk);
-		return launch_ch<PREC, DW, 3, 8, 64, 256>(g, p, a, b, st, chunk);
+		if (t.ty == 16 && t.tz == 64) return launch_ch<PREC, DW, 3, 16, 64, 256>(g, p, a, b, st, chunk, t.opt);
+		if (t.ty == 8 && t.tz == 32) return launch_ch<PREC, DW, 3, 8, 32, 256>(g, p, a, b, st, chunk, t.opt);
+		if (t.ty == 4 && t.tz == 128) return launch_ch<PREC, DW, 3, 4, 128, 256>(g, p, a, b, st, chunk, t.opt);
+		return launch_ch<PREC, DW, 3, 8, 64, 256>(g, p, a, b, st, chunk, t.opt);
 	}
-	if (t.tz == 64) return launch_ch<PREC, DW, 2, 1, 64, 64>(g, p, a, b, st, chunk);
-	if (t.tz == 128) return launch_ch<PREC, DW, 2, 1, 128, 128>(g, p, a, b, st, chunk);
-	return launch_ch<PREC, DW, 2, 1, 256, 256>(g, p, a, b, st, chunk);
+	if (t.tz == 64) return launch_ch<PREC, DW, 2, 1, 64, 64>(g, p, a, b, st, chunk, t.opt);
+	if (t.tz == 128) return launch_ch<PREC, DW, 2, 1, 128, 128>(g, p, a, b, st, chunk, t.opt);
+	return launch_ch<PREC, DW, 2, 1, 256, 256>(g, p, a, b, st, chunk, t.opt);
 }
 
 template <int PREC, int TZ>

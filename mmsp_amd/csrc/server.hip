@@ -79,7 +79,7 @@ struct mmspx_server {
 	int64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
 	int worker_error = 0;
 	// 64-wide 2-D strips: small 2-D requests would leave wider strips mostly idle
-	BatchTile tile{64, 16, 16, 8};
+	BatchTile tile{64, 16, 16, 8, 0};
 
 	double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };

@@ -16,9 +16,19 @@ __device__ __forceinline__ int wrap(int v, int n)
 }
 
 // ---- Cahn-Hilliard, one (TY x TZ) tile marching over planes [xb, xe) ---------
-// Periodic grids only. phi planes: global -> registers (one plane ahead) ->
-// 4-slot LDS ring with a 2-cell halo; mu: 3-slot LDS ring with a 1-cell halo;
-// c' written one plane behind. DIM 2 uses a single row (TY = 1, no axis-1 halo).
+// Periodic grids only; DIM 2 uses a single row (TY = 1, no axis-1 halo).
+// Two bodies with identical arithmetic (and so identical bits):
+//
+// ch_tile (LDS rings): phi planes go global -> registers (one plane ahead) ->
+// a 4-slot LDS ring with a 2-cell halo; mu goes to a 3-slot LDS ring with a
+// 1-cell halo; c' is written one plane behind. Every neighbour, the axis-0
+// ones included, is read from LDS.
+//
+// ch_tile_rq (register queues): each thread owns fixed points of the mu
+// region and keeps phi at planes P-1, P, P+1 and mu at P-2, P-1, P of those
+// points in registers, so only the in-plane neighbours go through LDS (one
+// phi plane and one mu plane, each double-buffered): fewer LDS operations per
+// cell and half the LDS footprint, hence more workgroups per CU.
 template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
 __device__ __forceinline__ void ch_tile(const typename Form<PREC>::S* __restrict__ a,
                                         typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
@@ -79,6 +89,7 @@ __device__ __forceinline__ void ch_tile(const typename Form<PREC>::S* __restrict
 	load_plane(xb, pre);
 
 	for (int P = xb - 1; P <= xe; P++) {
+		// phi(P+1) is in registers: publish it, then start loading phi(P+2)
 		store_plane((P + 1) & 3, pre);
 		if (P + 2 <= xe + 1) load_plane(P + 2, pre);
 		__syncthreads();
@@ -126,6 +137,163 @@ __device__ __forceinline__ void ch_tile(const typename Form<PREC>::S* __restrict
 				}
 			}
 		}
+	}
+}
+
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+__device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restrict__ a,
+                                           typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                           int y0, int z0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	constexpr int HY = DIM == 3 ? 2 : 0;
+	constexpr int PY = TY + 2 * HY, PZ = TZ + 4;   // phi plane tile (2-cell halo)
+	constexpr int MY = TY + HY, MZ = TZ + 2;       // mu region (1-cell halo): the threads' own points
+	constexpr int PN = PY * PZ, MN = MY * MZ;
+	constexpr int RN = PN - MN;                    // phi tile points outside the mu region
+	constexpr int MREG = (MN + NT - 1) / NT;
+	constexpr int RREG = (RN + NT - 1) / NT;
+	__shared__ S sphi[2][PN];
+	__shared__ S smu[2][MN];
+
+	const int nx = g.n[0];
+	const int ny = DIM == 3 ? g.n[1] : 1;
+	const int nz = DIM == 3 ? g.n[2] : g.n[1];
+	const int64_t plane = (int64_t)ny * nz;
+
+	// own points: mu-region index e = tid + k*NT; (r, c) in the mu region is
+	// phi tile (r + HY/2, c + 1), i.e. node (y0 - HY/2 + r, z0 - 1 + c)
+	int ooff[MREG], oq[MREG];
+	bool oout[MREG];
+#pragma unroll
+	for (int k = 0; k < MREG; k++) {
+		const int e = threadIdx.x + k * NT;
+		ooff[k] = 0;
+		oq[k] = 0;
+		oout[k] = false;
+		if (e < MN) {
+			const int r = e / MZ, c = e - (e / MZ) * MZ;
+			const int y = DIM == 3 ? y0 - HY / 2 + r : 0;
+			const int z = z0 - 1 + c;
+			ooff[k] = (DIM == 3 ? wrap(y, ny) : 0) * nz + wrap(z, nz);
+			oq[k] = (r + HY / 2) * PZ + (c + 1);
+			const bool inner = (DIM == 3 ? (r >= 1 && r <= TY) : true) && c >= 1 && c <= TZ;
+			oout[k] = inner && (DIM == 3 ? y < ny : true) && z < nz;
+		}
+	}
+	// ring points: the phi tile's rows 0 and PY-1 and columns 0 and PZ-1
+	int roff[RREG], rq[RREG];
+#pragma unroll
+	for (int j = 0; j < RREG; j++) {
+		const int t = threadIdx.x + j * NT;
+		roff[j] = 0;
+		rq[j] = 0;
+		if (t < RN) {
+			int pr, pc;
+			if (DIM == 3) {
+				if (t < PZ) {
+					pr = 0;
+					pc = t;
+				} else if (t < 2 * PZ) {
+					pr = PY - 1;
+					pc = t - PZ;
+				} else {
+					const int u = t - 2 * PZ;
+					pr = 1 + u / 2;
+					pc = (u & 1) ? PZ - 1 : 0;
+				}
+			} else {
+				pr = 0;
+				pc = t ? PZ - 1 : 0;
+			}
+			const int y = DIM == 3 ? wrap(y0 - HY + pr, ny) : 0;
+			roff[j] = y * nz + wrap(z0 - 2 + pc, nz);
+			rq[j] = pr * PZ + pc;
+		}
+	}
+	auto load_own = [&](int x, S* v) {
+		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
+#pragma unroll
+		for (int k = 0; k < MREG; k++)
+			if (threadIdx.x + k * NT < MN) v[k] = src[ooff[k]];
+	};
+	auto load_ring = [&](int x, S* v) {
+		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
+#pragma unroll
+		for (int j = 0; j < RREG; j++)
+			if (threadIdx.x + j * NT < RN) v[j] = src[roff[j]];
+	};
+
+	S fm[MREG], fc[MREG], fp[MREG], fn[MREG];  // phi at P-1, P, P+1; P+2 in flight
+	S mm[MREG], mc[MREG], mp[MREG];            // mu at P-2, P-1, P
+	S rc[RREG], rn[RREG];                      // ring phi at P; P+1 in flight
+#pragma unroll
+	for (int k = 0; k < MREG; k++) fn[k] = mm[k] = mc[k] = mp[k] = 0;
+#pragma unroll
+	for (int j = 0; j < RREG; j++) rn[j] = 0;
+	// the previous tile of a persistent loop may still read the planes
+	__syncthreads();
+	load_own(xb - 2, fm);
+	load_own(xb - 1, fc);
+	load_own(xb, fp);
+	load_ring(xb - 1, rc);
+
+	for (int P = xb - 1; P <= xe; P++) {
+		// loads for the next iteration go out first
+		if (P + 2 <= xe + 1) load_own(P + 2, fn);
+		if (P + 1 <= xe) load_ring(P + 1, rn);
+		S* __restrict__ lphi = sphi[P & 1];
+#pragma unroll
+		for (int k = 0; k < MREG; k++)
+			if (threadIdx.x + k * NT < MN) lphi[oq[k]] = fc[k];
+#pragma unroll
+		for (int j = 0; j < RREG; j++)
+			if (threadIdx.x + j * NT < RN) lphi[rq[j]] = rc[j];
+		__syncthreads();
+		// mu(P) at the own points: axis-0 neighbours from the queue, the rest from LDS
+		S* __restrict__ lmu = smu[P & 1];
+#pragma unroll
+		for (int k = 0; k < MREG; k++) {
+			if (threadIdx.x + k * NT < MN) {
+				const int q = oq[k];
+				const S y = fc[k];
+				typename F::L lap = F::lap_zero();
+				F::lap_add(lap, g.w[0], g.wf[0], fp[k], y, fm[k]);
+				if (DIM == 3) F::lap_add(lap, g.w[1], g.wf[1], lphi[q + PZ], y, lphi[q - PZ]);
+				F::lap_add(lap, g.w[DIM - 1], g.wf[DIM - 1], lphi[q + 1], y, lphi[q - 1]);
+				mp[k] = DW ? F::mu_dw(y, lap, p) : F::mu_poly(y, lap, p);
+				lmu[threadIdx.x + k * NT] = mp[k];
+			}
+		}
+		__syncthreads();
+		// c'(P-1) at the own interior points
+		if (P - 1 >= xb) {
+			const S* __restrict__ pmu = smu[(P - 1) & 1];
+			S* __restrict__ dst = b + (int64_t)(P - 1) * plane;
+#pragma unroll
+			for (int k = 0; k < MREG; k++) {
+				if (oout[k]) {
+					const int e = threadIdx.x + k * NT;
+					const S m = mc[k];
+					typename F::L lap = F::lap_zero();
+					F::lap_add(lap, g.w[0], g.wf[0], mp[k], m, mm[k]);
+					if (DIM == 3) F::lap_add(lap, g.w[1], g.wf[1], pmu[e + MZ], m, pmu[e - MZ]);
+					F::lap_add(lap, g.w[DIM - 1], g.wf[DIM - 1], pmu[e + 1], m, pmu[e - 1]);
+					dst[ooff[k]] = F::ch(fm[k], lap, p);
+				}
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < MREG; k++) {
+			fm[k] = fc[k];
+			fc[k] = fp[k];
+			fp[k] = fn[k];
+			mm[k] = mc[k];
+			mc[k] = mp[k];
+		}
+#pragma unroll
+		for (int j = 0; j < RREG; j++) rc[j] = rn[j];
 	}
 }
 

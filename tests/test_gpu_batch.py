@@ -25,8 +25,9 @@ def seeded(shape, seed, lo, hi, q=0, f32=True):
     return a.astype(np.float32) if f32 else a
 
 
+@pytest.mark.parametrize("tile", [None, (128, 16, 4, 0), (256, 32, 8, 1024), (64, 8, 16, 300)])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_batch_allen_cahn_uneven_members(prec):
+def test_batch_allen_cahn_uneven_members(prec, tile):
     shapes = [(64, 96), (33, 47), (128, 128), (16, 200), (300, 20), (7, 5)]
     kappas = [0.5, 1.0, 1.3, 2.0, 0.8, 1.7]
     say("[gpu] batch AC %s members=%s" % (prec, shapes))
@@ -34,6 +35,8 @@ def test_batch_allen_cahn_uneven_members(prec):
     init = [seeded(s, 40 + m, -0.5, 0.5, f32=prec == "f32") for m, s in enumerate(shapes)]
     with mx.Batch(shapes, "ac", precision=prec) as b:
         b.set_params(params)
+        if tile:
+            b.set_tile(*tile)
         for m, a in enumerate(init):
             b.upload(m, a)
         b.step(37, graph_steps=10)  # three graph replays + 7 direct steps
@@ -51,6 +54,7 @@ def test_batch_allen_cahn_boundaries_and_alone_vs_in_mix():
     init = [seeded(s, 60 + m, -1, 1) for m, s in enumerate(shapes)]
     with mx.Batch(shapes, "ac", precision="f32", bcs=bcs) as b:
         b.set_params(params)
+        b.set_tile(256, 32, 8, 512)  # runs of consecutive nodes through the boundary mapping
         for m, a in enumerate(init):
             b.upload(m, a)
         b.step(20, graph_steps=4)
