@@ -1,7 +1,13 @@
 """Benchmark of the explicit update path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ch3d|ac2d|gg2d]
-                    [--no-cpu-baseline] [--no-tune]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload ch3d|ch3d_f32|ac2d|gg2d|batch_ac|serve]
+                    [--no-cpu-baseline] [--tune | --tile TY,TZ,CHUNK[,OPT]]
+                    [--requests N] [--hbm-budget-gb G]
+
+--tune runs the autotuner before timing instead of using the default tile;
+--tile fixes the fused kernel's tile. --requests and --hbm-budget-gb size
+the served stream.
 
 Default workload: 3-D Cahn-Hilliard, grid<3,double> 256^3 periodic,
 mu = 2c(1-c)(1-2c) - kappa*lap(c), dc/dt = M*lap(mu), kappa = M = dx = 1,
@@ -39,6 +45,9 @@ WORKLOADS = {
     "ch3d": dict(model="ch_dw", shape=(256, 256, 256), prec="f64", params=[1.0, 1.0, 0.005], steps=2000,
                  seed=20260102, lo=0.45, hi=0.55, fields=1, bytes_per_cell=16,
                  name="cahn-hilliard-3d", cpu_steps=2),
+    "ch3d_f32": dict(model="ch_dw", shape=(256, 256, 256), prec="f32", params=[1.0, 1.0, 0.005], steps=2000,
+                     seed=20260102, lo=0.45, hi=0.55, fields=1, bytes_per_cell=8,
+                     name="cahn-hilliard-3d-fp32", cpu_steps=2),
     "ac2d": dict(model="ac", shape=(512, 512), prec="f64", params=[1.0, 1.0, 1.0, 1.0, 0.01], steps=1000,
                  seed=20260101, lo=-0.01, hi=0.01, fields=1, bytes_per_cell=16,
                  name="allen-cahn-2d", cpu_steps=100),
@@ -305,7 +314,7 @@ def main():
     ap.add_argument("--hbm-budget-gb", type=int, default=128, help="serve: device memory the server may hold")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="store_true", help="run the autotuner first instead of the tuned default tile")
-    ap.add_argument("--tile", default=None, help="ty,tz,chunk for the fused kernel")
+    ap.add_argument("--tile", default=None, help="ty,tz,chunk[,opt] for the tiled kernel (opt 1: register queues)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path is the per-GPU kernel; "
@@ -332,7 +341,10 @@ def main():
     g.upload(init.astype(g.dtype).reshape(g.host_shape))
     tune_report = ""
     if args.tile:
-        g.set_tile(*[int(v) for v in args.tile.split(",")])
+        v = [int(x) for x in args.tile.split(",")]
+        g.set_tile(*v[:3])
+        if len(v) > 3:
+            g.set_tile_options(bool(v[3]))
     elif args.tune and w["model"] in ("ch_dw", "ch"):
         tune_report = g.autotune(w["model"], w["params"], trial_steps=10)
         log(tune_report)
@@ -377,8 +389,16 @@ def main():
     per_launch = alg_bytes / lps
     achieved = per_launch / mean_launch_s / 1e9
     value = nodes * K / dev_s
-    kernel_tag = "%s_%s" % ("ch_fused" if tile["variant"] == 2 and w["model"] in ("ch", "ch_dw") else w["model"],
-                             args.workload)
+    base = {"ch_dw": "ch", "ch": "ch", "ac": "ac", "gg": "gg"}[w["model"]]
+    if tile["variant"] != 2:
+        kname = base + "_generic"
+    elif base == "ch":
+        kname = "ch_fused"
+    elif base == "gg" and tile["opt"] == 1:
+        kname = "gg_tile2d"
+    else:
+        kname = base + "_tiled"
+    kernel_tag = "%s_%s" % (kname, args.workload)
     out = {
         "metric": "cell-updates/s",
         "value": value,

@@ -349,7 +349,9 @@ int mmspx_step(mmspx_state* s, int model, const double* params, int np, int step
 int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
-	TuneChoice t{2, ty, tz, chunk, s->tuned ? s->tile.opt : 0};
+	// ty > 1 on a 2-D grid names a grain-growth 2-D tile (tile body 1)
+	const int opt = (s->geom.dim == 2 && ty > 1) ? 1 : (s->tuned ? s->tile.opt : 0);
+	TuneChoice t{2, ty, tz, chunk, opt};
 	if (!tile_known(s->geom, t)) return fail(MMSPX_EINVAL, "no compiled tile of that shape for this grid");
 	s->tile = t;
 	s->tuned = true;

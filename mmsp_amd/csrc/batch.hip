@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(TZ) gg_batch(const BMember* __restrict__ mem, 
 	}
 }
 
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, bool RQ>
 __global__ void __launch_bounds__(NT) ch_batch(const BMember* __restrict__ mem, const BItem* __restrict__ items,
                                                int nitems, int parity)
 {
@@ -87,12 +87,17 @@ __global__ void __launch_bounds__(NT) ch_batch(const BMember* __restrict__ mem, 
 		const BItem w = items[it];
 		const BMember& m = mem[w.m];
 		const int src = (parity + m.phase) & 1;
-		ch_tile<PREC, DW, DIM, TY, TZ, NT>((const S*)m.buf[src], (S*)m.buf[src ^ 1], m.g, m.p, w.y0, w.z0, w.xb, w.xe);
+		if (RQ)
+			ch_tile_rq<PREC, DW, DIM, TY, TZ, NT>((const S*)m.buf[src], (S*)m.buf[src ^ 1], m.g, m.p, w.y0, w.z0, w.xb,
+			                                      w.xe);
+		else
+			ch_tile<PREC, DW, DIM, TY, TZ, NT>((const S*)m.buf[src], (S*)m.buf[src ^ 1], m.g, m.p, w.y0, w.z0, w.xb, w.xe);
 	}
 }
 
 // tile shapes of the batched Cahn-Hilliard kernels (fixed; the strips' width
-// and chunk depth are what the tuner varies)
+// and chunk depth are what the tuner varies). 3-D members run the
+// register-queue body, the faster one at 16 x 64 in both precisions.
 constexpr int kCh3TY = 16, kCh3TZ = 64, kCh3NT = 256;
 
 bool batch_model_ok(int model, const Geom& g, std::string* why)
@@ -165,9 +170,9 @@ static void launch_strips(int model, int fields, const BMember* mem, const BItem
 	} else {
 		const bool dw = model == MMSPX_CH_DW;
 		if (dw)
-			hipLaunchKernelGGL((ch_batch<PREC, true, 2, 1, TZ, TZ>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity);
+			hipLaunchKernelGGL((ch_batch<PREC, true, 2, 1, TZ, TZ, false>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity);
 		else
-			hipLaunchKernelGGL((ch_batch<PREC, false, 2, 1, TZ, TZ>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity);
+			hipLaunchKernelGGL((ch_batch<PREC, false, 2, 1, TZ, TZ, false>), dim3(blocks), dim3(TZ), 0, st, mem, items, n, parity);
 	}
 }
 
@@ -175,10 +180,10 @@ template <int PREC>
 static void launch_ch3(bool dw, const BMember* mem, const BItem* items, int n, int parity, int blocks, hipStream_t st)
 {
 	if (dw)
-		hipLaunchKernelGGL((ch_batch<PREC, true, 3, kCh3TY, kCh3TZ, kCh3NT>), dim3(blocks), dim3(kCh3NT), 0, st, mem, items,
+		hipLaunchKernelGGL((ch_batch<PREC, true, 3, kCh3TY, kCh3TZ, kCh3NT, true>), dim3(blocks), dim3(kCh3NT), 0, st, mem, items,
 		                   n, parity);
 	else
-		hipLaunchKernelGGL((ch_batch<PREC, false, 3, kCh3TY, kCh3TZ, kCh3NT>), dim3(blocks), dim3(kCh3NT), 0, st, mem,
+		hipLaunchKernelGGL((ch_batch<PREC, false, 3, kCh3TY, kCh3TZ, kCh3NT, true>), dim3(blocks), dim3(kCh3NT), 0, st, mem,
 		                   items, n, parity);
 }
 

@@ -69,6 +69,17 @@ __global__ void __launch_bounds__(TZ) gg_tiled(const typename Form<PREC>::S* __r
 	gg_strip<PREC, Q, TZ>(a, b, g, p, s * TZ, xb, min(xb + chunk_len, g.n[0]));
 }
 
+template <int PREC, int Q, int TX, int TC, int NT>
+__global__ void __launch_bounds__(NT) gg_tile2d(const typename Form<PREC>::S* __restrict__ a,
+                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p, int tiles_c,
+                                                int ntiles)
+{
+	const int tile = xcd_tile(ntiles);
+	if (tile < 0) return;
+	const int tc = tile % tiles_c, tr = tile / tiles_c;
+	gg_tile<PREC, Q, TX, TC, NT>(a, b, g, p, tr * TX, tc * TC);
+}
+
 static bool all_periodic(const Geom& g)
 {
 	for (int k = 0; k < g.dim; k++)
@@ -97,13 +108,19 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 {
 	if (!fused_supported(g, prec, model)) return TuneChoice{1, 0, 0, 0, 0};
 	const int64_t nodes = g.nodes;
-	// Allen-Cahn and grain growth: the one-thread-per-node kernels beat the
-	// strips at every size swept (512^2 FP64, 2048^2 FP32, 2048^2 x 32 FP32);
-	// the strips stay selectable (mmspx_set_variant 2) and are what batches run.
-	if (model == MMSPX_AC || model == MMSPX_GG) return TuneChoice{1, 0, 0, 0, 0};
+	// Allen-Cahn: the one-thread-per-node kernel beats the strips at every size
+	// swept (512^2 FP64, 2048^2 FP32); the strips stay selectable
+	// (mmspx_set_variant 2) and are what batches run.
+	if (model == MMSPX_AC) return TuneChoice{1, 0, 0, 0, 0};
+	// grain growth with 32 fields (2048^2 FP32): the 8 x 64 two-dimensional
+	// tile takes 330 us per step against 388 us for one thread per node and
+	// 530 us for the strips; other field counts keep the generic kernel
+	if (model == MMSPX_GG) return g.fields == 32 ? TuneChoice{2, 8, 64, 8, 1} : TuneChoice{1, 0, 0, 0, 0};
 	if (g.dim == 2) return TuneChoice{2, 1, 64, 16, 0};
 	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16, 0};
-	return TuneChoice{2, 16, 64, prec == MMSPX_F64 ? 32 : 16, 0};
+	// 256^3: register-queue body, 16 x 64 tiles, 16-plane chunks (FP64 57 us
+	// per step against 64 us for the LDS-ring body's best tile; FP32 38 us)
+	return TuneChoice{2, 16, 64, 16, 1};
 }
 
 bool tile_known(const Geom& g, const TuneChoice& t)
@@ -112,7 +129,9 @@ bool tile_known(const Geom& g, const TuneChoice& t)
 	if (g.dim == 3)
 		return (t.ty == 8 && t.tz == 64) || (t.ty == 16 && t.tz == 64) || (t.ty == 8 && t.tz == 32) ||
 		       (t.ty == 4 && t.tz == 128);
-	if (g.dim == 2) return t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256);
+	if (g.dim == 2)
+		return (t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256)) ||
+		       (t.opt == 1 && (t.ty == 8 || t.ty == 16) && t.tz == 64);
 	return false;
 }
 
@@ -131,6 +150,11 @@ int tile_candidates(const Geom& g, TuneChoice* out, int max)
 		for (int tz : {64, 128, 256})
 			for (int c : {16, 32, 64, 128})
 				for (int opt = 0; opt < 2; opt++) add(1, tz, c, opt);
+		// grain growth: 2-D tiles (the strips above ignore opt for it)
+		if (gg_q_compiled(g.fields) && g.fields > 1) {
+			add(8, 64, 8, 1);
+			add(16, 64, 16, 1);
+		}
 	}
 	return n;
 }
@@ -204,6 +228,33 @@ static int launch_gg_q(const Geom& g, const ModelParams& p, const void* a, void*
 	return 0;
 }
 
+template <int PREC, int Q, int TX>
+static int launch_gg_tile_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st)
+{
+	using S = typename Form<PREC>::S;
+	constexpr int TC = 64, NT = 256;
+	const int tiles_c = (g.n[1] + TC - 1) / TC;
+	const int ntiles = tiles_c * ((g.n[0] + TX - 1) / TX);
+	const int blocks = (ntiles + 7) / 8 * 8;
+	hipLaunchKernelGGL((gg_tile2d<PREC, Q, TX, TC, NT>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g, p,
+	                   tiles_c, ntiles);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+template <int PREC, int TX>
+static int launch_gg_tile(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st)
+{
+	switch (g.fields) {
+	case 2: return launch_gg_tile_q<PREC, 2, TX>(g, p, a, b, st);
+	case 4: return launch_gg_tile_q<PREC, 4, TX>(g, p, a, b, st);
+	case 8: return launch_gg_tile_q<PREC, 8, TX>(g, p, a, b, st);
+	case 16: return launch_gg_tile_q<PREC, 16, TX>(g, p, a, b, st);
+	case 32: return launch_gg_tile_q<PREC, 32, TX>(g, p, a, b, st);
+	}
+	return fail(MMSPX_EUNSUPPORTED, "no tiled grain-growth kernel for this field count");
+}
+
 template <int PREC, int TZ>
 static int launch_gg_tz(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
@@ -229,6 +280,8 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		if (t.tz == 128) return launch_ac<PREC, 128>(g, p, a, b, st, chunk);
 		return launch_ac<PREC, 256>(g, p, a, b, st, chunk);
 	case MMSPX_GG:
+		// opt 1: 2-D tiles of ty (8 or 16) rows x 64 columns; otherwise marching strips
+		if (t.opt & 1) return t.ty == 16 ? launch_gg_tile<PREC, 16>(g, p, a, b, st) : launch_gg_tile<PREC, 8>(g, p, a, b, st);
 		if (t.tz == 64) return launch_gg_tz<PREC, 64>(g, p, a, b, st, chunk);
 		if (t.tz == 256 && PREC != MMSPX_F64) return launch_gg_tz<PREC, 256>(g, p, a, b, st, chunk);
 		return launch_gg_tz<PREC, 128>(g, p, a, b, st, chunk);

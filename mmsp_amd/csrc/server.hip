@@ -1,25 +1,34 @@
 // server.hip -- continuous batching of solves on one GPU.
 //
 // The reference runs one solve per process invocation (MMSP.main.hpp:38-354);
-// here a worker thread owns the device and serves a stream of requests:
-//   * submitted requests wait in a FIFO queue;
-//   * at every round boundary the worker admits queued requests while slots
-//     are free and their device footprint (two state buffers) fits the HBM
-//     budget; a request that could never fit is rejected;
-//   * admitted state stays resident on the device: every resident request
-//     advances in the same launches (one persistent launch per step for the
-//     2-D members, one for the 3-D members, all members' tiles in one work
-//     list), so the work of requests that arrived at different times overlaps;
-//   * a round runs until the next request finishes or `chunk_steps` steps,
-//     whichever comes first; finished requests get their FP64 compensated mass
-//     and their state copied out, and their memory is freed for admission.
+// here the library serves a stream of requests with three threads, each with
+// its own stream:
+//   * submitted requests wait in a FIFO queue, their initial state already in
+//     the server's precision (the submitting thread converts it);
+//   * the worker admits queued requests while slots are free and their device
+//     footprint (two state buffers) fits the HBM budget; a request that could
+//     never fit is rejected;
+//   * the uploader allocates an admitted request's buffers (stream-ordered,
+//     from the server's own pool), copies its state in and takes its FP64
+//     compensated mass; the request is then resident;
+//   * the worker advances all resident requests together in rounds: a round
+//     lasts until the next resident request finishes or `chunk_steps` steps,
+//     and is one persistent launch per step over every member's tiles (2-D
+//     and 3-D members in two launches). It is planned on the host and queued
+//     on the compute stream; up to two rounds are queued ahead, so the device
+//     does not wait for the host between rounds;
+//   * when a request's last round is queued, the downloader waits for that
+//     round, takes the FP64 mass of the final state, copies it out and frees
+//     its buffers; only then is its memory counted free for admission.
+// Uploads and downloads therefore overlap the compute of the other requests.
 // Each member's buffers alternate on the global step parity plus a phase set
-// at admission, so joining mid-stream needs no copy. The tile bodies are the
+// when it joins, so joining mid-stream needs no copy. The tile bodies are the
 // single-grid kernels' (tiles.hpp): a request's result does not depend on what
 // else was resident.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdint>
 #include <deque>
 #include <map>
 #include <mutex>
@@ -30,22 +39,51 @@
 
 namespace mmspx {
 
+constexpr int kSlots = 4;        // work-list buffers cycled by queued rounds
+constexpr int kRoundsAhead = 2;  // rounds queued on the compute stream at most
+
 struct Request {
 	int64_t id = 0;
 	mmspx_desc desc{};
 	Geom g{};
 	ModelParams p{};
 	int steps = 0;
-	int done = 0;
-	int status = 0;  // 0 queued, 1 running, 2 done, 3 rejected, 4 cancelled
+	int done = 0;     // steps of rounds the device has finished
+	int planned = 0;  // steps queued
+	int status = 0;   // 0 queued, 1 running, 2 done, 3 rejected, 4 cancelled
 	bool cancel = false;
 	double t_submit = 0, t_admit = 0, t_done = 0;
 	double mass0 = 0, mass1 = 0;
-	size_t bytes = 0;  // one state buffer
-	char* dev = nullptr;
+	size_t bytes = 0;     // one state buffer (server precision, padded to 256 B)
+	char* dev = nullptr;  // buf[0] at dev, buf[1] at dev + bytes
 	int phase = 0;
-	std::vector<double> init;
-	std::vector<double> result;
+	int64_t gstep_end = 0;           // global step after its last round
+	hipEvent_t ev_last = nullptr;    // recorded after its last round (or when dropped)
+	std::vector<char> init;          // initial state, until uploaded
+	std::vector<char> out;           // final state
+};
+
+struct Slot {
+	BMember* d_mem = nullptr;
+	BItem* d_items = nullptr;
+	BMember* h_mem = nullptr;  // pinned mirrors, copied asynchronously
+	BItem* h_items = nullptr;
+	int mem_cap = 0, items_cap = 0;
+	hipEvent_t done = nullptr;  // the last round that read this slot
+	bool used = false;
+};
+
+struct Round {
+	hipEvent_t ev = nullptr;
+	int steps = 0;
+	std::vector<Request*> members;
+};
+
+// A copy-side job: a request's retirement (mass, copy out, free) or, for a
+// cancelled one, only the free.
+struct IoJob {
+	Request* r;
+	bool drop;
 };
 
 }  // namespace mmspx
@@ -55,162 +93,364 @@ using namespace mmspx;
 struct mmspx_server {
 	mmspx_server_config cfg{};
 	std::mutex mu;
-	std::condition_variable cv_work, cv_done;
+	std::condition_variable cv_work, cv_done, cv_up, cv_down;
 	std::deque<Request*> queue;
 	std::map<int64_t, Request*> all;
-	std::vector<Request*> active;
-	std::thread worker;
+	std::thread worker, uploader, downloader;
 	bool stop = false;
 	int64_t next_id = 1;
 	std::chrono::steady_clock::time_point t0;
-	hipStream_t st = nullptr;
+	hipStream_t st = nullptr;   // compute (worker)
+	hipStream_t ust = nullptr;  // uploads
+	hipStream_t dst = nullptr;  // downloads
+	hipMemPool_t pool = nullptr;
+	double* upart = nullptr;    // reduction partials of the uploader and the downloader
+	double* dpart = nullptr;
 	int device = 0;
 	int cus = 256;
 	size_t in_use = 0;
-	int64_t gstep = 0;
-	BMember* d_mem = nullptr;
-	int mem_cap = 0;
-	BItem* d_items = nullptr;
-	int items_cap = 0;
-	double* part = nullptr;
-	double* part_host = nullptr;
-	void* conv = nullptr;  // staging for double <-> float conversion
-	size_t conv_bytes = 0;
 	int64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
 	int worker_error = 0;
 	// 64-wide 2-D strips: small 2-D requests would leave wider strips mostly idle
 	BatchTile tile{64, 16, 16, 8, 0};
+
+	// hand-offs between the threads (under mu)
+	std::deque<Request*> up_jobs;   // admitted, to upload
+	std::vector<Request*> ready;    // uploaded, to join the next round
+	std::deque<IoJob> down_jobs;
+	int uploading = 0;              // admitted and not yet ready
+
+	// worker-only state
+	std::vector<Request*> resident;
+	std::deque<Round> rounds;
+	int64_t gstep = 0;  // global steps queued so far
+	Slot slot[kSlots];
+	int next_slot = 0;
 
 	double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
 static size_t elem_of(const mmspx_server* sv) { return sv->cfg.precision == MMSPX_F64 ? 8 : 4; }
 
-static int ensure_conv(mmspx_server* sv, size_t bytes)
+static void finish(mmspx_server* sv, Request* r, int status)
 {
-	if (sv->conv_bytes >= bytes) return 0;
-	if (sv->conv) (void)hipFree(sv->conv);
-	sv->conv = nullptr;
-	sv->conv_bytes = 0;
-	if (hipMalloc(&sv->conv, bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed (staging)");
-	sv->conv_bytes = bytes;
+	// caller holds sv->mu
+	r->status = status;
+	r->t_done = sv->now();
+	if (status == 2) sv->stats[1] += 1;
+	if (status == 3) sv->stats[2] += 1;
+	sv->cv_done.notify_all();
+}
+
+// ---- uploader ----------------------------------------------------------------
+
+static int upload_one(mmspx_server* sv, Request* r, std::vector<double>& part_host)
+{
+	MMSPX_HIP(hipMallocFromPoolAsync((void**)&r->dev, 2 * r->bytes, sv->pool, sv->ust));
+	// the data goes to buf[0]; the worker sets the phase that makes it the source
+	MMSPX_HIP(hipMemcpyAsync(r->dev, r->init.data(), r->init.size(), hipMemcpyHostToDevice, sv->ust));
+	int rc = launch_reduce(r->g, sv->cfg.precision, MMSPX_MASS, r->p, r->dev, sv->upart, sv->ust);
+	if (rc) return rc;
+	MMSPX_HIP(hipMemcpyAsync(part_host.data(), sv->upart, part_host.size() * sizeof(double), hipMemcpyDeviceToHost,
+	                         sv->ust));
+	MMSPX_HIP(hipStreamSynchronize(sv->ust));
+	r->mass0 = sum_partials(part_host.data(), kRedBlocks);
 	return 0;
 }
 
-static double member_mass(mmspx_server* sv, Request* r, const void* state)
+static void uploader_main(mmspx_server* sv)
 {
-	if (launch_reduce(r->g, sv->cfg.precision, MMSPX_MASS, r->p, state, sv->part, sv->st)) return 0.0;
-	if (hipMemcpyAsync(sv->part_host, sv->part, 2 * kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, sv->st) !=
-	        hipSuccess ||
-	    hipStreamSynchronize(sv->st) != hipSuccess)
-		return 0.0;
-	return sum_partials(sv->part_host, kRedBlocks);
-}
-
-// Device side of admission (worker thread, lock not held).
-static int admit_upload(mmspx_server* sv, Request* r)
-{
-	if (hipMalloc((void**)&r->dev, 2 * r->bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");
-	const size_t n = (size_t)r->g.nodes;
-	int rc = ensure_conv(sv, n * sizeof(double));
-	if (rc) return rc;
-	MMSPX_HIP(hipMemcpyAsync(sv->conv, r->init.data(), n * sizeof(double), hipMemcpyHostToDevice, sv->st));
-	// data goes to buf[0]; phase makes buf[(gstep + phase) & 1] == buf[0]
-	rc = launch_convert(sv->conv, 1, r->dev, sv->cfg.precision == MMSPX_F64, (int64_t)n, sv->st);
-	if (rc) return rc;
-	r->phase = (int)(sv->gstep & 1);
-	r->mass0 = member_mass(sv, r, r->dev);
-	return 0;
-}
-
-static void* cur_state(const mmspx_server* sv, const Request* r)
-{
-	return r->dev + (((sv->gstep + r->phase) & 1) ? r->bytes : 0);
-}
-
-static int retire_download(mmspx_server* sv, Request* r)
-{
-	void* s = cur_state(sv, r);
-	r->mass1 = member_mass(sv, r, s);
-	const size_t n = (size_t)r->g.nodes;
-	int rc = ensure_conv(sv, n * sizeof(double));
-	if (rc) return rc;
-	rc = launch_convert(s, sv->cfg.precision == MMSPX_F64, sv->conv, 1, (int64_t)n, sv->st);
-	if (rc) return rc;
-	r->result.resize(n);
-	MMSPX_HIP(hipMemcpyAsync(r->result.data(), sv->conv, n * sizeof(double), hipMemcpyDeviceToHost, sv->st));
-	MMSPX_HIP(hipStreamSynchronize(sv->st));
-	return 0;
-}
-
-static void free_member(mmspx_server* sv, Request* r)
-{
-	if (r->dev) (void)hipFree(r->dev);
-	r->dev = nullptr;
-	std::lock_guard<std::mutex> lk(sv->mu);
-	sv->in_use -= 2 * r->bytes;
-}
-
-static int run_round(mmspx_server* sv)
-{
-	std::vector<Request*> act;
-	{
+	(void)hipSetDevice(sv->device);
+	std::vector<double> part_host(2 * kRedBlocks);
+	for (;;) {
+		Request* r;
+		{
+			std::unique_lock<std::mutex> lk(sv->mu);
+			sv->cv_up.wait(lk, [&] { return sv->stop || !sv->up_jobs.empty(); });
+			if (sv->stop) return;
+			r = sv->up_jobs.front();
+			sv->up_jobs.pop_front();
+		}
+		const int rc = upload_one(sv, r, part_host);
 		std::lock_guard<std::mutex> lk(sv->mu);
-		act = sv->active;
+		sv->uploading -= 1;
+		if (rc) {
+			if (!sv->worker_error) sv->worker_error = rc;
+			if (r->dev) (void)hipFreeAsync(r->dev, sv->ust);
+			r->dev = nullptr;
+			sv->in_use -= 2 * r->bytes;
+			finish(sv, r, 3);
+		} else {
+			r->init.clear();
+			r->init.shrink_to_fit();
+			r->status = 1;
+			r->t_admit = sv->now();
+			sv->stats[0] += 1;
+			sv->ready.push_back(r);
+		}
+		sv->cv_work.notify_all();
 	}
+}
+
+// ---- downloader --------------------------------------------------------------
+
+static int download_one(mmspx_server* sv, Request* r, std::vector<double>& part_host)
+{
+	MMSPX_HIP(hipStreamWaitEvent(sv->dst, r->ev_last, 0));
+	const void* s = r->dev + (((r->gstep_end + r->phase) & 1) ? r->bytes : 0);
+	int rc = launch_reduce(r->g, sv->cfg.precision, MMSPX_MASS, r->p, s, sv->dpart, sv->dst);
+	if (rc) return rc;
+	MMSPX_HIP(hipMemcpyAsync(part_host.data(), sv->dpart, part_host.size() * sizeof(double), hipMemcpyDeviceToHost,
+	                         sv->dst));
+	r->out.resize((size_t)r->g.nodes * elem_of(sv));
+	MMSPX_HIP(hipMemcpyAsync(r->out.data(), s, r->out.size(), hipMemcpyDeviceToHost, sv->dst));
+	MMSPX_HIP(hipStreamSynchronize(sv->dst));
+	r->mass1 = sum_partials(part_host.data(), kRedBlocks);
+	return 0;
+}
+
+static void downloader_main(mmspx_server* sv)
+{
+	(void)hipSetDevice(sv->device);
+	std::vector<double> part_host(2 * kRedBlocks);
+	for (;;) {
+		IoJob job;
+		{
+			std::unique_lock<std::mutex> lk(sv->mu);
+			sv->cv_down.wait(lk, [&] { return sv->stop || !sv->down_jobs.empty(); });
+			if (sv->stop) return;
+			job = sv->down_jobs.front();
+			sv->down_jobs.pop_front();
+		}
+		Request* r = job.r;
+		int rc = 0;
+		if (job.drop)
+			rc = hipStreamWaitEvent(sv->dst, r->ev_last, 0) == hipSuccess ? 0 : fail(MMSPX_EDEVICE, "hipStreamWaitEvent failed");
+		else
+			rc = download_one(sv, r, part_host);
+		// after the wait above, the free follows every launch that used the buffers
+		if (r->dev && hipFreeAsync(r->dev, sv->dst) != hipSuccess && !rc) rc = fail(MMSPX_EDEVICE, "hipFreeAsync failed");
+		(void)hipEventDestroy(r->ev_last);
+		std::lock_guard<std::mutex> lk(sv->mu);
+		r->dev = nullptr;
+		r->ev_last = nullptr;
+		if (!job.drop) r->done = r->steps;
+		sv->in_use -= 2 * r->bytes;
+		if (rc && !sv->worker_error) sv->worker_error = rc;
+		finish(sv, r, job.drop ? 4 : (rc ? 3 : 2));
+		sv->cv_work.notify_all();
+	}
+}
+
+// ---- worker ------------------------------------------------------------------
+
+static int ensure_slot(Slot& sl, int nmem, int nitems)
+{
+	if (nmem > sl.mem_cap) {
+		const int cap = std::max(nmem, 2 * sl.mem_cap + 16);
+		if (sl.d_mem) (void)hipFree(sl.d_mem);
+		if (sl.h_mem) (void)hipHostFree(sl.h_mem);
+		sl.d_mem = nullptr;
+		sl.h_mem = nullptr;
+		sl.mem_cap = 0;
+		if (hipMalloc((void**)&sl.d_mem, sizeof(BMember) * cap) != hipSuccess ||
+		    hipHostMalloc((void**)&sl.h_mem, sizeof(BMember) * cap, 0) != hipSuccess)
+			return fail(MMSPX_ENOMEM, "allocation failed (member table)");
+		sl.mem_cap = cap;
+	}
+	if (nitems > sl.items_cap) {
+		const int cap = std::max(nitems, 2 * sl.items_cap + 1024);
+		if (sl.d_items) (void)hipFree(sl.d_items);
+		if (sl.h_items) (void)hipHostFree(sl.h_items);
+		sl.d_items = nullptr;
+		sl.h_items = nullptr;
+		sl.items_cap = 0;
+		if (hipMalloc((void**)&sl.d_items, sizeof(BItem) * cap) != hipSuccess ||
+		    hipHostMalloc((void**)&sl.h_items, sizeof(BItem) * cap, 0) != hipSuccess)
+			return fail(MMSPX_ENOMEM, "allocation failed (work list)");
+		sl.items_cap = cap;
+	}
+	return 0;
+}
+
+// Plans one round over the resident members and queues it on the compute stream;
+// members whose last step it holds are handed to the downloader.
+static int enqueue_round(mmspx_server* sv)
+{
+	std::vector<Request*>& act = sv->resident;
 	int steps = sv->cfg.chunk_steps;
-	for (Request* r : act) steps = std::min(steps, r->steps - r->done);
-	if (steps <= 0) steps = 0;
-	std::vector<BMember> mem(act.size());
+	for (Request* r : act) steps = std::min(steps, r->steps - r->planned);
+	steps = std::max(steps, 0);
+	Slot& sl = sv->slot[sv->next_slot];
+	sv->next_slot = (sv->next_slot + 1) % kSlots;
+	// the pinned mirrors are rewritten below: the copy that read them last must be done
+	if (sl.used) MMSPX_HIP(hipEventSynchronize(sl.done));
 	std::vector<BItem> l2, l3;
+	for (size_t m = 0; m < act.size(); m++) member_items((int)m, act[m]->g, sv->cfg.model, sv->tile, l2, l3);
+	const int n2 = (int)l2.size(), n3 = (int)l3.size();
+	int rc = ensure_slot(sl, (int)act.size(), n2 + n3);
+	if (rc) return rc;
 	for (size_t m = 0; m < act.size(); m++) {
-		Request* r = act[m];
-		mem[m].buf[0] = r->dev;
-		mem[m].buf[1] = r->dev + r->bytes;
-		mem[m].g = r->g;
-		mem[m].p = r->p;
-		mem[m].phase = r->phase;
-		member_items((int)m, r->g, sv->cfg.model, sv->tile, l2, l3);
+		const Request* r = act[m];
+		BMember& bm = sl.h_mem[m];
+		bm.buf[0] = r->dev;
+		bm.buf[1] = r->dev + r->bytes;
+		bm.g = r->g;
+		bm.p = r->p;
+		bm.phase = r->phase;
 	}
-	if ((int)mem.size() > sv->mem_cap) {
-		if (sv->d_mem) (void)hipFree(sv->d_mem);
-		sv->d_mem = nullptr;
-		sv->mem_cap = 0;
-		if (hipMalloc((void**)&sv->d_mem, sizeof(BMember) * mem.size()) != hipSuccess)
-			return fail(MMSPX_ENOMEM, "device allocation failed (member table)");
-		sv->mem_cap = (int)mem.size();
-	}
-	const int nit = (int)(l2.size() + l3.size());
-	if (nit > sv->items_cap) {
-		if (sv->d_items) (void)hipFree(sv->d_items);
-		sv->d_items = nullptr;
-		sv->items_cap = 0;
-		if (hipMalloc((void**)&sv->d_items, sizeof(BItem) * (size_t)nit) != hipSuccess)
-			return fail(MMSPX_ENOMEM, "device allocation failed (work list)");
-		sv->items_cap = nit;
-	}
-	const int n2 = nit - (int)l3.size(), n3 = (int)l3.size();
-	l2 = xcd_order(l2);
-	{
-		const std::vector<BItem> o3 = xcd_order(l3);
-		l2.insert(l2.end(), o3.begin(), o3.end());
-	}
-	if (!mem.empty()) {
-		MMSPX_HIP(hipMemcpyAsync(sv->d_mem, mem.data(), sizeof(BMember) * mem.size(), hipMemcpyHostToDevice, sv->st));
-		MMSPX_HIP(hipMemcpyAsync(sv->d_items, l2.data(), sizeof(BItem) * l2.size(), hipMemcpyHostToDevice, sv->st));
-	}
+	const std::vector<BItem> o2 = xcd_order(l2), o3 = xcd_order(l3);
+	std::copy(o2.begin(), o2.end(), sl.h_items);
+	std::copy(o3.begin(), o3.end(), sl.h_items + n2);
+	MMSPX_HIP(hipMemcpyAsync(sl.d_mem, sl.h_mem, sizeof(BMember) * act.size(), hipMemcpyHostToDevice, sv->st));
+	MMSPX_HIP(hipMemcpyAsync(sl.d_items, sl.h_items, sizeof(BItem) * (size_t)(n2 + n3), hipMemcpyHostToDevice, sv->st));
 	int64_t launches = 0;
-	for (int k = 0; k < steps && !act.empty(); k++) {
-		int rc = launch_batch_step(sv->cfg.model, sv->cfg.precision, 1, sv->tile, sv->d_mem, sv->d_items, n2,
-		                           sv->d_items + n2, n3, (int)((sv->gstep + k) & 1), sv->cus, sv->st, &launches);
+	for (int k = 0; k < steps; k++) {
+		rc = launch_batch_step(sv->cfg.model, sv->cfg.precision, 1, sv->tile, sl.d_mem, sl.d_items, n2,
+		                       sl.d_items + n2, n3, (int)((sv->gstep + k) & 1), sv->cus, sv->st, &launches);
 		if (rc) return rc;
 	}
-	MMSPX_HIP(hipStreamSynchronize(sv->st));
+	Round rd;
+	rd.steps = steps;
+	MMSPX_HIP(hipEventCreateWithFlags(&rd.ev, hipEventDisableTiming));
+	MMSPX_HIP(hipEventRecord(rd.ev, sv->st));
+	MMSPX_HIP(hipEventRecord(sl.done, sv->st));
+	sl.used = true;
 	sv->gstep += steps;
+	std::vector<Request*> keep;
+	std::vector<IoJob> fin;
+	for (Request* r : act) {
+		r->planned += steps;
+		if (r->planned < r->steps) {
+			keep.push_back(r);
+			continue;
+		}
+		r->gstep_end = sv->gstep;
+		MMSPX_HIP(hipEventCreateWithFlags(&r->ev_last, hipEventDisableTiming));
+		MMSPX_HIP(hipEventRecord(r->ev_last, sv->st));
+		fin.push_back({r, false});
+	}
+	// a finishing member may be fetched and deleted once downloaded: the round
+	// keeps only the members that stay
+	rd.members = keep;
+	act.swap(keep);
+	sv->rounds.push_back(std::move(rd));
 	std::lock_guard<std::mutex> lk(sv->mu);
-	for (Request* r : act) r->done += steps;
 	sv->stats[3] += 1;
 	sv->stats[6] += launches;
+	for (const IoJob& j : fin) sv->down_jobs.push_back(j);
+	if (!fin.empty()) sv->cv_down.notify_all();
+	return 0;
+}
+
+// Retires finished rounds from the front of the queue (the oldest one is waited for if `block`).
+static int reap_rounds(mmspx_server* sv, bool block)
+{
+	while (!sv->rounds.empty()) {
+		Round& rd = sv->rounds.front();
+		if (block) {
+			MMSPX_HIP(hipEventSynchronize(rd.ev));
+			block = false;
+		} else {
+			const hipError_t e = hipEventQuery(rd.ev);
+			if (e == hipErrorNotReady) break;
+			if (e != hipSuccess) return fail(MMSPX_EDEVICE, std::string("round: ") + hipGetErrorString(e));
+		}
+		{
+			std::lock_guard<std::mutex> lk(sv->mu);
+			for (Request* r : rd.members) r->done += rd.steps;
+		}
+		(void)hipEventDestroy(rd.ev);
+		sv->rounds.pop_front();
+	}
+	return 0;
+}
+
+// Cancelled members leave before the next round; their buffers are freed behind the rounds queued so far.
+static int drop_cancelled(mmspx_server* sv)
+{
+	std::vector<Request*> keep;
+	std::vector<IoJob> drop;
+	{
+		std::lock_guard<std::mutex> lk(sv->mu);
+		for (Request* r : sv->resident) {
+			if (r->cancel) drop.push_back({r, true});
+			else keep.push_back(r);
+		}
+	}
+	for (IoJob& j : drop) {
+		MMSPX_HIP(hipEventCreateWithFlags(&j.r->ev_last, hipEventDisableTiming));
+		MMSPX_HIP(hipEventRecord(j.r->ev_last, sv->st));
+	}
+	sv->resident.swap(keep);
+	if (!drop.empty()) {
+		std::lock_guard<std::mutex> lk(sv->mu);
+		for (const IoJob& j : drop) sv->down_jobs.push_back(j);
+		sv->cv_down.notify_all();
+	}
+	return 0;
+}
+
+// One pass of the worker: admit, take in uploaded requests, queue or retire a round.
+// Returns 1 when the server stops, a negative code on a device error.
+static int worker_pass(mmspx_server* sv)
+{
+	{
+		std::unique_lock<std::mutex> lk(sv->mu);
+		const bool idle = sv->resident.empty() && sv->rounds.empty() && sv->ready.empty();
+		if (idle)
+			sv->cv_work.wait_for(lk, std::chrono::milliseconds(20),
+			                     [&] { return sv->stop || !sv->queue.empty() || !sv->ready.empty(); });
+		if (sv->stop) return 1;
+		// cancelled requests leave the queue wherever they stand
+		for (auto it = sv->queue.begin(); it != sv->queue.end();) {
+			if ((*it)->cancel) {
+				finish(sv, *it, 4);
+				it = sv->queue.erase(it);
+			} else {
+				++it;
+			}
+		}
+		// admission: FIFO while slots and memory allow
+		int live = (int)(sv->resident.size() + sv->ready.size()) + sv->uploading;
+		bool admitted = false;
+		while (!sv->queue.empty() && live < sv->cfg.max_slots) {
+			Request* r = sv->queue.front();
+			const size_t need = 2 * r->bytes;
+			if (need > sv->cfg.hbm_budget) {
+				sv->queue.pop_front();
+				finish(sv, r, 3);
+				continue;
+			}
+			if (sv->in_use + need > sv->cfg.hbm_budget) break;
+			sv->queue.pop_front();
+			sv->in_use += need;
+			sv->stats[4] = std::max<int64_t>(sv->stats[4], (int64_t)sv->in_use);
+			sv->uploading += 1;
+			sv->up_jobs.push_back(r);
+			live += 1;
+			admitted = true;
+		}
+		if (admitted) sv->cv_up.notify_all();
+		// uploaded requests join the next round with their data in buf[0]
+		for (Request* r : sv->ready) {
+			r->phase = (int)(sv->gstep & 1);
+			sv->resident.push_back(r);
+		}
+		sv->ready.clear();
+	}
+	int rc = reap_rounds(sv, false);
+	if (rc) return rc;
+	rc = drop_cancelled(sv);
+	if (rc) return rc;
+	if (!sv->resident.empty() && (int)sv->rounds.size() < kRoundsAhead) {
+		{
+			std::lock_guard<std::mutex> lk(sv->mu);
+			sv->stats[5] = std::max<int64_t>(sv->stats[5], (int64_t)sv->resident.size());
+		}
+		return enqueue_round(sv);
+	}
+	if (!sv->rounds.empty()) return reap_rounds(sv, true);
 	return 0;
 }
 
@@ -218,91 +458,20 @@ static void worker_main(mmspx_server* sv)
 {
 	(void)hipSetDevice(sv->device);
 	for (;;) {
-		std::vector<Request*> admit;
-		{
-			std::unique_lock<std::mutex> lk(sv->mu);
-			if (sv->active.empty())
-				sv->cv_work.wait_for(lk, std::chrono::milliseconds(50), [&] { return sv->stop || !sv->queue.empty(); });
-			if (sv->stop) break;
-			// admission: FIFO while slots and memory allow
-			while (!sv->queue.empty() && (int)(sv->active.size() + admit.size()) < sv->cfg.max_slots) {
-				Request* r = sv->queue.front();
-				if (r->cancel) {
-					sv->queue.pop_front();
-					r->status = 4;
-					r->t_done = sv->now();
-					sv->cv_done.notify_all();
-					continue;
-				}
-				const size_t need = 2 * r->bytes;
-				if (need > sv->cfg.hbm_budget) {
-					sv->queue.pop_front();
-					r->status = 3;
-					r->t_done = sv->now();
-					sv->stats[2] += 1;
-					sv->cv_done.notify_all();
-					continue;
-				}
-				if (sv->in_use + need > sv->cfg.hbm_budget) break;
-				sv->queue.pop_front();
-				sv->in_use += need;
-				sv->stats[4] = std::max<int64_t>(sv->stats[4], (int64_t)sv->in_use);
-				admit.push_back(r);
-			}
-		}
-		for (Request* r : admit) {
-			int rc = admit_upload(sv, r);
-			std::lock_guard<std::mutex> lk(sv->mu);
-			if (rc) {
-				sv->worker_error = rc;
-				r->status = 3;
-				r->t_done = sv->now();
-				sv->stats[2] += 1;
-				sv->cv_done.notify_all();
-				if (r->dev) (void)hipFree(r->dev);
-				r->dev = nullptr;
-				sv->in_use -= 2 * r->bytes;
-				continue;
-			}
-			r->status = 1;
-			r->t_admit = sv->now();
-			r->init.clear();
-			r->init.shrink_to_fit();
-			sv->active.push_back(r);
-			sv->stats[0] += 1;
-			sv->stats[5] = std::max<int64_t>(sv->stats[5], (int64_t)sv->active.size());
-		}
-		bool any;
-		{
-			std::lock_guard<std::mutex> lk(sv->mu);
-			any = !sv->active.empty();
-		}
-		if (!any) continue;
-		int rc = run_round(sv);
-		if (rc) {
-			std::lock_guard<std::mutex> lk(sv->mu);
-			sv->worker_error = rc;
-		}
-		// retire finished and cancelled members
-		std::vector<Request*> fin;
-		{
-			std::lock_guard<std::mutex> lk(sv->mu);
-			std::vector<Request*> keep;
-			for (Request* r : sv->active) {
-				if (r->done >= r->steps || r->cancel || rc) fin.push_back(r);
-				else keep.push_back(r);
-			}
-			sv->active.swap(keep);
-		}
-		for (Request* r : fin) {
-			int rc2 = (r->cancel || rc) ? 0 : retire_download(sv, r);
-			free_member(sv, r);
-			std::lock_guard<std::mutex> lk(sv->mu);
-			r->t_done = sv->now();
-			r->status = (r->cancel || rc || rc2) ? (r->cancel ? 4 : 3) : 2;
-			if (r->status == 2) sv->stats[1] += 1;
-			sv->cv_done.notify_all();
-		}
+		const int rc = worker_pass(sv);
+		if (rc == 0) continue;
+		(void)hipStreamSynchronize(sv->st);
+		if (rc == 1) break;
+		// a device error: fail every request not yet finished and stop serving
+		std::lock_guard<std::mutex> lk(sv->mu);
+		if (!sv->worker_error) sv->worker_error = rc;
+		for (auto& kv : sv->all)
+			if (kv.second->status < 2) finish(sv, kv.second, 3);
+		sv->queue.clear();
+		sv->stop = true;
+		sv->cv_up.notify_all();
+		sv->cv_down.notify_all();
+		break;
 	}
 }
 
@@ -323,19 +492,43 @@ int mmspx_server_create(const mmspx_server_config* cfg, mmspx_server** out)
 	sv->cfg = *cfg;
 	sv->t0 = std::chrono::steady_clock::now();
 	int rc = 0;
-	if (hipGetDevice(&sv->device) != hipSuccess || hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking) != hipSuccess)
+	if (hipGetDevice(&sv->device) != hipSuccess || hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking) != hipSuccess ||
+	    hipStreamCreateWithFlags(&sv->ust, hipStreamNonBlocking) != hipSuccess ||
+	    hipStreamCreateWithFlags(&sv->dst, hipStreamNonBlocking) != hipSuccess)
 		rc = fail(MMSPX_EDEVICE, "HIP stream setup failed");
-	if (rc == 0 && (hipMalloc((void**)&sv->part, 2 * kRedBlocks * sizeof(double)) != hipSuccess ||
-	                hipHostMalloc((void**)&sv->part_host, 2 * kRedBlocks * sizeof(double), 0) != hipSuccess))
+	for (int k = 0; k < kSlots && rc == 0; k++)
+		if (hipEventCreateWithFlags(&sv->slot[k].done, hipEventDisableTiming) != hipSuccess)
+			rc = fail(MMSPX_EDEVICE, "HIP event setup failed");
+	if (rc == 0) {
+		hipMemPoolProps props{};
+		props.allocType = hipMemAllocationTypePinned;
+		props.handleTypes = hipMemHandleTypeNone;
+		props.location.type = hipMemLocationTypeDevice;
+		props.location.id = sv->device;
+		// keep freed blocks reserved (up to the budget) for the next admissions
+		uint64_t keep = cfg->hbm_budget;
+		if (hipMemPoolCreate(&sv->pool, &props) != hipSuccess ||
+		    hipMemPoolSetAttribute(sv->pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess)
+			rc = fail(MMSPX_EDEVICE, "HIP memory pool setup failed");
+	}
+	if (rc == 0 && (hipMalloc((void**)&sv->upart, 2 * kRedBlocks * sizeof(double)) != hipSuccess ||
+	                hipMalloc((void**)&sv->dpart, 2 * kRedBlocks * sizeof(double)) != hipSuccess))
 		rc = fail(MMSPX_ENOMEM, "device allocation failed");
 	if (rc) {
-		if (sv->part) (void)hipFree(sv->part);
-		if (sv->part_host) (void)hipHostFree(sv->part_host);
+		for (int k = 0; k < kSlots; k++)
+			if (sv->slot[k].done) (void)hipEventDestroy(sv->slot[k].done);
+		if (sv->upart) (void)hipFree(sv->upart);
+		if (sv->dpart) (void)hipFree(sv->dpart);
+		if (sv->pool) (void)hipMemPoolDestroy(sv->pool);
 		if (sv->st) (void)hipStreamDestroy(sv->st);
+		if (sv->ust) (void)hipStreamDestroy(sv->ust);
+		if (sv->dst) (void)hipStreamDestroy(sv->dst);
 		delete sv;
 		return rc;
 	}
 	sv->cus = device_cus();
+	sv->uploader = std::thread(uploader_main, sv);
+	sv->downloader = std::thread(downloader_main, sv);
 	sv->worker = std::thread(worker_main, sv);
 	*out = sv;
 	return 0;
@@ -349,18 +542,36 @@ void mmspx_server_destroy(mmspx_server* sv)
 		sv->stop = true;
 	}
 	sv->cv_work.notify_all();
+	sv->cv_up.notify_all();
+	sv->cv_down.notify_all();
 	if (sv->worker.joinable()) sv->worker.join();
+	if (sv->uploader.joinable()) sv->uploader.join();
+	if (sv->downloader.joinable()) sv->downloader.join();
 	(void)hipStreamSynchronize(sv->st);
+	(void)hipStreamSynchronize(sv->ust);
+	(void)hipStreamSynchronize(sv->dst);
+	for (auto& rd : sv->rounds)
+		if (rd.ev) (void)hipEventDestroy(rd.ev);
 	for (auto& kv : sv->all) {
-		if (kv.second->dev) (void)hipFree(kv.second->dev);
-		delete kv.second;
+		Request* r = kv.second;
+		if (r->dev) (void)hipFreeAsync(r->dev, sv->dst);
+		if (r->ev_last) (void)hipEventDestroy(r->ev_last);
+		delete r;
 	}
-	if (sv->d_mem) (void)hipFree(sv->d_mem);
-	if (sv->d_items) (void)hipFree(sv->d_items);
-	if (sv->part) (void)hipFree(sv->part);
-	if (sv->part_host) (void)hipHostFree(sv->part_host);
-	if (sv->conv) (void)hipFree(sv->conv);
-	if (sv->st) (void)hipStreamDestroy(sv->st);
+	(void)hipStreamSynchronize(sv->dst);
+	for (auto& sl : sv->slot) {
+		if (sl.d_mem) (void)hipFree(sl.d_mem);
+		if (sl.d_items) (void)hipFree(sl.d_items);
+		if (sl.h_mem) (void)hipHostFree(sl.h_mem);
+		if (sl.h_items) (void)hipHostFree(sl.h_items);
+		if (sl.done) (void)hipEventDestroy(sl.done);
+	}
+	(void)hipFree(sv->upart);
+	(void)hipFree(sv->dpart);
+	(void)hipMemPoolDestroy(sv->pool);
+	(void)hipStreamDestroy(sv->st);
+	(void)hipStreamDestroy(sv->ust);
+	(void)hipStreamDestroy(sv->dst);
 	delete sv;
 }
 
@@ -387,9 +598,20 @@ int mmspx_server_submit(mmspx_server* sv, const mmspx_desc* desc, const double* 
 		return rc;
 	}
 	r->steps = steps;
-	r->bytes = ((size_t)r->g.nodes * elem_of(sv) + 255) / 256 * 256;
-	r->init.assign(init, init + r->g.nodes);
+	const size_t n = (size_t)r->g.nodes;
+	r->bytes = (n * elem_of(sv) + 255) / 256 * 256;
+	r->init.resize(n * elem_of(sv));
+	if (sv->cfg.precision == MMSPX_F64) {
+		std::copy(init, init + n, (double*)r->init.data());
+	} else {
+		float* f = (float*)r->init.data();
+		for (size_t i = 0; i < n; i++) f[i] = (float)init[i];
+	}
 	std::lock_guard<std::mutex> lk(sv->mu);
+	if (sv->stop) {
+		delete r;
+		return fail(MMSPX_EBUSY, "the server has stopped");
+	}
 	r->id = sv->next_id++;
 	r->t_submit = sv->now();
 	sv->all[r->id] = r;
@@ -403,7 +625,7 @@ static void fill_info(const Request* r, mmspx_request_info* info)
 {
 	info->id = r->id;
 	info->status = r->status;
-	info->steps_done = r->done;
+	info->steps_done = r->status == 2 ? r->steps : r->done;
 	info->t_submit = r->t_submit;
 	info->t_admit = r->t_admit;
 	info->t_done = r->t_done;
@@ -451,14 +673,24 @@ int mmspx_server_info(mmspx_server* sv, int64_t id, mmspx_request_info* info)
 int mmspx_server_fetch(mmspx_server* sv, int64_t id, double* out, size_t count)
 {
 	if (!sv || !out) return fail(MMSPX_EINVAL, "null argument");
-	std::lock_guard<std::mutex> lk(sv->mu);
-	auto it = sv->all.find(id);
-	if (it == sv->all.end()) return fail(MMSPX_ENOTFOUND, "unknown request id");
-	Request* r = it->second;
-	if (r->status != 2) return fail(MMSPX_EBUSY, "request has not completed");
-	if (count != r->result.size()) return fail(MMSPX_EINVAL, "output size does not match the request's grid");
-	std::copy(r->result.begin(), r->result.end(), out);
-	sv->all.erase(it);
+	Request* r = nullptr;
+	{
+		std::lock_guard<std::mutex> lk(sv->mu);
+		auto it = sv->all.find(id);
+		if (it == sv->all.end()) return fail(MMSPX_ENOTFOUND, "unknown request id");
+		r = it->second;
+		if (r->status != 2) return fail(MMSPX_EBUSY, "request has not completed");
+		if (count != (size_t)r->g.nodes) return fail(MMSPX_EINVAL, "output size does not match the request's grid");
+		sv->all.erase(it);
+	}
+	// the fetching thread converts to double
+	if (sv->cfg.precision == MMSPX_F64) {
+		const double* d = (const double*)r->out.data();
+		std::copy(d, d + count, out);
+	} else {
+		const float* f = (const float*)r->out.data();
+		for (size_t i = 0; i < count; i++) out[i] = f[i];
+	}
 	delete r;
 	return 0;
 }

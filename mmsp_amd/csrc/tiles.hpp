@@ -422,4 +422,114 @@ __device__ __forceinline__ void gg_strip(const typename Form<PREC>::S* __restric
 	}
 }
 
+// ---- grain growth, 2-D tile of TX rows (axis 0) x TC columns (axis 1) ---------
+// Each thread owns CPT cells. It loads the cells' Q field values once (all
+// loads in flight together), forms sum_j phi_j^2 in the reference's order,
+// then, field by field, publishes its values into a double-buffered LDS tile
+// whose one-cell ring is loaded (one field ahead) by the threads in turn; the
+// laplacian reads the tile, the update uses the registers. One global read
+// and one write per field value (the ring reads hit in L2). Neighbours outside
+// the grid come through check_boundary's mapping; positions more than one
+// cell outside are never read by a cell of the grid and load a dummy in-range
+// value.
+__device__ __forceinline__ int edge_map(int x, int n, int b0, int b1)
+{
+	if (x >= -1 && x <= n) return bc_map(x, n, b0, b1);
+	return x < 0 ? 0 : n - 1;
+}
+
+template <int PREC, int Q, int TX, int TC, int NT>
+__device__ __forceinline__ void gg_tile(const typename Form<PREC>::S* __restrict__ a,
+                                        typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                        int x0, int y0)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	static_assert((TX * TC) % NT == 0, "tile cells must divide evenly among threads");
+	constexpr int CPT = TX * TC / NT;
+	constexpr int PC = TC + 2, PN = (TX + 2) * PC;
+	constexpr int RN = 2 * PC + 2 * TX;
+	constexpr int RREG = (RN + NT - 1) / NT;
+	__shared__ S tile[2][PN];
+	const int nx = g.n[0], ny = g.n[1];
+	const int64_t N = (int64_t)nx * ny;
+
+	int node[CPT], q[CPT];
+	bool own[CPT];
+#pragma unroll
+	for (int k = 0; k < CPT; k++) {
+		const int e = threadIdx.x + k * NT;
+		const int r = e / TC, c = e - (e / TC) * TC;
+		const int x = x0 + r, y = y0 + c;
+		own[k] = x < nx && y < ny;
+		node[k] = edge_map(x, nx, g.b0[0], g.b1[0]) * ny + edge_map(y, ny, g.b0[1], g.b1[1]);
+		q[k] = (r + 1) * PC + (c + 1);
+	}
+	int rnode[RREG], rq[RREG];
+#pragma unroll
+	for (int j = 0; j < RREG; j++) {
+		const int t = threadIdx.x + j * NT;
+		rnode[j] = 0;
+		rq[j] = 0;
+		if (t < RN) {
+			int pr, pc;
+			if (t < PC) {
+				pr = 0;
+				pc = t;
+			} else if (t < 2 * PC) {
+				pr = TX + 1;
+				pc = t - PC;
+			} else {
+				const int u = t - 2 * PC;
+				pr = 1 + u / 2;
+				pc = (u & 1) ? PC - 1 : 0;
+			}
+			rnode[j] = edge_map(x0 - 1 + pr, nx, g.b0[0], g.b1[0]) * ny + edge_map(y0 - 1 + pc, ny, g.b0[1], g.b1[1]);
+			rq[j] = pr * PC + pc;
+		}
+	}
+	S v[CPT][Q];
+#pragma unroll
+	for (int f = 0; f < Q; f++)
+#pragma unroll
+		for (int k = 0; k < CPT; k++) v[k][f] = a[f * N + node[k]];
+	typename F::Sum sum[CPT];
+#pragma unroll
+	for (int k = 0; k < CPT; k++) {
+		sum[k] = 0;
+#pragma unroll
+		for (int f = 0; f < Q; f++) sum[k] += v[k][f] * v[k][f];
+	}
+	S ring[RREG], rnext[RREG];
+#pragma unroll
+	for (int j = 0; j < RREG; j++) ring[j] = threadIdx.x + j * NT < RN ? a[rnode[j]] : S(0);
+#pragma unroll
+	for (int f = 0; f < Q; f++) {
+		if (f + 1 < Q) {
+#pragma unroll
+			for (int j = 0; j < RREG; j++)
+				if (threadIdx.x + j * NT < RN) rnext[j] = a[(f + 1) * N + rnode[j]];
+		}
+		S* __restrict__ t = tile[f & 1];
+#pragma unroll
+		for (int k = 0; k < CPT; k++) t[q[k]] = v[k][f];
+#pragma unroll
+		for (int j = 0; j < RREG; j++)
+			if (threadIdx.x + j * NT < RN) t[rq[j]] = ring[j];
+		__syncthreads();
+#pragma unroll
+		for (int k = 0; k < CPT; k++) {
+			if (own[k]) {
+				const S y = v[k][f];
+				typename F::L lap = F::lap_zero();
+				F::lap_add(lap, g.w[0], g.wf[0], t[q[k] + PC], y, t[q[k] - PC]);
+				F::lap_add(lap, g.w[1], g.wf[1], t[q[k] + 1], y, t[q[k] - 1]);
+				b[f * N + node[k]] = F::gg(y, sum[k], lap, p);
+			}
+		}
+#pragma unroll
+		for (int j = 0; j < RREG; j++) ring[j] = rnext[j];
+	}
+}
+
 }  // namespace mmspx

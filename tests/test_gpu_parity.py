@@ -179,6 +179,27 @@ def test_fused_ch_options_and_directions(prec):
         assert np.array_equal(g.download(), exp)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
+def test_grain_growth_2d_tiles(prec):
+    """The 2-D grain-growth tiles (8x64 and 16x64 cells), partial tiles and
+    every boundary condition, against the oracle bitwise."""
+    cases = [((70, 300), 4, [(N, M), (D, P)]), ((40, 150), 32, None), ((9, 130), 8, [(M, M), (N, D)]),
+             ((33, 64), 2, [(P, P), (M, N)])]
+    for shape, q, bc in cases:
+        say("[gpu] gg tile %s q=%d %s" % (shape, q, prec))
+        a = O.uniform(int(np.prod(shape)) * q, 300 + q, 0.0, 1.0).reshape(shape + (q,))
+        if prec != "f64":
+            a = a.astype(np.float32)
+        exp = O.gg(a, 0.01, 9, form=prec, bc=bc)
+        for tx in (8, 16):
+            with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+                g.set_variant(2)
+                g.set_tile(tx, 64, tx)
+                g.upload(a)
+                g.step("gg", [0.01], 9)
+                assert np.array_equal(g.download(), exp), (shape, q, tx)
+
+
 def test_upload_download_layouts():
     say("[gpu] layouts")
     a = O.uniform(7 * 6 * 5 * 3, 4, 0, 1).reshape(7, 6, 5, 3)
