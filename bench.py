@@ -4,10 +4,12 @@
                     [--workload ch3d|ch3d_f32|ac2d|gg2d|batch_ac|serve]
                     [--no-cpu-baseline] [--tune | --tile TY,TZ,CHUNK[,OPT]]
                     [--requests N] [--hbm-budget-gb G]
+                    [--graph-steps G] [--no-sequential]
 
 --tune runs the autotuner before timing instead of using the default tile;
 --tile fixes the fused kernel's tile. --requests and --hbm-budget-gb size
-the served stream.
+the served stream. --graph-steps and --no-sequential shape batch_ac runs
+made for profiling (direct launches; no FP64 one-after-another comparison).
 
 Default workload: 3-D Cahn-Hilliard, grid<3,double> 256^3 periodic,
 mu = 2c(1-c)(1-2c) - kappa*lap(c), dc/dt = M*lap(mu), kappa = M = dx = 1,
@@ -117,7 +119,7 @@ def bench_batch_ac(args):
 
     sides, kap = c4_members()
     K = args.steps if args.steps is not None else 1000
-    G = 100
+    G = args.graph_steps
     shapes = [(s, s) for s in sides]
     params = np.array([[1.0, 1.0, k, 1.0, 0.01] for k in kap])
     inits = [mx.fill_uniform(s * s, 20260105 + m, -0.01, 0.01).reshape(s, s) for m, s in enumerate(sides)]
@@ -142,7 +144,7 @@ def bench_batch_ac(args):
     # the same solves, sequentially, plain FP64 single grids
     seq_s = 0.0
     err = 0.0
-    for m, s in enumerate(sides):
+    for m, s in enumerate([] if args.no_sequential else sides):
         with mx.Grid((s, s), precision="f64") as g:
             g.upload(inits[m])
             g.step("ac", params[m], 2)
@@ -173,9 +175,9 @@ def bench_batch_ac(args):
             "graph_steps": G,
             "tile": tile,
             "solves_per_s": len(sides) / dev_s,
-            "fp64_sequential_solves_per_s": len(sides) / seq_s,
-            "speedup_vs_fp64_sequential": seq_s / dev_s,
-            "max_normwise_err_vs_fp64": err,
+            "fp64_sequential_solves_per_s": len(sides) / seq_s if seq_s else None,
+            "speedup_vs_fp64_sequential": seq_s / dev_s if seq_s else None,
+            "max_normwise_err_vs_fp64": err if seq_s else None,
             "wall_s_timed": wall_s,
         },
         "roofline": {
@@ -308,11 +310,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=100,
+                    help="untimed steps first (100 or more also builds the captured graph that small grids replay)")
     ap.add_argument("--workload", default="ch3d", choices=sorted(WORKLOADS) + ["batch_ac", "serve"])
     ap.add_argument("--requests", type=int, default=200, help="serve: requests in the trace")
     ap.add_argument("--hbm-budget-gb", type=int, default=128, help="serve: device memory the server may hold")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph-steps", type=int, default=100, help="batch_ac: steps per captured graph (0: direct launches)")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batch_ac: skip the FP64 one-after-another comparison (for profiling the batch kernel)")
     ap.add_argument("--tune", action="store_true", help="run the autotuner first instead of the tuned default tile")
     ap.add_argument("--tile", default=None, help="ty,tz,chunk[,opt] for the tiled kernel (opt 1: register queues)")
     args = ap.parse_args()
@@ -394,6 +400,8 @@ def main():
         kname = base + "_generic"
     elif base == "ch":
         kname = "ch_fused"
+    elif tile["opt"] == 2:
+        kname = "wave_strips"
     elif base == "gg" and tile["opt"] == 1:
         kname = "gg_tile2d"
     else:

@@ -146,12 +146,22 @@ void mmspx_fill_uniform(double* out, int64_t n, uint64_t seed, double lo, double
  * (MMSPX_EUNSUPPORTED where none exists for the model/shape/boundaries). */
 int mmspx_set_variant(mmspx_state* s, int variant);
 /* Tile of the fused kernels: ty x tz cells of the (axis 1, last axis) plane,
- * marching `chunk` planes along axis 0 (2-D grids: ty = 1). */
+ * marching `chunk` planes along axis 0. 2-D grids: ty = 1 selects strips of
+ * tz columns (tz = 62: the wave strips of Allen-Cahn and grain growth);
+ * ty = 8 or 16 with tz = 64 the 2-D grain-growth tiles; ty = 16 or 32 with
+ * tz = 48 the temporal blocks of periodic Allen-Cahn, which take `chunk`
+ * (at most 8) steps per pass over HBM. */
 int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
-/* Fused Cahn-Hilliard tile body: 0 keeps every neighbour in LDS rings, 1
- * keeps the axis-0 neighbours in per-thread register queues. Same bits. */
-int mmspx_set_tile_options(mmspx_state* s, int register_queues);
-/* Returns variant + 16 * tile body (1: register queues). */
+/* Fused Cahn-Hilliard tile body, all with the same bits: 0 keeps every
+ * neighbour in LDS rings; 1 keeps the axis-0 neighbours in per-thread
+ * register queues; 3-D 32 x 64 tiles only: 2 and 3 are the register queues
+ * with 768 and 1024 threads per tile (fewer points, fewer registers each). */
+int mmspx_set_tile_options(mmspx_state* s, int body);
+/* Returns variant + 16 * tile body (1: register queues, or the 2-D
+ * grain-growth tile; 2: wave strips, or 3-D register queues on 768 threads;
+ * 3: temporal blocks, or 3-D register queues on 1024 threads). Before any
+ * tile is set, reports what the last step ran (the Cahn-Hilliard default
+ * before any step). */
 int mmspx_get_tile(const mmspx_state* s, int* ty, int* tz, int* chunk);
 /* Project autotuner: times every candidate tile for `model` on this state's
  * shape (trial_steps each, state restored afterwards), keeps the fastest and
@@ -192,11 +202,14 @@ int mmspx_batch_reduce(mmspx_batch* b, int member, int what, double* out, int no
 int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t report_len);
 /* Sets the work decomposition directly: strips of strip_width (64, 128, 256)
  * columns by rows_per_item rows with blocks_per_cu persistent workgroups per
- * CU, or, for Allen-Cahn with cells > 0, runs of `cells` consecutive nodes. */
+ * CU, or, for Allen-Cahn with cells > 0, runs of `cells` consecutive nodes.
+ * strip_width 48 selects temporal blocks (periodic Allen-Cahn members only):
+ * 16 x 48-node blocks taking rows_per_item (1..8) steps per pass over HBM. */
 int mmspx_batch_set_tile(mmspx_batch* b, int strip_width, int rows_per_item, int blocks_per_cu, int cells);
 /* Returns 1 when the tile came from the autotuner, 0 for the default. For
  * Allen-Cahn batches run by runs of consecutive nodes, strip_width and
- * blocks_per_cu are 0 and rows_per_item holds the nodes per workgroup. */
+ * blocks_per_cu are 0 and rows_per_item holds the nodes per workgroup; for
+ * temporal blocks strip_width is 48 and rows_per_item the steps per pass. */
 int mmspx_batch_get_tile(const mmspx_batch* b, int* strip_width, int* rows_per_item, int* blocks_per_cu);
 
 /* ---- serving: continuous batching of solves ------------------------------ */

@@ -279,8 +279,10 @@ class Grid:
     def set_tile(self, ty, tz, chunk):
         check(lib.mmspx_set_tile(self._h, ty, tz, chunk))
 
-    def set_tile_options(self, register_queues=False):
-        check(lib.mmspx_set_tile_options(self._h, 1 if register_queues else 0))
+    def set_tile_options(self, register_queues=False, body=None):
+        """Cahn-Hilliard tile body: 0 LDS rings, 1 register queues; 3-D 32 x 64
+        tiles: 2 and 3 register queues on 768 and 1024 threads."""
+        check(lib.mmspx_set_tile_options(self._h, int(body) if body is not None else (1 if register_queues else 0)))
 
     def get_tile(self):
         ty, tz, ch = C.c_int(), C.c_int(), C.c_int()
@@ -388,6 +390,9 @@ class Batch:
         return buf.value.decode()
 
     def set_tile(self, strip_width=256, rows_per_item=32, blocks_per_cu=8, cells=0):
+        """Strips (width 64/128/256), runs of `cells` nodes (Allen-Cahn), or with
+        strip_width 48 temporal blocks taking rows_per_item steps per pass
+        (periodic Allen-Cahn)."""
         check(lib.mmspx_batch_set_tile(self._h, strip_width, rows_per_item, blocks_per_cu, cells))
 
     def get_tile(self):

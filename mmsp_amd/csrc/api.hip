@@ -256,6 +256,24 @@ int mmspx_set_variant(mmspx_state* s, int variant)
 	return 0;
 }
 
+// Splits n steps into passes of at most spl steps; with `even`, into an even
+// number of passes (a captured graph must leave the buffers where it found
+// them). Splitting the largest pass keeps every pass within spl.
+static std::vector<int> plan_passes(int n, int spl, bool even)
+{
+	std::vector<int> v;
+	for (int left = n; left > 0; left -= std::min(left, spl)) v.push_back(std::min(left, spl));
+	if (even && (v.size() & 1)) {
+		auto it = std::max_element(v.begin(), v.end());
+		if (*it >= 2) {
+			const int h = *it / 2;
+			*it -= h;
+			v.insert(it, h);
+		}
+	}
+	return v;
+}
+
 static int step_impl(mmspx_state* s, int model, const double* params, int np, int steps)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
@@ -279,6 +297,9 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	} else if (s->variant == 2) {
 		return fail(MMSPX_EUNSUPPORTED, "no tiled/fused kernel for this model, shape and boundary condition");
 	}
+	s->last = fused ? tc : TuneChoice{1, 0, 0, 0, 0};
+	s->has_last = true;
+	const int spl = fused ? fused_steps_per_launch(model, tc) : 1;
 	if ((model == MMSPX_CH || model == MMSPX_CH_DW) && !fused && !s->aux) {
 		if (hipMalloc(&s->aux, s->bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");
 	}
@@ -302,9 +323,10 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 			MMSPX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
 			int64_t nl = 0;
 			void *a = s->cur, *b = s->nxt;
-			for (int k = 0; k < kGraphSteps && rc == 0; k++) {
-				rc = fused ? launch_fused_step(g, s->desc.precision, p, a, b, s->stream, tc, &nl)
+			for (int n : plan_passes(kGraphSteps, spl, true)) {
+				rc = fused ? launch_fused_step(g, s->desc.precision, p, a, b, s->stream, tc, n, &nl)
 				           : launch_generic_step(g, s->desc.precision, p, a, b, s->aux, s->stream, &nl);
+				if (rc) break;
 				std::swap(a, b);
 			}
 			hipError_t e = hipStreamEndCapture(s->stream, &graph);
@@ -331,12 +353,14 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 			steps -= kGraphSteps;
 		}
 	}
-	for (int k = 0; k < steps; k++) {
-		rc = fused ? launch_fused_step(g, s->desc.precision, p, s->cur, s->nxt, s->stream, tc, &s->launches)
+	while (steps > 0) {
+		const int n = std::min(steps, spl);
+		rc = fused ? launch_fused_step(g, s->desc.precision, p, s->cur, s->nxt, s->stream, tc, n, &s->launches)
 		           : launch_generic_step(g, s->desc.precision, p, s->cur, s->nxt, s->aux, s->stream, &s->launches);
 		if (rc) return rc;
 		std::swap(s->cur, s->nxt);
-		s->steps_run++;
+		s->steps_run += n;
+		steps -= n;
 	}
 	return 0;
 }
@@ -349,8 +373,14 @@ int mmspx_step(mmspx_state* s, int model, const double* params, int np, int step
 int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
-	// ty > 1 on a 2-D grid names a grain-growth 2-D tile (tile body 1)
-	const int opt = (s->geom.dim == 2 && ty > 1) ? 1 : (s->tuned ? s->tile.opt : 0);
+	// on a 2-D grid ty > 1 names a grain-growth 2-D tile (body 1) or, with
+	// tz == kTbCols, Allen-Cahn temporal blocks (body 3); tz == kWaveOut the
+	// wave strips (body 2); otherwise the Cahn-Hilliard body choice is kept
+	int opt = (s->tuned && s->tile.opt == 1) ? 1 : 0;
+	if (s->geom.dim == 2 && ty > 1 && tz == kTbCols) opt = 3;
+	else if (s->geom.dim == 2 && ty > 1) opt = 1;
+	else if (s->geom.dim == 2 && tz == kWaveOut) opt = 2;
+	else if (s->geom.dim == 2 && s->tuned && s->tile.ty > 1) opt = 0;
 	TuneChoice t{2, ty, tz, chunk, opt};
 	if (!tile_known(s->geom, t)) return fail(MMSPX_EINVAL, "no compiled tile of that shape for this grid");
 	s->tile = t;
@@ -358,21 +388,22 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 	return 0;
 }
 
-int mmspx_set_tile_options(mmspx_state* s, int register_queues)
+int mmspx_set_tile_options(mmspx_state* s, int body)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
+	if (body < 0 || body > (s->geom.dim == 3 ? 3 : 1)) return fail(MMSPX_EINVAL, "unknown tile body for this grid");
 	if (!s->tuned) {
 		s->tile = default_choice(s->geom, s->desc.precision, MMSPX_CH_DW);
 		s->tuned = true;
 	}
-	s->tile.opt = register_queues ? 1 : 0;
+	s->tile.opt = body;
 	return 0;
 }
 
 int mmspx_get_tile(const mmspx_state* s, int* ty, int* tz, int* chunk)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
-	TuneChoice t = s->tuned ? s->tile : default_choice(s->geom, s->desc.precision, MMSPX_CH_DW);
+	TuneChoice t = s->tuned ? s->tile : s->has_last ? s->last : default_choice(s->geom, s->desc.precision, MMSPX_CH_DW);
 	if (ty) *ty = t.ty;
 	if (tz) *tz = t.tz;
 	if (chunk) *chunk = t.chunk;
@@ -424,8 +455,8 @@ int mmspx_autotune(mmspx_state* s, int model, const double* params, int np, int 
 	int rc = make_params(model, params, np, &p);
 	if (rc) return rc;
 	std::string rep;
-	TuneChoice cand[32];
-	int nc = fused_supported(s->geom, s->desc.precision, model) ? tile_candidates(s->geom, cand, 32) : 0;
+	TuneChoice cand[64];
+	int nc = fused_supported(s->geom, s->desc.precision, model) ? tile_candidates(s->geom, s->desc.precision, model, cand, 64) : 0;
 	// keep the state: every trial starts from it and it is restored at the end
 	void* saved = nullptr;
 	if (hipMalloc(&saved, s->bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");

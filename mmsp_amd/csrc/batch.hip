@@ -65,6 +65,24 @@ __global__ void __launch_bounds__(256) ac_batch_cells(const BMember* __restrict_
 	}
 }
 
+// Temporal blocks (periodic Allen-Cahn): item = member m, tile origin (y0
+// rows, z0 columns) of a kTbBatchRows x kTbCols block; `steps` steps per pass.
+constexpr int kTbBatchRows = 16;
+
+template <int PREC>
+__global__ void __launch_bounds__(256) ac_batch_tb(const BMember* __restrict__ mem, const BItem* __restrict__ items,
+                                                   int nitems, int parity, int steps)
+{
+	using S = typename Form<PREC>::S;
+	for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+		const BItem w = items[it];
+		const BMember& m = mem[w.m];
+		const int src = (parity + m.phase) & 1;
+		ac_tblock<PREC, kTbBatchRows, kTbMaxSteps, 256>((const S*)m.buf[src], (S*)m.buf[src ^ 1], m.g, m.p, w.y0, w.z0,
+		                                                steps);
+	}
+}
+
 template <int PREC, int Q, int TZ>
 __global__ void __launch_bounds__(TZ) gg_batch(const BMember* __restrict__ mem, const BItem* __restrict__ items,
                                                int nitems, int parity)
@@ -142,6 +160,11 @@ void member_items(int m, const Geom& g, int model, const BatchTile& t, std::vect
 	// 2-D: strips of the contiguous axis (axis 1). The strip bodies take the
 	// strip origin as y0; the 2-D Cahn-Hilliard tile names that axis z.
 	const bool ch2 = model == MMSPX_CH || model == MMSPX_CH_DW;
+	if (model == MMSPX_AC && t.tb > 0) {
+		for (int x0 = 0; x0 < nx; x0 += kTbBatchRows)
+			for (int c0 = 0; c0 < g.n[1]; c0 += kTbCols) out2.push_back(BItem{m, x0, c0, 0, 0});
+		return;
+	}
 	if (model == MMSPX_AC && t.cells > 0) {
 		for (int f = 0; f < g.nodes; f += t.cells) out2.push_back(BItem{m, f, std::min(t.cells, g.nodes - f), 0, 0});
 		return;
@@ -189,9 +212,12 @@ static void launch_ch3(bool dw, const BMember* mem, const BItem* items, int n, i
 
 template <int PREC>
 static void launch_prec(int model, int fields, const BatchTile& t, const BMember* mem, const BItem* l2, int n2,
-                        const BItem* l3, int n3, int parity, int cus, hipStream_t st)
+                        const BItem* l3, int n3, int parity, int cus, hipStream_t st, int nsteps)
 {
-	if (n2 > 0 && model == MMSPX_AC && t.cells > 0) {
+	if (n2 > 0 && model == MMSPX_AC && t.tb > 0) {
+		const int blocks = std::max(1, std::min(n2, cus * t.bpc));
+		hipLaunchKernelGGL((ac_batch_tb<PREC>), dim3(blocks), dim3(256), 0, st, mem, l2, n2, parity, nsteps);
+	} else if (n2 > 0 && model == MMSPX_AC && t.cells > 0) {
 		hipLaunchKernelGGL((ac_batch_cells<PREC>), dim3(n2), dim3(256), 0, st, mem, l2, parity);
 	} else if (n2 > 0) {
 		const int blocks = std::max(1, std::min(n2, cus * t.bpc));
@@ -209,12 +235,14 @@ static void launch_prec(int model, int fields, const BatchTile& t, const BMember
 }
 
 int launch_batch_step(int model, int prec, int fields, const BatchTile& t, const BMember* mem, const BItem* l2, int n2,
-                      const BItem* l3, int n3, int parity, int cus, hipStream_t st, int64_t* launches)
+                      const BItem* l3, int n3, int parity, int cus, hipStream_t st, int64_t* launches, int nsteps)
 {
+	if (nsteps < 1 || nsteps > std::max(1, std::min(t.tb, kTbMaxSteps)))
+		return fail(MMSPX_EINVAL, "steps per pass exceed what the batch tile takes");
 	switch (prec) {
-	case MMSPX_F64: launch_prec<MMSPX_F64>(model, fields, t, mem, l2, n2, l3, n3, parity, cus, st); break;
-	case MMSPX_F32: launch_prec<MMSPX_F32>(model, fields, t, mem, l2, n2, l3, n3, parity, cus, st); break;
-	case MMSPX_F32REF: launch_prec<MMSPX_F32REF>(model, fields, t, mem, l2, n2, l3, n3, parity, cus, st); break;
+	case MMSPX_F64: launch_prec<MMSPX_F64>(model, fields, t, mem, l2, n2, l3, n3, parity, cus, st, nsteps); break;
+	case MMSPX_F32: launch_prec<MMSPX_F32>(model, fields, t, mem, l2, n2, l3, n3, parity, cus, st, nsteps); break;
+	case MMSPX_F32REF: launch_prec<MMSPX_F32REF>(model, fields, t, mem, l2, n2, l3, n3, parity, cus, st, nsteps); break;
 	default: return fail(MMSPX_EINVAL, "unknown precision");
 	}
 	MMSPX_HIP(hipGetLastError());
@@ -265,6 +293,7 @@ struct mmspx_batch {
 	bool tuned = false;
 	int cus = 256;
 	int64_t steps_done = 0, launches = 0;
+	int64_t passes_done = 0;  // launches per item list so far: the buffer parity
 	hipStream_t st = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -318,29 +347,42 @@ static int upload_plan(mmspx_batch* b)
 	return 0;
 }
 
+static int steps_per_pass(const mmspx_batch* b) { return b->tile.tb > 0 ? std::min(b->tile.tb, kTbMaxSteps) : 1; }
+
 static int enqueue_steps(mmspx_batch* b, int steps)
 {
-	for (int k = 0; k < steps; k++) {
-		const int parity = (int)((b->steps_done + k) & 1);
+	const int spp = steps_per_pass(b);
+	for (int left = steps; left > 0;) {
+		const int n = std::min(left, spp);
+		const int parity = (int)(b->passes_done & 1);
 		int rc = launch_batch_step(b->model, b->prec, b->fields, b->tile, b->d_mem, b->d_items, b->n2,
-		                           b->d_items + b->n2, b->n3, parity, b->cus, b->st, &b->launches);
+		                           b->d_items + b->n2, b->n3, parity, b->cus, b->st, &b->launches, n);
 		if (rc) return rc;
+		b->passes_done += 1;
+		b->steps_done += n;
+		left -= n;
 	}
-	b->steps_done += steps;
 	return 0;
 }
+
+static int graph_passes(const mmspx_batch* b, int gsteps) { return (gsteps + steps_per_pass(b) - 1) / steps_per_pass(b); }
 
 static int build_graphs(mmspx_batch* b, int gsteps)
 {
 	drop_graphs(b);
+	const int spp = steps_per_pass(b);
 	for (int start = 0; start < 2; start++) {
 		hipGraph_t graph = nullptr;
 		MMSPX_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
 		int64_t dummy = 0;
 		int rc = 0;
-		for (int k = 0; k < gsteps && rc == 0; k++)
+		int k = 0;
+		for (int left = gsteps; left > 0 && rc == 0; k++) {
+			const int n = std::min(left, spp);
 			rc = launch_batch_step(b->model, b->prec, b->fields, b->tile, b->d_mem, b->d_items, b->n2, b->d_items + b->n2,
-			                       b->n3, (start + k) & 1, b->cus, b->st, &dummy);
+			                       b->n3, (start + k) & 1, b->cus, b->st, &dummy, n);
+			left -= n;
+		}
 		hipError_t e = hipStreamEndCapture(b->st, &graph);
 		if (rc) {
 			if (graph) (void)hipGraphDestroy(graph);
@@ -365,10 +407,12 @@ static int batch_step_impl(mmspx_batch* b, int steps, int gsteps)
 			if (rc) return rc;
 		}
 		const int reps = steps / gsteps;
+		const int gp = graph_passes(b, gsteps);
 		for (int r = 0; r < reps; r++) {
-			MMSPX_HIP(hipGraphLaunch(b->graph[b->steps_done & 1], b->st));
+			MMSPX_HIP(hipGraphLaunch(b->graph[b->passes_done & 1], b->st));
 			b->steps_done += gsteps;
-			b->launches += (int64_t)gsteps * ((b->n2 > 0) + (b->n3 > 0));
+			b->passes_done += gp;
+			b->launches += (int64_t)gp * ((b->n2 > 0) + (b->n3 > 0));
 		}
 		steps -= reps * gsteps;
 	}
@@ -455,7 +499,7 @@ int mmspx_batch_create(const mmspx_desc* descs, int count, int model, mmspx_batc
 	return 0;
 }
 
-static void* cur_buf(mmspx_batch* b, int m) { return b->arena + b->off[m] + ((b->steps_done & 1) ? b->bytes[m] : 0); }
+static void* cur_buf(mmspx_batch* b, int m) { return b->arena + b->off[m] + ((b->passes_done & 1) ? b->bytes[m] : 0); }
 
 int mmspx_batch_upload(mmspx_batch* b, int m, const void* host, int layout)
 {
@@ -466,7 +510,7 @@ int mmspx_batch_upload(mmspx_batch* b, int m, const void* host, int layout)
 	if (g.fields == 1 || layout == MMSPX_FIELD_MAJOR) {
 		MMSPX_HIP(hipMemcpyAsync(dst, host, n, hipMemcpyHostToDevice, b->st));
 	} else {
-		void* spare = b->arena + b->off[m] + ((b->steps_done & 1) ? 0 : b->bytes[m]);
+		void* spare = b->arena + b->off[m] + ((b->passes_done & 1) ? 0 : b->bytes[m]);
 		MMSPX_HIP(hipMemcpyAsync(spare, host, n, hipMemcpyHostToDevice, b->st));
 		int rc = launch_transpose(spare, dst, g.nodes, g.fields, b->prec == MMSPX_F64 ? 8 : 4, true, b->st);
 		if (rc) return rc;
@@ -484,7 +528,7 @@ int mmspx_batch_download(mmspx_batch* b, int m, void* host, int layout)
 	if (g.fields == 1 || layout == MMSPX_FIELD_MAJOR) {
 		MMSPX_HIP(hipMemcpyAsync(host, src, n, hipMemcpyDeviceToHost, b->st));
 	} else {
-		void* spare = b->arena + b->off[m] + ((b->steps_done & 1) ? 0 : b->bytes[m]);
+		void* spare = b->arena + b->off[m] + ((b->passes_done & 1) ? 0 : b->bytes[m]);
 		int rc = launch_transpose(src, spare, g.nodes, g.fields, b->prec == MMSPX_F64 ? 8 : 4, false, b->st);
 		if (rc) return rc;
 		MMSPX_HIP(hipMemcpyAsync(host, spare, n, hipMemcpyDeviceToHost, b->st));
@@ -538,9 +582,25 @@ int mmspx_batch_reduce(mmspx_batch* b, int m, int what, double* out, int nout)
 	return 0;
 }
 
+static bool all_members_periodic(const mmspx_batch* b)
+{
+	for (const Geom& g : b->geom)
+		for (int k = 0; k < g.dim; k++)
+			if (g.b0[k] != MMSPX_PERIODIC || g.b1[k] != MMSPX_PERIODIC) return false;
+	return true;
+}
+
 int mmspx_batch_set_tile(mmspx_batch* b, int tz, int chunk, int blocks_per_cu, int cells)
 {
 	if (!b) return fail(MMSPX_EINVAL, "null batch");
+	if (tz == kTbCols) {  // temporal blocks: chunk = steps per pass
+		if (b->model != MMSPX_AC || !all_members_periodic(b) || chunk < 1 || chunk > kTbMaxSteps || blocks_per_cu < 1)
+			return fail(MMSPX_EINVAL, "temporal blocks: periodic Allen-Cahn members, 1..8 steps per pass");
+		BatchTile t{tz, 32, b->tile.chunk3, blocks_per_cu, 0};
+		t.tb = chunk;
+		b->tile = t;
+		return upload_plan(b);
+	}
 	if ((tz != 64 && tz != 128 && tz != 256) || chunk < 1 || blocks_per_cu < 1 || cells < 0 ||
 	    (cells > 0 && b->model != MMSPX_AC))
 		return fail(MMSPX_EINVAL, "strip width 64/128/256, positive rows and workgroups; cell runs for Allen-Cahn only");
@@ -551,9 +611,10 @@ int mmspx_batch_set_tile(mmspx_batch* b, int tz, int chunk, int blocks_per_cu, i
 int mmspx_batch_get_tile(const mmspx_batch* b, int* tz, int* chunk, int* blocks_per_cu)
 {
 	if (!b) return fail(MMSPX_EINVAL, "null batch");
-	if (tz) *tz = b->tile.cells > 0 ? 0 : b->tile.tz;
-	if (chunk) *chunk = b->tile.cells > 0 ? b->tile.cells : b->tile.chunk;
-	if (blocks_per_cu) *blocks_per_cu = b->tile.cells > 0 ? 0 : b->tile.bpc;
+	const BatchTile& t = b->tile;
+	if (tz) *tz = t.tb > 0 ? kTbCols : t.cells > 0 ? 0 : t.tz;
+	if (chunk) *chunk = t.tb > 0 ? t.tb : t.cells > 0 ? t.cells : t.chunk;
+	if (blocks_per_cu) *blocks_per_cu = t.cells > 0 ? 0 : t.bpc;
 	return b->tuned ? 1 : 0;
 }
 
@@ -569,7 +630,7 @@ int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t r
 		b->staging_bytes = b->arena_bytes;
 	}
 	MMSPX_HIP(hipMemcpyAsync(b->staging, b->arena, b->arena_bytes, hipMemcpyDeviceToDevice, b->st));
-	const int64_t steps0 = b->steps_done;
+	const int64_t steps0 = b->steps_done, passes0 = b->passes_done;
 	const BatchTile keep = b->tile;
 	std::vector<BatchTile> cand;
 	for (int tz : {64, 128, 256})
@@ -577,6 +638,13 @@ int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t r
 			for (int bpc : {4, 8, 16}) cand.push_back(BatchTile{tz, chunk, 16, bpc, 0});
 	if (b->model == MMSPX_AC)
 		for (int cells : {256, 512, 1024, 2048, 4096}) cand.push_back(BatchTile{256, 32, 16, 8, cells});
+	if (b->model == MMSPX_AC && all_members_periodic(b))
+		for (int tb : {4, 8})
+			for (int bpc : {4, 8, 16}) {
+				BatchTile t{kTbCols, 32, 16, bpc, 0};
+				t.tb = tb;
+				cand.push_back(t);
+			}
 	std::string rep;
 	double best = 1e30;
 	BatchTile best_t = keep;
@@ -584,13 +652,16 @@ int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t r
 	for (const BatchTile& t : cand) {
 		b->tile = t;
 		rc = upload_plan(b);
+		// temporal blocks are timed over whole passes
+		const int steps = t.tb > 0 ? std::max(trial_steps, 2 * t.tb) / t.tb * t.tb : trial_steps;
 		if (rc == 0) rc = enqueue_steps(b, 2);
 		double sec = 0.0;
-		if (rc == 0) rc = mmspx_batch_step_timed(b, trial_steps, 0, &sec);
+		if (rc == 0) rc = mmspx_batch_step_timed(b, steps, 0, &sec);
 		if (rc) break;
-		char line[128];
-		std::snprintf(line, sizeof line, "tz=%d chunk=%d blocks_per_cu=%d cells=%d us_per_step=%.2f\n", t.tz, t.chunk,
-		              t.bpc, t.cells, sec / trial_steps * 1e6);
+		sec = sec / steps * trial_steps;
+		char line[160];
+		std::snprintf(line, sizeof line, "tz=%d chunk=%d blocks_per_cu=%d cells=%d steps_per_pass=%d us_per_step=%.2f\n",
+		              t.tz, t.chunk, t.bpc, t.cells, t.tb, sec / trial_steps * 1e6);
 		rep += line;
 		if (sec < best) {
 			best = sec;
@@ -599,6 +670,7 @@ int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t r
 	}
 	b->tile = rc ? keep : best_t;
 	b->steps_done = steps0;
+	b->passes_done = passes0;
 	int rc2 = upload_plan(b);
 	if (hipMemcpyAsync(b->arena, b->staging, b->arena_bytes, hipMemcpyDeviceToDevice, b->st) != hipSuccess ||
 	    hipStreamSynchronize(b->st) != hipSuccess)

@@ -27,15 +27,17 @@ struct BItem {
 // Cahn-Hilliard kernel, and the persistent grid size in workgroups per CU.
 struct BatchTile {
 	int tz, chunk, chunk3, bpc;
-	int cells;  // > 0: 2-D Allen-Cahn by runs of `cells` consecutive nodes, one workgroup each
+	int cells;   // > 0: 2-D Allen-Cahn by runs of `cells` consecutive nodes, one workgroup each
+	int tb = 0;  // > 0: periodic 2-D Allen-Cahn by temporal blocks, up to `tb` steps per pass
 };
 
 bool batch_model_ok(int model, const Geom& g, std::string* why);
 void member_items(int m, const Geom& g, int model, const BatchTile& t, std::vector<BItem>& out2,
                   std::vector<BItem>& out3);
-// One step for every item: list l2 (2-D strips/rows) and l3 (3-D tiles).
+// One pass over every item: list l2 (2-D strips/rows/blocks) and l3 (3-D
+// tiles). A pass is one step, or with temporal blocks `nsteps` (<= t.tb) steps.
 int launch_batch_step(int model, int prec, int fields, const BatchTile& t, const BMember* mem, const BItem* l2, int n2,
-                      const BItem* l3, int n3, int parity, int cus, hipStream_t st, int64_t* launches);
+                      const BItem* l3, int n3, int parity, int cus, hipStream_t st, int64_t* launches, int nsteps = 1);
 int device_cus();
 // Reorders a spatially ordered work list so that the persistent launch (item
 // p runs on workgroup p mod gridDim, and workgroup ids are dealt round-robin

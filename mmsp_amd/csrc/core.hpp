@@ -79,6 +79,8 @@ struct mmspx_state {
 	int variant = 0;
 	bool tuned = false;      // a tile chosen by mmspx_set_tile / mmspx_autotune
 	mmspx::TuneChoice tile{};
+	mmspx::TuneChoice last{};  // what the last step ran (reported by mmspx_get_tile when not tuned)
+	bool has_last = false;
 	int64_t launches = 0;
 	int64_t steps_run = 0;
 	// captured multi-step graph for small grids (launch-bound), keyed by what
@@ -96,14 +98,24 @@ namespace mmspx {
 constexpr int kRedBlocks = 512;
 constexpr int kGraphSteps = 50;  // steps per captured graph (even: buffers return to place)
 constexpr int kRedThreads = 256;
+// output columns of one wave strip (2-D Allen-Cahn and grain growth): a wave's
+// 64 lanes hold these columns plus one halo column on each side
+constexpr int kWaveOut = 62;
+// temporal blocks (2-D periodic Allen-Cahn): tile columns, and the most steps
+// one pass takes (tile plus halo fits the 64 lanes of a wave)
+constexpr int kTbCols = 48;
+constexpr int kTbMaxSteps = 8;
 
 // ---- kernel launchers (kernels_generic.hip, kernels_fused.hip, reduce.hip) ----
 // Each returns 0 or a negative code; all enqueue on `st`.
 int launch_generic_step(const Geom& g, int prec, const ModelParams& p, const void* a, void* b, void* aux,
                         hipStream_t st, int64_t* launches);
 bool fused_supported(const Geom& g, int prec, int model);
+// One launch of the tiled kernels; it advances `nsteps` steps, which must not
+// exceed fused_steps_per_launch (1 except for temporal blocks).
 int launch_fused_step(const Geom& g, int prec, const ModelParams& p, const void* a, void* b, hipStream_t st,
-                      const TuneChoice& t, int64_t* launches);
+                      const TuneChoice& t, int nsteps, int64_t* launches);
+int fused_steps_per_launch(int model, const TuneChoice& t);
 TuneChoice default_choice(const Geom& g, int prec, int model);
 int launch_reduce(const Geom& g, int prec, int what, const ModelParams& p, const void* a, double* part,
                   hipStream_t st);
@@ -112,7 +124,7 @@ int launch_transpose(const void* src, void* dst, int nodes, int fields, size_t e
 int launch_convert(const void* src, int src_f64, void* dst, int dst_f64, int64_t n, hipStream_t st);
 int launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
 bool tile_known(const Geom& g, const TuneChoice& t);
-int tile_candidates(const Geom& g, TuneChoice* out, int max);
+int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max);
 
 // host-side compensated sum of the (sum, comp) partial pairs
 double sum_partials(const double* part, int count);

@@ -42,6 +42,38 @@ __device__ __forceinline__ double cube_cr(double x)
 	return hi + lo;
 }
 
+// ---- lane shifts across one 64-lane wave (DPP wave_shr:1 / wave_shl:1) -----
+// from_prev: lane i receives lane i-1's value; from_next: lane i+1's. The lane
+// with no source (lane 0, resp. 63) keeps `edge`. Callers keep every lane of
+// the wave active at the call (the builtin is convergent, so the compiler does
+// not move it under a branch).
+__device__ __forceinline__ int dpp_from_prev(int v, int edge)
+{
+	return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_from_next(int v, int edge)
+{
+	return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float from_prev(float v, float edge)
+{
+	return __builtin_bit_cast(float, dpp_from_prev(__builtin_bit_cast(int, v), __builtin_bit_cast(int, edge)));
+}
+__device__ __forceinline__ float from_next(float v, float edge)
+{
+	return __builtin_bit_cast(float, dpp_from_next(__builtin_bit_cast(int, v), __builtin_bit_cast(int, edge)));
+}
+__device__ __forceinline__ double from_prev(double v, double edge)
+{
+	const int2 s = __builtin_bit_cast(int2, v), e = __builtin_bit_cast(int2, edge);
+	return __builtin_bit_cast(double, make_int2(dpp_from_prev(s.x, e.x), dpp_from_prev(s.y, e.y)));
+}
+__device__ __forceinline__ double from_next(double v, double edge)
+{
+	const int2 s = __builtin_bit_cast(int2, v), e = __builtin_bit_cast(int2, edge);
+	return __builtin_bit_cast(double, make_int2(dpp_from_next(s.x, e.x), dpp_from_next(s.y, e.y)));
+}
+
 // ---- precision forms -----------------------------------------------------
 template <int PREC> struct Form;
 

@@ -15,6 +15,8 @@
 // dealt round-robin over the 8 XCDs) form a contiguous run of tiles with the
 // axis-0 chunk slowest: an XCD sweeps whole planes of one chunk, so halo reads
 // of neighbouring tiles hit its own L2.
+#include <algorithm>
+
 #include "tiles.hpp"
 
 namespace mmspx {
@@ -26,7 +28,8 @@ __device__ __forceinline__ int xcd_tile(int ntiles)
 	return tile < ntiles ? tile : -1;
 }
 
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, bool RQ>
+// BODY 0: LDS rings, 1: register queues
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, int BODY>
 __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __restrict__ a,
                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
                                                int chunk_len, int tiles_y, int tiles_z, int ntiles)
@@ -39,7 +42,7 @@ __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __r
 	const int chunk = t2 / tiles_y;
 	const int xb = chunk * chunk_len;
 	const int xe = min(xb + chunk_len, g.n[0]);
-	if (RQ)
+	if constexpr (BODY == 1)
 		ch_tile_rq<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
 	else
 		ch_tile<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
@@ -80,6 +83,37 @@ __global__ void __launch_bounds__(NT) gg_tile2d(const typename Form<PREC>::S* __
 	gg_tile<PREC, Q, TX, TC, NT>(a, b, g, p, tr * TX, tc * TC);
 }
 
+// Wave strips (2-D Allen-Cahn and grain growth): one wave per (strip, chunk)
+// item, four waves per workgroup, strips kWaveOut columns wide. Items run
+// strip-fastest, so a workgroup's waves and an XCD's run of workgroups cover
+// neighbouring strips of the same rows (shared halo columns in one L2).
+template <int PREC, int Q, bool GG>
+__global__ void __launch_bounds__(256) wave_strips(const typename Form<PREC>::S* __restrict__ a,
+                                                   typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
+                                                   int chunk_len, int strips, int nitems, int ntiles)
+{
+	const int tile = xcd_tile(ntiles);
+	if (tile < 0) return;
+	const int item = tile * 4 + (int)(threadIdx.x >> 6);
+	if (item >= nitems) return;  // a whole wave; the waves share nothing
+	const int s = item % strips, chunk = item / strips;
+	const int xb = chunk * chunk_len;
+	strip_wave<PREC, Q, GG>(a, b, g, p, s * kWaveOut, xb, min(xb + chunk_len, g.n[0]));
+}
+
+// Temporal blocks (2-D periodic Allen-Cahn): one workgroup per TX x kTbCols
+// tile, `steps` (<= kTbMaxSteps) steps per launch.
+template <int PREC, int TX>
+__global__ void __launch_bounds__(256) ac_tblocks(const typename Form<PREC>::S* __restrict__ a,
+                                                  typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
+                                                  int tiles_y, int ntiles, int steps)
+{
+	const int tile = xcd_tile(ntiles);
+	if (tile < 0) return;
+	const int ty = tile % tiles_y, tx = tile / tiles_y;
+	ac_tblock<PREC, TX, kTbMaxSteps, 256>(a, b, g, p, tx * TX, ty * kTbCols, steps);
+}
+
 static bool all_periodic(const Geom& g)
 {
 	for (int k = 0; k < g.dim; k++)
@@ -101,65 +135,87 @@ bool fused_supported(const Geom& g, int prec, int model)
 	return false;
 }
 
-// Defaults come from the autotuner's sweep on MI355X: for the 256^3 periodic
-// CH grid 16x64 tiles with 32-plane chunks were fastest; grids too small to
-// fill the chip with them take narrower tiles and shorter chunks.
+// Defaults come from sweeps of the candidates below on MI355X (the figures
+// quoted are microseconds per step from those sweeps).
 TuneChoice default_choice(const Geom& g, int prec, int model)
 {
 	if (!fused_supported(g, prec, model)) return TuneChoice{1, 0, 0, 0, 0};
 	const int64_t nodes = g.nodes;
-	// Allen-Cahn: the one-thread-per-node kernel beats the strips at every size
-	// swept (512^2 FP64, 2048^2 FP32); the strips stay selectable
-	// (mmspx_set_variant 2) and are what batches run.
-	if (model == MMSPX_AC) return TuneChoice{1, 0, 0, 0, 0};
-	// grain growth with 32 fields (2048^2 FP32): the 8 x 64 two-dimensional
-	// tile takes 330 us per step against 388 us for one thread per node and
-	// 530 us for the strips; other field counts keep the generic kernel
-	if (model == MMSPX_GG) return g.fields == 32 ? TuneChoice{2, 8, 64, 8, 1} : TuneChoice{1, 0, 0, 0, 0};
+	// Allen-Cahn: periodic grids take temporal blocks of 16 x 48 nodes and 8
+	// steps per pass (512^2: 2.05 us per step in FP64 against 4.58 us for one
+	// thread per node, FP32 1.71 against 3.37); other boundaries keep the
+	// one-thread-per-node kernel, which beats the strips at every size swept.
+	if (model == MMSPX_AC) return all_periodic(g) ? TuneChoice{2, 16, kTbCols, 8, 3} : TuneChoice{1, 0, 0, 0, 0};
+	// grain growth with 32 fields (2048^2 FP32): wave strips with 12-row
+	// chunks take 273 us per step against 334 us for the 8 x 64 tile, 388 us
+	// for one thread per node and 530 us for the LDS-row strips. 32 FP64
+	// fields do not fit the strips' registers: the 8 x 64 tile. Other field
+	// counts keep the generic kernel.
+	if (model == MMSPX_GG && g.fields == 32)
+		return prec == MMSPX_F64 ? TuneChoice{2, 8, 64, 8, 1} : TuneChoice{2, 1, kWaveOut, 12, 2};
+	if (model == MMSPX_GG) return TuneChoice{1, 0, 0, 0, 0};
 	if (g.dim == 2) return TuneChoice{2, 1, 64, 16, 0};
 	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16, 0};
-	// 256^3: register-queue body, 16 x 64 tiles, 16-plane chunks (FP64 57 us
-	// per step against 64 us for the LDS-ring body's best tile; FP32 38 us)
-	return TuneChoice{2, 16, 64, 16, 1};
+	// 256^3: register queues on 32 x 64 tiles, 16-plane chunks. FP64 with 768
+	// threads per tile (54.7 us per step, against 56.9 us with 512 threads,
+	// 57.6 us for 16 x 64 tiles and 64 us for the LDS-ring body's best tile);
+	// FP32 with 512 threads (37.8 us, against 38.3 us with 768)
+	return TuneChoice{2, 32, 64, 16, prec == MMSPX_F64 ? 2 : 1};
 }
 
 bool tile_known(const Geom& g, const TuneChoice& t)
 {
 	if (t.chunk < 2 || t.chunk > 4096) return false;
 	if (g.dim == 3)
-		return (t.ty == 8 && t.tz == 64) || (t.ty == 16 && t.tz == 64) || (t.ty == 8 && t.tz == 32) ||
-		       (t.ty == 4 && t.tz == 128);
+		return (t.ty == 8 && t.tz == 64) || (t.ty == 16 && t.tz == 64) || (t.ty == 32 && t.tz == 64) ||
+		       (t.ty == 8 && t.tz == 32) || (t.ty == 4 && t.tz == 128);
 	if (g.dim == 2)
-		return (t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256)) ||
+		return (t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256 || t.tz == kWaveOut)) ||
+		       ((t.ty == 16 || t.ty == 32) && t.tz == kTbCols && t.chunk <= kTbMaxSteps) ||
 		       (t.opt == 1 && (t.ty == 8 || t.ty == 16) && t.tz == 64);
 	return false;
 }
 
-int tile_candidates(const Geom& g, TuneChoice* out, int max)
+int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max)
 {
 	int n = 0;
 	auto add = [&](int ty, int tz, int chunk, int opt) {
 		if (n < max) out[n++] = TuneChoice{2, ty, tz, chunk, opt};
 	};
+	const bool ch = model == MMSPX_CH || model == MMSPX_CH_DW;
 	if (g.dim == 3) {
 		const int tiles[4][2] = {{16, 64}, {8, 64}, {8, 32}, {4, 128}};
 		for (auto& t : tiles)
 			for (int c : {16, 32, 64})
 				for (int opt = 0; opt < 2; opt++) add(t[0], t[1], c, opt);
+		// 32 x 64: register queues only (the LDS rings would leave one
+		// workgroup per CU), on 512, 768 and 1024 threads
+		for (int c : {12, 16, 22, 32})
+			for (int body = 1; body <= 3; body++) add(32, 64, c, body);
+		for (int c : {20, 22, 24}) add(16, 64, c, 1);
 	} else if (g.dim == 2) {
+		// LDS-row strips (Cahn-Hilliard: both tile bodies)
 		for (int tz : {64, 128, 256})
 			for (int c : {16, 32, 64, 128})
-				for (int opt = 0; opt < 2; opt++) add(1, tz, c, opt);
-		// grain growth: 2-D tiles (the strips above ignore opt for it)
-		if (gg_q_compiled(g.fields) && g.fields > 1) {
+				for (int opt = 0; opt < (ch ? 2 : 1); opt++) add(1, tz, c, opt);
+		// grain growth: 2-D tiles
+		if (model == MMSPX_GG && g.fields > 1) {
 			add(8, 64, 8, 1);
 			add(16, 64, 16, 1);
 		}
+		// Allen-Cahn and grain growth: wave strips, where compiled
+		const bool wave = model == MMSPX_AC || (model == MMSPX_GG && (g.fields < 32 || prec != MMSPX_F64));
+		if (wave)
+			for (int c : {8, 16, 24, 32, 64}) add(1, kWaveOut, c, 2);
+		// periodic Allen-Cahn: temporal blocks (chunk = steps per pass)
+		if (model == MMSPX_AC && all_periodic(g))
+			for (int tx : {16, 32})
+				for (int k : {4, 8}) add(tx, kTbCols, k, 3);
 	}
 	return n;
 }
 
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, bool RQ>
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, int BODY>
 static int launch_ch_body(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
 	using S = typename Form<PREC>::S;
@@ -172,18 +228,19 @@ static int launch_ch_body(const Geom& g, const ModelParams& p, const void* a, vo
 	const int ntiles = tiles_y * tiles_z * chunks;
 	// round the grid up to a multiple of 8 so the XCD remap applies; extra ids exit
 	const int blocks = (ntiles + 7) / 8 * 8;
-	hipLaunchKernelGGL((ch_fused<PREC, DW, DIM, TY, TZ, NT, RQ>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g,
+	hipLaunchKernelGGL((ch_fused<PREC, DW, DIM, TY, TZ, NT, BODY>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g,
 	                   p, chunk, tiles_y, tiles_z, ntiles);
 	MMSPX_HIP(hipGetLastError());
 	return 0;
 }
 
-// opt 1 selects the register-queue body
+// opt 1 selects the register-queue body (opt 2 and 3, register queues with
+// more threads per tile, are dispatched in launch_ch_prec)
 template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
 static int launch_ch(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk, int opt)
 {
-	if (opt & 1) return launch_ch_body<PREC, DW, DIM, TY, TZ, NT, true>(g, p, a, b, st, chunk);
-	return launch_ch_body<PREC, DW, DIM, TY, TZ, NT, false>(g, p, a, b, st, chunk);
+	if (opt == 1) return launch_ch_body<PREC, DW, DIM, TY, TZ, NT, 1>(g, p, a, b, st, chunk);
+	return launch_ch_body<PREC, DW, DIM, TY, TZ, NT, 0>(g, p, a, b, st, chunk);
 }
 
 template <int PREC, bool DW>
@@ -191,12 +248,21 @@ static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, vo
                           const TuneChoice& t)
 {
 	const int chunk = t.chunk > 0 ? t.chunk : 32;
+	if (g.dim == 3 && t.opt >= 2) {
+		// register queues on 32 x 64 tiles with more threads (fewer own points
+		// each): opt 2: 768 threads, opt 3: 1024
+		if (t.ty != 32 || t.tz != 64) return fail(MMSPX_EINVAL, "tile bodies 2 and 3 run 32 x 64 tiles");
+		if (t.opt == 3) return launch_ch_body<PREC, DW, 3, 32, 64, 1024, 1>(g, p, a, b, st, chunk);
+		return launch_ch_body<PREC, DW, 3, 32, 64, 768, 1>(g, p, a, b, st, chunk);
+	}
 	if (g.dim == 3) {
 		if (t.ty == 16 && t.tz == 64) return launch_ch<PREC, DW, 3, 16, 64, 256>(g, p, a, b, st, chunk, t.opt);
+		if (t.ty == 32 && t.tz == 64) return launch_ch<PREC, DW, 3, 32, 64, 512>(g, p, a, b, st, chunk, t.opt);
 		if (t.ty == 8 && t.tz == 32) return launch_ch<PREC, DW, 3, 8, 32, 256>(g, p, a, b, st, chunk, t.opt);
 		if (t.ty == 4 && t.tz == 128) return launch_ch<PREC, DW, 3, 4, 128, 256>(g, p, a, b, st, chunk, t.opt);
 		return launch_ch<PREC, DW, 3, 8, 64, 256>(g, p, a, b, st, chunk, t.opt);
 	}
+	if (t.tz == kWaveOut) return fail(MMSPX_EUNSUPPORTED, "the wave strips run Allen-Cahn and grain growth");
 	if (t.tz == 64) return launch_ch<PREC, DW, 2, 1, 64, 64>(g, p, a, b, st, chunk, t.opt);
 	if (t.tz == 128) return launch_ch<PREC, DW, 2, 1, 128, 128>(g, p, a, b, st, chunk, t.opt);
 	return launch_ch<PREC, DW, 2, 1, 256, 256>(g, p, a, b, st, chunk, t.opt);
@@ -268,20 +334,75 @@ static int launch_gg_tz(const Geom& g, const ModelParams& p, const void* a, void
 	return fail(MMSPX_EUNSUPPORTED, "no tiled grain-growth kernel for this field count");
 }
 
+template <int PREC, int TX>
+static int launch_tb(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int steps)
+{
+	using S = typename Form<PREC>::S;
+	const int tiles_y = (g.n[1] + kTbCols - 1) / kTbCols;
+	const int ntiles = tiles_y * ((g.n[0] + TX - 1) / TX);
+	const int blocks = (ntiles + 7) / 8 * 8;
+	hipLaunchKernelGGL((ac_tblocks<PREC, TX>), dim3(blocks), dim3(256), 0, st, (const S*)a, (S*)b, g, p, tiles_y, ntiles,
+	                   steps);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+template <int PREC, int Q, bool GG>
+static int launch_wave_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	using S = typename Form<PREC>::S;
+	const int strips = (g.n[1] + kWaveOut - 1) / kWaveOut;
+	const int nitems = strips * ((g.n[0] + chunk - 1) / chunk);
+	const int ntiles = (nitems + 3) / 4;
+	const int blocks = (ntiles + 7) / 8 * 8;
+	hipLaunchKernelGGL((wave_strips<PREC, Q, GG>), dim3(blocks), dim3(256), 0, st, (const S*)a, (S*)b, g, p, chunk,
+	                   strips, nitems, ntiles);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+// The wave strips keep four rows of every field in registers: compiled for up
+// to 128 bytes of field values per cell (FP32 up to 32 fields, FP64 up to 16).
 template <int PREC>
-static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, const TuneChoice& t)
+static int launch_wave(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	constexpr bool wide = sizeof(typename Form<PREC>::S) == 4;
+	if (p.model == MMSPX_AC) return launch_wave_q<PREC, 1, false>(g, p, a, b, st, chunk);
+	switch (g.fields) {
+	case 2: return launch_wave_q<PREC, 2, true>(g, p, a, b, st, chunk);
+	case 4: return launch_wave_q<PREC, 4, true>(g, p, a, b, st, chunk);
+	case 8: return launch_wave_q<PREC, 8, true>(g, p, a, b, st, chunk);
+	case 16: return launch_wave_q<PREC, 16, true>(g, p, a, b, st, chunk);
+	case 32:
+		if constexpr (wide) return launch_wave_q<PREC, 32, true>(g, p, a, b, st, chunk);
+		break;
+	}
+	return fail(MMSPX_EUNSUPPORTED, "no wave-strip kernel for this field count and precision");
+}
+
+template <int PREC>
+static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, const TuneChoice& t,
+                       int nsteps)
 {
 	const int chunk = t.chunk > 0 ? t.chunk : 32;
 	switch (p.model) {
 	case MMSPX_CH: return launch_ch_prec<PREC, false>(g, p, a, b, st, t);
 	case MMSPX_CH_DW: return launch_ch_prec<PREC, true>(g, p, a, b, st, t);
 	case MMSPX_AC:
+		if (t.tz == kTbCols) {
+			if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "temporal blocks need periodic boundaries");
+			if (nsteps < 1 || nsteps > kTbMaxSteps) return fail(MMSPX_EINVAL, "temporal blocks take 1..8 steps per pass");
+			return t.ty == 32 ? launch_tb<PREC, 32>(g, p, a, b, st, nsteps) : launch_tb<PREC, 16>(g, p, a, b, st, nsteps);
+		}
+		if (t.tz == kWaveOut) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		if (t.tz == 64) return launch_ac<PREC, 64>(g, p, a, b, st, chunk);
 		if (t.tz == 128) return launch_ac<PREC, 128>(g, p, a, b, st, chunk);
 		return launch_ac<PREC, 256>(g, p, a, b, st, chunk);
 	case MMSPX_GG:
-		// opt 1: 2-D tiles of ty (8 or 16) rows x 64 columns; otherwise marching strips
-		if (t.opt & 1) return t.ty == 16 ? launch_gg_tile<PREC, 16>(g, p, a, b, st) : launch_gg_tile<PREC, 8>(g, p, a, b, st);
+		// ty > 1: 2-D tiles of ty (8 or 16) rows x 64 columns; tz == kWaveOut:
+		// wave strips; otherwise LDS-row marching strips
+		if (t.ty > 1) return t.ty == 16 ? launch_gg_tile<PREC, 16>(g, p, a, b, st) : launch_gg_tile<PREC, 8>(g, p, a, b, st);
+		if (t.tz == kWaveOut) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		if (t.tz == 64) return launch_gg_tz<PREC, 64>(g, p, a, b, st, chunk);
 		if (t.tz == 256 && PREC != MMSPX_F64) return launch_gg_tz<PREC, 256>(g, p, a, b, st, chunk);
 		return launch_gg_tz<PREC, 128>(g, p, a, b, st, chunk);
@@ -289,14 +410,20 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 	return fail(MMSPX_EINVAL, "unknown model");
 }
 
+int fused_steps_per_launch(int model, const TuneChoice& t)
+{
+	if (model == MMSPX_AC && t.tz == kTbCols) return std::max(1, std::min(t.chunk, kTbMaxSteps));
+	return 1;
+}
+
 int launch_fused_step(const Geom& g, int prec, const ModelParams& p, const void* a, void* b, hipStream_t st,
-                      const TuneChoice& t, int64_t* launches)
+                      const TuneChoice& t, int nsteps, int64_t* launches)
 {
 	int rc;
 	switch (prec) {
-	case MMSPX_F64: rc = launch_prec<MMSPX_F64>(g, p, a, b, st, t); break;
-	case MMSPX_F32: rc = launch_prec<MMSPX_F32>(g, p, a, b, st, t); break;
-	case MMSPX_F32REF: rc = launch_prec<MMSPX_F32REF>(g, p, a, b, st, t); break;
+	case MMSPX_F64: rc = launch_prec<MMSPX_F64>(g, p, a, b, st, t, nsteps); break;
+	case MMSPX_F32: rc = launch_prec<MMSPX_F32>(g, p, a, b, st, t, nsteps); break;
+	case MMSPX_F32REF: rc = launch_prec<MMSPX_F32REF>(g, p, a, b, st, t, nsteps); break;
 	default: return fail(MMSPX_EINVAL, "unknown precision");
 	}
 	if (rc == 0) *launches += 1;

@@ -297,6 +297,62 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 	}
 }
 
+// ---- 2-D Allen-Cahn, temporal blocks (periodic): K <= KMAX steps per pass ----
+// A TX x kTbCols tile and its K-cell halo (at most 64 columns: one wave per
+// row) are read once into LDS; the block then takes K steps in LDS, the valid
+// region shrinking by one cell per step (cells near the rim are recomputed by
+// the neighbouring tiles), and writes the tile. Every value at every step is
+// computed from its four neighbours at the previous step in the reference's
+// order (allen-cahn.cpp:68-71), so the result is bitwise that of K single
+// steps, while HBM sees one read and one write per cell per K steps. Rows and
+// columns past the grid's end hold wrapped nodes (consistent periodic images);
+// only stores are masked.
+template <int PREC, int TX, int KMAX, int NT>
+__device__ __forceinline__ void ac_tblock(const typename Form<PREC>::S* __restrict__ a,
+                                          typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                          int x0, int y0, int K)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	constexpr int TY = kTbCols;
+	static_assert(TY + 2 * KMAX <= 64, "a region row must fit one wave");
+	constexpr int NW = NT / 64;
+	__shared__ S buf[2][TX + 2 * KMAX][64];
+	const int nx = g.n[0], ny = g.n[1];
+	const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+	const int RX = TX + 2 * K, RY = TY + 2 * K;
+	const int yc = wrap(y0 - K + l, ny);
+	// the previous tile of a persistent loop may still read the buffers
+	__syncthreads();
+	for (int r = w; r < RX; r += NW)
+		if (l < RY) buf[0][r][l] = a[(int64_t)wrap(x0 - K + r, nx) * ny + yc];
+	__syncthreads();
+	for (int s = 1; s <= K; s++) {
+		const S(*src)[64] = buf[(s - 1) & 1];
+		S(*dst)[64] = buf[s & 1];
+		const bool act = l >= s && l < RY - s;
+		// every lane of a wave runs each row (the column neighbours come from
+		// the adjacent lanes by DPP); lanes 0 and 63 are never in the valid
+		// region, so no lane needs a neighbour from outside the wave
+		for (int r = s + w; r < RX - s; r += NW) {
+			const S y = src[r][l];
+			const S yl = from_prev(y, y), yh = from_next(y, y);
+			typename F::L lap = F::lap_zero();
+			F::lap_add(lap, g.w[0], g.wf[0], src[r + 1][l], y, src[r - 1][l]);
+			F::lap_add(lap, g.w[1], g.wf[1], yh, y, yl);
+			if (act) dst[r][l] = F::ac(y, lap, p);
+		}
+		__syncthreads();
+	}
+	const S(*res)[64] = buf[K & 1];
+	const int y = y0 + l - K;
+	if (l >= K && l < K + TY && y < ny)
+		for (int r = w; r < TX; r += NW) {
+			const int x = x0 + r;
+			if (x < nx) b[(int64_t)x * ny + y] = res[r + K][l];
+		}
+}
+
 // ---- 2-D strips: TZ contiguous columns [y0, y0+TZ) marching rows [xb, xe) ----
 // Each thread owns one column; the axis-0 neighbours stay in registers (a
 // three-row window), the axis-1 neighbours come from a double-buffered LDS
@@ -529,6 +585,72 @@ __device__ __forceinline__ void gg_tile(const typename Form<PREC>::S* __restrict
 		}
 #pragma unroll
 		for (int j = 0; j < RREG; j++) ring[j] = rnext[j];
+	}
+}
+
+// ---- 2-D wave strips: one wave per (strip, chunk) item, any boundary ----------
+// Lane l holds column c = c0 - 1 + l: lanes 1..kWaveOut produce output, lanes 0
+// and 63 hold the strip's halo columns, read like every other column (through
+// check_boundary's mapping outside the grid). Axis-1 neighbours move between
+// lanes by DPP wave shifts; axis-0 neighbours stay in registers (rows x-1, x,
+// x+1, with row x+2 in flight). No LDS and no barriers: each wave runs on its
+// own. GG: Q fields stored field-major, sum_j phi_j^2 formed in the
+// reference's order before the field updates (graingrowth.cpp:91-107);
+// otherwise Allen-Cahn (Q = 1, allen-cahn.cpp:68-71).
+
+template <int PREC, int Q, bool GG>
+__device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restrict__ a,
+                                           typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                           int c0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	const int lane = threadIdx.x & 63;
+	const int nx = g.n[0], ny = g.n[1];
+	const int64_t N = (int64_t)nx * ny;
+	const int c = c0 - 1 + lane;
+	const int cm = edge_map(c, ny, g.b0[1], g.b1[1]);
+	const bool out = lane >= 1 && lane <= kWaveOut && c < ny;
+	auto roff = [&](int x) { return (int64_t)edge_map(x, nx, g.b0[0], g.b1[0]) * ny + cm; };
+	S w0[Q], w1[Q], w2[Q], w3[Q];
+	{
+		const int64_t r0 = roff(xb - 1), r1 = roff(xb), r2 = roff(xb + 1);
+#pragma unroll
+		for (int j = 0; j < Q; j++) {
+			w0[j] = a[j * N + r0];
+			w1[j] = a[j * N + r1];
+			w2[j] = a[j * N + r2];
+			w3[j] = 0;
+		}
+	}
+	for (int x = xb; x < xe; x++) {
+		// row x+2 only when row x+1 is computed here (then x+2 <= nx, at most one past the edge)
+		if (x + 1 < xe) {
+			const int64_t r3 = roff(x + 2);
+#pragma unroll
+			for (int j = 0; j < Q; j++) w3[j] = a[j * N + r3];
+		}
+		typename F::Sum sum = 0;
+		if (GG) {
+#pragma unroll
+			for (int j = 0; j < Q; j++) sum += w1[j] * w1[j];
+		}
+		const int64_t o = (int64_t)x * ny + c;
+#pragma unroll
+		for (int j = 0; j < Q; j++) {
+			const S y = w1[j];
+			const S yl = from_prev(y, y), yh = from_next(y, y);
+			typename F::L lap = F::lap_zero();
+			F::lap_add(lap, g.w[0], g.wf[0], w2[j], y, w0[j]);
+			F::lap_add(lap, g.w[1], g.wf[1], yh, y, yl);
+			if (out) b[j * N + o] = GG ? F::gg(y, sum, lap, p) : F::ac(y, lap, p);
+		}
+#pragma unroll
+		for (int j = 0; j < Q; j++) {
+			w0[j] = w1[j];
+			w1[j] = w2[j];
+			w2[j] = w3[j];
+		}
 	}
 }
 

@@ -25,7 +25,8 @@ def seeded(shape, seed, lo, hi, q=0, f32=True):
     return a.astype(np.float32) if f32 else a
 
 
-@pytest.mark.parametrize("tile", [None, (128, 16, 4, 0), (256, 32, 8, 1024), (64, 8, 16, 300)])
+@pytest.mark.parametrize("tile", [None, (128, 16, 4, 0), (256, 32, 8, 1024), (64, 8, 16, 300), (48, 8, 4, 0),
+                                  (48, 3, 8, 0)])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_batch_allen_cahn_uneven_members(prec, tile):
     shapes = [(64, 96), (33, 47), (128, 128), (16, 200), (300, 20), (7, 5)]
@@ -54,6 +55,8 @@ def test_batch_allen_cahn_boundaries_and_alone_vs_in_mix():
     init = [seeded(s, 60 + m, -1, 1) for m, s in enumerate(shapes)]
     with mx.Batch(shapes, "ac", precision="f32", bcs=bcs) as b:
         b.set_params(params)
+        with pytest.raises(mx.MmspxError):
+            b.set_tile(48, 8, 4, 0)  # temporal blocks need periodic members
         b.set_tile(256, 32, 8, 512)  # runs of consecutive nodes through the boundary mapping
         for m, a in enumerate(init):
             b.upload(m, a)

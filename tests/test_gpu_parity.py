@@ -146,7 +146,7 @@ def test_fused_ch_options_and_directions(prec):
         a = a.astype(np.float32)
     p = [1.0, 1.0, 0.005]
     exp13 = O.ch(a, p, 13, mu=O.MU_DW, form=prec)
-    for tile in [(16, 64, 16), (8, 32, 16), (8, 64, 32), (4, 128, 64)]:
+    for tile in [(16, 64, 16), (8, 32, 16), (8, 64, 32), (4, 128, 64), (32, 64, 12), (16, 64, 22)]:
         for rq in (False, True):
             with mx.Grid(shape, precision=prec) as g:
                 g.set_variant(2)
@@ -155,6 +155,34 @@ def test_fused_ch_options_and_directions(prec):
                 g.upload(a)
                 g.step("ch_dw", p, 13)
                 assert np.array_equal(g.download(), exp13), (tile, rq)
+    # register queues on 32 x 64 tiles with 768 and 1024 threads: partial
+    # tiles in both axes, a short last chunk, a grid smaller than one tile
+    small = (12, 10, 20)
+    a_s = O.uniform(int(np.prod(small)), 79, 0.45, 0.55).reshape(small)
+    if prec == "f32":
+        a_s = a_s.astype(np.float32)
+    exp_s = O.ch(a_s, p, 13, mu=O.MU_DW, form=prec)
+    for body in (2, 3):
+        for chunk in (7, 16):
+            for arr, ex in ((a, exp13), (a_s, exp_s)):
+                with mx.Grid(arr.shape, precision=prec) as g:
+                    g.set_variant(2)
+                    g.set_tile(32, 64, chunk)
+                    g.set_tile_options(body=body)
+                    g.upload(arr)
+                    g.step("ch_dw", p, 13)
+                    assert np.array_equal(g.download(), ex), (arr.shape, body, chunk)
+    with mx.Grid(shape, precision=prec) as g:  # the polynomial mu form too
+        g.set_variant(2)
+        g.set_tile(32, 64, 16)
+        g.set_tile_options(body=2)
+        pp = [1.0, 1.0, 1.0, 1.0, 0.005]
+        b0 = O.uniform(int(np.prod(shape)), 80, -1.0, 1.0).reshape(shape)
+        if prec == "f32":
+            b0 = b0.astype(np.float32)
+        g.upload(b0)
+        g.step("ch", pp, 9)
+        assert np.array_equal(g.download(), O.ch(b0, pp, 9, mu=O.MU_POLY, form=prec))
     for shape2 in [(40, 300), (30, 64), (17, 5)]:  # 2-D rows, wider and narrower than a tile
         a2 = O.uniform(int(np.prod(shape2)), 78, 0.45, 0.55).reshape(shape2)
         if prec == "f32":
@@ -198,6 +226,83 @@ def test_grain_growth_2d_tiles(prec):
                 g.upload(a)
                 g.step("gg", [0.01], 9)
                 assert np.array_equal(g.download(), exp), (shape, q, tx)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
+def test_wave_strips(prec):
+    """The 62-column wave strips (2-D Allen-Cahn and grain growth): partial
+    strips, grids narrower than a strip, chunks that leave a short last chunk,
+    every boundary condition; bitwise against the oracle."""
+    ac_cases = [((70, 300), [(N, M), (D, P)]), ((40, 62), None), ((9, 5), [(M, M), (N, D)]), ((33, 125), [(P, P), (M, N)])]
+    for shape, bc in ac_cases:
+        say("[gpu] wave strips ac %s %s" % (shape, prec))
+        a = O.uniform(int(np.prod(shape)), 400, -1.0, 1.0).reshape(shape)
+        if prec != "f64":
+            a = a.astype(np.float32)
+        p = [1.0, 1.0, 1.0, 1.0, 0.01]
+        exp = O.ac(a, p, 11, form=prec, bc=bc)
+        for chunk in (8, 24):
+            with mx.Grid(shape, precision=prec, bc=bc) as g:
+                g.set_variant(2)
+                g.set_tile(1, 62, chunk)
+                g.upload(a)
+                g.step("ac", p, 11)
+                assert np.array_equal(g.download(), exp), (shape, chunk)
+    gg_cases = [((70, 300), 4, [(N, M), (D, P)]), ((40, 150), 16, None), ((9, 130), 8, [(M, M), (N, D)]),
+                ((33, 64), 2, [(P, P), (M, N)])]
+    if prec != "f64":
+        gg_cases.append(((37, 200), 32, [(P, P), (N, P)]))
+    for shape, q, bc in gg_cases:
+        say("[gpu] wave strips gg %s q=%d %s" % (shape, q, prec))
+        a = O.uniform(int(np.prod(shape)) * q, 500 + q, 0.0, 1.0).reshape(shape + (q,))
+        if prec != "f64":
+            a = a.astype(np.float32)
+        exp = O.gg(a, 0.01, 9, form=prec, bc=bc)
+        for chunk in (8, 16):
+            with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+                g.set_variant(2)
+                g.set_tile(1, 62, chunk)
+                g.upload(a)
+                g.step("gg", [0.01], 9)
+                assert np.array_equal(g.download(), exp), (shape, q, chunk)
+    if prec == "f64":  # 32 FP64 fields do not fit the strip's registers: refused, not run
+        with mx.Grid((16, 70), fields=32, vector=True, precision="f64") as g:
+            g.set_variant(2)
+            g.set_tile(1, 62, 8)
+            g.upload(np.zeros((16, 70, 32)))
+            with pytest.raises(mx.MmspxError):
+                g.step("gg", [0.01], 1)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
+def test_ac_temporal_blocks(prec):
+    """Periodic Allen-Cahn temporal blocks (16 or 32 rows x 48 columns, up to
+    8 steps per pass): partial tiles, a grid smaller than one tile, step counts
+    that leave a short last pass, and graph-replayed runs; bitwise against the
+    oracle's single steps."""
+    p = [1.0, 1.0, 1.0, 1.0, 0.01]
+    for shape in [(64, 64), (70, 130), (9, 5), (512, 512)]:
+        say("[gpu] temporal blocks %s %s" % (shape, prec))
+        a = O.uniform(int(np.prod(shape)), 600, -1.0, 1.0).reshape(shape)
+        if prec != "f64":
+            a = a.astype(np.float32)
+        steps = 13 if shape[0] < 512 else 107
+        exp = O.ac(a, p, steps, form=prec)
+        for tx in (16, 32):
+            for k in (4, 8):
+                with mx.Grid(shape, precision=prec) as g:
+                    g.set_variant(2)
+                    g.set_tile(tx, 48, k)
+                    g.upload(a)
+                    g.step("ac", p, steps)
+                    assert np.array_equal(g.download(), exp), (shape, tx, k)
+                    assert g.get_tile()["opt"] == 3
+    with mx.Grid((40, 50), precision=prec, bc=[(N, N), (P, P)]) as g:  # periodic boundaries only
+        g.set_variant(2)
+        g.set_tile(16, 48, 8)
+        g.upload(np.zeros((40, 50)))
+        with pytest.raises(mx.MmspxError):
+            g.step("ac", p, 8)
 
 
 def test_upload_download_layouts():
