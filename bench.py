@@ -240,11 +240,14 @@ def bench_serve(args):
             ids[k] = sv.submit(inits[k], params, r["steps"])
 
     th = threading.Thread(target=submitter)
-    c0 = sv.clock()
     th.start()
     th.join()
     infos = [sv.wait(i, 600) for i in ids]
     st = sv.stats()
+    # every 10th request: its served FP32 result against the same solve run
+    # alone on the plain FP64 single-grid path, which also times that path
+    sample = [k for k in range(n) if k % 10 == 0 and infos[k]["status"] == mx.DONE]
+    served = {k: sv.fetch(ids[k]) for k in sample}
     sv.close()
     done = [i for i in infos if i["status"] == mx.DONE]
     lat = np.array([i["t_done"] - i["t_submit"] for i in done])
@@ -252,6 +255,17 @@ def bench_serve(args):
     mass = [abs(i["mass1"] - i["mass0"]) / abs(i["mass0"]) for i in done]
     value = len(done) / span
     cu = cells / span
+    err, seq_s, seq_cells = 0.0, 0.0, 0
+    for k in sample:
+        r = trace[k]
+        with mx.Grid(r["shape"], precision="f64") as g:
+            g.upload(inits[k].astype(np.float32).astype(np.float64))  # the served start, rounded to float
+            sec, _ = g.step_timed("ch_dw", params, r["steps"])
+            r64 = g.download()
+        seq_s += sec
+        seq_cells += int(np.prod(r["shape"])) * r["steps"]
+        err = max(err, float(np.abs(served[k] - r64).max() / np.abs(r64).max()))
+    seq_est = seq_s * cells / seq_cells if seq_cells else None
     return {
         "metric": "solves/s",
         "value": value,
@@ -278,6 +292,10 @@ def bench_serve(args):
             "cell_updates_per_s": cu,
             "total_cell_updates": cells,
             "max_rel_mass_drift_fp32": float(max(mass)),
+            "fp64_check_sample": len(sample),
+            "max_normwise_err_vs_fp64_sample": err if sample else None,
+            "fp64_sequential_s_estimated": seq_est,
+            "speedup_vs_fp64_sequential_estimated": seq_est / span if seq_est else None,
             "peak_concurrent": st["peak_concurrent"],
             "peak_bytes": st["peak_bytes"],
             "rounds": st["rounds"],
@@ -355,7 +373,6 @@ def main():
         tune_report = g.autotune(w["model"], w["params"], trial_steps=10)
         log(tune_report)
         g.upload(init.astype(g.dtype).reshape(g.host_shape))
-    tile = g.get_tile()
     m0 = g.mass()
 
     g.step(w["model"], w["params"], W)
@@ -364,6 +381,7 @@ def main():
     dev_s, launches = g.step_timed(w["model"], w["params"], K)
     wall_s = time.perf_counter() - t0
     m1 = g.mass()
+    tile = g.get_tile()  # what the timed steps ran
     state = g.download()
     finite = bool(np.all(np.isfinite(state)))
     err64 = None
@@ -400,8 +418,10 @@ def main():
         kname = base + "_generic"
     elif base == "ch":
         kname = "ch_fused"
-    elif tile["opt"] == 2:
+    elif tile["opt"] == 2 and base != "ch":
         kname = "wave_strips"
+    elif tile["opt"] == 3 and base == "ac":
+        kname = "ac_tblocks"
     elif base == "gg" and tile["opt"] == 1:
         kname = "gg_tile2d"
     else:
@@ -433,7 +453,8 @@ def main():
             "launches_per_step": launches / K,
             "wall_s_timed": wall_s,
             "hbm_frac_of_value": value * w["bytes_per_cell"] / (HBM_PEAK_GBS * 1e9),
-            "mass_rel_change": float(np.max(np.abs(m1 - m0) / np.abs(m0))),
+            # conserved by Cahn-Hilliard only
+            "mass_rel_change": float(np.max(np.abs(m1 - m0) / np.abs(m0))) if w["model"] in ("ch", "ch_dw") else None,
             "max_normwise_err_vs_fp64": err64,
             "finite": finite,
         },
