@@ -248,7 +248,7 @@ int mmspx_server_fetch(mmspx_server* sv, int64_t id, double* out, size_t count);
 int mmspx_server_info(mmspx_server* sv, int64_t id, mmspx_request_info* info);
 double mmspx_server_clock(mmspx_server* sv);
 /* Counters: [0] admitted, [1] completed, [2] rejected, [3] rounds, [4] peak bytes,
- * [5] peak concurrent requests, [6] kernel launches. */
+ * [5] peak requests resident at once, [6] kernel launches. */
 int mmspx_server_stats(mmspx_server* sv, int64_t* out, int nout);
 
 #ifdef __cplusplus

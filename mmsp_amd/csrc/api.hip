@@ -378,8 +378,8 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 	// wave strips (body 2); otherwise the Cahn-Hilliard body choice is kept
 	int opt = (s->tuned && s->tile.opt == 1) ? 1 : 0;
 	if (s->geom.dim == 2 && ty > 1 && tz == kTbCols) opt = 3;
-	else if (s->geom.dim == 2 && ty > 1) opt = 1;
 	else if (s->geom.dim == 2 && tz == kWaveOut) opt = 2;
+	else if (s->geom.dim == 2 && ty > 1) opt = 1;
 	else if (s->geom.dim == 2 && s->tuned && s->tile.ty > 1) opt = 0;
 	TuneChoice t{2, ty, tz, chunk, opt};
 	if (!tile_known(s->geom, t)) return fail(MMSPX_EINVAL, "no compiled tile of that shape for this grid");

@@ -113,6 +113,22 @@ __global__ void __launch_bounds__(NT) ch_batch(const BMember* __restrict__ mem, 
 	}
 }
 
+// 2-D Cahn-Hilliard by wave strips: one wave per item (member m, direction
+// y0: 1 marches downward, first column z0, rows [xb, xe)); four waves per
+// workgroup walk the list independently.
+template <int PREC, bool DW>
+__global__ void __launch_bounds__(256) ch_batch_wave(const BMember* __restrict__ mem, const BItem* __restrict__ items,
+                                                     int nitems, int parity)
+{
+	using S = typename Form<PREC>::S;
+	for (int it = blockIdx.x * 4 + (int)(threadIdx.x >> 6); it < nitems; it += gridDim.x * 4) {
+		const BItem w = items[it];
+		const BMember& m = mem[w.m];
+		const int src = (parity + m.phase) & 1;
+		ch_wave<PREC, DW>((const S*)m.buf[src], (S*)m.buf[src ^ 1], m.g, m.p, w.z0, w.xb, w.xe, w.y0 != 0);
+	}
+}
+
 // tile shapes of the batched Cahn-Hilliard kernels (fixed; the strips' width
 // and chunk depth are what the tuner varies). 3-D members run the
 // register-queue body, the faster one at 16 x 64 in both precisions.
@@ -169,6 +185,12 @@ void member_items(int m, const Geom& g, int model, const BatchTile& t, std::vect
 		for (int f = 0; f < g.nodes; f += t.cells) out2.push_back(BItem{m, f, std::min(t.cells, g.nodes - f), 0, 0});
 		return;
 	}
+	if (ch2 && t.tz == kChWaveOut) {
+		for (int xb = 0, k = 0; xb < nx; xb += t.chunk, k++)
+			for (int c0 = 0; c0 < g.n[1]; c0 += kChWaveOut)
+				out2.push_back(BItem{m, k & 1, c0, xb, std::min(xb + t.chunk, nx)});
+		return;
+	}
 	for (int xb = 0; xb < nx; xb += t.chunk)
 		for (int c0 = 0; c0 < g.n[1]; c0 += t.tz) {
 			const int xe = std::min(xb + t.chunk, nx);
@@ -200,6 +222,16 @@ static void launch_strips(int model, int fields, const BMember* mem, const BItem
 }
 
 template <int PREC>
+static void launch_ch2_waves(int model, const BMember* mem, const BItem* items, int n, int parity, int blocks,
+                             hipStream_t st)
+{
+	if (model == MMSPX_CH_DW)
+		hipLaunchKernelGGL((ch_batch_wave<PREC, true>), dim3(blocks), dim3(256), 0, st, mem, items, n, parity);
+	else
+		hipLaunchKernelGGL((ch_batch_wave<PREC, false>), dim3(blocks), dim3(256), 0, st, mem, items, n, parity);
+}
+
+template <int PREC>
 static void launch_ch3(bool dw, const BMember* mem, const BItem* items, int n, int parity, int blocks, hipStream_t st)
 {
 	if (dw)
@@ -219,6 +251,9 @@ static void launch_prec(int model, int fields, const BatchTile& t, const BMember
 		hipLaunchKernelGGL((ac_batch_tb<PREC>), dim3(blocks), dim3(256), 0, st, mem, l2, n2, parity, nsteps);
 	} else if (n2 > 0 && model == MMSPX_AC && t.cells > 0) {
 		hipLaunchKernelGGL((ac_batch_cells<PREC>), dim3(n2), dim3(256), 0, st, mem, l2, parity);
+	} else if (n2 > 0 && (model == MMSPX_CH || model == MMSPX_CH_DW) && t.tz == kChWaveOut) {
+		const int blocks = std::max(1, std::min((n2 + 3) / 4, cus * t.bpc));
+		launch_ch2_waves<PREC>(model, mem, l2, n2, parity, blocks, st);
 	} else if (n2 > 0) {
 		const int blocks = std::max(1, std::min(n2, cus * t.bpc));
 		if (t.tz == 64)
@@ -478,6 +513,8 @@ int mmspx_batch_create(const mmspx_desc* descs, int count, int model, mmspx_batc
 		total += 2 * bytes;
 	}
 	b->params.assign(count, ModelParams{});
+	// Cahn-Hilliard: 2-D members by wave strips (faster than the LDS row strips)
+	if (model == MMSPX_CH || model == MMSPX_CH_DW) b->tile = BatchTile{kChWaveOut, 8, 16, 8, 0};
 	b->cus = device_cus();
 	b->arena_bytes = total;
 	const size_t red = (size_t)b->fields * 2 * kRedBlocks * sizeof(double);
@@ -601,9 +638,11 @@ int mmspx_batch_set_tile(mmspx_batch* b, int tz, int chunk, int blocks_per_cu, i
 		b->tile = t;
 		return upload_plan(b);
 	}
-	if ((tz != 64 && tz != 128 && tz != 256) || chunk < 1 || blocks_per_cu < 1 || cells < 0 ||
-	    (cells > 0 && b->model != MMSPX_AC))
-		return fail(MMSPX_EINVAL, "strip width 64/128/256, positive rows and workgroups; cell runs for Allen-Cahn only");
+	const bool ch = b->model == MMSPX_CH || b->model == MMSPX_CH_DW;
+	if ((tz != 64 && tz != 128 && tz != 256 && !(ch && tz == kChWaveOut)) || chunk < 1 || blocks_per_cu < 1 ||
+	    cells < 0 || (cells > 0 && b->model != MMSPX_AC))
+		return fail(MMSPX_EINVAL, "strip width 64/128/256 (60: Cahn-Hilliard wave strips), positive rows and "
+		                          "workgroups; cell runs for Allen-Cahn only");
 	b->tile = BatchTile{tz, chunk, b->tile.chunk3, blocks_per_cu, cells};
 	return upload_plan(b);
 }
@@ -636,6 +675,9 @@ int mmspx_batch_autotune(mmspx_batch* b, int trial_steps, char* report, size_t r
 	for (int tz : {64, 128, 256})
 		for (int chunk : {16, 32, 64, 128})
 			for (int bpc : {4, 8, 16}) cand.push_back(BatchTile{tz, chunk, 16, bpc, 0});
+	if (b->model == MMSPX_CH || b->model == MMSPX_CH_DW)
+		for (int chunk : {8, 16, 32})
+			for (int bpc : {4, 8}) cand.push_back(BatchTile{kChWaveOut, chunk, 16, bpc, 0});
 	if (b->model == MMSPX_AC)
 		for (int cells : {256, 512, 1024, 2048, 4096}) cand.push_back(BatchTile{256, 32, 16, 8, cells});
 	if (b->model == MMSPX_AC && all_members_periodic(b))

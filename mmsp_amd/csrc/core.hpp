@@ -101,6 +101,8 @@ constexpr int kRedThreads = 256;
 // output columns of one wave strip (2-D Allen-Cahn and grain growth): a wave's
 // 64 lanes hold these columns plus one halo column on each side
 constexpr int kWaveOut = 62;
+// output columns of one Cahn-Hilliard wave strip (two halo lanes each side)
+constexpr int kChWaveOut = 60;
 // temporal blocks (2-D periodic Allen-Cahn): tile columns, and the most steps
 // one pass takes (tile plus halo fits the 64 lanes of a wave)
 constexpr int kTbCols = 48;

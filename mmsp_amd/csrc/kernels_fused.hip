@@ -28,7 +28,7 @@ __device__ __forceinline__ int xcd_tile(int ntiles)
 	return tile < ntiles ? tile : -1;
 }
 
-// BODY 0: LDS rings, 1: register queues
+// BODY 0: LDS rings, 1: register queues, 2: register queues with two planes of loads in flight
 template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, int BODY>
 __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __restrict__ a,
                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
@@ -42,7 +42,9 @@ __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __r
 	const int chunk = t2 / tiles_y;
 	const int xb = chunk * chunk_len;
 	const int xe = min(xb + chunk_len, g.n[0]);
-	if constexpr (BODY == 1)
+	if constexpr (BODY == 2)
+		ch_tile_rq<PREC, DW, DIM, TY, TZ, NT, 2>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
+	else if constexpr (BODY == 1)
 		ch_tile_rq<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
 	else
 		ch_tile<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
@@ -114,6 +116,23 @@ __global__ void __launch_bounds__(256) ac_tblocks(const typename Form<PREC>::S* 
 	ac_tblock<PREC, TX, kTbMaxSteps, 256>(a, b, g, p, tx * TX, ty * kTbCols, steps);
 }
 
+// Cahn-Hilliard wave strips (2-D periodic): one wave per (strip, chunk) item,
+// four waves per workgroup, kChWaveOut columns per strip; odd chunks march
+// downward.
+template <int PREC, bool DW>
+__global__ void __launch_bounds__(256) ch_waves(const typename Form<PREC>::S* __restrict__ a,
+                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
+                                                int chunk_len, int strips, int nitems, int ntiles)
+{
+	const int tile = xcd_tile(ntiles);
+	if (tile < 0) return;
+	const int item = tile * 4 + (int)(threadIdx.x >> 6);
+	if (item >= nitems) return;  // a whole wave; the waves share nothing
+	const int s = item % strips, chunk = item / strips;
+	const int xb = chunk * chunk_len;
+	ch_wave<PREC, DW>(a, b, g, p, s * kChWaveOut, xb, min(xb + chunk_len, g.n[0]), chunk & 1);
+}
+
 static bool all_periodic(const Geom& g)
 {
 	for (int k = 0; k < g.dim; k++)
@@ -154,7 +173,10 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	if (model == MMSPX_GG && g.fields == 32)
 		return prec == MMSPX_F64 ? TuneChoice{2, 8, 64, 8, 1} : TuneChoice{2, 1, kWaveOut, 12, 2};
 	if (model == MMSPX_GG) return TuneChoice{1, 0, 0, 0, 0};
-	if (g.dim == 2) return TuneChoice{2, 1, 64, 16, 0};
+	// 2-D Cahn-Hilliard: wave strips (1024^2 FP64 7.3 us per step against 13.5
+	// us for the LDS row strips; 4096^2 FP64 53 us against 69 us, FP32 34 us
+	// against 56 us); shorter chunks for smaller grids
+	if (g.dim == 2) return TuneChoice{2, 1, kChWaveOut, nodes < ((int64_t)1 << 22) ? 8 : 16, 0};
 	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16, 0};
 	// 256^3: register queues on 32 x 64 tiles, 16-plane chunks. FP64 with 768
 	// threads per tile (54.7 us per step, against 56.9 us with 512 threads,
@@ -170,7 +192,7 @@ bool tile_known(const Geom& g, const TuneChoice& t)
 		return (t.ty == 8 && t.tz == 64) || (t.ty == 16 && t.tz == 64) || (t.ty == 32 && t.tz == 64) ||
 		       (t.ty == 8 && t.tz == 32) || (t.ty == 4 && t.tz == 128);
 	if (g.dim == 2)
-		return (t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256 || t.tz == kWaveOut)) ||
+		return (t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256 || t.tz == kWaveOut || t.tz == kChWaveOut)) ||
 		       ((t.ty == 16 || t.ty == 32) && t.tz == kTbCols && t.chunk <= kTbMaxSteps) ||
 		       (t.opt == 1 && (t.ty == 8 || t.ty == 16) && t.tz == 64);
 	return false;
@@ -198,6 +220,9 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		for (int tz : {64, 128, 256})
 			for (int c : {16, 32, 64, 128})
 				for (int opt = 0; opt < (ch ? 2 : 1); opt++) add(1, tz, c, opt);
+		// Cahn-Hilliard wave strips
+		if (ch)
+			for (int c : {8, 16, 32, 64}) add(1, kChWaveOut, c, 0);
 		// grain growth: 2-D tiles
 		if (model == MMSPX_GG && g.fields > 1) {
 			add(8, 64, 8, 1);
@@ -244,6 +269,20 @@ static int launch_ch(const Geom& g, const ModelParams& p, const void* a, void* b
 }
 
 template <int PREC, bool DW>
+static int launch_ch_waves(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	using S = typename Form<PREC>::S;
+	const int strips = (g.n[1] + kChWaveOut - 1) / kChWaveOut;
+	const int nitems = strips * ((g.n[0] + chunk - 1) / chunk);
+	const int ntiles = (nitems + 3) / 4;
+	const int blocks = (ntiles + 7) / 8 * 8;
+	hipLaunchKernelGGL((ch_waves<PREC, DW>), dim3(blocks), dim3(256), 0, st, (const S*)a, (S*)b, g, p, chunk, strips,
+	                   nitems, ntiles);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+template <int PREC, bool DW>
 static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st,
                           const TuneChoice& t)
 {
@@ -252,7 +291,7 @@ static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, vo
 		// register queues on 32 x 64 tiles with more threads (fewer own points
 		// each): opt 2: 768 threads, opt 3: 1024
 		if (t.ty != 32 || t.tz != 64) return fail(MMSPX_EINVAL, "tile bodies 2 and 3 run 32 x 64 tiles");
-		if (t.opt == 3) return launch_ch_body<PREC, DW, 3, 32, 64, 1024, 1>(g, p, a, b, st, chunk);
+		if (t.opt == 3) return launch_ch_body<PREC, DW, 3, 32, 64, 768, 2>(g, p, a, b, st, chunk);
 		return launch_ch_body<PREC, DW, 3, 32, 64, 768, 1>(g, p, a, b, st, chunk);
 	}
 	if (g.dim == 3) {
@@ -262,7 +301,8 @@ static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, vo
 		if (t.ty == 4 && t.tz == 128) return launch_ch<PREC, DW, 3, 4, 128, 256>(g, p, a, b, st, chunk, t.opt);
 		return launch_ch<PREC, DW, 3, 8, 64, 256>(g, p, a, b, st, chunk, t.opt);
 	}
-	if (t.tz == kWaveOut) return fail(MMSPX_EUNSUPPORTED, "the wave strips run Allen-Cahn and grain growth");
+	if (t.tz == kWaveOut) return fail(MMSPX_EUNSUPPORTED, "the 62-column wave strips run Allen-Cahn and grain growth");
+	if (t.tz == kChWaveOut) return launch_ch_waves<PREC, DW>(g, p, a, b, st, chunk);
 	if (t.tz == 64) return launch_ch<PREC, DW, 2, 1, 64, 64>(g, p, a, b, st, chunk, t.opt);
 	if (t.tz == 128) return launch_ch<PREC, DW, 2, 1, 128, 128>(g, p, a, b, st, chunk, t.opt);
 	return launch_ch<PREC, DW, 2, 1, 256, 256>(g, p, a, b, st, chunk, t.opt);
@@ -401,8 +441,8 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 	case MMSPX_GG:
 		// ty > 1: 2-D tiles of ty (8 or 16) rows x 64 columns; tz == kWaveOut:
 		// wave strips; otherwise LDS-row marching strips
-		if (t.ty > 1) return t.ty == 16 ? launch_gg_tile<PREC, 16>(g, p, a, b, st) : launch_gg_tile<PREC, 8>(g, p, a, b, st);
 		if (t.tz == kWaveOut) return launch_wave<PREC>(g, p, a, b, st, chunk);
+		if (t.ty > 1) return t.ty == 16 ? launch_gg_tile<PREC, 16>(g, p, a, b, st) : launch_gg_tile<PREC, 8>(g, p, a, b, st);
 		if (t.tz == 64) return launch_gg_tz<PREC, 64>(g, p, a, b, st, chunk);
 		if (t.tz == 256 && PREC != MMSPX_F64) return launch_gg_tz<PREC, 256>(g, p, a, b, st, chunk);
 		return launch_gg_tz<PREC, 128>(g, p, a, b, st, chunk);

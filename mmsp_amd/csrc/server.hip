@@ -11,18 +11,20 @@
 //   * the uploader allocates an admitted request's buffers (stream-ordered,
 //     from the server's own pool), copies its state in and takes its FP64
 //     compensated mass; the request is then resident;
-//   * the worker advances all resident requests together in rounds: a round
-//     lasts until the next resident request finishes or `chunk_steps` steps,
-//     and is one persistent launch per step over every member's tiles (2-D
-//     and 3-D members in two launches). It is planned on the host and queued
-//     on the compute stream; up to two rounds are queued ahead, so the device
-//     does not wait for the host between rounds;
+//   * the worker advances resident requests in rounds. Each round takes the
+//     requests with the least remaining work until it holds enough nodes per
+//     step to fill the GPU (plus any request left out for too long); it lasts
+//     until one of them finishes or `chunk_steps` steps, and is one
+//     persistent launch per step over every member's tiles (2-D and 3-D
+//     members in two launches). It is planned on the host and queued on the
+//     compute stream; up to two rounds are queued ahead, so the device does
+//     not wait for the host between rounds;
 //   * when a request's last round is queued, the downloader waits for that
 //     round, takes the FP64 mass of the final state, copies it out and frees
 //     its buffers; only then is its memory counted free for admission.
 // Uploads and downloads therefore overlap the compute of the other requests.
-// Each member's buffers alternate on the global step parity plus a phase set
-// when it joins, so joining mid-stream needs no copy. The tile bodies are the
+// A member's state after s steps sits in buf[s & 1], so joining mid-stream or
+// sitting out rounds needs no copy. The tile bodies are the
 // single-grid kernels' (tiles.hpp): a request's result does not depend on what
 // else was resident.
 #include <algorithm>
@@ -41,6 +43,8 @@ namespace mmspx {
 
 constexpr int kSlots = 4;        // work-list buffers cycled by queued rounds
 constexpr int kRoundsAhead = 2;  // rounds queued on the compute stream at most
+constexpr int64_t kRoundNodes = (int64_t)1 << 25;  // nodes per step a round aims to hold
+constexpr int kMaxIdleRounds = 64;                 // a resident request advances at least this often
 
 struct Request {
 	int64_t id = 0;
@@ -56,8 +60,7 @@ struct Request {
 	double mass0 = 0, mass1 = 0;
 	size_t bytes = 0;     // one state buffer (server precision, padded to 256 B)
 	char* dev = nullptr;  // buf[0] at dev, buf[1] at dev + bytes
-	int phase = 0;
-	int64_t gstep_end = 0;           // global step after its last round
+	int idle_rounds = 0;             // rounds planned since it last advanced
 	hipEvent_t ev_last = nullptr;    // recorded after its last round (or when dropped)
 	std::vector<char> init;          // initial state, until uploaded
 	std::vector<char> out;           // final state
@@ -111,8 +114,8 @@ struct mmspx_server {
 	size_t in_use = 0;
 	int64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
 	int worker_error = 0;
-	// 64-wide 2-D strips: small 2-D requests would leave wider strips mostly idle
-	BatchTile tile{64, 16, 16, 8, 0};
+	// 2-D members by 60-column wave strips of 8 rows, 3-D members by 16-plane chunks
+	BatchTile tile{kChWaveOut, 8, 16, 8, 0};
 
 	// hand-offs between the threads (under mu)
 	std::deque<Request*> up_jobs;   // admitted, to upload
@@ -147,7 +150,7 @@ static void finish(mmspx_server* sv, Request* r, int status)
 static int upload_one(mmspx_server* sv, Request* r, std::vector<double>& part_host)
 {
 	MMSPX_HIP(hipMallocFromPoolAsync((void**)&r->dev, 2 * r->bytes, sv->pool, sv->ust));
-	// the data goes to buf[0]; the worker sets the phase that makes it the source
+	// the data goes to buf[0], the state after 0 steps
 	MMSPX_HIP(hipMemcpyAsync(r->dev, r->init.data(), r->init.size(), hipMemcpyHostToDevice, sv->ust));
 	int rc = launch_reduce(r->g, sv->cfg.precision, MMSPX_MASS, r->p, r->dev, sv->upart, sv->ust);
 	if (rc) return rc;
@@ -197,7 +200,7 @@ static void uploader_main(mmspx_server* sv)
 static int download_one(mmspx_server* sv, Request* r, std::vector<double>& part_host)
 {
 	MMSPX_HIP(hipStreamWaitEvent(sv->dst, r->ev_last, 0));
-	const void* s = r->dev + (((r->gstep_end + r->phase) & 1) ? r->bytes : 0);
+	const void* s = r->dev + ((r->steps & 1) ? r->bytes : 0);
 	int rc = launch_reduce(r->g, sv->cfg.precision, MMSPX_MASS, r->p, s, sv->dpart, sv->dst);
 	if (rc) return rc;
 	MMSPX_HIP(hipMemcpyAsync(part_host.data(), sv->dpart, part_host.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -273,11 +276,36 @@ static int ensure_slot(Slot& sl, int nmem, int nitems)
 	return 0;
 }
 
-// Plans one round over the resident members and queues it on the compute stream;
-// members whose last step it holds are handed to the downloader.
+// Picks the members that advance in the next round: resident requests in
+// order of least remaining work (nodes x steps left) until the round holds
+// kRoundNodes nodes per step, enough to fill the GPU, plus any request that
+// has sat out kMaxIdleRounds rounds. Short requests thus finish in a few
+// rounds instead of sharing every launch with the longest ones, and each
+// launch is still wide enough to keep the chip busy.
+static std::vector<Request*> pick_round(mmspx_server* sv)
+{
+	std::vector<Request*> order = sv->resident;
+	auto left = [](const Request* r) { return (double)r->g.nodes * (double)(r->steps - r->planned); };
+	std::stable_sort(order.begin(), order.end(), [&](const Request* x, const Request* y) { return left(x) < left(y); });
+	std::vector<Request*> pick;
+	int64_t nodes = 0;
+	for (Request* r : order) {
+		if (nodes < kRoundNodes || r->idle_rounds >= kMaxIdleRounds) {
+			pick.push_back(r);
+			nodes += r->g.nodes;
+		}
+	}
+	return pick;
+}
+
+// Plans one round over the picked members and queues it on the compute stream;
+// members whose last step it holds are handed to the downloader. A member's
+// state after s steps sits in buf[s & 1] (buf[0] holds the uploaded state),
+// so members that sat out rounds need no copy: the phase sent with each
+// round maps the launch parity onto the member's own step count.
 static int enqueue_round(mmspx_server* sv)
 {
-	std::vector<Request*>& act = sv->resident;
+	std::vector<Request*> act = pick_round(sv);
 	int steps = sv->cfg.chunk_steps;
 	for (Request* r : act) steps = std::min(steps, r->steps - r->planned);
 	steps = std::max(steps, 0);
@@ -297,7 +325,7 @@ static int enqueue_round(mmspx_server* sv)
 		bm.buf[1] = r->dev + r->bytes;
 		bm.g = r->g;
 		bm.p = r->p;
-		bm.phase = r->phase;
+		bm.phase = (int)((r->planned - sv->gstep) & 1);
 	}
 	const std::vector<BItem> o2 = xcd_order(l2), o3 = xcd_order(l3);
 	std::copy(o2.begin(), o2.end(), sl.h_items);
@@ -319,13 +347,14 @@ static int enqueue_round(mmspx_server* sv)
 	sv->gstep += steps;
 	std::vector<Request*> keep;
 	std::vector<IoJob> fin;
+	for (Request* r : sv->resident) r->idle_rounds += 1;
 	for (Request* r : act) {
+		r->idle_rounds = 0;
 		r->planned += steps;
 		if (r->planned < r->steps) {
 			keep.push_back(r);
 			continue;
 		}
-		r->gstep_end = sv->gstep;
 		MMSPX_HIP(hipEventCreateWithFlags(&r->ev_last, hipEventDisableTiming));
 		MMSPX_HIP(hipEventRecord(r->ev_last, sv->st));
 		fin.push_back({r, false});
@@ -333,7 +362,12 @@ static int enqueue_round(mmspx_server* sv)
 	// a finishing member may be fetched and deleted once downloaded: the round
 	// keeps only the members that stay
 	rd.members = keep;
-	act.swap(keep);
+	{
+		std::vector<Request*> stay;
+		for (Request* r : sv->resident)
+			if (r->planned < r->steps) stay.push_back(r);
+		sv->resident.swap(stay);
+	}
 	sv->rounds.push_back(std::move(rd));
 	std::lock_guard<std::mutex> lk(sv->mu);
 	sv->stats[3] += 1;
@@ -432,9 +466,8 @@ static int worker_pass(mmspx_server* sv)
 			admitted = true;
 		}
 		if (admitted) sv->cv_up.notify_all();
-		// uploaded requests join the next round with their data in buf[0]
+		// uploaded requests are resident from the next round on
 		for (Request* r : sv->ready) {
-			r->phase = (int)(sv->gstep & 1);
 			sv->resident.push_back(r);
 		}
 		sv->ready.clear();

@@ -140,7 +140,8 @@ __device__ __forceinline__ void ch_tile(const typename Form<PREC>::S* __restrict
 	}
 }
 
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+// PD: planes of loads in flight ahead of the window (1: phi(P+2), 2: also phi(P+3)).
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, int PD = 1>
 __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restrict__ a,
                                            typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
                                            int y0, int z0, int xb, int xe)
@@ -226,8 +227,10 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 	};
 
 	S fm[MREG], fc[MREG], fp[MREG], fn[MREG];  // phi at P-1, P, P+1; P+2 in flight
+	S fn2[PD > 1 ? MREG : 1];                  // phi at P+3 in flight (PD 2)
 	S mm[MREG], mc[MREG], mp[MREG];            // mu at P-2, P-1, P
 	S rc[RREG], rn[RREG];                      // ring phi at P; P+1 in flight
+	S rn2[PD > 1 ? RREG : 1];                  // ring phi at P+2 in flight (PD 2)
 #pragma unroll
 	for (int k = 0; k < MREG; k++) fn[k] = mm[k] = mc[k] = mp[k] = 0;
 #pragma unroll
@@ -238,11 +241,20 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 	load_own(xb - 1, fc);
 	load_own(xb, fp);
 	load_ring(xb - 1, rc);
+	if (PD > 1) {
+		load_own(xb + 1, fn);  // xb + 1 <= xe
+		load_ring(xb, rn);
+	}
 
 	for (int P = xb - 1; P <= xe; P++) {
-		// loads for the next iteration go out first
-		if (P + 2 <= xe + 1) load_own(P + 2, fn);
-		if (P + 1 <= xe) load_ring(P + 1, rn);
+		// loads for the next iterations go out first
+		if (PD > 1) {
+			if (P + 3 <= xe + 1) load_own(P + 3, fn2);
+			if (P + 2 <= xe) load_ring(P + 2, rn2);
+		} else {
+			if (P + 2 <= xe + 1) load_own(P + 2, fn);
+			if (P + 1 <= xe) load_ring(P + 1, rn);
+		}
 		S* __restrict__ lphi = sphi[P & 1];
 #pragma unroll
 		for (int k = 0; k < MREG; k++)
@@ -289,12 +301,90 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 			fm[k] = fc[k];
 			fc[k] = fp[k];
 			fp[k] = fn[k];
+			if (PD > 1) fn[k] = fn2[k];
 			mm[k] = mc[k];
 			mc[k] = mp[k];
 		}
 #pragma unroll
-		for (int j = 0; j < RREG; j++) rc[j] = rn[j];
+		for (int j = 0; j < RREG; j++) {
+			rc[j] = rn[j];
+			if (PD > 1) rn[j] = rn2[j];
+		}
 	}
+}
+
+// ---- 2-D Cahn-Hilliard, wave strips (periodic): one wave per (strip, chunk) --
+// Lane l holds column c0 - 2 + l: mu is formed on lanes 1..62 and c' written
+// on lanes 2..61 (kChWaveOut columns per strip); column neighbours come from
+// the adjacent lanes by DPP, row neighbours from a register window: phi at
+// rows x, x+d, x+2d (the next rows in flight) and mu at x-d, x, x+d while row
+// x is written. The same per-node arithmetic as the two-pass form
+// (cahn-hilliard.cpp:69-76), so the same bits; no LDS and no barriers. Odd
+// chunks march downward (d = -1) so that two neighbouring chunks read their
+// shared boundary rows at about the same time.
+template <int PREC, bool DW, bool DOWN>
+__device__ __forceinline__ void ch_wave_dir(const typename Form<PREC>::S* __restrict__ a,
+                                            typename Form<PREC>::S* __restrict__ b, const Geom& g,
+                                            const ModelParams& p, int c0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	constexpr int d = DOWN ? -1 : 1;
+	constexpr int PD = 4;  // rows in flight beyond the window
+	const int lane = threadIdx.x & 63;
+	const int nx = g.n[0], ny = g.n[1];
+	const int c = c0 - 2 + lane;
+	const unsigned cm = (unsigned)wrap(c, ny);
+	const bool out = lane >= 2 && lane < 2 + kChWaveOut && c < ny;
+	auto ld = [&](int x) { return a[(size_t)wrap(x, nx) * ny + cm]; };
+	// mu at row r from phi at r-d (pb), r (pc), r+d (pa)
+	auto mu = [&](S pb, S pc, S pa) {
+		const S zl = from_prev(pc, pc), zh = from_next(pc, pc);
+		typename F::L lap = F::lap_zero();
+		F::lap_add(lap, g.w[0], g.wf[0], DOWN ? pb : pa, pc, DOWN ? pa : pb);
+		F::lap_add(lap, g.w[1], g.wf[1], zh, pc, zl);
+		return DW ? F::mu_dw(pc, lap, p) : F::mu_poly(pc, lap, p);
+	};
+	const int x0 = DOWN ? xe - 1 : xb, x1 = DOWN ? xb - 1 : xe;
+	const int n = xe - xb;  // rows written
+	// phi at x0-2d .. x0+2d, then mu at x0-d and x0
+	const S fm2 = ld(x0 - 2 * d), fm1 = ld(x0 - d);
+	S f0 = ld(x0), f1 = ld(x0 + d), f2 = ld(x0 + 2 * d);
+	// q[k] = phi(x + (3+k)d) in flight; it becomes f2 when row x+(k+1)d is
+	// written, so it is read only if that row is in the chunk
+	S q[PD];
+#pragma unroll
+	for (int k = 0; k < PD; k++) q[k] = (k + 1 < n) ? ld(x0 + (3 + k) * d) : S(0);
+	S mb = mu(fm2, fm1, f0);  // mu(x0 - d)
+	S mc = mu(fm1, f0, f1);   // mu(x0)
+	int i = 0;
+	for (int x = x0; x != x1; x += d, i++) {
+		const S ma = mu(f0, f1, f2);  // mu(x+d)
+		const S zl = from_prev(mc, mc), zh = from_next(mc, mc);
+		typename F::L lap = F::lap_zero();
+		F::lap_add(lap, g.w[0], g.wf[0], DOWN ? mb : ma, mc, DOWN ? ma : mb);
+		F::lap_add(lap, g.w[1], g.wf[1], zh, mc, zl);
+		if (out) b[(size_t)x * ny + c] = F::ch(f0, lap, p);
+		mb = mc;
+		mc = ma;
+		f0 = f1;
+		f1 = f2;
+		f2 = q[0];
+#pragma unroll
+		for (int k = 0; k + 1 < PD; k++) q[k] = q[k + 1];
+		q[PD - 1] = (i + PD + 1 < n) ? ld(x + (3 + PD) * d) : S(0);
+	}
+}
+
+template <int PREC, bool DW>
+__device__ __forceinline__ void ch_wave(const typename Form<PREC>::S* __restrict__ a,
+                                        typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                        int c0, int xb, int xe, bool down)
+{
+	if (down)
+		ch_wave_dir<PREC, DW, true>(a, b, g, p, c0, xb, xe);
+	else
+		ch_wave_dir<PREC, DW, false>(a, b, g, p, c0, xb, xe);
 }
 
 // ---- 2-D Allen-Cahn, temporal blocks (periodic): K <= KMAX steps per pass ----
@@ -607,35 +697,37 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 	using S = typename F::S;
 	const int lane = threadIdx.x & 63;
 	const int nx = g.n[0], ny = g.n[1];
-	const int64_t N = (int64_t)nx * ny;
+	const size_t N = (size_t)nx * ny;
 	const int c = c0 - 1 + lane;
-	const int cm = edge_map(c, ny, g.b0[1], g.b1[1]);
+	const unsigned cm = (unsigned)edge_map(c, ny, g.b0[1], g.b1[1]);
 	const bool out = lane >= 1 && lane <= kWaveOut && c < ny;
-	auto roff = [&](int x) { return (int64_t)edge_map(x, nx, g.b0[0], g.b1[0]) * ny + cm; };
-	S w0[Q], w1[Q], w2[Q], w3[Q];
+	// offsets inside one field plane (fewer than 2^31 nodes)
+	auto roff = [&](int x) { return (unsigned)edge_map(x, nx, g.b0[0], g.b1[0]) * (unsigned)ny + cm; };
+	S w0[Q], w1[Q], w2[Q], w3[Q];  // rows x-1, x, x+1; x+2 in flight
 	{
-		const int64_t r0 = roff(xb - 1), r1 = roff(xb), r2 = roff(xb + 1);
+		const unsigned r0 = roff(xb - 1), r1 = roff(xb), r2 = roff(xb + 1);
 #pragma unroll
 		for (int j = 0; j < Q; j++) {
-			w0[j] = a[j * N + r0];
-			w1[j] = a[j * N + r1];
-			w2[j] = a[j * N + r2];
+			const S* __restrict__ aj = a + j * N;
+			w0[j] = aj[r0];
+			w1[j] = aj[r1];
+			w2[j] = aj[r2];
 			w3[j] = 0;
 		}
 	}
 	for (int x = xb; x < xe; x++) {
 		// row x+2 only when row x+1 is computed here (then x+2 <= nx, at most one past the edge)
 		if (x + 1 < xe) {
-			const int64_t r3 = roff(x + 2);
+			const unsigned r3 = roff(x + 2);
 #pragma unroll
-			for (int j = 0; j < Q; j++) w3[j] = a[j * N + r3];
+			for (int j = 0; j < Q; j++) w3[j] = (a + j * N)[r3];
 		}
 		typename F::Sum sum = 0;
 		if (GG) {
 #pragma unroll
 			for (int j = 0; j < Q; j++) sum += w1[j] * w1[j];
 		}
-		const int64_t o = (int64_t)x * ny + c;
+		const unsigned o = (unsigned)x * (unsigned)ny + (unsigned)c;
 #pragma unroll
 		for (int j = 0; j < Q; j++) {
 			const S y = w1[j];
@@ -643,7 +735,7 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 			typename F::L lap = F::lap_zero();
 			F::lap_add(lap, g.w[0], g.wf[0], w2[j], y, w0[j]);
 			F::lap_add(lap, g.w[1], g.wf[1], yh, y, yl);
-			if (out) b[j * N + o] = GG ? F::gg(y, sum, lap, p) : F::ac(y, lap, p);
+			if (out) (b + j * N)[o] = GG ? F::gg(y, sum, lap, p) : F::ac(y, lap, p);
 		}
 #pragma unroll
 		for (int j = 0; j < Q; j++) {

@@ -71,14 +71,17 @@ def test_batch_allen_cahn_boundaries_and_alone_vs_in_mix():
         assert np.array_equal(got[m], O.ac(init[m], params[m], 20, form="f32", bc=bcs[m]))
 
 
+@pytest.mark.parametrize("tile", [None, (60, 3, 4, 0), (64, 16, 8, 0)])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_batch_cahn_hilliard_2d_and_3d_members(prec):
+def test_batch_cahn_hilliard_2d_and_3d_members(prec, tile):
     shapes = [(64, 64), (20, 36, 40), (100, 70), (16, 16, 16), (33, 65, 70), (40, 600), (9, 20, 300), (70, 257)]
-    say("[gpu] batch CH %s" % prec)
+    say("[gpu] batch CH %s tile=%s" % (prec, tile))
     params = np.array([[1.0, 1.0, 0.005]] * len(shapes))
     init = [seeded(s, 80 + m, 0.45, 0.55, f32=prec == "f32") for m, s in enumerate(shapes)]
     with mx.Batch(shapes, "ch_dw", precision=prec) as b:
         b.set_params(params)
+        if tile:
+            b.set_tile(*tile)  # 2-D members by 60-column wave strips (the default) or LDS row strips
         for m, a in enumerate(init):
             b.upload(m, a)
         m0 = [b.mass(m)[0] for m in range(len(shapes))]

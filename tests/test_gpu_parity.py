@@ -197,6 +197,15 @@ def test_fused_ch_options_and_directions(prec):
                     g.upload(a2)
                     g.step("ch_dw", p, 11)
                     assert np.array_equal(g.download(), exp2), (shape2, tz, rq)
+        # wave strips (60 columns; odd chunks march downward; chunks of 2, 3, 8
+        # rows and one chunk of the whole grid)
+        for chunk in (2, 3, 8, 64):
+            with mx.Grid(shape2, precision=prec) as g:
+                g.set_variant(2)
+                g.set_tile(1, 60, chunk)
+                g.upload(a2)
+                g.step("ch_dw", p, 11)
+                assert np.array_equal(g.download(), exp2), (shape2, chunk)
     exp = O.ch(a, p, 107, mu=O.MU_DW, form=prec)
     with mx.Grid(shape, precision=prec) as g:
         g.set_variant(2)
