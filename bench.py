@@ -372,7 +372,7 @@ def main():
         v = [int(x) for x in args.tile.split(",")]
         g.set_tile(*v[:3])
         if len(v) > 3:
-            g.set_tile_options(bool(v[3]))
+            g.set_tile_options(body=v[3])
     elif args.tune and w["model"] in ("ch_dw", "ch"):
         tune_report = g.autotune(w["model"], w["params"], trial_steps=10)
         log(tune_report)

@@ -281,7 +281,7 @@ class Grid:
 
     def set_tile_options(self, register_queues=False, body=None):
         """Cahn-Hilliard tile body: 0 LDS rings, 1 register queues; 3-D 32 x 64
-        tiles: 2 and 3 register queues on 768 and 1024 threads."""
+        tiles: 2 register queues on 768 threads."""
         check(lib.mmspx_set_tile_options(self._h, int(body) if body is not None else (1 if register_queues else 0)))
 
     def get_tile(self):

@@ -391,7 +391,7 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 int mmspx_set_tile_options(mmspx_state* s, int body)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
-	if (body < 0 || body > (s->geom.dim == 3 ? 3 : 1)) return fail(MMSPX_EINVAL, "unknown tile body for this grid");
+	if (body < 0 || body > (s->geom.dim == 3 ? 2 : 1)) return fail(MMSPX_EINVAL, "unknown tile body for this grid");
 	if (!s->tuned) {
 		s->tile = default_choice(s->geom, s->desc.precision, MMSPX_CH_DW);
 		s->tuned = true;

@@ -28,7 +28,7 @@ __device__ __forceinline__ int xcd_tile(int ntiles)
 	return tile < ntiles ? tile : -1;
 }
 
-// BODY 0: LDS rings, 1: register queues, 2: register queues with two planes of loads in flight
+// BODY 0: LDS rings, 1: register queues
 template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, int BODY>
 __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __restrict__ a,
                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
@@ -42,9 +42,7 @@ __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __r
 	const int chunk = t2 / tiles_y;
 	const int xb = chunk * chunk_len;
 	const int xe = min(xb + chunk_len, g.n[0]);
-	if constexpr (BODY == 2)
-		ch_tile_rq<PREC, DW, DIM, TY, TZ, NT, 2>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
-	else if constexpr (BODY == 1)
+	if constexpr (BODY == 1)
 		ch_tile_rq<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
 	else
 		ch_tile<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
@@ -179,9 +177,11 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	if (g.dim == 2) return TuneChoice{2, 1, kChWaveOut, nodes < ((int64_t)1 << 22) ? 8 : 16, 0};
 	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16, 0};
 	// 256^3: register queues on 32 x 64 tiles, 16-plane chunks. FP64 with 768
-	// threads per tile (54.7 us per step, against 56.9 us with 512 threads,
-	// 57.6 us for 16 x 64 tiles and 64 us for the LDS-ring body's best tile);
-	// FP32 with 512 threads (37.8 us, against 38.3 us with 768)
+	// threads per tile (54.1-55.0 us per step, against 55.9-57.1 us with 512
+	// threads, 57.6 us for 16 x 64 tiles, 61.5 us with 1024 threads and 64 us
+	// for the LDS-ring body's best tile; neither a second plane of loads in
+	// flight nor odd chunks marching downward gained); FP32 with 512 threads
+	// (37.0-37.8 us, 36.4-38.6 us with 768: within the noise)
 	return TuneChoice{2, 32, 64, 16, prec == MMSPX_F64 ? 2 : 1};
 }
 
@@ -211,9 +211,9 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 			for (int c : {16, 32, 64})
 				for (int opt = 0; opt < 2; opt++) add(t[0], t[1], c, opt);
 		// 32 x 64: register queues only (the LDS rings would leave one
-		// workgroup per CU), on 512, 768 and 1024 threads
+		// workgroup per CU), on 512 and 768 threads
 		for (int c : {12, 16, 22, 32})
-			for (int body = 1; body <= 3; body++) add(32, 64, c, body);
+			for (int body = 1; body <= 2; body++) add(32, 64, c, body);
 		for (int c : {20, 22, 24}) add(16, 64, c, 1);
 	} else if (g.dim == 2) {
 		// LDS-row strips (Cahn-Hilliard: both tile bodies)
@@ -287,11 +287,9 @@ static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, vo
                           const TuneChoice& t)
 {
 	const int chunk = t.chunk > 0 ? t.chunk : 32;
-	if (g.dim == 3 && t.opt >= 2) {
-		// register queues on 32 x 64 tiles with more threads (fewer own points
-		// each): opt 2: 768 threads, opt 3: 1024
-		if (t.ty != 32 || t.tz != 64) return fail(MMSPX_EINVAL, "tile bodies 2 and 3 run 32 x 64 tiles");
-		if (t.opt == 3) return launch_ch_body<PREC, DW, 3, 32, 64, 768, 2>(g, p, a, b, st, chunk);
+	if (g.dim == 3 && t.opt == 2) {
+		// register queues on 32 x 64 tiles with 768 threads (fewer own points each)
+		if (t.ty != 32 || t.tz != 64) return fail(MMSPX_EINVAL, "tile body 2 runs 32 x 64 tiles");
 		return launch_ch_body<PREC, DW, 3, 32, 64, 768, 1>(g, p, a, b, st, chunk);
 	}
 	if (g.dim == 3) {

@@ -140,8 +140,7 @@ __device__ __forceinline__ void ch_tile(const typename Form<PREC>::S* __restrict
 	}
 }
 
-// PD: planes of loads in flight ahead of the window (1: phi(P+2), 2: also phi(P+3)).
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, int PD = 1>
+template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
 __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restrict__ a,
                                            typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
                                            int y0, int z0, int xb, int xe)
@@ -227,10 +226,8 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 	};
 
 	S fm[MREG], fc[MREG], fp[MREG], fn[MREG];  // phi at P-1, P, P+1; P+2 in flight
-	S fn2[PD > 1 ? MREG : 1];                  // phi at P+3 in flight (PD 2)
 	S mm[MREG], mc[MREG], mp[MREG];            // mu at P-2, P-1, P
 	S rc[RREG], rn[RREG];                      // ring phi at P; P+1 in flight
-	S rn2[PD > 1 ? RREG : 1];                  // ring phi at P+2 in flight (PD 2)
 #pragma unroll
 	for (int k = 0; k < MREG; k++) fn[k] = mm[k] = mc[k] = mp[k] = 0;
 #pragma unroll
@@ -241,20 +238,11 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 	load_own(xb - 1, fc);
 	load_own(xb, fp);
 	load_ring(xb - 1, rc);
-	if (PD > 1) {
-		load_own(xb + 1, fn);  // xb + 1 <= xe
-		load_ring(xb, rn);
-	}
 
 	for (int P = xb - 1; P <= xe; P++) {
-		// loads for the next iterations go out first
-		if (PD > 1) {
-			if (P + 3 <= xe + 1) load_own(P + 3, fn2);
-			if (P + 2 <= xe) load_ring(P + 2, rn2);
-		} else {
-			if (P + 2 <= xe + 1) load_own(P + 2, fn);
-			if (P + 1 <= xe) load_ring(P + 1, rn);
-		}
+		// loads for the next iteration go out first
+		if (P + 2 <= xe + 1) load_own(P + 2, fn);
+		if (P + 1 <= xe) load_ring(P + 1, rn);
 		S* __restrict__ lphi = sphi[P & 1];
 #pragma unroll
 		for (int k = 0; k < MREG; k++)
@@ -301,15 +289,11 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 			fm[k] = fc[k];
 			fc[k] = fp[k];
 			fp[k] = fn[k];
-			if (PD > 1) fn[k] = fn2[k];
 			mm[k] = mc[k];
 			mc[k] = mp[k];
 		}
 #pragma unroll
-		for (int j = 0; j < RREG; j++) {
-			rc[j] = rn[j];
-			if (PD > 1) rn[j] = rn2[j];
-		}
+		for (int j = 0; j < RREG; j++) rc[j] = rn[j];
 	}
 }
 
