@@ -147,15 +147,17 @@ void mmspx_fill_uniform(double* out, int64_t n, uint64_t seed, double lo, double
 int mmspx_set_variant(mmspx_state* s, int variant);
 /* Tile of the fused kernels: ty x tz cells of the (axis 1, last axis) plane,
  * marching `chunk` planes along axis 0. 2-D grids: ty = 1 selects strips of
- * tz columns (tz = 62: the wave strips of Allen-Cahn and grain growth);
- * ty = 8 or 16 with tz = 64 the 2-D grain-growth tiles; ty = 16 or 32 with
- * tz = 48 the temporal blocks of periodic Allen-Cahn, which take `chunk`
- * (at most 8) steps per pass over HBM. */
+ * tz columns (tz = 60: the wave strips of Cahn-Hilliard); ty = 8 or 16 with
+ * tz = 64 the 2-D grain-growth tiles; ty = 16 or 32 with tz = 48 the
+ * temporal blocks of periodic Allen-Cahn, which take `chunk` (at most 8)
+ * steps per pass over HBM. */
 int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
-/* Fused Cahn-Hilliard tile body, all with the same bits: 0 keeps every
- * neighbour in LDS rings; 1 keeps the axis-0 neighbours in per-thread
- * register queues; 3-D 32 x 64 tiles only: 2 is the register queues with 768
- * threads per tile (fewer points, fewer registers each). */
+/* Tile body, all with the same bits. Cahn-Hilliard: 0 keeps every neighbour
+ * in LDS rings; 1 keeps the axis-0 neighbours in per-thread register queues;
+ * 3-D 32 x 64 tiles only: 2 is the register queues with 768 threads per tile
+ * (fewer points, fewer registers each). 2-D Allen-Cahn and grain growth with
+ * 1 x 64 strips: 2 selects the wave strips (one 64-column strip per wave,
+ * neighbours across lanes by DPP, no LDS). */
 int mmspx_set_tile_options(mmspx_state* s, int body);
 /* Returns variant + 16 * tile body (1: register queues, or the 2-D
  * grain-growth tile; 2: wave strips, or 3-D register queues on 768 threads;
@@ -217,7 +219,8 @@ typedef struct mmspx_server mmspx_server;
 typedef struct mmspx_server_config {
 	size_t hbm_budget;   /* bytes of device memory the server may hold for request state */
 	int max_slots;       /* most requests resident at once */
-	int chunk_steps;     /* steps per scheduling round before admitting/retiring */
+	int chunk_steps;     /* most steps in one round; a round advances the resident requests with the
+	                        least remaining work (nodes x steps left), as many as fill the GPU */
 	int model;           /* MMSPX_CH_DW, MMSPX_CH or MMSPX_AC */
 	int precision;       /* MMSPX_F32 or MMSPX_F64 */
 } mmspx_server_config;
@@ -233,7 +236,8 @@ typedef struct mmspx_request_info {
 } mmspx_request_info;
 
 int mmspx_server_create(const mmspx_server_config* cfg, mmspx_server** out);
-/* Stops the worker after the current round; pending requests are cancelled. */
+/* Stops the server's threads (the rounds already queued finish on the device)
+ * and frees every request, finished or not. */
 void mmspx_server_destroy(mmspx_server* sv);
 /* Queue a solve. `init` (node-major doubles, converted to the server's
  * precision) is copied before return. params as for mmspx_step. */

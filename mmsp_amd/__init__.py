@@ -280,8 +280,9 @@ class Grid:
         check(lib.mmspx_set_tile(self._h, ty, tz, chunk))
 
     def set_tile_options(self, register_queues=False, body=None):
-        """Cahn-Hilliard tile body: 0 LDS rings, 1 register queues; 3-D 32 x 64
-        tiles: 2 register queues on 768 threads."""
+        """Tile body: Cahn-Hilliard 0 LDS rings, 1 register queues, 2 register
+        queues on 768 threads (3-D 32 x 64 tiles); 2-D Allen-Cahn and grain
+        growth on 1 x 64 strips: 2 wave strips."""
         check(lib.mmspx_set_tile_options(self._h, int(body) if body is not None else (1 if register_queues else 0)))
 
     def get_tile(self):

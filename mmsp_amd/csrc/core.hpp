@@ -98,9 +98,8 @@ namespace mmspx {
 constexpr int kRedBlocks = 512;
 constexpr int kGraphSteps = 50;  // steps per captured graph (even: buffers return to place)
 constexpr int kRedThreads = 256;
-// output columns of one wave strip (2-D Allen-Cahn and grain growth): a wave's
-// 64 lanes hold these columns plus one halo column on each side
-constexpr int kWaveOut = 62;
+// columns of one wave strip (2-D Allen-Cahn and grain growth): one per lane
+constexpr int kWaveCols = 64;
 // output columns of one Cahn-Hilliard wave strip (two halo lanes each side)
 constexpr int kChWaveOut = 60;
 // temporal blocks (2-D periodic Allen-Cahn): tile columns, and the most steps

@@ -74,6 +74,17 @@ __device__ __forceinline__ double from_next(double v, double edge)
 	return __builtin_bit_cast(double, make_int2(dpp_from_next(s.x, e.x), dpp_from_next(s.y, e.y)));
 }
 
+// lane k's value of v, the same in every lane (k a wave-uniform lane index)
+__device__ __forceinline__ float lane_value(float v, int k)
+{
+	return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
+}
+__device__ __forceinline__ double lane_value(double v, int k)
+{
+	const int2 s = __builtin_bit_cast(int2, v);
+	return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(s.x, k), __builtin_amdgcn_readlane(s.y, k)));
+}
+
 // ---- precision forms -----------------------------------------------------
 template <int PREC> struct Form;
 

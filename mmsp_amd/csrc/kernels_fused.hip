@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(NT) gg_tile2d(const typename Form<PREC>::S* __
 }
 
 // Wave strips (2-D Allen-Cahn and grain growth): one wave per (strip, chunk)
-// item, four waves per workgroup, strips kWaveOut columns wide. Items run
+// item, four waves per workgroup, strips kWaveCols columns wide. Items run
 // strip-fastest, so a workgroup's waves and an XCD's run of workgroups cover
 // neighbouring strips of the same rows (shared halo columns in one L2).
 template <int PREC, int Q, bool GG>
@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(256) wave_strips(const typename Form<PREC>::S*
 	if (item >= nitems) return;  // a whole wave; the waves share nothing
 	const int s = item % strips, chunk = item / strips;
 	const int xb = chunk * chunk_len;
-	strip_wave<PREC, Q, GG>(a, b, g, p, s * kWaveOut, xb, min(xb + chunk_len, g.n[0]));
+	strip_wave<PREC, Q, GG>(a, b, g, p, s * kWaveCols, xb, min(xb + chunk_len, g.n[0]));
 }
 
 // Temporal blocks (2-D periodic Allen-Cahn): one workgroup per TX x kTbCols
@@ -163,13 +163,14 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// thread per node, FP32 1.71 against 3.37); other boundaries keep the
 	// one-thread-per-node kernel, which beats the strips at every size swept.
 	if (model == MMSPX_AC) return all_periodic(g) ? TuneChoice{2, 16, kTbCols, 8, 3} : TuneChoice{1, 0, 0, 0, 0};
-	// grain growth with 32 fields (2048^2 FP32): wave strips with 12-row
-	// chunks take 273 us per step against 334 us for the 8 x 64 tile, 388 us
-	// for one thread per node and 530 us for the LDS-row strips. 32 FP64
-	// fields do not fit the strips' registers: the 8 x 64 tile. Other field
-	// counts keep the generic kernel.
+	// grain growth with 32 fields (2048^2 FP32): 64-column wave strips with
+	// 32-row chunks take 260-264 us per step against 272 us for 62-column
+	// strips (halo lanes, misaligned rows), 331-338 us for the 8 x 64 tile,
+	// 388 us for one thread per node and 530 us for the LDS-row strips. 32
+	// FP64 fields do not fit the strips' registers: the 8 x 64 tile. Other
+	// field counts keep the generic kernel.
 	if (model == MMSPX_GG && g.fields == 32)
-		return prec == MMSPX_F64 ? TuneChoice{2, 8, 64, 8, 1} : TuneChoice{2, 1, kWaveOut, 12, 2};
+		return prec == MMSPX_F64 ? TuneChoice{2, 8, 64, 8, 1} : TuneChoice{2, 1, kWaveCols, 32, 2};
 	if (model == MMSPX_GG) return TuneChoice{1, 0, 0, 0, 0};
 	// 2-D Cahn-Hilliard: wave strips (1024^2 FP64 7.3 us per step against 13.5
 	// us for the LDS row strips; 4096^2 FP64 53 us against 69 us, FP32 34 us
@@ -192,7 +193,7 @@ bool tile_known(const Geom& g, const TuneChoice& t)
 		return (t.ty == 8 && t.tz == 64) || (t.ty == 16 && t.tz == 64) || (t.ty == 32 && t.tz == 64) ||
 		       (t.ty == 8 && t.tz == 32) || (t.ty == 4 && t.tz == 128);
 	if (g.dim == 2)
-		return (t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256 || t.tz == kWaveOut || t.tz == kChWaveOut)) ||
+		return (t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256 || t.tz == kChWaveOut)) ||
 		       ((t.ty == 16 || t.ty == 32) && t.tz == kTbCols && t.chunk <= kTbMaxSteps) ||
 		       (t.opt == 1 && (t.ty == 8 || t.ty == 16) && t.tz == 64);
 	return false;
@@ -231,7 +232,7 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		// Allen-Cahn and grain growth: wave strips, where compiled
 		const bool wave = model == MMSPX_AC || (model == MMSPX_GG && (g.fields < 32 || prec != MMSPX_F64));
 		if (wave)
-			for (int c : {8, 16, 24, 32, 64}) add(1, kWaveOut, c, 2);
+			for (int c : {8, 12, 16, 24, 32}) add(1, kWaveCols, c, 2);
 		// periodic Allen-Cahn: temporal blocks (chunk = steps per pass)
 		if (model == MMSPX_AC && all_periodic(g))
 			for (int tx : {16, 32})
@@ -299,7 +300,6 @@ static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, vo
 		if (t.ty == 4 && t.tz == 128) return launch_ch<PREC, DW, 3, 4, 128, 256>(g, p, a, b, st, chunk, t.opt);
 		return launch_ch<PREC, DW, 3, 8, 64, 256>(g, p, a, b, st, chunk, t.opt);
 	}
-	if (t.tz == kWaveOut) return fail(MMSPX_EUNSUPPORTED, "the 62-column wave strips run Allen-Cahn and grain growth");
 	if (t.tz == kChWaveOut) return launch_ch_waves<PREC, DW>(g, p, a, b, st, chunk);
 	if (t.tz == 64) return launch_ch<PREC, DW, 2, 1, 64, 64>(g, p, a, b, st, chunk, t.opt);
 	if (t.tz == 128) return launch_ch<PREC, DW, 2, 1, 128, 128>(g, p, a, b, st, chunk, t.opt);
@@ -389,7 +389,7 @@ template <int PREC, int Q, bool GG>
 static int launch_wave_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
 	using S = typename Form<PREC>::S;
-	const int strips = (g.n[1] + kWaveOut - 1) / kWaveOut;
+	const int strips = (g.n[1] + kWaveCols - 1) / kWaveCols;
 	const int nitems = strips * ((g.n[0] + chunk - 1) / chunk);
 	const int ntiles = (nitems + 3) / 4;
 	const int blocks = (ntiles + 7) / 8 * 8;
@@ -432,14 +432,14 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 			if (nsteps < 1 || nsteps > kTbMaxSteps) return fail(MMSPX_EINVAL, "temporal blocks take 1..8 steps per pass");
 			return t.ty == 32 ? launch_tb<PREC, 32>(g, p, a, b, st, nsteps) : launch_tb<PREC, 16>(g, p, a, b, st, nsteps);
 		}
-		if (t.tz == kWaveOut) return launch_wave<PREC>(g, p, a, b, st, chunk);
+		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		if (t.tz == 64) return launch_ac<PREC, 64>(g, p, a, b, st, chunk);
 		if (t.tz == 128) return launch_ac<PREC, 128>(g, p, a, b, st, chunk);
 		return launch_ac<PREC, 256>(g, p, a, b, st, chunk);
 	case MMSPX_GG:
-		// ty > 1: 2-D tiles of ty (8 or 16) rows x 64 columns; tz == kWaveOut:
-		// wave strips; otherwise LDS-row marching strips
-		if (t.tz == kWaveOut) return launch_wave<PREC>(g, p, a, b, st, chunk);
+		// ty > 1: 2-D tiles of ty (8 or 16) rows x 64 columns; body 2 with 64
+		// columns: wave strips; otherwise LDS-row marching strips
+		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		if (t.ty > 1) return t.ty == 16 ? launch_gg_tile<PREC, 16>(g, p, a, b, st) : launch_gg_tile<PREC, 8>(g, p, a, b, st);
 		if (t.tz == 64) return launch_gg_tz<PREC, 64>(g, p, a, b, st, chunk);
 		if (t.tz == 256 && PREC != MMSPX_F64) return launch_gg_tz<PREC, 256>(g, p, a, b, st, chunk);

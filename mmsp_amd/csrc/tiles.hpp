@@ -663,15 +663,18 @@ __device__ __forceinline__ void gg_tile(const typename Form<PREC>::S* __restrict
 }
 
 // ---- 2-D wave strips: one wave per (strip, chunk) item, any boundary ----------
-// Lane l holds column c = c0 - 1 + l: lanes 1..kWaveOut produce output, lanes 0
-// and 63 hold the strip's halo columns, read like every other column (through
-// check_boundary's mapping outside the grid). Axis-1 neighbours move between
-// lanes by DPP wave shifts; axis-0 neighbours stay in registers (rows x-1, x,
-// x+1, with row x+2 in flight). No LDS and no barriers: each wave runs on its
+// Lane l holds column c0 + l of a 64-column strip (c0 a multiple of 64, so
+// every row segment a wave reads or writes is two whole 128-byte lines).
+// Axis-1 neighbours move between lanes by DPP wave shifts. The two columns
+// beside the strip, for all Q fields of a row, fit one register: lane k < Q
+// loads field k of the column left of the strip, lane Q + k field k of the
+// column right of it, one row ahead like the rest; lanes 0 and 63 take them
+// from there by readlane. Axis-0 neighbours stay in registers (rows x-1, x,
+// x+1, with row x+2 in flight). Neighbours outside the grid come through
+// check_boundary's mapping. No LDS and no barriers: each wave runs on its
 // own. GG: Q fields stored field-major, sum_j phi_j^2 formed in the
 // reference's order before the field updates (graingrowth.cpp:91-107);
 // otherwise Allen-Cahn (Q = 1, allen-cahn.cpp:68-71).
-
 template <int PREC, int Q, bool GG>
 __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restrict__ a,
                                            typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
@@ -679,33 +682,48 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 {
 	using F = Form<PREC>;
 	using S = typename F::S;
+	static_assert(2 * Q <= 64, "the beside-the-strip columns of all fields must fit one register");
 	const int lane = threadIdx.x & 63;
 	const int nx = g.n[0], ny = g.n[1];
 	const size_t N = (size_t)nx * ny;
-	const int c = c0 - 1 + lane;
+	const int c = c0 + lane;
+	const bool out = c < ny;
+	// the lane's own column; lanes past the grid's end hold mapped columns:
+	// the first of them, column ny, is the last column's right neighbour
 	const unsigned cm = (unsigned)edge_map(c, ny, g.b0[1], g.b1[1]);
-	const bool out = lane >= 1 && lane <= kWaveOut && c < ny;
+	// beside-the-strip loads: lane k < 2Q, field k mod Q, left side for k < Q
+	const bool hl = lane < 2 * Q;
+	const unsigned ch = (unsigned)edge_map(lane < Q ? c0 - 1 : c0 + 64, ny, g.b0[1], g.b1[1]);
+	const S* __restrict__ ah = a + (size_t)(hl ? lane % Q : 0) * N;
 	// offsets inside one field plane (fewer than 2^31 nodes)
-	auto roff = [&](int x) { return (unsigned)edge_map(x, nx, g.b0[0], g.b1[0]) * (unsigned)ny + cm; };
+	auto rrow = [&](int x) { return (unsigned)edge_map(x, nx, g.b0[0], g.b1[0]) * (unsigned)ny; };
 	S w0[Q], w1[Q], w2[Q], w3[Q];  // rows x-1, x, x+1; x+2 in flight
+	S h1 = 0, h2 = 0;              // beside-the-strip columns of rows x and x+1
 	{
-		const unsigned r0 = roff(xb - 1), r1 = roff(xb), r2 = roff(xb + 1);
+		const unsigned r0 = rrow(xb - 1), r1 = rrow(xb), r2 = rrow(xb + 1);
 #pragma unroll
 		for (int j = 0; j < Q; j++) {
 			const S* __restrict__ aj = a + j * N;
-			w0[j] = aj[r0];
-			w1[j] = aj[r1];
-			w2[j] = aj[r2];
+			w0[j] = aj[r0 + cm];
+			w1[j] = aj[r1 + cm];
+			w2[j] = aj[r2 + cm];
 			w3[j] = 0;
+		}
+		if (hl) {
+			h1 = ah[r1 + ch];
+			h2 = ah[r2 + ch];
 		}
 	}
 	for (int x = xb; x < xe; x++) {
 		// row x+2 only when row x+1 is computed here (then x+2 <= nx, at most one past the edge)
-		if (x + 1 < xe) {
-			const unsigned r3 = roff(x + 2);
+		const bool more = x + 1 < xe;
+		if (more) {
+			const unsigned r3 = rrow(x + 2);
 #pragma unroll
-			for (int j = 0; j < Q; j++) w3[j] = (a + j * N)[r3];
+			for (int j = 0; j < Q; j++) w3[j] = (a + j * N)[r3 + cm];
 		}
+		S h3 = 0;
+		if (more && hl) h3 = ah[rrow(x + 2) + ch];
 		typename F::Sum sum = 0;
 		if (GG) {
 #pragma unroll
@@ -715,7 +733,7 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 #pragma unroll
 		for (int j = 0; j < Q; j++) {
 			const S y = w1[j];
-			const S yl = from_prev(y, y), yh = from_next(y, y);
+			const S yl = from_prev(y, lane_value(h1, j)), yh = from_next(y, lane_value(h1, Q + j));
 			typename F::L lap = F::lap_zero();
 			F::lap_add(lap, g.w[0], g.wf[0], w2[j], y, w0[j]);
 			F::lap_add(lap, g.w[1], g.wf[1], yh, y, yl);
@@ -727,6 +745,8 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 			w1[j] = w2[j];
 			w2[j] = w3[j];
 		}
+		h1 = h2;
+		h2 = h3;
 	}
 }
 

@@ -239,10 +239,11 @@ def test_grain_growth_2d_tiles(prec):
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
 def test_wave_strips(prec):
-    """The 62-column wave strips (2-D Allen-Cahn and grain growth): partial
+    """The 64-column wave strips (2-D Allen-Cahn and grain growth): partial
     strips, grids narrower than a strip, chunks that leave a short last chunk,
     every boundary condition; bitwise against the oracle."""
-    ac_cases = [((70, 300), [(N, M), (D, P)]), ((40, 62), None), ((9, 5), [(M, M), (N, D)]), ((33, 125), [(P, P), (M, N)])]
+    ac_cases = [((70, 300), [(N, M), (D, P)]), ((40, 64), None), ((9, 5), [(M, M), (N, D)]), ((33, 129), [(P, P), (M, N)]),
+                ((20, 128), [(M, N), (N, M)])]
     for shape, bc in ac_cases:
         say("[gpu] wave strips ac %s %s" % (shape, prec))
         a = O.uniform(int(np.prod(shape)), 400, -1.0, 1.0).reshape(shape)
@@ -253,7 +254,8 @@ def test_wave_strips(prec):
         for chunk in (8, 24):
             with mx.Grid(shape, precision=prec, bc=bc) as g:
                 g.set_variant(2)
-                g.set_tile(1, 62, chunk)
+                g.set_tile(1, 64, chunk)
+                g.set_tile_options(body=2)  # wave strips
                 g.upload(a)
                 g.step("ac", p, 11)
                 assert np.array_equal(g.download(), exp), (shape, chunk)
@@ -270,14 +272,16 @@ def test_wave_strips(prec):
         for chunk in (8, 16):
             with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
                 g.set_variant(2)
-                g.set_tile(1, 62, chunk)
+                g.set_tile(1, 64, chunk)
+                g.set_tile_options(body=2)  # wave strips
                 g.upload(a)
                 g.step("gg", [0.01], 9)
                 assert np.array_equal(g.download(), exp), (shape, q, chunk)
     if prec == "f64":  # 32 FP64 fields do not fit the strip's registers: refused, not run
         with mx.Grid((16, 70), fields=32, vector=True, precision="f64") as g:
             g.set_variant(2)
-            g.set_tile(1, 62, 8)
+            g.set_tile(1, 64, 8)
+            g.set_tile_options(body=2)
             g.upload(np.zeros((16, 70, 32)))
             with pytest.raises(mx.MmspxError):
                 g.step("gg", [0.01], 1)
