@@ -137,6 +137,31 @@ int mmspx_read_dat(const char* filename, int precision, mmspx_state** out);
 /* Reads just the header of a .dat file into *desc (no device work). */
 int mmspx_peek_dat(const char* filename, mmspx_desc* desc);
 
+/* ---- checkpoints that overlap stepping ----
+ * Replaces grid::output (MMSP.grid.cpp:1358-1402, write_buffer :1818-1909) in
+ * the driver loop of MMSP.main.hpp:278-350, which deflates the whole grid on
+ * one core between update() calls. The file has the serial layout (one block)
+ * and reads back through grid::input and the reference's converters.
+ *
+ * mmspx_checkpoint_begin snapshots the state on the state's stream (one extra
+ * state-sized device allocation, stream-ordered) and returns; later
+ * mmspx_step calls on the state run while host threads download and write the
+ * snapshot. `threads` host threads deflate it (0: one per host core): with
+ * one, the zlib stream is compress2 level 9's, byte for byte the reference's
+ * file; with more, the payload is deflated in chunks that form one zlib
+ * stream (identical payload, different compressed bytes). compress=0 writes
+ * a raw block. mmspx_checkpoint_wait joins the writer, returns its status and
+ * the seconds from begin to the file's close, and frees the handle; every
+ * begun checkpoint must be waited for. The state may be stepped, read or
+ * destroyed while its checkpoints are in flight. */
+typedef struct mmspx_checkpoint mmspx_checkpoint;
+int mmspx_checkpoint_begin(mmspx_state* s, const char* filename, int compress, int threads, mmspx_checkpoint** out);
+int mmspx_checkpoint_wait(mmspx_checkpoint* c, double* seconds);
+/* The same writer over a host array of nodes in row-major order, each node's
+ * fields contiguous (no device work). */
+int mmspx_write_dat_host(const mmspx_desc* desc, const void* node_major, const char* filename, int compress,
+                         int threads);
+
 /* Seeded inputs: element i = lo + (hi-lo) * u_i, u_i = top 53 bits of the
  * (i+1)-th SplitMix64 output from state `seed`, over [0,1). */
 void mmspx_fill_uniform(double* out, int64_t n, uint64_t seed, double lo, double hi);

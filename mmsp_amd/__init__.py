@@ -103,6 +103,9 @@ def _load():
         "mmspx_write_dat": (C.c_int, [vp, C.c_char_p, C.c_int]),
         "mmspx_read_dat": (C.c_int, [C.c_char_p, C.c_int, P(vp)]),
         "mmspx_peek_dat": (C.c_int, [C.c_char_p, P(Desc)]),
+        "mmspx_checkpoint_begin": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, P(vp)]),
+        "mmspx_checkpoint_wait": (C.c_int, [vp, dp]),
+        "mmspx_write_dat_host": (C.c_int, [P(Desc), vp, C.c_char_p, C.c_int, C.c_int]),
         "mmspx_fill_uniform": (None, [dp, C.c_int64, C.c_uint64, C.c_double, C.c_double]),
         "mmspx_set_variant": (C.c_int, [vp, C.c_int]),
         "mmspx_set_tile": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
@@ -319,6 +322,50 @@ class Grid:
 
     def write_dat(self, path, compress=True):
         check(lib.mmspx_write_dat(self._h, path.encode(), 1 if compress else 0))
+
+    def checkpoint(self, path, compress=True, threads=0):
+        """Starts a .dat checkpoint of the current state and returns at once;
+        steps issued afterwards overlap the download and the writing. Call
+        wait() on the result (threads: host threads deflating, 0 = one per
+        core; 1 writes compress2's exact bytes)."""
+        h = C.c_void_p()
+        check(lib.mmspx_checkpoint_begin(self._h, path.encode(), 1 if compress else 0, int(threads), C.byref(h)))
+        return Checkpoint(h)
+
+
+class Checkpoint:
+    """A checkpoint in flight (State.checkpoint)."""
+
+    def __init__(self, h):
+        self._h = h
+
+    def wait(self):
+        """Joins the writer; returns the seconds from begin to the file's close."""
+        if self._h is None:
+            raise RuntimeError("checkpoint already waited for")
+        sec = C.c_double()
+        h, self._h = self._h, None
+        check(lib.mmspx_checkpoint_wait(h, C.byref(sec)))
+        return sec.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            lib.mmspx_checkpoint_wait(self._h, None)
+            self._h = None
+
+
+def write_dat_host(path, array, shape, fields=1, vector=False, precision="f64", bc=None, dx=None, g0=None,
+                   compress=True, threads=0):
+    """Writes a host array (nodes in row-major order, each node's fields
+    contiguous) as a serial-layout .dat with `threads` deflating threads."""
+    d = make_desc(shape, fields, vector, precision, bc, dx, g0)
+    dt = np.float64 if d.precision == F64 else np.float32
+    a = np.ascontiguousarray(array, dtype=dt)
+    n = int(np.prod(shape)) * (int(fields) if vector else 1)
+    if a.size != n:
+        raise ValueError("array size %d does not match grid %s x %d" % (a.size, tuple(shape), fields))
+    check(lib.mmspx_write_dat_host(C.byref(d), a.ctypes.data_as(C.c_void_p), path.encode(), 1 if compress else 0,
+                                   int(threads)))
 
 
 class Batch:
