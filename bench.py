@@ -1,7 +1,7 @@
 """Benchmark of the explicit update path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload ch3d|ch3d_f32|ac2d|gg2d|batch_ac|serve|ckpt]
+                    [--workload ch3d|ch3d_f32|ac2d|gg2d|model_b3d|batch_ac|serve|ckpt]
                     [--no-cpu-baseline] [--tune | --tile TY,TZ,CHUNK[,OPT]]
                     [--requests N] [--hbm-budget-gb G]
                     [--graph-steps G] [--no-sequential] [--ckpt-threads T]
@@ -57,6 +57,11 @@ WORKLOADS = {
                  name="allen-cahn-2d", cpu_steps=100),
     "gg2d": dict(model="gg", shape=(2048, 2048), prec="f32", params=[0.05], steps=500, seed=20260103,
                  lo=0.0, hi=0.01, fields=32, bytes_per_cell=256, name="grain-growth-2d-q32", cpu_steps=2),
+    # model B with the example's parameters (model_B.cpp:86-92: r = u = K = M = 1,
+    # dt = 0.01, kT = 0.01, dV = 1, so sigma = sqrt(2 kT / (dt dV))) on the C2 grid
+    "model_b3d": dict(model="ch_noise", shape=(256, 256, 256), prec="f64",
+                      params=[1.0, 1.0, 1.0, 1.0, 0.01, math.sqrt(2.0 * 0.01 / (0.01 * 1.0)), 20260107], steps=2000,
+                      seed=20260108, lo=-0.01, hi=0.01, fields=1, bytes_per_cell=16, name="model-b-3d", cpu_steps=2),
 }
 
 
@@ -71,7 +76,8 @@ def cpu_baseline(w, init):
 
     shape, steps = w["shape"], w["cpu_steps"]
     nodes = int(np.prod(shape))
-    if O.ref_available():
+    ref_bin = {"ch_dw": "ch", "ac": "ac", "gg": "gg", "ch_noise": "mb"}[w["model"]]
+    if O.ref_available() and os.path.exists(os.path.join(O.REF_DIR, "ref_" + ref_bin)):
         with tempfile.TemporaryDirectory() as td:
             fin = os.path.join(td, "in.dat")
             vec = w["fields"] > 1
@@ -79,8 +85,9 @@ def cpu_baseline(w, init):
             if w["prec"] != "f64":
                 data = data.astype(np.float32)
             dat.write_dat(fin, dat.make_grid(data, vector=vec), compress=False)
-            model = {"ch_dw": "ch", "ac": "ac", "gg": "gg"}[w["model"]]
-            variant = "config"
+            model = {"ch_dw": "ch", "ac": "ac", "gg": "gg", "ch_noise": "mb"}[w["model"]]
+            # model B: the example's own update (its parameters are the workload's)
+            variant = "stock" if model == "mb" else "config"
             sec = O.ref_run(model, fin, "-", steps, variant, w["params"], timeout=600)
         kind, what = "reference", "oracle/_ref reference update() compiled from the reference sources"
     else:
@@ -89,6 +96,8 @@ def cpu_baseline(w, init):
             O.ch(init.reshape(shape), w["params"], steps, mu=O.MU_DW)
         elif w["model"] == "ac":
             O.ac(init.reshape(shape), w["params"], steps)
+        elif w["model"] == "ch_noise":
+            O.ch_noise(init.reshape(shape), w["params"], steps)
         else:
             O.gg(init.reshape(shape + (w["fields"],)), w["params"][0], steps)
         sec = time.perf_counter() - t0
@@ -526,7 +535,7 @@ def main():
     per_launch = alg_bytes / lps
     achieved = per_launch / mean_launch_s / 1e9
     value = nodes * K / dev_s
-    base = {"ch_dw": "ch", "ch": "ch", "ac": "ac", "gg": "gg"}[w["model"]]
+    base = {"ch_dw": "ch", "ch": "ch", "ch_noise": "ch", "ac": "ac", "gg": "gg"}[w["model"]]
     if tile["variant"] != 2:
         kname = base + "_generic"
     elif base == "ch":
@@ -568,6 +577,7 @@ def main():
             "hbm_frac_of_value": value * w["bytes_per_cell"] / (HBM_PEAK_GBS * 1e9),
             # conserved by Cahn-Hilliard only
             "mass_rel_change": float(np.max(np.abs(m1 - m0) / np.abs(m0))) if w["model"] in ("ch", "ch_dw") else None,
+            "mass_abs_change": float(np.max(np.abs(m1 - m0))) if w["model"] == "ch_noise" else None,
             "max_normwise_err_vs_fp64": err64,
             "finite": finite,
         },

@@ -69,9 +69,27 @@ enum {
 	                     mu = -r*c + u*c^3 - K*lap(c);  new = c + dt*M*lap(mu) */
 	MMSPX_CH_DW = 2,  /* Cahn-Hilliard, double-well mu.      p = {kappa, M, dt}
 	                     mu = 2c(1-c)(1-2c) - kappa*lap(c); new = c + dt*M*lap(mu) */
-	MMSPX_GG = 3      /* grain growth, graingrowth.cpp:84-105. p = {dt}
+	MMSPX_GG = 3,     /* grain growth, graingrowth.cpp:84-105. p = {dt}
 	                     new_j = phi_j - dt*(-phi_j - phi_j^3 + 2*(phi_j*sum_k phi_k^2 - lap phi_j)) */
+	/* Stochastic forms: a Gaussian noise term xi = sigma*g(seed, step, node)
+	 * added where the reference adds gaussian(0, std). p = {r, u, K, M, dt,
+	 * sigma, seed}; seed an integer in [0, 2^53). */
+	MMSPX_AC_NOISE = 4, /* model A, model_A.cpp:74-105 (sigma = sqrt(2kT/(dt dV)))
+	                       new = phi - dt*M*(-r*phi + u*phi^3 - K*lap(phi) + xi) */
+	MMSPX_CH_NOISE = 5  /* model B, model_B.cpp:74-111 (sigma = sqrt(2kT/(dt dV)));
+	                       spinodal.cpp:74-107 is r = u = K = 1, M = 1/(2 dV), sigma = sqrt(eps dt/dV)
+	                       mu = -r*c + u*c^3 - K*lap(c) + xi;  new = c + dt*M*lap(mu) */
 };
+/* The noise: g(seed, step, node) is a standard normal deviate (float), the
+ * Box-Muller transform of the first two words of one Philox4x32-10 block
+ * (Salmon et al., SC'11) with key {seed low, seed high} and counter {node low,
+ * node high, step low, step high}. `step` counts the state's steps since its
+ * creation (mmspx_step calls continue it; mmspx_autotune leaves it as it
+ * was); `node` is the node's index in the reference's order. The noise does
+ * not depend on the kernel, its tiles or the launch: every path gives the
+ * same bits. The reference's gaussian() (model_A.cpp:13-39) draws from rand()
+ * in node order by the polar method, so trajectories agree in distribution
+ * only. Every precision form; single grids only (not batches or servers). */
 
 /* ---- host array layouts for upload/download ---- */
 enum {

@@ -85,6 +85,8 @@ int model_nparams(int model)
 	case MMSPX_CH: return 5;
 	case MMSPX_CH_DW: return 3;
 	case MMSPX_GG: return 1;
+	case MMSPX_AC_NOISE: return 7;
+	case MMSPX_CH_NOISE: return 7;
 	}
 	return -1;
 }
@@ -99,7 +101,16 @@ int make_params(int model, const double* p, int np, ModelParams* out)
 	ModelParams m;
 	std::memset(&m, 0, sizeof(m));
 	m.model = model;
-	if (model == MMSPX_AC || model == MMSPX_CH) {
+	if (noisy_model(model)) {
+		// p = {r, u, K, M, dt, sigma, seed}
+		if (!(p[5] >= 0.0)) return fail(MMSPX_EINVAL, "the noise amplitude sigma must be >= 0");
+		if (!(p[6] >= 0.0 && p[6] < 9007199254740992.0 && p[6] == std::floor(p[6])))
+			return fail(MMSPX_EINVAL, "the noise seed must be an integer in [0, 2^53)");
+		m.sigma = p[5];
+		m.sigmaf = (float)p[5];
+		m.seed = (uint64_t)p[6];
+	}
+	if (model == MMSPX_AC || model == MMSPX_CH || noisy_model(model)) {
 		// p = {r, u, K, M, dt}; the reference writes dt*M*(...) and -r*phi
 		m.nr = -p[0];
 		m.u = p[1];
@@ -156,7 +167,7 @@ size_t mmspx_footprint(const mmspx_desc* d, int model)
 	size_t nodes = 1;
 	for (int k = 0; k < d->dim; k++) nodes *= (size_t)d->n[k];
 	const size_t q = d->vector ? (size_t)d->fields : 1;
-	size_t bufs = (model == MMSPX_CH || model == MMSPX_CH_DW) ? 3 : 2;
+	size_t bufs = ch_family(model) ? 3 : 2;
 	return bufs * nodes * q * elem + q * 2 * kRedBlocks * sizeof(double);
 }
 
@@ -300,13 +311,14 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	s->last = fused ? tc : TuneChoice{1, 0, 0, 0, 0};
 	s->has_last = true;
 	const int spl = fused ? fused_steps_per_launch(model, tc) : 1;
-	if ((model == MMSPX_CH || model == MMSPX_CH_DW) && !fused && !s->aux) {
+	if (ch_family(model) && !fused && !s->aux) {
 		if (hipMalloc(&s->aux, s->bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");
 	}
 	// Small grids are launch-bound: replay an even number of steps from a
-	// captured graph (the buffers are back in place after each replay).
+	// captured graph (the buffers are back in place after each replay). Not
+	// the stochastic forms: each launch carries its step number.
 	const size_t small_bytes = (size_t)64 << 20;
-	if (steps >= 2 * kGraphSteps && s->bytes <= small_bytes) {
+	if (steps >= 2 * kGraphSteps && s->bytes <= small_bytes && !noisy_model(model)) {
 		const ModelParams& q = s->graph_params;
 		const bool same_params = q.nr == p.nr && q.u == p.u && q.K == p.K && q.dtM == p.dtM && q.kappa == p.kappa &&
 		                         q.dt == p.dt && q.nrf == p.nrf && q.uf == p.uf && q.Kf == p.Kf && q.dtMf == p.dtMf &&
@@ -355,6 +367,7 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	}
 	while (steps > 0) {
 		const int n = std::min(steps, spl);
+		p.step = (uint64_t)s->steps_run;
 		rc = fused ? launch_fused_step(g, s->desc.precision, p, s->cur, s->nxt, s->stream, tc, n, &s->launches)
 		           : launch_generic_step(g, s->desc.precision, p, s->cur, s->nxt, s->aux, s->stream, &s->launches);
 		if (rc) return rc;
@@ -463,6 +476,7 @@ int mmspx_autotune(mmspx_state* s, int model, const double* params, int np, int 
 	const int old_variant = s->variant;
 	const bool old_tuned = s->tuned;
 	const TuneChoice old_tile = s->tile;
+	const int64_t old_steps = s->steps_run;  // the stochastic forms' noise continues from here
 	double best = 1e30;
 	TuneChoice best_t{1, 0, 0, 0, 0};
 	for (int k = -1; k < nc && rc == 0; k++) {
@@ -492,6 +506,7 @@ int mmspx_autotune(mmspx_state* s, int model, const double* params, int np, int 
 	(void)hipStreamSynchronize(s->stream);
 	(void)hipFree(saved);
 	s->variant = old_variant;
+	s->steps_run = old_steps;
 	if (rc) {
 		s->tuned = old_tuned;
 		s->tile = old_tile;

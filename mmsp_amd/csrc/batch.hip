@@ -156,6 +156,9 @@ bool batch_model_ok(int model, const Geom& g, std::string* why)
 			if (g.b0[k] != MMSPX_PERIODIC || g.b1[k] != MMSPX_PERIODIC)
 				return no("batched Cahn-Hilliard needs periodic members");
 		return true;
+	case MMSPX_AC_NOISE:
+	case MMSPX_CH_NOISE:
+		return no("batches do not run the stochastic forms (each member would need its own step count)");
 	}
 	return no("unknown model");
 }

@@ -48,7 +48,16 @@ struct ModelParams {
 	double nr, u, K, dtM, kappa; // nr = -r; dtM = dt*M (left-to-right as in the reference)
 	double dt;                   // grain growth
 	float nrf, uf, Kf, dtMf, kappaf, dtf;
+	// stochastic forms: noise amplitude, Philox key and the state's step
+	// count at the launch (set per launch)
+	double sigma;
+	float sigmaf;
+	uint64_t seed, step;
 };
+
+inline bool noisy_model(int model) { return model == MMSPX_AC_NOISE || model == MMSPX_CH_NOISE; }
+inline bool ch_family(int model) { return model == MMSPX_CH || model == MMSPX_CH_DW || model == MMSPX_CH_NOISE; }
+inline bool ac_family(int model) { return model == MMSPX_AC || model == MMSPX_AC_NOISE; }
 int make_params(int model, const double* p, int np, ModelParams* out);
 int model_nparams(int model);
 

@@ -85,6 +85,87 @@ __device__ __forceinline__ double lane_value(double v, int k)
 	return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(s.x, k), __builtin_amdgcn_readlane(s.y, k)));
 }
 
+// ---- noise of the stochastic forms -------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): ten rounds, each two 32 x 32 -> 64-bit products
+// of words 0 and 2 with the round multipliers, the high halves xored with
+// words 1 and 3 and the round key; the key advances by the Weyl constants
+// between rounds. c is the counter in and the four output words out.
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+	for (int r = 0; r < 10; r++) {
+		const uint32_t m0 = 0xD2511F53u, m1 = 0xCD9E8D57u;
+		const uint32_t lo0 = m0 * c[0], hi0 = __umulhi(m0, c[0]);
+		const uint32_t lo1 = m1 * c[2], hi1 = __umulhi(m1, c[2]);
+		const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+		c[0] = n0;
+		c[1] = lo1;
+		c[2] = n2;
+		c[3] = lo0;
+		k0 += 0x9E3779B9u;
+		k1 += 0xBB67AE85u;
+	}
+}
+
+// natural log of u in (0, 1], a normal float, from +, -, *, / only (so the
+// host restatement gives the same bits): u = m 2^e with m in (1/sqrt 2,
+// sqrt 2], ln m = 2 atanh(s), s = (m-1)/(m+1), |s| <= 0.1716, by its series to
+// s^9 (float).
+__device__ __forceinline__ float ln_unit(float u)
+{
+	const uint32_t bits = __float_as_uint(u);
+	int e = (int)(bits >> 23) - 127;
+	float m = __uint_as_float((bits & 0x7FFFFFu) | 0x3F800000u);
+	if (m > 1.41421356f) {
+		m = m * 0.5f;
+		e += 1;
+	}
+	const float s = (m - 1.0f) / (m + 1.0f), z = s * s;
+	float q = 1.0f / 9.0f;
+	q = 1.0f / 7.0f + z * q;
+	q = 1.0f / 5.0f + z * q;
+	q = 1.0f / 3.0f + z * q;
+	q = 1.0f + z * q;
+	return (float)e * 0.693147182f + 2.0f * s * q;
+}
+
+// cos(2 pi t) for t in [0, 1) a multiple of 2^-24: quadrant and octant
+// symmetries reduce it exactly to sin or cos of theta in [0, pi/4], where the
+// Taylor series to theta^10 (cos) or theta^9 (sin) is below float rounding.
+__device__ __forceinline__ float cos_2pi(float t)
+{
+	const float v = t * 4.0f;
+	const int q = (int)v;
+	const float f = v - (float)q;
+	const float gq = (q & 1) ? 1.0f - f : f;  // cos(2 pi t) = +-cos(gq pi/2)
+	const bool hi = gq > 0.5f;
+	const float h = hi ? 1.0f - gq : gq;      // cos(gq pi/2) = sin(h pi/2) when hi
+	const float th = h * 1.57079633f, z = th * th;
+	const float cs = 1.0f + z * (-1.0f / 2.0f + z * (1.0f / 24.0f + z * (-1.0f / 720.0f +
+	                 z * (1.0f / 40320.0f + z * (-1.0f / 3628800.0f)))));
+	const float sn = th * (1.0f + z * (-1.0f / 6.0f + z * (1.0f / 120.0f + z * (-1.0f / 5040.0f +
+	                 z * (1.0f / 362880.0f)))));
+	const float c = hi ? sn : cs;
+	return (q == 1 || q == 2) ? -c : c;
+}
+
+// Standard normal deviate for (seed, step, node): one Philox4x32-10 block,
+// key {seed low, seed high}, counter {node low, node high, step low, step
+// high}; Box-Muller on its first two words, u1 = (w0 + 1/2) 2^-32 in (0, 1],
+// u2 = (w1 >> 8) 2^-24 in [0, 1): g = sqrt(-2 ln u1) cos(2 pi u2), in float.
+// The reference's gaussian() (model_A.cpp:13-39) transforms uniforms by
+// Marsaglia's polar method, whose rejection loop would leave the lanes of a
+// wave waiting on each other; both give exact normal deviates.
+__device__ __forceinline__ float normal_deviate(uint64_t seed, uint64_t step, uint64_t node)
+{
+	uint32_t c[4] = {(uint32_t)node, (uint32_t)(node >> 32), (uint32_t)step, (uint32_t)(step >> 32)};
+	philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+	const float u1 = ((float)c[0] + 0.5f) * 0x1p-32f;
+	const float u2 = (float)(c[1] >> 8) * 0x1p-24f;
+	return sqrtf(-2.0f * ln_unit(u1)) * cos_2pi(u2);
+}
+
 // ---- precision forms -----------------------------------------------------
 template <int PREC> struct Form;
 
@@ -115,6 +196,17 @@ template <> struct Form<MMSPX_F64> {
 	{
 		return y - p.dt * (-y - cube_cr(y) + 2.0 * (y * sum - lap));
 	}
+	// model_A.cpp:99-100 and model_B.cpp:100-101: noise = gaussian(0, sigma) = 0 + g*sigma
+	__device__ static inline S ac_noise(S y, L lap, float gd, const ModelParams& p)
+	{
+		const double xi = 0.0 + (double)gd * p.sigma;
+		return y - p.dtM * (p.nr * y + p.u * cube_cr(y) - p.K * lap + xi);
+	}
+	__device__ static inline S mu_noise(S c, L lap, float gd, const ModelParams& p)
+	{
+		const double xi = 0.0 + (double)gd * p.sigma;
+		return p.nr * c + p.u * cube_cr(c) - p.K * lap + xi;
+	}
 };
 
 // float storage, float arithmetic in the same order (the fast FP32 path)
@@ -144,6 +236,17 @@ template <> struct Form<MMSPX_F32> {
 	__device__ static inline S gg(S y, Sum sum, L lap, const ModelParams& p)
 	{
 		return y - p.dtf * (-y - y * y * y + 2.0f * (y * sum - lap));
+	}
+	__device__ static inline S ac_noise(S y, L lap, float gd, const ModelParams& p)
+	{
+		const float xi = gd * p.sigmaf;
+		float c3 = y * y * y;
+		return y - p.dtMf * (p.nrf * y + p.uf * c3 - p.Kf * lap + xi);
+	}
+	__device__ static inline S mu_noise(S c, L lap, float gd, const ModelParams& p)
+	{
+		const float xi = gd * p.sigmaf;
+		return p.nrf * c + p.uf * (c * c * c) - p.Kf * lap + xi;
 	}
 };
 
@@ -185,6 +288,38 @@ template <> struct Form<MMSPX_F32REF> {
 		float inner = y * sum - lap;
 		return (float)(d - p.dt * (-d - cube_cr(d) + 2.0 * (double)inner));
 	}
+	// T noise = gaussian(...): the double deviate rounded to float, then
+	// promoted again in the double expression
+	__device__ static inline S ac_noise(S y, L lap, float gd, const ModelParams& p)
+	{
+		double d = y;
+		const float xi = (float)(0.0 + (double)gd * p.sigma);
+		return (float)(d - p.dtM * (p.nr * d + p.u * cube_cr(d) - p.K * (double)lap + (double)xi));
+	}
+	__device__ static inline S mu_noise(S c, L lap, float gd, const ModelParams& p)
+	{
+		double d = c;
+		const float xi = (float)(0.0 + (double)gd * p.sigma);
+		return (float)(p.nr * d + p.u * cube_cr(d) - p.K * (double)lap + (double)xi);
+	}
 };
+
+// mu of the Cahn-Hilliard family: MU 0 the reference's polynomial, 1 the
+// double well, 2 the polynomial plus noise (node = the node's index)
+template <int PREC, int MU>
+__device__ __forceinline__ typename Form<PREC>::S mu_of(typename Form<PREC>::S c, typename Form<PREC>::L lap,
+                                                        const ModelParams& p, uint64_t node)
+{
+	using F = Form<PREC>;
+	if constexpr (MU == 2) {
+		return F::mu_noise(c, lap, normal_deviate(p.seed, p.step, node), p);
+	} else if constexpr (MU == 1) {
+		(void)node;
+		return F::mu_dw(c, lap, p);
+	} else {
+		(void)node;
+		return F::mu_poly(c, lap, p);
+	}
+}
 
 }  // namespace mmspx

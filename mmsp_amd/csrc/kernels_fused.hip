@@ -28,8 +28,9 @@ __device__ __forceinline__ int xcd_tile(int ntiles)
 	return tile < ntiles ? tile : -1;
 }
 
-// BODY 0: LDS rings, 1: register queues
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, int BODY>
+// BODY 0: LDS rings, 1: register queues; MU as mu_of (0 polynomial, 1 double
+// well, 2 polynomial plus noise)
+template <int PREC, int MU, int DIM, int TY, int TZ, int NT, int BODY>
 __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __restrict__ a,
                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
                                                int chunk_len, int tiles_y, int tiles_z, int ntiles)
@@ -43,9 +44,9 @@ __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __r
 	const int xb = chunk * chunk_len;
 	const int xe = min(xb + chunk_len, g.n[0]);
 	if constexpr (BODY == 1)
-		ch_tile_rq<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
+		ch_tile_rq<PREC, MU, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
 	else
-		ch_tile<PREC, DW, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
+		ch_tile<PREC, MU, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
 }
 
 template <int PREC, int TZ>
@@ -87,7 +88,7 @@ __global__ void __launch_bounds__(NT) gg_tile2d(const typename Form<PREC>::S* __
 // item, four waves per workgroup, strips kWaveCols columns wide. Items run
 // strip-fastest, so a workgroup's waves and an XCD's run of workgroups cover
 // neighbouring strips of the same rows (shared halo columns in one L2).
-template <int PREC, int Q, bool GG>
+template <int PREC, int Q, int KIND>
 __global__ void __launch_bounds__(256) wave_strips(const typename Form<PREC>::S* __restrict__ a,
                                                    typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
                                                    int chunk_len, int strips, int nitems, int ntiles)
@@ -98,7 +99,7 @@ __global__ void __launch_bounds__(256) wave_strips(const typename Form<PREC>::S*
 	if (item >= nitems) return;  // a whole wave; the waves share nothing
 	const int s = item % strips, chunk = item / strips;
 	const int xb = chunk * chunk_len;
-	strip_wave<PREC, Q, GG>(a, b, g, p, s * kWaveCols, xb, min(xb + chunk_len, g.n[0]));
+	strip_wave<PREC, Q, KIND>(a, b, g, p, s * kWaveCols, xb, min(xb + chunk_len, g.n[0]));
 }
 
 // Temporal blocks (2-D periodic Allen-Cahn): one workgroup per TX x kTbCols
@@ -117,7 +118,7 @@ __global__ void __launch_bounds__(256) ac_tblocks(const typename Form<PREC>::S* 
 // Cahn-Hilliard wave strips (2-D periodic): one wave per (strip, chunk) item,
 // four waves per workgroup, kChWaveOut columns per strip; odd chunks march
 // downward.
-template <int PREC, bool DW>
+template <int PREC, int MU>
 __global__ void __launch_bounds__(256) ch_waves(const typename Form<PREC>::S* __restrict__ a,
                                                 typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
                                                 int chunk_len, int strips, int nitems, int ntiles)
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(256) ch_waves(const typename Form<PREC>::S* __
 	if (item >= nitems) return;  // a whole wave; the waves share nothing
 	const int s = item % strips, chunk = item / strips;
 	const int xb = chunk * chunk_len;
-	ch_wave<PREC, DW>(a, b, g, p, s * kChWaveOut, xb, min(xb + chunk_len, g.n[0]), chunk & 1);
+	ch_wave<PREC, MU>(a, b, g, p, s * kChWaveOut, xb, min(xb + chunk_len, g.n[0]), chunk & 1);
 }
 
 static bool all_periodic(const Geom& g)
@@ -145,8 +146,10 @@ bool fused_supported(const Geom& g, int prec, int model)
 	(void)prec;
 	switch (model) {
 	case MMSPX_CH:
-	case MMSPX_CH_DW: return g.dim >= 2 && all_periodic(g);
-	case MMSPX_AC: return g.dim == 2;
+	case MMSPX_CH_DW:
+	case MMSPX_CH_NOISE: return g.dim >= 2 && all_periodic(g);
+	case MMSPX_AC:
+	case MMSPX_AC_NOISE: return g.dim == 2;
 	case MMSPX_GG: return g.dim == 2 && gg_q_compiled(g.fields);
 	}
 	return false;
@@ -163,6 +166,9 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// thread per node, FP32 1.71 against 3.37); other boundaries keep the
 	// one-thread-per-node kernel, which beats the strips at every size swept.
 	if (model == MMSPX_AC) return all_periodic(g) ? TuneChoice{2, 16, kTbCols, 8, 3} : TuneChoice{1, 0, 0, 0, 0};
+	// model A: the noise costs arithmetic per node and step, so the wave
+	// strips (one load and one store per node) rather than one thread per node
+	if (model == MMSPX_AC_NOISE) return TuneChoice{2, 1, kWaveCols, 16, 2};
 	// grain growth with 32 fields (2048^2 FP32): 64-column wave strips with
 	// 32-row chunks take 260-264 us per step against 272 us for 62-column
 	// strips (halo lanes, misaligned rows), 331-338 us for the 8 x 64 tile,
@@ -205,7 +211,12 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 	auto add = [&](int ty, int tz, int chunk, int opt) {
 		if (n < max) out[n++] = TuneChoice{2, ty, tz, chunk, opt};
 	};
-	const bool ch = model == MMSPX_CH || model == MMSPX_CH_DW;
+	const bool ch = ch_family(model);
+	if (g.dim == 2 && model == MMSPX_AC_NOISE) {
+		// model A: wave strips only
+		for (int c : {8, 12, 16, 24, 32}) add(1, kWaveCols, c, 2);
+		return n;
+	}
 	if (g.dim == 3) {
 		const int tiles[4][2] = {{16, 64}, {8, 64}, {8, 32}, {4, 128}};
 		for (auto& t : tiles)
@@ -241,7 +252,7 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 	return n;
 }
 
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT, int BODY>
+template <int PREC, int MU, int DIM, int TY, int TZ, int NT, int BODY>
 static int launch_ch_body(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
 	using S = typename Form<PREC>::S;
@@ -254,7 +265,7 @@ static int launch_ch_body(const Geom& g, const ModelParams& p, const void* a, vo
 	const int ntiles = tiles_y * tiles_z * chunks;
 	// round the grid up to a multiple of 8 so the XCD remap applies; extra ids exit
 	const int blocks = (ntiles + 7) / 8 * 8;
-	hipLaunchKernelGGL((ch_fused<PREC, DW, DIM, TY, TZ, NT, BODY>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g,
+	hipLaunchKernelGGL((ch_fused<PREC, MU, DIM, TY, TZ, NT, BODY>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g,
 	                   p, chunk, tiles_y, tiles_z, ntiles);
 	MMSPX_HIP(hipGetLastError());
 	return 0;
@@ -262,14 +273,14 @@ static int launch_ch_body(const Geom& g, const ModelParams& p, const void* a, vo
 
 // opt 1 selects the register-queue body (opt 2 and 3, register queues with
 // more threads per tile, are dispatched in launch_ch_prec)
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+template <int PREC, int MU, int DIM, int TY, int TZ, int NT>
 static int launch_ch(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk, int opt)
 {
-	if (opt == 1) return launch_ch_body<PREC, DW, DIM, TY, TZ, NT, 1>(g, p, a, b, st, chunk);
-	return launch_ch_body<PREC, DW, DIM, TY, TZ, NT, 0>(g, p, a, b, st, chunk);
+	if (opt == 1) return launch_ch_body<PREC, MU, DIM, TY, TZ, NT, 1>(g, p, a, b, st, chunk);
+	return launch_ch_body<PREC, MU, DIM, TY, TZ, NT, 0>(g, p, a, b, st, chunk);
 }
 
-template <int PREC, bool DW>
+template <int PREC, int MU>
 static int launch_ch_waves(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
 	using S = typename Form<PREC>::S;
@@ -277,13 +288,13 @@ static int launch_ch_waves(const Geom& g, const ModelParams& p, const void* a, v
 	const int nitems = strips * ((g.n[0] + chunk - 1) / chunk);
 	const int ntiles = (nitems + 3) / 4;
 	const int blocks = (ntiles + 7) / 8 * 8;
-	hipLaunchKernelGGL((ch_waves<PREC, DW>), dim3(blocks), dim3(256), 0, st, (const S*)a, (S*)b, g, p, chunk, strips,
+	hipLaunchKernelGGL((ch_waves<PREC, MU>), dim3(blocks), dim3(256), 0, st, (const S*)a, (S*)b, g, p, chunk, strips,
 	                   nitems, ntiles);
 	MMSPX_HIP(hipGetLastError());
 	return 0;
 }
 
-template <int PREC, bool DW>
+template <int PREC, int MU>
 static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st,
                           const TuneChoice& t)
 {
@@ -291,19 +302,19 @@ static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, vo
 	if (g.dim == 3 && t.opt == 2) {
 		// register queues on 32 x 64 tiles with 768 threads (fewer own points each)
 		if (t.ty != 32 || t.tz != 64) return fail(MMSPX_EINVAL, "tile body 2 runs 32 x 64 tiles");
-		return launch_ch_body<PREC, DW, 3, 32, 64, 768, 1>(g, p, a, b, st, chunk);
+		return launch_ch_body<PREC, MU, 3, 32, 64, 768, 1>(g, p, a, b, st, chunk);
 	}
 	if (g.dim == 3) {
-		if (t.ty == 16 && t.tz == 64) return launch_ch<PREC, DW, 3, 16, 64, 256>(g, p, a, b, st, chunk, t.opt);
-		if (t.ty == 32 && t.tz == 64) return launch_ch<PREC, DW, 3, 32, 64, 512>(g, p, a, b, st, chunk, t.opt);
-		if (t.ty == 8 && t.tz == 32) return launch_ch<PREC, DW, 3, 8, 32, 256>(g, p, a, b, st, chunk, t.opt);
-		if (t.ty == 4 && t.tz == 128) return launch_ch<PREC, DW, 3, 4, 128, 256>(g, p, a, b, st, chunk, t.opt);
-		return launch_ch<PREC, DW, 3, 8, 64, 256>(g, p, a, b, st, chunk, t.opt);
+		if (t.ty == 16 && t.tz == 64) return launch_ch<PREC, MU, 3, 16, 64, 256>(g, p, a, b, st, chunk, t.opt);
+		if (t.ty == 32 && t.tz == 64) return launch_ch<PREC, MU, 3, 32, 64, 512>(g, p, a, b, st, chunk, t.opt);
+		if (t.ty == 8 && t.tz == 32) return launch_ch<PREC, MU, 3, 8, 32, 256>(g, p, a, b, st, chunk, t.opt);
+		if (t.ty == 4 && t.tz == 128) return launch_ch<PREC, MU, 3, 4, 128, 256>(g, p, a, b, st, chunk, t.opt);
+		return launch_ch<PREC, MU, 3, 8, 64, 256>(g, p, a, b, st, chunk, t.opt);
 	}
-	if (t.tz == kChWaveOut) return launch_ch_waves<PREC, DW>(g, p, a, b, st, chunk);
-	if (t.tz == 64) return launch_ch<PREC, DW, 2, 1, 64, 64>(g, p, a, b, st, chunk, t.opt);
-	if (t.tz == 128) return launch_ch<PREC, DW, 2, 1, 128, 128>(g, p, a, b, st, chunk, t.opt);
-	return launch_ch<PREC, DW, 2, 1, 256, 256>(g, p, a, b, st, chunk, t.opt);
+	if (t.tz == kChWaveOut) return launch_ch_waves<PREC, MU>(g, p, a, b, st, chunk);
+	if (t.tz == 64) return launch_ch<PREC, MU, 2, 1, 64, 64>(g, p, a, b, st, chunk, t.opt);
+	if (t.tz == 128) return launch_ch<PREC, MU, 2, 1, 128, 128>(g, p, a, b, st, chunk, t.opt);
+	return launch_ch<PREC, MU, 2, 1, 256, 256>(g, p, a, b, st, chunk, t.opt);
 }
 
 template <int PREC, int TZ>
@@ -385,7 +396,7 @@ static int launch_tb(const Geom& g, const ModelParams& p, const void* a, void* b
 	return 0;
 }
 
-template <int PREC, int Q, bool GG>
+template <int PREC, int Q, int KIND>
 static int launch_wave_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
 	using S = typename Form<PREC>::S;
@@ -393,7 +404,7 @@ static int launch_wave_q(const Geom& g, const ModelParams& p, const void* a, voi
 	const int nitems = strips * ((g.n[0] + chunk - 1) / chunk);
 	const int ntiles = (nitems + 3) / 4;
 	const int blocks = (ntiles + 7) / 8 * 8;
-	hipLaunchKernelGGL((wave_strips<PREC, Q, GG>), dim3(blocks), dim3(256), 0, st, (const S*)a, (S*)b, g, p, chunk,
+	hipLaunchKernelGGL((wave_strips<PREC, Q, KIND>), dim3(blocks), dim3(256), 0, st, (const S*)a, (S*)b, g, p, chunk,
 	                   strips, nitems, ntiles);
 	MMSPX_HIP(hipGetLastError());
 	return 0;
@@ -405,14 +416,15 @@ template <int PREC>
 static int launch_wave(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
 	constexpr bool wide = sizeof(typename Form<PREC>::S) == 4;
-	if (p.model == MMSPX_AC) return launch_wave_q<PREC, 1, false>(g, p, a, b, st, chunk);
+	if (p.model == MMSPX_AC) return launch_wave_q<PREC, 1, 0>(g, p, a, b, st, chunk);
+	if (p.model == MMSPX_AC_NOISE) return launch_wave_q<PREC, 1, 2>(g, p, a, b, st, chunk);
 	switch (g.fields) {
-	case 2: return launch_wave_q<PREC, 2, true>(g, p, a, b, st, chunk);
-	case 4: return launch_wave_q<PREC, 4, true>(g, p, a, b, st, chunk);
-	case 8: return launch_wave_q<PREC, 8, true>(g, p, a, b, st, chunk);
-	case 16: return launch_wave_q<PREC, 16, true>(g, p, a, b, st, chunk);
+	case 2: return launch_wave_q<PREC, 2, 1>(g, p, a, b, st, chunk);
+	case 4: return launch_wave_q<PREC, 4, 1>(g, p, a, b, st, chunk);
+	case 8: return launch_wave_q<PREC, 8, 1>(g, p, a, b, st, chunk);
+	case 16: return launch_wave_q<PREC, 16, 1>(g, p, a, b, st, chunk);
 	case 32:
-		if constexpr (wide) return launch_wave_q<PREC, 32, true>(g, p, a, b, st, chunk);
+		if constexpr (wide) return launch_wave_q<PREC, 32, 1>(g, p, a, b, st, chunk);
 		break;
 	}
 	return fail(MMSPX_EUNSUPPORTED, "no wave-strip kernel for this field count and precision");
@@ -424,8 +436,13 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 {
 	const int chunk = t.chunk > 0 ? t.chunk : 32;
 	switch (p.model) {
-	case MMSPX_CH: return launch_ch_prec<PREC, false>(g, p, a, b, st, t);
-	case MMSPX_CH_DW: return launch_ch_prec<PREC, true>(g, p, a, b, st, t);
+	case MMSPX_CH: return launch_ch_prec<PREC, 0>(g, p, a, b, st, t);
+	case MMSPX_CH_DW: return launch_ch_prec<PREC, 1>(g, p, a, b, st, t);
+	case MMSPX_CH_NOISE: return launch_ch_prec<PREC, 2>(g, p, a, b, st, t);
+	case MMSPX_AC_NOISE:
+		// model A: wave strips only (the noise is drawn per node and step)
+		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
+		return fail(MMSPX_EUNSUPPORTED, "Allen-Cahn with noise runs the wave strips (1 x 64, body 2) or the generic kernel");
 	case MMSPX_AC:
 		if (t.tz == kTbCols) {
 			if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "temporal blocks need periodic boundaries");

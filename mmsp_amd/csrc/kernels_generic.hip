@@ -56,7 +56,8 @@ __device__ __forceinline__ typename Form<PREC>::L lap_at(const Geom& g, const ty
 	return lap;
 }
 
-template <int DIM, int PREC>
+// NOISE: model A (model_A.cpp:97-101), the deviate drawn for (step, node i)
+template <int DIM, int PREC, bool NOISE>
 __global__ void __launch_bounds__(256) ac_generic(const typename Form<PREC>::S* __restrict__ a,
                                                   typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p)
 {
@@ -64,11 +65,15 @@ __global__ void __launch_bounds__(256) ac_generic(const typename Form<PREC>::S* 
 	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.nodes; i += gridDim.x * blockDim.x) {
 		int x[3];
 		coords<DIM>(g, i, x);
-		b[i] = F::ac(a[i], lap_at<DIM, PREC>(g, a, i, x), p);
+		if constexpr (NOISE)
+			b[i] = F::ac_noise(a[i], lap_at<DIM, PREC>(g, a, i, x), normal_deviate(p.seed, p.step, (uint64_t)i), p);
+		else
+			b[i] = F::ac(a[i], lap_at<DIM, PREC>(g, a, i, x), p);
 	}
 }
 
-template <int DIM, int PREC, bool DW>
+// MU as mu_of: 0 the reference's polynomial, 1 double well, 2 polynomial plus noise
+template <int DIM, int PREC, int MU>
 __global__ void __launch_bounds__(256) ch_mu_generic(const typename Form<PREC>::S* __restrict__ a,
                                                      typename Form<PREC>::S* __restrict__ mu, Geom g, ModelParams p)
 {
@@ -77,7 +82,7 @@ __global__ void __launch_bounds__(256) ch_mu_generic(const typename Form<PREC>::
 		int x[3];
 		coords<DIM>(g, i, x);
 		typename F::L lap = lap_at<DIM, PREC>(g, a, i, x);
-		mu[i] = DW ? F::mu_dw(a[i], lap, p) : F::mu_poly(a[i], lap, p);
+		mu[i] = mu_of<PREC, MU>(a[i], lap, p, (uint64_t)i);
 	}
 }
 
@@ -147,16 +152,25 @@ static int launch_dim_prec(const Geom& g, const ModelParams& p, const void* av, 
 	const int blocks = grid_blocks(g.nodes);
 	switch (p.model) {
 	case MMSPX_AC:
-		hipLaunchKernelGGL((ac_generic<DIM, PREC>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
+		hipLaunchKernelGGL((ac_generic<DIM, PREC, false>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
+		*launches += 1;
+		break;
+	case MMSPX_AC_NOISE:
+		hipLaunchKernelGGL((ac_generic<DIM, PREC, true>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
 		*launches += 1;
 		break;
 	case MMSPX_CH:
-		hipLaunchKernelGGL((ch_mu_generic<DIM, PREC, false>), dim3(blocks), dim3(256), 0, st, a, aux, g, p);
+		hipLaunchKernelGGL((ch_mu_generic<DIM, PREC, 0>), dim3(blocks), dim3(256), 0, st, a, aux, g, p);
+		hipLaunchKernelGGL((ch_update_generic<DIM, PREC>), dim3(blocks), dim3(256), 0, st, a, aux, b, g, p);
+		*launches += 2;
+		break;
+	case MMSPX_CH_NOISE:
+		hipLaunchKernelGGL((ch_mu_generic<DIM, PREC, 2>), dim3(blocks), dim3(256), 0, st, a, aux, g, p);
 		hipLaunchKernelGGL((ch_update_generic<DIM, PREC>), dim3(blocks), dim3(256), 0, st, a, aux, b, g, p);
 		*launches += 2;
 		break;
 	case MMSPX_CH_DW:
-		hipLaunchKernelGGL((ch_mu_generic<DIM, PREC, true>), dim3(blocks), dim3(256), 0, st, a, aux, g, p);
+		hipLaunchKernelGGL((ch_mu_generic<DIM, PREC, 1>), dim3(blocks), dim3(256), 0, st, a, aux, g, p);
 		hipLaunchKernelGGL((ch_update_generic<DIM, PREC>), dim3(blocks), dim3(256), 0, st, a, aux, b, g, p);
 		*launches += 2;
 		break;

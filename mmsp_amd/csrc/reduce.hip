@@ -152,7 +152,8 @@ int launch_reduce(const Geom& g, int prec, int what, const ModelParams& p, const
 		return 0;
 	}
 	EParams ep;
-	if (p.model == MMSPX_AC || p.model == MMSPX_CH) {
+	if (p.model == MMSPX_AC || p.model == MMSPX_CH || noisy_model(p.model)) {
+		// the noisy forms: the free energy of their deterministic part
 		ep.kind = 0;
 		ep.r = -p.nr;
 		ep.u = p.u;

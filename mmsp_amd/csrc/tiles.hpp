@@ -29,7 +29,7 @@ __device__ __forceinline__ int wrap(int v, int n)
 // points in registers, so only the in-plane neighbours go through LDS (one
 // phi plane and one mu plane, each double-buffered): fewer LDS operations per
 // cell and half the LDS footprint, hence more workgroups per CU.
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+template <int PREC, int MU, int DIM, int TY, int TZ, int NT>
 __device__ __forceinline__ void ch_tile(const typename Form<PREC>::S* __restrict__ a,
                                         typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
                                         int y0, int z0, int xb, int xe)
@@ -109,7 +109,11 @@ __device__ __forceinline__ void ch_tile(const typename Form<PREC>::S* __restrict
 					F::lap_add(lap, g.w[0], g.wf[0], fp[q], y, fm[q]);
 					if (DIM == 3) F::lap_add(lap, g.w[1], g.wf[1], fc[q + PZ], y, fc[q - PZ]);
 					F::lap_add(lap, g.w[DIM - 1], g.wf[DIM - 1], fc[q + 1], y, fc[q - 1]);
-					mu[e] = DW ? F::mu_dw(y, lap, p) : F::mu_poly(y, lap, p);
+					const uint64_t node = MU == 2 ? (uint64_t)wrap(P, nx) * plane +
+					                                    (DIM == 3 ? wrap(y0 - HY / 2 + r, ny) * nz : 0) +
+					                                    wrap(z0 - 1 + c, nz)
+					                              : 0;
+					mu[e] = mu_of<PREC, MU>(y, lap, p, node);
 				}
 			}
 		}
@@ -140,7 +144,7 @@ __device__ __forceinline__ void ch_tile(const typename Form<PREC>::S* __restrict
 	}
 }
 
-template <int PREC, bool DW, int DIM, int TY, int TZ, int NT>
+template <int PREC, int MU, int DIM, int TY, int TZ, int NT>
 __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restrict__ a,
                                            typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
                                            int y0, int z0, int xb, int xe)
@@ -262,7 +266,7 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 				F::lap_add(lap, g.w[0], g.wf[0], fp[k], y, fm[k]);
 				if (DIM == 3) F::lap_add(lap, g.w[1], g.wf[1], lphi[q + PZ], y, lphi[q - PZ]);
 				F::lap_add(lap, g.w[DIM - 1], g.wf[DIM - 1], lphi[q + 1], y, lphi[q - 1]);
-				mp[k] = DW ? F::mu_dw(y, lap, p) : F::mu_poly(y, lap, p);
+				mp[k] = mu_of<PREC, MU>(y, lap, p, MU == 2 ? (uint64_t)wrap(P, nx) * plane + ooff[k] : 0);
 				lmu[threadIdx.x + k * NT] = mp[k];
 			}
 		}
@@ -306,7 +310,7 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 // (cahn-hilliard.cpp:69-76), so the same bits; no LDS and no barriers. Odd
 // chunks march downward (d = -1) so that two neighbouring chunks read their
 // shared boundary rows at about the same time.
-template <int PREC, bool DW, bool DOWN>
+template <int PREC, int MU, bool DOWN>
 __device__ __forceinline__ void ch_wave_dir(const typename Form<PREC>::S* __restrict__ a,
                                             typename Form<PREC>::S* __restrict__ b, const Geom& g,
                                             const ModelParams& p, int c0, int xb, int xe)
@@ -322,12 +326,12 @@ __device__ __forceinline__ void ch_wave_dir(const typename Form<PREC>::S* __rest
 	const bool out = lane >= 2 && lane < 2 + kChWaveOut && c < ny;
 	auto ld = [&](int x) { return a[(size_t)wrap(x, nx) * ny + cm]; };
 	// mu at row r from phi at r-d (pb), r (pc), r+d (pa)
-	auto mu = [&](S pb, S pc, S pa) {
+	auto mu = [&](int r, S pb, S pc, S pa) {
 		const S zl = from_prev(pc, pc), zh = from_next(pc, pc);
 		typename F::L lap = F::lap_zero();
 		F::lap_add(lap, g.w[0], g.wf[0], DOWN ? pb : pa, pc, DOWN ? pa : pb);
 		F::lap_add(lap, g.w[1], g.wf[1], zh, pc, zl);
-		return DW ? F::mu_dw(pc, lap, p) : F::mu_poly(pc, lap, p);
+		return mu_of<PREC, MU>(pc, lap, p, MU == 2 ? (uint64_t)wrap(r, nx) * ny + cm : 0);
 	};
 	const int x0 = DOWN ? xe - 1 : xb, x1 = DOWN ? xb - 1 : xe;
 	const int n = xe - xb;  // rows written
@@ -339,11 +343,11 @@ __device__ __forceinline__ void ch_wave_dir(const typename Form<PREC>::S* __rest
 	S q[PD];
 #pragma unroll
 	for (int k = 0; k < PD; k++) q[k] = (k + 1 < n) ? ld(x0 + (3 + k) * d) : S(0);
-	S mb = mu(fm2, fm1, f0);  // mu(x0 - d)
-	S mc = mu(fm1, f0, f1);   // mu(x0)
+	S mb = mu(x0 - d, fm2, fm1, f0);  // mu(x0 - d)
+	S mc = mu(x0, fm1, f0, f1);       // mu(x0)
 	int i = 0;
 	for (int x = x0; x != x1; x += d, i++) {
-		const S ma = mu(f0, f1, f2);  // mu(x+d)
+		const S ma = mu(x + d, f0, f1, f2);  // mu(x+d)
 		const S zl = from_prev(mc, mc), zh = from_next(mc, mc);
 		typename F::L lap = F::lap_zero();
 		F::lap_add(lap, g.w[0], g.wf[0], DOWN ? mb : ma, mc, DOWN ? ma : mb);
@@ -360,15 +364,15 @@ __device__ __forceinline__ void ch_wave_dir(const typename Form<PREC>::S* __rest
 	}
 }
 
-template <int PREC, bool DW>
+template <int PREC, int MU>
 __device__ __forceinline__ void ch_wave(const typename Form<PREC>::S* __restrict__ a,
                                         typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
                                         int c0, int xb, int xe, bool down)
 {
 	if (down)
-		ch_wave_dir<PREC, DW, true>(a, b, g, p, c0, xb, xe);
+		ch_wave_dir<PREC, MU, true>(a, b, g, p, c0, xb, xe);
 	else
-		ch_wave_dir<PREC, DW, false>(a, b, g, p, c0, xb, xe);
+		ch_wave_dir<PREC, MU, false>(a, b, g, p, c0, xb, xe);
 }
 
 // ---- 2-D Allen-Cahn, temporal blocks (periodic): K <= KMAX steps per pass ----
@@ -672,10 +676,11 @@ __device__ __forceinline__ void gg_tile(const typename Form<PREC>::S* __restrict
 // from there by readlane. Axis-0 neighbours stay in registers (rows x-1, x,
 // x+1, with row x+2 in flight). Neighbours outside the grid come through
 // check_boundary's mapping. No LDS and no barriers: each wave runs on its
-// own. GG: Q fields stored field-major, sum_j phi_j^2 formed in the
-// reference's order before the field updates (graingrowth.cpp:91-107);
-// otherwise Allen-Cahn (Q = 1, allen-cahn.cpp:68-71).
-template <int PREC, int Q, bool GG>
+// own. KIND 1, grain growth: Q fields stored field-major, sum_j phi_j^2
+// formed in the reference's order before the field updates
+// (graingrowth.cpp:91-107); KIND 0, Allen-Cahn (Q = 1, allen-cahn.cpp:68-71);
+// KIND 2, model A (Allen-Cahn plus noise, model_A.cpp:97-101).
+template <int PREC, int Q, int KIND>
 __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restrict__ a,
                                            typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
                                            int c0, int xb, int xe)
@@ -725,11 +730,12 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 		S h3 = 0;
 		if (more && hl) h3 = ah[rrow(x + 2) + ch];
 		typename F::Sum sum = 0;
-		if (GG) {
+		if (KIND == 1) {
 #pragma unroll
 			for (int j = 0; j < Q; j++) sum += w1[j] * w1[j];
 		}
 		const unsigned o = (unsigned)x * (unsigned)ny + (unsigned)c;
+		const float gd = KIND == 2 && out ? normal_deviate(p.seed, p.step, o) : 0.0f;
 #pragma unroll
 		for (int j = 0; j < Q; j++) {
 			const S y = w1[j];
@@ -737,7 +743,10 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 			typename F::L lap = F::lap_zero();
 			F::lap_add(lap, g.w[0], g.wf[0], w2[j], y, w0[j]);
 			F::lap_add(lap, g.w[1], g.wf[1], yh, y, yl);
-			if (out) (b + j * N)[o] = GG ? F::gg(y, sum, lap, p) : F::ac(y, lap, p);
+			if (out)
+				(b + j * N)[o] = KIND == 1 ? F::gg(y, sum, lap, p)
+				                 : KIND == 2 ? F::ac_noise(y, lap, gd, p)
+				                             : F::ac(y, lap, p);
 		}
 #pragma unroll
 		for (int j = 0; j < Q; j++) {

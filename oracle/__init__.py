@@ -68,6 +68,20 @@ def lib():
         L.orc_ac_steps_f32ref.argtypes = [G, fp, fp, dp, C.c_int, C.c_int]
         L.orc_ac_steps_f32.argtypes = [G, fp, fp, dp, C.c_int]
         L.orc_ch_steps_f64.argtypes = [G, dp, dp, dp, dp, C.c_int, C.c_int, C.c_int]
+        u32p = P(C.c_uint32)
+        L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.orc_ln_unit.restype = C.c_float
+        L.orc_ln_unit.argtypes = [C.c_float]
+        L.orc_cos_2pi.restype = C.c_float
+        L.orc_cos_2pi.argtypes = [C.c_float]
+        L.orc_normal.restype = C.c_float
+        L.orc_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        L.orc_ac_noise_steps_f64.argtypes = [G, dp, dp, dp, C.c_int, C.c_uint64, C.c_int]
+        L.orc_ac_noise_steps_f32ref.argtypes = [G, fp, fp, dp, C.c_int, C.c_uint64, C.c_int]
+        L.orc_ac_noise_steps_f32.argtypes = [G, fp, fp, dp, C.c_int, C.c_uint64]
+        L.orc_ch_noise_steps_f64.argtypes = [G, dp, dp, dp, dp, C.c_int, C.c_uint64, C.c_int]
+        L.orc_ch_noise_steps_f32ref.argtypes = [G, fp, fp, fp, dp, C.c_int, C.c_uint64, C.c_int]
+        L.orc_ch_noise_steps_f32.argtypes = [G, fp, fp, fp, dp, C.c_int, C.c_uint64]
         L.orc_ch_steps_f32ref.argtypes = [G, fp, fp, fp, dp, C.c_int, C.c_int, C.c_int]
         L.orc_ch_steps_f32.argtypes = [G, fp, fp, fp, dp, C.c_int, C.c_int]
         L.orc_gg_steps_f64.argtypes = [G, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int]
@@ -165,6 +179,71 @@ def ch(a, params, steps, mu=MU_DW, form="f64", cube=CUBE_CR, bc=None, dx=None):
         y, m = np.empty_like(x), np.empty_like(x)
         lib().orc_ch_steps_f32(C.byref(g), _p(x, C.c_float), _p(y, C.c_float), _p(m, C.c_float),
                                _p(p, C.c_double), mu, steps)
+    return x
+
+
+# ---- stochastic forms: p = {r, u, K, M, dt, sigma, seed} ----------------------
+def philox4x32_10(counter, key):
+    c = np.ascontiguousarray(np.asarray(counter, dtype=np.uint32))
+    k = np.ascontiguousarray(np.asarray(key, dtype=np.uint32))
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(_p(c, C.c_uint32), _p(k, C.c_uint32), _p(out, C.c_uint32))
+    return [int(v) for v in out]
+
+
+def ln_unit(u: float) -> float:
+    return lib().orc_ln_unit(u)
+
+
+def cos_2pi(t: float) -> float:
+    return lib().orc_cos_2pi(t)
+
+
+def normal(seed: int, step: int, node: int) -> float:
+    return lib().orc_normal(seed, step, node)
+
+
+def ac_noise(a, params, steps, form="f64", step0=0, cube=CUBE_CR, bc=None, dx=None):
+    """Model A from the state `a`; the noise of step k is drawn for step number step0 + k."""
+    g = grid_desc(a.shape, bc, dx)
+    p = _params(params)
+    if form == "f64":
+        x = np.array(a, dtype=np.float64, order="C")
+        y = np.empty_like(x)
+        lib().orc_ac_noise_steps_f64(C.byref(g), _p(x, C.c_double), _p(y, C.c_double), _p(p, C.c_double), steps,
+                                     step0, cube)
+    elif form == "f32ref":
+        x = np.array(a, dtype=np.float32, order="C")
+        y = np.empty_like(x)
+        lib().orc_ac_noise_steps_f32ref(C.byref(g), _p(x, C.c_float), _p(y, C.c_float), _p(p, C.c_double), steps,
+                                        step0, cube)
+    else:
+        x = np.array(a, dtype=np.float32, order="C")
+        y = np.empty_like(x)
+        lib().orc_ac_noise_steps_f32(C.byref(g), _p(x, C.c_float), _p(y, C.c_float), _p(p, C.c_double), steps,
+                                     step0)
+    return x
+
+
+def ch_noise(a, params, steps, form="f64", step0=0, cube=CUBE_CR, bc=None, dx=None):
+    """Model B from the state `a` (spinodal: r = u = K = 1, M = 1/(2 dV))."""
+    g = grid_desc(a.shape, bc, dx)
+    p = _params(params)
+    if form == "f64":
+        x = np.array(a, dtype=np.float64, order="C")
+        y, m = np.empty_like(x), np.empty_like(x)
+        lib().orc_ch_noise_steps_f64(C.byref(g), _p(x, C.c_double), _p(y, C.c_double), _p(m, C.c_double),
+                                     _p(p, C.c_double), steps, step0, cube)
+    elif form == "f32ref":
+        x = np.array(a, dtype=np.float32, order="C")
+        y, m = np.empty_like(x), np.empty_like(x)
+        lib().orc_ch_noise_steps_f32ref(C.byref(g), _p(x, C.c_float), _p(y, C.c_float), _p(m, C.c_float),
+                                        _p(p, C.c_double), steps, step0, cube)
+    else:
+        x = np.array(a, dtype=np.float32, order="C")
+        y, m = np.empty_like(x), np.empty_like(x)
+        lib().orc_ch_noise_steps_f32(C.byref(g), _p(x, C.c_float), _p(y, C.c_float), _p(m, C.c_float),
+                                     _p(p, C.c_double), steps, step0)
     return x
 
 

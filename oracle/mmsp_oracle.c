@@ -14,6 +14,10 @@
  *   Allen-Cahn update     examples/phase_transitions/allen-cahn/allen-cahn.cpp:58-73
  *   Cahn-Hilliard update  examples/phase_transitions/cahn-hilliard/explicit/cahn-hilliard.cpp:59-80
  *   grain growth update   examples/coarsening/grain_growth/isotropic/phase_field/graingrowth.cpp:84-110
+ *   model A / model B     examples/phase_transitions/model_A/model_A.cpp:74-105,
+ *                         examples/phase_transitions/model_B/model_B.cpp:74-111 (noise: normal
+ *                         deviates as gaussian(), model_A.cpp:13-39, from Philox4x32-10 and
+ *                         Box-Muller instead of rand() and the polar method)
  *
  * Cube: the reference evaluates pow(phi,3) with glibc pow (a call, checked in
  * the disassembly of the reference build). ORC_CUBE_POW reproduces that;
@@ -295,6 +299,202 @@ void orc_ch_steps_f32(const orc_grid* g, float* a, float* b, float* mu, const do
 		const float dt = (float)(form == ORC_MU_POLY ? p[4] : p[2]);
 		for (int64_t i = 0; i < n; i++)
 			b[i] = a[i] + dt * M * orc_laplacian_f32(g, mu, i);
+		memcpy(a, b, (size_t)n * sizeof(float));
+	}
+}
+
+/* ---- noise of the stochastic forms ---------------------------------------- */
+/* Philox4x32-10 (Salmon, Moraes, Dror and Shaw, SC'11): ten rounds; in each,
+ * words 0 and 2 are multiplied (32 x 32 -> 64 bits) by the round
+ * multipliers, the high halves xored with words 1 and 3 and the key, and the
+ * key advanced by the Weyl constants. Pinned by the known-answer vectors of
+ * the Random123 distribution. */
+void orc_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out)
+{
+	uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+	for (int r = 0; r < 10; r++) {
+		const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+		const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+		c0 = n0;
+		c1 = (uint32_t)p1;
+		c2 = n2;
+		c3 = (uint32_t)p0;
+		k0 += 0x9E3779B9u;
+		k1 += 0xBB67AE85u;
+	}
+	out[0] = c0;
+	out[1] = c1;
+	out[2] = c2;
+	out[3] = c3;
+}
+
+/* ln u for u in (0, 1], a normal float, from +, -, *, / only: u = m 2^e, m in
+ * (1/sqrt2, sqrt2], ln m = 2 atanh s, s = (m-1)/(m+1), series to s^9. */
+float orc_ln_unit(float u)
+{
+	uint32_t bits;
+	memcpy(&bits, &u, 4);
+	int e = (int)(bits >> 23) - 127;
+	const uint32_t mb = (bits & 0x7FFFFFu) | 0x3F800000u;
+	float m;
+	memcpy(&m, &mb, 4);
+	if (m > 1.41421356f) {
+		m = m * 0.5f;
+		e += 1;
+	}
+	const float s = (m - 1.0f) / (m + 1.0f), z = s * s;
+	float q = 1.0f / 9.0f;
+	q = 1.0f / 7.0f + z * q;
+	q = 1.0f / 5.0f + z * q;
+	q = 1.0f / 3.0f + z * q;
+	q = 1.0f + z * q;
+	return (float)e * 0.693147182f + 2.0f * s * q;
+}
+
+/* cos(2 pi t), t in [0, 1) a multiple of 2^-24: exact reduction by quadrant
+ * and octant to theta in [0, pi/4], Taylor series to theta^10 / theta^9. */
+float orc_cos_2pi(float t)
+{
+	const float v = t * 4.0f;
+	const int q = (int)v;
+	const float f = v - (float)q;
+	const float gq = (q & 1) ? 1.0f - f : f;
+	const int hi = gq > 0.5f;
+	const float h = hi ? 1.0f - gq : gq;
+	const float th = h * 1.57079633f, z = th * th;
+	const float cs = 1.0f + z * (-1.0f / 2.0f + z * (1.0f / 24.0f + z * (-1.0f / 720.0f +
+	                 z * (1.0f / 40320.0f + z * (-1.0f / 3628800.0f)))));
+	const float sn = th * (1.0f + z * (-1.0f / 6.0f + z * (1.0f / 120.0f + z * (-1.0f / 5040.0f +
+	                 z * (1.0f / 362880.0f)))));
+	const float c = hi ? sn : cs;
+	return (q == 1 || q == 2) ? -c : c;
+}
+
+/* Standard normal deviate for (seed, step, node): Box-Muller on the first two
+ * words of Philox4x32-10 with key {seed lo, seed hi} and counter {node lo,
+ * node hi, step lo, step hi}: u1 = (w0 + 1/2) 2^-32, u2 = (w1 >> 8) 2^-24,
+ * g = sqrt(-2 ln u1) cos(2 pi u2) in float. (The reference's gaussian(),
+ * model_A.cpp:13-39, uses the polar method on rand(): the same distribution.) */
+float orc_normal(uint64_t seed, uint64_t step, uint64_t node)
+{
+	const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+	const uint32_t ctr[4] = {(uint32_t)node, (uint32_t)(node >> 32), (uint32_t)step, (uint32_t)(step >> 32)};
+	uint32_t w[4];
+	orc_philox4x32_10(ctr, key, w);
+	const float u1 = ((float)w[0] + 0.5f) * 0x1p-32f;
+	const float u2 = (float)(w[1] >> 8) * 0x1p-24f;
+	return sqrtf(-2.0f * orc_ln_unit(u1)) * orc_cos_2pi(u2);
+}
+
+/* ---- model A: p = {r, u, K, M, dt, sigma, seed}; step0 = the state's step count
+ * new = phi - dt*M*(-r*phi + u*phi^3 - K*lap + noise), noise = 0 + g*sigma
+ * (model_A.cpp:98-100) */
+void orc_ac_noise_steps_f64(const orc_grid* g, double* a, double* b, const double* p, int steps, uint64_t step0,
+                            int cube)
+{
+	const double r = p[0], u = p[1], K = p[2], M = p[3], dt = p[4], sigma = p[5];
+	const uint64_t seed = (uint64_t)p[6];
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			double phi = a[i];
+			double noise = 0.0 + (double)orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma;
+			b[i] = phi - dt * M * (-r * phi + u * cube64(phi, cube) - K * orc_laplacian_f64(g, a, i) + noise);
+		}
+		memcpy(a, b, (size_t)n * sizeof(double));
+	}
+}
+
+/* T = float: the deviate is rounded to float (T noise) and promoted again */
+void orc_ac_noise_steps_f32ref(const orc_grid* g, float* a, float* b, const double* p, int steps, uint64_t step0,
+                               int cube)
+{
+	const double r = p[0], u = p[1], K = p[2], M = p[3], dt = p[4], sigma = p[5];
+	const uint64_t seed = (uint64_t)p[6];
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			float phi = a[i];
+			float lap = orc_laplacian_f32ref(g, a, i);
+			float noise = (float)(0.0 + (double)orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma);
+			b[i] = (float)((double)phi -
+			               dt * M * (-r * (double)phi + u * cube64((double)phi, cube) - K * (double)lap + (double)noise));
+		}
+		memcpy(a, b, (size_t)n * sizeof(float));
+	}
+}
+
+void orc_ac_noise_steps_f32(const orc_grid* g, float* a, float* b, const double* p, int steps, uint64_t step0)
+{
+	const float r = (float)p[0], u = (float)p[1], K = (float)p[2], M = (float)p[3], dt = (float)p[4];
+	const float sigma = (float)p[5];
+	const uint64_t seed = (uint64_t)p[6];
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			float phi = a[i];
+			float c3 = phi * phi * phi;
+			float noise = orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma;
+			b[i] = phi - dt * M * (-r * phi + u * c3 - K * orc_laplacian_f32(g, a, i) + noise);
+		}
+		memcpy(a, b, (size_t)n * sizeof(float));
+	}
+}
+
+/* ---- model B: p = {r, u, K, M, dt, sigma, seed}
+ * mu = -r*phi + u*phi^3 - K*lap + noise (model_B.cpp:99-101); new = old + dt*M*lap(mu) (:105-106).
+ * spinodal.cpp:94-102 is r = u = K = 1, M = 1/(2 dV), sigma = sqrt(eps dt/dV). */
+void orc_ch_noise_steps_f64(const orc_grid* g, double* a, double* b, double* mu, const double* p, int steps,
+                            uint64_t step0, int cube)
+{
+	const double sigma = p[5];
+	const uint64_t seed = (uint64_t)p[6];
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			double c = a[i];
+			double noise = 0.0 + (double)orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma;
+			mu[i] = -p[0] * c + p[1] * cube64(c, cube) - p[2] * orc_laplacian_f64(g, a, i) + noise;
+		}
+		for (int64_t i = 0; i < n; i++) b[i] = a[i] + p[4] * p[3] * orc_laplacian_f64(g, mu, i);
+		memcpy(a, b, (size_t)n * sizeof(double));
+	}
+}
+
+void orc_ch_noise_steps_f32ref(const orc_grid* g, float* a, float* b, float* mu, const double* p, int steps,
+                               uint64_t step0, int cube)
+{
+	const double sigma = p[5];
+	const uint64_t seed = (uint64_t)p[6];
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			double c = a[i];
+			double lap = orc_laplacian_f32ref(g, a, i);
+			float noise = (float)(0.0 + (double)orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma);
+			mu[i] = (float)(-p[0] * c + p[1] * cube64(c, cube) - p[2] * lap + (double)noise);
+		}
+		for (int64_t i = 0; i < n; i++)
+			b[i] = (float)((double)a[i] + p[4] * p[3] * (double)orc_laplacian_f32ref(g, mu, i));
+		memcpy(a, b, (size_t)n * sizeof(float));
+	}
+}
+
+void orc_ch_noise_steps_f32(const orc_grid* g, float* a, float* b, float* mu, const double* p, int steps,
+                            uint64_t step0)
+{
+	const float sigma = (float)p[5];
+	const uint64_t seed = (uint64_t)p[6];
+	const int64_t n = nodes_of(g);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			float c = a[i];
+			float lap = orc_laplacian_f32(g, a, i);
+			float noise = orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma;
+			mu[i] = -(float)p[0] * c + (float)p[1] * (c * c * c) - (float)p[2] * lap + noise;
+		}
+		const float M = (float)p[3], dt = (float)p[4];
+		for (int64_t i = 0; i < n; i++) b[i] = a[i] + dt * M * orc_laplacian_f32(g, mu, i);
 		memcpy(a, b, (size_t)n * sizeof(float));
 	}
 }

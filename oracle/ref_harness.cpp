@@ -8,6 +8,8 @@
 //   -DHARNESS_AC  includes examples/phase_transitions/allen-cahn/allen-cahn.cpp
 //   -DHARNESS_CH  includes examples/phase_transitions/cahn-hilliard/explicit/cahn-hilliard.cpp
 //   -DHARNESS_GG  includes examples/coarsening/grain_growth/isotropic/phase_field/graingrowth.cpp
+//   -DHARNESS_MA  includes examples/phase_transitions/model_A/model_A.cpp (stock only)
+//   -DHARNESS_MB  includes examples/phase_transitions/model_B/model_B.cpp (stock only)
 // MMSP_MAIN is predefined so that the reference's main() (MMSP.main.hpp:38) is
 // skipped; this file supplies a small main() instead.
 //
@@ -35,8 +37,12 @@
 #include "cahn-hilliard.cpp"
 #elif defined(HARNESS_GG)
 #include "graingrowth.cpp"
+#elif defined(HARNESS_MA)
+#include "model_A.cpp"
+#elif defined(HARNESS_MB)
+#include "model_B.cpp"
 #else
-#error "define one of HARNESS_AC, HARNESS_CH, HARNESS_GG"
+#error "define one of HARNESS_AC, HARNESS_CH, HARNESS_GG, HARNESS_MA, HARNESS_MB"
 #endif
 
 namespace harness {
@@ -44,7 +50,7 @@ namespace harness {
 using MMSP::grid;
 
 static std::vector<double> P;  // config parameters from the command line
-static double par(size_t i, double dflt) { return i < P.size() ? P[i] : dflt; }
+[[maybe_unused]] static double par(size_t i, double dflt) { return i < P.size() ? P[i] : dflt; }
 
 #if defined(HARNESS_AC)
 // Allen-Cahn with parameters; same expression and order as allen-cahn.cpp:70.
@@ -110,9 +116,20 @@ template <int dim, typename T> void update_config(grid<dim, MMSP::vector<T> >& o
 }
 #endif
 
+#if defined(HARNESS_MA) || defined(HARNESS_MB)
+// model A / model B: only the stock update (hard-coded parameters); the noise
+// comes from rand(), seeded by srand(HARNESS_SEED) when that is set
+template <class G> void update_config(G&, int)
+{
+	std::fprintf(stderr, "ref harness: model A/B run the stock update only\n");
+	std::exit(2);
+}
+#endif
+
 template <class G> int run(const char* in, const char* out, int steps, bool stock)
 {
 	G g(in);
+	if (const char* sd = std::getenv("HARNESS_SEED")) std::srand((unsigned)std::atoi(sd));
 	auto t0 = std::chrono::steady_clock::now();
 	if (stock)
 		MMSP::update(g, steps);
