@@ -44,6 +44,9 @@
 #error "mmsp_gpu.hpp drives one serial grid per process; MPI builds of MMSP are not supported"
 #endif
 
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <iostream>
 #include <string>
@@ -163,6 +166,46 @@ void cahn_hilliard_double_well(MMSP::grid<dim, T>& grid, int steps, double kappa
 {
 	const double p[3] = {kappa, M, dt};
 	run_scalar(grid, MMSPX_CH_DW, p, 3, steps);
+}
+
+// Noise seed of one update() call, drawn from rand() -- the generator the
+// reference's gaussian() uses, seeded once by MMSP.main.hpp -- so successive
+// update() calls get fresh noise and srand() fixes a run. The state made for
+// the call counts its steps from 0; the seed keeps calls apart.
+inline double next_noise_seed()
+{
+	const uint64_t a = (uint64_t)std::rand(), b = (uint64_t)std::rand();
+	return (double)(((a << 31) ^ b) & ((uint64_t(1) << 53) - 1));
+}
+
+// Model A, model_A.cpp:74-105: new = phi - dt*M*(-r*phi + u*phi^3 - K*lap(phi) + noise),
+// noise = gaussian(0, sqrt(2 kT/(dt dV))).
+template <int dim, typename T>
+void model_a(MMSP::grid<dim, T>& grid, int steps, double r = 1.0, double u = 1.0, double K = 1.0, double M = 1.0,
+             double dt = 0.01, double kT = 0.01, double dV = 1.0)
+{
+	const double p[7] = {r, u, K, M, dt, std::sqrt(2.0 * kT / (dt * dV)), next_noise_seed()};
+	run_scalar(grid, MMSPX_AC_NOISE, p, 7, steps);
+}
+
+// Model B, model_B.cpp:74-111: mu = -r*phi + u*phi^3 - K*lap(phi) + noise;
+// new = phi + dt*M*lap(mu), noise = gaussian(0, sqrt(2 kT/(dt dV))).
+template <int dim, typename T>
+void model_b(MMSP::grid<dim, T>& grid, int steps, double r = 1.0, double u = 1.0, double K = 1.0, double M = 1.0,
+             double dt = 0.01, double kT = 0.01, double dV = 1.0)
+{
+	const double p[7] = {r, u, K, M, dt, std::sqrt(2.0 * kT / (dt * dV)), next_noise_seed()};
+	run_scalar(grid, MMSPX_CH_NOISE, p, 7, steps);
+}
+
+// Spinodal decomposition, spinodal.cpp:74-107: model B with r = u = K = 1 and
+// dt/(2 dV) in place of dt*M (the same bits when dV is a power of two),
+// noise = gaussian(0, sqrt(epsilon dt/dV)).
+template <int dim, typename T>
+void spinodal(MMSP::grid<dim, T>& grid, int steps, double dt = 0.01, double dV = 1.0, double epsilon = 0.05)
+{
+	const double p[7] = {1.0, 1.0, 1.0, 1.0 / (2.0 * dV), dt, std::sqrt(epsilon * dt / dV), next_noise_seed()};
+	run_scalar(grid, MMSPX_CH_NOISE, p, 7, steps);
 }
 
 // Grain growth on grid<dim,vector<T> >, graingrowth.cpp:73-111. The per-node

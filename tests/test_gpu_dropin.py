@@ -5,6 +5,7 @@ headers) compiled with their update() bodies replaced by include/mmsp_gpu.hpp
 the stock reference programs (oracle/_ref/allen-cahn ...), run side by side:
 same names, same headers, data within 1e-12 (FP64) and bitwise equal to the
 oracle with the device's cube."""
+import math
 import os
 import re
 import shutil
@@ -98,3 +99,44 @@ def test_dropin_program_matches_stock_program(name, model, tmp_path):
     assert np.array_equal(last, exp)
     # and the stock program's last file equals the reference's golden output
     assert np.array_equal(dat.read_dat(str(ds / files_s[-1])).data, dat.read_dat(golden_path(name, "out")).data)
+
+
+
+# The examples' own parameters: model_A.cpp:85-91, model_B.cpp:86-92, spinodal.cpp:86-88.
+STOCHASTIC = {
+    "model_A": (O.ac_noise, [1.0, 1.0, 1.0, 1.0, 0.01, math.sqrt(2.0 * 0.01 / 0.01)]),
+    "model_B": (O.ch_noise, [1.0, 1.0, 1.0, 1.0, 0.01, math.sqrt(2.0 * 0.01 / 0.01)]),
+    "spinodal": (O.ch_noise, [1.0, 1.0, 1.0, 0.5, 0.01, math.sqrt(0.05 * 0.01)]),
+}
+
+
+@pytest.mark.parametrize("prog", ["model_A", "model_B", "spinodal"])
+def test_stochastic_dropin_programs_agree_in_distribution(prog, tmp_path):
+    """model_A, model_B and spinodal with the GPU update beside the stock
+    programs (which seed rand() from the clock): the same checkpoint names,
+    headers and progress output, and from phi = 0 on 256^2 after 100 steps the
+    mean of phi^2 of both within 5 % of the oracle's mean over four seeds
+    (the spread of one run is about 1 %)."""
+    gpu = prog + "-gpu"
+    if not (have(prog) and have(gpu)):
+        pytest.skip("reference programs not built here")
+    say("[gpu] stochastic drop-in %s" % prog)
+    n, steps, inc = 256, 100, 50
+    outs = {}
+    for p in (prog, gpu):
+        d = tmp_path / p
+        d.mkdir()
+        dat.write_dat(str(d / "start.dat"), dat.make_grid(np.zeros((n, n))), compress=True)
+        out = run(p, str(d), ["start.dat", str(steps), str(inc)])
+        outs[p] = (sorted(f for f in os.listdir(d) if f != "start.dat"), out, d)
+    files_s, out_s, ds = outs[prog]
+    files_g, out_g, dg = outs[gpu]
+    assert files_g == files_s and len(files_g) == steps // inc
+    assert progress_shape(out_g) == progress_shape(out_s)
+    f, p = STOCHASTIC[prog]
+    expect = float(np.mean([(f(np.zeros((n, n)), p + [seed], steps) ** 2).mean() for seed in range(1, 5)]))
+    for dd in (ds, dg):
+        a = dat.read_dat(str(dd / files_s[-1]))
+        assert (a.dim, a.dx, a.g0) == (2, [1.0, 1.0], [0, 0])
+        v = float((a.data ** 2).mean())
+        assert abs(v / expect - 1) < 0.05, (dd, v, expect)
