@@ -76,9 +76,20 @@ enum {
 	 * sigma, seed}; seed an integer in [0, 2^53). */
 	MMSPX_AC_NOISE = 4, /* model A, model_A.cpp:74-105 (sigma = sqrt(2kT/(dt dV)))
 	                       new = phi - dt*M*(-r*phi + u*phi^3 - K*lap(phi) + xi) */
-	MMSPX_CH_NOISE = 5  /* model B, model_B.cpp:74-111 (sigma = sqrt(2kT/(dt dV)));
+	MMSPX_CH_NOISE = 5, /* model B, model_B.cpp:74-111 (sigma = sqrt(2kT/(dt dV)));
 	                       spinodal.cpp:74-107 is r = u = K = 1, M = 1/(2 dV), sigma = sqrt(eps dt/dV)
 	                       mu = -r*c + u*c^3 - K*lap(c) + xi;  new = c + dt*M*lap(mu) */
+	/* Vector-laplacian models (vector grids; forms MMSPX_F64 and MMSPX_F32): */
+	MMSPX_ZENER = 6,    /* grain growth with Zener pinning, zener_pinning/isotropic/phase_field/zener.cpp:104-141.
+	                       p = {dt}; field 0 (the particles) keeps its values, fields j >= 1 as MMSPX_GG */
+	MMSPX_OSTWALD = 7   /* Ostwald ripening, ostwald_ripening/isotropic/phase_field/ostwald.cpp:56-123.
+	                       p = {dt, L, D, Calpha, Cbeta, A, delta, epsilon, kappa}; field 0 is the
+	                       concentration C, fields j >= 1 the order parameters; Cmatrix = (Calpha+Cbeta)/2,
+	                       B = A/(Cbeta-Cmatrix)^2, gamma = 2/(Cbeta-Calpha)^2, Dalpha = Dbeta = gamma/delta^2;
+	                       w = -A(C-Cm) + B(C-Cm)^3 + Dalpha(C-Calpha)^3 + Dbeta(C-Cbeta)^3
+	                           - gamma(C-Calpha) sum_j phi_j^2 - kappa lap C;  C' = C + dt*D*lap(w);
+	                       phi_j' = phi_j - dt*L*(-gamma(C-Calpha)^2 + delta phi_j^3
+	                                + epsilon phi_j (sum - phi_j^2) - kappa lap phi_j) */
 };
 /* The noise: g(seed, step, node) is a standard normal deviate (float), the
  * Box-Muller transform of the first two words of one Philox4x32-10 block

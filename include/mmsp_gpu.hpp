@@ -208,9 +208,10 @@ void spinodal(MMSP::grid<dim, T>& grid, int steps, double dt = 0.01, double dV =
 	run_scalar(grid, MMSPX_CH_NOISE, p, 7, steps);
 }
 
-// Grain growth on grid<dim,vector<T> >, graingrowth.cpp:73-111. The per-node
-// vectors are packed node-major (the reference's node order) for the upload.
-template <int dim, typename T> void grain_growth(MMSP::grid<dim, MMSP::vector<T> >& grid, int steps, double dt = 0.01)
+// Vector grids grid<dim,vector<T> >: the per-node vectors are packed
+// node-major (the reference's node order) for the upload and unpacked after.
+template <int dim, typename T>
+void run_vector(MMSP::grid<dim, MMSP::vector<T> >& grid, int model, const double* p, int np, int steps)
 {
 	const int n = nodes(grid);
 	const int q = fields(grid);
@@ -228,14 +229,39 @@ template <int dim, typename T> void grain_growth(MMSP::grid<dim, MMSP::vector<T>
 	mmspx_state* s = nullptr;
 	check(mmspx_create(&d, &s), "mmspx_create");
 	check(mmspx_upload(s, host.data(), MMSPX_NODE_MAJOR), "mmspx_upload");
-	const double p[1] = {dt};
-	run_with_progress(s, MMSPX_GG, p, 1, steps);
+	run_with_progress(s, model, p, np, steps);
 	check(mmspx_download(s, host.data(), MMSPX_NODE_MAJOR), "mmspx_download");
 	mmspx_destroy(s);
 	for (int i = 0; i < n; i++) {
 		MMSP::vector<T>& v = grid(i);
 		for (int j = 0; j < q; j++) v[j] = host[(size_t)i * q + j];
 	}
+}
+
+// Grain growth on grid<dim,vector<T> >, graingrowth.cpp:73-111.
+template <int dim, typename T> void grain_growth(MMSP::grid<dim, MMSP::vector<T> >& grid, int steps, double dt = 0.01)
+{
+	const double p[1] = {dt};
+	run_vector(grid, MMSPX_GG, p, 1, steps);
+}
+
+// Grain growth with Zener pinning, zener_pinning/isotropic/phase_field/zener.cpp:104-141:
+// field 0 (the particles) has zero mobility.
+template <int dim, typename T> void zener(MMSP::grid<dim, MMSP::vector<T> >& grid, int steps, double dt = 0.01)
+{
+	const double p[1] = {dt};
+	run_vector(grid, MMSPX_ZENER, p, 1, steps);
+}
+
+// Ostwald ripening, ostwald_ripening/isotropic/phase_field/ostwald.cpp:56-123;
+// field 0 is the concentration. Defaults: the example's constants (:66-84).
+template <int dim, typename T>
+void ostwald(MMSP::grid<dim, MMSP::vector<T> >& grid, int steps, double dt = 0.001, double L = 1.0, double D = 1.0,
+             double Calpha = 0.05, double Cbeta = 0.95, double A = 2.0, double delta = 1.0, double epsilon = 3.0,
+             double kappa = 2.0)
+{
+	const double p[9] = {dt, L, D, Calpha, Cbeta, A, delta, epsilon, kappa};
+	run_vector(grid, MMSPX_OSTWALD, p, 9, steps);
 }
 
 }  // namespace mmsp_gpu

@@ -87,6 +87,8 @@ int model_nparams(int model)
 	case MMSPX_GG: return 1;
 	case MMSPX_AC_NOISE: return 7;
 	case MMSPX_CH_NOISE: return 7;
+	case MMSPX_ZENER: return 1;
+	case MMSPX_OSTWALD: return 9;
 	}
 	return -1;
 }
@@ -126,6 +128,26 @@ int make_params(int model, const double* p, int np, ModelParams* out)
 		m.dtM = p[2] * p[1];
 		m.kappaf = (float)p[0];
 		m.dtMf = (float)p[2] * (float)p[1];
+	} else if (model == MMSPX_OSTWALD) {
+		// p = {dt, L, D, Calpha, Cbeta, A, delta, epsilon, kappa}; the derived
+		// constants as ostwald.cpp:68-82 computes them (pow(x, 2) included)
+		const double dt = p[0], L = p[1], D = p[2], Ca = p[3], Cb = p[4], A = p[5], delta = p[6];
+		const double Cm = 0.5 * (Ca + Cb);
+		const double gamma = 2.0 / std::pow(Cb - Ca, 2);
+		m.ow[OW_A] = A;
+		m.ow[OW_B] = A / std::pow(Cb - Cm, 2);
+		m.ow[OW_G] = gamma;
+		m.ow[OW_DA] = gamma / std::pow(delta, 2);
+		m.ow[OW_DB] = gamma / std::pow(delta, 2);
+		m.ow[OW_DELTA] = delta;
+		m.ow[OW_EPS] = p[7];
+		m.ow[OW_K] = p[8];
+		m.ow[OW_CA] = Ca;
+		m.ow[OW_CB] = Cb;
+		m.ow[OW_CM] = Cm;
+		m.ow[OW_DTD] = dt * D;
+		m.ow[OW_DTL] = dt * L;
+		for (int k = 0; k < 13; k++) m.owf[k] = (float)m.ow[k];
 	} else {
 		m.dt = p[0];
 		m.dtf = (float)p[0];
@@ -168,6 +190,8 @@ size_t mmspx_footprint(const mmspx_desc* d, int model)
 	for (int k = 0; k < d->dim; k++) nodes *= (size_t)d->n[k];
 	const size_t q = d->vector ? (size_t)d->fields : 1;
 	size_t bufs = ch_family(model) ? 3 : 2;
+	// Ostwald ripening: one scalar plane for w (the state-sized auxiliary buffer)
+	if (model == MMSPX_OSTWALD) bufs = 3;
 	return bufs * nodes * q * elem + q * 2 * kRedBlocks * sizeof(double);
 }
 
@@ -292,8 +316,12 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	ModelParams p;
 	int rc = make_params(model, params, np, &p);
 	if (rc) return rc;
-	if (model == MMSPX_GG) {
-		if (!s->desc.vector) return fail(MMSPX_EINVAL, "grain growth needs a vector grid");
+	if (vector_model(model)) {
+		if (!s->desc.vector) return fail(MMSPX_EINVAL, "grain growth, Zener pinning and Ostwald ripening need a vector grid");
+		if (model == MMSPX_OSTWALD && s->geom.fields < 2)
+			return fail(MMSPX_EINVAL, "Ostwald ripening needs a concentration field and at least one order parameter");
+		if ((model == MMSPX_ZENER || model == MMSPX_OSTWALD) && s->desc.precision == MMSPX_F32REF)
+			return fail(MMSPX_EUNSUPPORTED, "Zener pinning and Ostwald ripening run the MMSPX_F64 and MMSPX_F32 forms");
 	} else if (s->desc.vector) {
 		return fail(MMSPX_EINVAL, "Allen-Cahn and Cahn-Hilliard need a scalar grid");
 	}
@@ -311,7 +339,7 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	s->last = fused ? tc : TuneChoice{1, 0, 0, 0, 0};
 	s->has_last = true;
 	const int spl = fused ? fused_steps_per_launch(model, tc) : 1;
-	if (ch_family(model) && !fused && !s->aux) {
+	if ((ch_family(model) || model == MMSPX_OSTWALD) && !fused && !s->aux) {
 		if (hipMalloc(&s->aux, s->bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");
 	}
 	// Small grids are launch-bound: replay an even number of steps from a

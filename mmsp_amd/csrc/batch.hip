@@ -159,6 +159,9 @@ bool batch_model_ok(int model, const Geom& g, std::string* why)
 	case MMSPX_AC_NOISE:
 	case MMSPX_CH_NOISE:
 		return no("batches do not run the stochastic forms (each member would need its own step count)");
+	case MMSPX_ZENER:
+	case MMSPX_OSTWALD:
+		return no("batches do not run Zener pinning or Ostwald ripening");
 	}
 	return no("unknown model");
 }

@@ -53,7 +53,13 @@ struct ModelParams {
 	double sigma;
 	float sigmaf;
 	uint64_t seed, step;
+	// Ostwald ripening constants, derived as ostwald.cpp:68-82 derives them
+	double ow[13];
+	float owf[13];
 };
+enum { OW_A, OW_B, OW_G, OW_DA, OW_DB, OW_DELTA, OW_EPS, OW_K, OW_CA, OW_CB, OW_CM, OW_DTD, OW_DTL };
+
+inline bool vector_model(int model) { return model == MMSPX_GG || model == MMSPX_ZENER || model == MMSPX_OSTWALD; }
 
 inline bool noisy_model(int model) { return model == MMSPX_AC_NOISE || model == MMSPX_CH_NOISE; }
 inline bool ch_family(int model) { return model == MMSPX_CH || model == MMSPX_CH_DW || model == MMSPX_CH_NOISE; }

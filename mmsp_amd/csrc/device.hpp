@@ -207,6 +207,23 @@ template <> struct Form<MMSPX_F64> {
 		const double xi = 0.0 + (double)gd * p.sigma;
 		return p.nr * c + p.u * cube_cr(c) - p.K * lap + xi;
 	}
+	// Ostwald ripening, ostwald.cpp:97-99 (w), :108 (C) and :117-118 (order parameters);
+	// pow(x, 2) as x*x, pow(x, 3) as the correctly rounded cube
+	using OSum = double;
+	__device__ static inline S ow_w(S C, OSum sum, L lap, const ModelParams& p)
+	{
+		const double* o = p.ow;
+		return -o[OW_A] * (C - o[OW_CM]) + o[OW_B] * cube_cr(C - o[OW_CM]) + o[OW_DA] * cube_cr(C - o[OW_CA]) +
+		       o[OW_DB] * cube_cr(C - o[OW_CB]) - o[OW_G] * (C - o[OW_CA]) * sum - o[OW_K] * lap;
+	}
+	__device__ static inline S ow_phi(S v, S C, OSum sum, L lap, const ModelParams& p)
+	{
+		const double* o = p.ow;
+		const double d = C - o[OW_CA];
+		return v - o[OW_DTL] * (-o[OW_G] * (d * d) + o[OW_DELTA] * cube_cr(v) + o[OW_EPS] * v * (sum - v * v) -
+		                        o[OW_K] * lap);
+	}
+	__device__ static inline S ow_c(S C, L lapw, const ModelParams& p) { return C + p.ow[OW_DTD] * lapw; }
 };
 
 // float storage, float arithmetic in the same order (the fast FP32 path)
@@ -248,6 +265,22 @@ template <> struct Form<MMSPX_F32> {
 		const float xi = gd * p.sigmaf;
 		return p.nrf * c + p.uf * (c * c * c) - p.Kf * lap + xi;
 	}
+	using OSum = float;
+	__device__ static inline S ow_w(S C, OSum sum, L lap, const ModelParams& p)
+	{
+		const float* o = p.owf;
+		const float a = C - o[OW_CM], b = C - o[OW_CA], c = C - o[OW_CB];
+		return -o[OW_A] * a + o[OW_B] * (a * a * a) + o[OW_DA] * (b * b * b) + o[OW_DB] * (c * c * c) -
+		       o[OW_G] * b * sum - o[OW_K] * lap;
+	}
+	__device__ static inline S ow_phi(S v, S C, OSum sum, L lap, const ModelParams& p)
+	{
+		const float* o = p.owf;
+		const float d = C - o[OW_CA];
+		return v - o[OW_DTL] * (-o[OW_G] * (d * d) + o[OW_DELTA] * (v * v * v) + o[OW_EPS] * v * (sum - v * v) -
+		                        o[OW_K] * lap);
+	}
+	__device__ static inline S ow_c(S C, L lapw, const ModelParams& p) { return C + p.owf[OW_DTD] * lapw; }
 };
 
 // float storage with the reference's T=float semantics: the laplacian adds a

@@ -151,6 +151,9 @@ bool fused_supported(const Geom& g, int prec, int model)
 	case MMSPX_AC:
 	case MMSPX_AC_NOISE: return g.dim == 2;
 	case MMSPX_GG: return g.dim == 2 && gg_q_compiled(g.fields);
+	// Zener pinning: the wave strips, where their registers hold the fields
+	case MMSPX_ZENER:
+		return g.dim == 2 && gg_q_compiled(g.fields) && prec != MMSPX_F32REF && (g.fields < 32 || prec != MMSPX_F64);
 	}
 	return false;
 }
@@ -169,6 +172,7 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// model A: the noise costs arithmetic per node and step, so the wave
 	// strips (one load and one store per node) rather than one thread per node
 	if (model == MMSPX_AC_NOISE) return TuneChoice{2, 1, kWaveCols, 16, 2};
+	if (model == MMSPX_ZENER) return TuneChoice{2, 1, kWaveCols, 32, 2};
 	// grain growth with 32 fields (2048^2 FP32): 64-column wave strips with
 	// 32-row chunks take 260-264 us per step against 272 us for 62-column
 	// strips (halo lanes, misaligned rows), 331-338 us for the 8 x 64 tile,
@@ -212,8 +216,8 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		if (n < max) out[n++] = TuneChoice{2, ty, tz, chunk, opt};
 	};
 	const bool ch = ch_family(model);
-	if (g.dim == 2 && model == MMSPX_AC_NOISE) {
-		// model A: wave strips only
+	if (g.dim == 2 && (model == MMSPX_AC_NOISE || model == MMSPX_ZENER)) {
+		// model A and Zener pinning: wave strips only
 		for (int c : {8, 12, 16, 24, 32}) add(1, kWaveCols, c, 2);
 		return n;
 	}
@@ -418,6 +422,20 @@ static int launch_wave(const Geom& g, const ModelParams& p, const void* a, void*
 	constexpr bool wide = sizeof(typename Form<PREC>::S) == 4;
 	if (p.model == MMSPX_AC) return launch_wave_q<PREC, 1, 0>(g, p, a, b, st, chunk);
 	if (p.model == MMSPX_AC_NOISE) return launch_wave_q<PREC, 1, 2>(g, p, a, b, st, chunk);
+	if (p.model == MMSPX_ZENER) {
+		if constexpr (PREC != MMSPX_F32REF) {
+			switch (g.fields) {
+			case 2: return launch_wave_q<PREC, 2, 3>(g, p, a, b, st, chunk);
+			case 4: return launch_wave_q<PREC, 4, 3>(g, p, a, b, st, chunk);
+			case 8: return launch_wave_q<PREC, 8, 3>(g, p, a, b, st, chunk);
+			case 16: return launch_wave_q<PREC, 16, 3>(g, p, a, b, st, chunk);
+			case 32:
+				if constexpr (wide) return launch_wave_q<PREC, 32, 3>(g, p, a, b, st, chunk);
+				break;
+			}
+		}
+		return fail(MMSPX_EUNSUPPORTED, "no wave-strip kernel for this field count and precision");
+	}
 	switch (g.fields) {
 	case 2: return launch_wave_q<PREC, 2, 1>(g, p, a, b, st, chunk);
 	case 4: return launch_wave_q<PREC, 4, 1>(g, p, a, b, st, chunk);
@@ -439,6 +457,9 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 	case MMSPX_CH: return launch_ch_prec<PREC, 0>(g, p, a, b, st, t);
 	case MMSPX_CH_DW: return launch_ch_prec<PREC, 1>(g, p, a, b, st, t);
 	case MMSPX_CH_NOISE: return launch_ch_prec<PREC, 2>(g, p, a, b, st, t);
+	case MMSPX_ZENER:
+		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
+		return fail(MMSPX_EUNSUPPORTED, "Zener pinning runs the wave strips (1 x 64, body 2) or the generic kernel");
 	case MMSPX_AC_NOISE:
 		// model A: wave strips only (the noise is drawn per node and step)
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);

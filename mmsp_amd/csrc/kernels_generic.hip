@@ -100,8 +100,9 @@ __global__ void __launch_bounds__(256) ch_update_generic(const typename Form<PRE
 }
 
 // Grain growth on field-major storage (plane j at offset j*nodes): the
-// reference's per-node vector<T> becomes Q coalesced planes.
-template <int DIM, int PREC>
+// reference's per-node vector<T> becomes Q coalesced planes. ZEN: Zener
+// pinning (zener.cpp:117-135), field 0 -- the particles -- keeps its value.
+template <int DIM, int PREC, bool ZEN>
 __global__ void __launch_bounds__(256) gg_generic(const typename Form<PREC>::S* __restrict__ a,
                                                   typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p)
 {
@@ -128,8 +129,62 @@ __global__ void __launch_bounds__(256) gg_generic(const typename Form<PREC>::S* 
 			typename F::L lap = F::lap_zero();
 #pragma unroll
 			for (int d = 0; d < DIM; d++) F::lap_add(lap, g.w[d], g.wf[d], pl[ih[d]], y, pl[il[d]]);
-			b[j * N + i] = F::gg(y, sum, lap, p);
+			b[j * N + i] = (ZEN && j == 0) ? y : F::gg(y, sum, lap, p);
 		}
+	}
+}
+
+// Ostwald ripening (ostwald.cpp:87-122), field 0 the concentration C. Pass 1:
+// w (the reference's first node loop) and the order parameters' update (the
+// j-loop of its second), both from the old state; pass 2: C' = C + dt*D*lap(w).
+template <int DIM, int PREC>
+__global__ void __launch_bounds__(256) ow_pass1(const typename Form<PREC>::S* __restrict__ a,
+                                                typename Form<PREC>::S* __restrict__ w,
+                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	const int64_t N = g.nodes;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.nodes; i += gridDim.x * blockDim.x) {
+		int x[3];
+		coords<DIM>(g, i, x);
+		int ih[3], il[3];
+#pragma unroll
+		for (int d = 0; d < DIM; d++) {
+			ih[d] = nbr(g, i, x, d, +1);
+			il[d] = nbr(g, i, x, d, -1);
+		}
+		typename F::OSum sum = 0;
+		for (int j = 1; j < g.fields; j++) {
+			const S v = a[j * N + i];
+			sum += v * v;
+		}
+		const S C = a[i];
+		typename F::L lap0 = F::lap_zero();
+#pragma unroll
+		for (int d = 0; d < DIM; d++) F::lap_add(lap0, g.w[d], g.wf[d], a[ih[d]], C, a[il[d]]);
+		w[i] = F::ow_w(C, sum, lap0, p);
+		for (int j = 1; j < g.fields; j++) {
+			const S* __restrict__ pl = a + j * N;
+			const S v = pl[i];
+			typename F::L lap = F::lap_zero();
+#pragma unroll
+			for (int d = 0; d < DIM; d++) F::lap_add(lap, g.w[d], g.wf[d], pl[ih[d]], v, pl[il[d]]);
+			b[j * N + i] = F::ow_phi(v, C, sum, lap, p);
+		}
+	}
+}
+
+template <int DIM, int PREC>
+__global__ void __launch_bounds__(256) ow_pass2(const typename Form<PREC>::S* __restrict__ a,
+                                                const typename Form<PREC>::S* __restrict__ w,
+                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p)
+{
+	using F = Form<PREC>;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.nodes; i += gridDim.x * blockDim.x) {
+		int x[3];
+		coords<DIM>(g, i, x);
+		b[i] = F::ow_c(a[i], lap_at<DIM, PREC>(g, w, i, x), p);
 	}
 }
 
@@ -175,8 +230,21 @@ static int launch_dim_prec(const Geom& g, const ModelParams& p, const void* av, 
 		*launches += 2;
 		break;
 	case MMSPX_GG:
-		hipLaunchKernelGGL((gg_generic<DIM, PREC>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
+		hipLaunchKernelGGL((gg_generic<DIM, PREC, false>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
 		*launches += 1;
+		break;
+	case MMSPX_ZENER:
+		hipLaunchKernelGGL((gg_generic<DIM, PREC, true>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
+		*launches += 1;
+		break;
+	case MMSPX_OSTWALD:
+		if constexpr (PREC == MMSPX_F32REF) {
+			return fail(MMSPX_EUNSUPPORTED, "Ostwald ripening runs the MMSPX_F64 and MMSPX_F32 forms");
+		} else {
+			hipLaunchKernelGGL((ow_pass1<DIM, PREC>), dim3(blocks), dim3(256), 0, st, a, aux, b, g, p);
+			hipLaunchKernelGGL((ow_pass2<DIM, PREC>), dim3(blocks), dim3(256), 0, st, a, aux, b, g, p);
+			*launches += 2;
+		}
 		break;
 	default:
 		return fail(MMSPX_EINVAL, "unknown model");

@@ -162,10 +162,13 @@ int launch_reduce(const Geom& g, int prec, int what, const ModelParams& p, const
 		ep.kind = 1;
 		ep.r = ep.u = 0.0;
 		ep.kap = p.kappa;
-	} else {
+	} else if (p.model == MMSPX_GG || p.model == MMSPX_ZENER) {
+		// Zener pinning: the grain-growth functional (the particle field included)
 		ep.kind = 2;
 		ep.r = ep.u = 0.0;
 		ep.kap = 2.0;
+	} else {
+		return fail(MMSPX_EUNSUPPORTED, "no free energy for this model");
 	}
 	if (f64) return launch_energy<double>(g, ep, (const double*)a, part, st);
 	return launch_energy<float>(g, ep, (const float*)a, part, st);

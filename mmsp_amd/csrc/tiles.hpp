@@ -679,7 +679,8 @@ __device__ __forceinline__ void gg_tile(const typename Form<PREC>::S* __restrict
 // own. KIND 1, grain growth: Q fields stored field-major, sum_j phi_j^2
 // formed in the reference's order before the field updates
 // (graingrowth.cpp:91-107); KIND 0, Allen-Cahn (Q = 1, allen-cahn.cpp:68-71);
-// KIND 2, model A (Allen-Cahn plus noise, model_A.cpp:97-101).
+// KIND 2, model A (Allen-Cahn plus noise, model_A.cpp:97-101); KIND 3, Zener
+// pinning (grain growth with field 0 held, zener.cpp:117-135).
 template <int PREC, int Q, int KIND>
 __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restrict__ a,
                                            typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
@@ -730,7 +731,7 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 		S h3 = 0;
 		if (more && hl) h3 = ah[rrow(x + 2) + ch];
 		typename F::Sum sum = 0;
-		if (KIND == 1) {
+		if (KIND == 1 || KIND == 3) {
 #pragma unroll
 			for (int j = 0; j < Q; j++) sum += w1[j] * w1[j];
 		}
@@ -746,6 +747,7 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 			if (out)
 				(b + j * N)[o] = KIND == 1 ? F::gg(y, sum, lap, p)
 				                 : KIND == 2 ? F::ac_noise(y, lap, gd, p)
+				                 : KIND == 3 ? (j == 0 ? y : F::gg(y, sum, lap, p))
 				                             : F::ac(y, lap, p);
 		}
 #pragma unroll

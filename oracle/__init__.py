@@ -82,6 +82,10 @@ def lib():
         L.orc_ch_noise_steps_f64.argtypes = [G, dp, dp, dp, dp, C.c_int, C.c_uint64, C.c_int]
         L.orc_ch_noise_steps_f32ref.argtypes = [G, fp, fp, fp, dp, C.c_int, C.c_uint64, C.c_int]
         L.orc_ch_noise_steps_f32.argtypes = [G, fp, fp, fp, dp, C.c_int, C.c_uint64]
+        L.orc_zener_steps_f64.argtypes = [G, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int]
+        L.orc_zener_steps_f32.argtypes = [G, C.c_int, fp, fp, C.c_double, C.c_int]
+        L.orc_ostwald_steps_f64.argtypes = [G, C.c_int, dp, dp, dp, dp, C.c_int, C.c_int]
+        L.orc_ostwald_steps_f32.argtypes = [G, C.c_int, fp, fp, fp, dp, C.c_int]
         L.orc_ch_steps_f32ref.argtypes = [G, fp, fp, fp, dp, C.c_int, C.c_int, C.c_int]
         L.orc_ch_steps_f32.argtypes = [G, fp, fp, fp, dp, C.c_int, C.c_int]
         L.orc_gg_steps_f64.argtypes = [G, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int]
@@ -244,6 +248,44 @@ def ch_noise(a, params, steps, form="f64", step0=0, cube=CUBE_CR, bc=None, dx=No
         y, m = np.empty_like(x), np.empty_like(x)
         lib().orc_ch_noise_steps_f32(C.byref(g), _p(x, C.c_float), _p(y, C.c_float), _p(m, C.c_float),
                                      _p(p, C.c_double), steps, step0)
+    return x
+
+
+def zener(a, dt, steps, form="f64", cube=CUBE_CR, bc=None, dx=None):
+    """Zener pinning; `a` node-major (extents + (Q,)), field 0 the particles."""
+    shape, q = a.shape[:-1], a.shape[-1]
+    g = grid_desc(shape, bc, dx)
+    if form == "f64":
+        x = np.array(a, dtype=np.float64, order="C")
+        y = np.empty_like(x)
+        lib().orc_zener_steps_f64(C.byref(g), q, _p(x, C.c_double), _p(y, C.c_double), dt, steps, cube)
+    elif form == "f32":
+        x = np.array(a, dtype=np.float32, order="C")
+        y = np.empty_like(x)
+        lib().orc_zener_steps_f32(C.byref(g), q, _p(x, C.c_float), _p(y, C.c_float), dt, steps)
+    else:
+        raise ValueError("Zener pinning: forms f64 and f32")
+    return x
+
+
+def ostwald(a, params, steps, form="f64", cube=CUBE_CR, bc=None, dx=None):
+    """Ostwald ripening; `a` node-major, field 0 the concentration;
+    params = {dt, L, D, Calpha, Cbeta, A, delta, epsilon, kappa}."""
+    shape, q = a.shape[:-1], a.shape[-1]
+    g = grid_desc(shape, bc, dx)
+    p = _params(params)
+    if form == "f64":
+        x = np.array(a, dtype=np.float64, order="C")
+        y, w = np.empty_like(x), np.empty(int(np.prod(shape)), dtype=np.float64)
+        lib().orc_ostwald_steps_f64(C.byref(g), q, _p(x, C.c_double), _p(y, C.c_double), _p(w, C.c_double),
+                                    _p(p, C.c_double), steps, cube)
+    elif form == "f32":
+        x = np.array(a, dtype=np.float32, order="C")
+        y, w = np.empty_like(x), np.empty(int(np.prod(shape)), dtype=np.float32)
+        lib().orc_ostwald_steps_f32(C.byref(g), q, _p(x, C.c_float), _p(y, C.c_float), _p(w, C.c_float),
+                                    _p(p, C.c_double), steps)
+    else:
+        raise ValueError("Ostwald ripening: forms f64 and f32")
     return x
 
 

@@ -14,6 +14,8 @@
  *   Allen-Cahn update     examples/phase_transitions/allen-cahn/allen-cahn.cpp:58-73
  *   Cahn-Hilliard update  examples/phase_transitions/cahn-hilliard/explicit/cahn-hilliard.cpp:59-80
  *   grain growth update   examples/coarsening/grain_growth/isotropic/phase_field/graingrowth.cpp:84-110
+ *   Zener pinning         examples/coarsening/zener_pinning/isotropic/phase_field/zener.cpp:104-141
+ *   Ostwald ripening      examples/coarsening/ostwald_ripening/isotropic/phase_field/ostwald.cpp:56-123
  *   model A / model B     examples/phase_transitions/model_A/model_A.cpp:74-105,
  *                         examples/phase_transitions/model_B/model_B.cpp:74-111 (noise: normal
  *                         deviates as gaussian(), model_A.cpp:13-39, from Philox4x32-10 and
@@ -589,6 +591,145 @@ void orc_gg_steps_f32(const orc_grid* g, int Q, float* a, float* b, double dt64,
 				b[i * Q + j] = pj - dt * (-pj - pj * pj * pj + 2.0f * (pj * sum - lap[j]));
 			}
 		}
+		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(float));
+	}
+	free(lap);
+}
+
+/* ---- Zener pinning: zener.cpp:117-135, p = {dt} -------------------------- */
+/* grain growth with `double sum` and field 0 (the particles) held */
+void orc_zener_steps_f64(const orc_grid* g, int Q, double* a, double* b, double dt, int steps, int cube)
+{
+	const int64_t n = nodes_of(g);
+	double* lap = (double*)malloc(sizeof(double) * (size_t)Q);
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			const double* phi = a + i * Q;
+			gg_lap_f64(g, Q, a, i, lap);
+			double sum = 0.0;
+			for (int j = 0; j < Q; j++) sum += phi[j] * phi[j];
+			for (int j = 0; j < Q; j++)
+				b[i * Q + j] = j == 0 ? phi[j]
+				                      : phi[j] - dt * (-phi[j] - cube64(phi[j], cube) + 2.0 * (phi[j] * sum - lap[j]));
+		}
+		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(double));
+	}
+	free(lap);
+}
+
+/* the float form: float arithmetic in the same order */
+void orc_zener_steps_f32(const orc_grid* g, int Q, float* a, float* b, double dt64, int steps)
+{
+	const int64_t n = nodes_of(g);
+	const float dt = (float)dt64;
+	float* lap = (float*)malloc(sizeof(float) * (size_t)Q);
+	int x[3] = {0, 0, 0};
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			const float* phi = a + i * Q;
+			position(g, i, x);
+			for (int j = 0; j < Q; j++) lap[j] = 0.0f;
+			for (int d = 0; d < g->dim; d++) {
+				const float* yh = a + neighbour(g, x, d, +1) * Q;
+				const float* yl = a + neighbour(g, x, d, -1) * Q;
+				float w = (float)weight_of(g, d);
+				for (int j = 0; j < Q; j++) lap[j] += w * (yh[j] - 2.0f * phi[j] + yl[j]);
+			}
+			float sum = 0.0f;
+			for (int j = 0; j < Q; j++) sum += phi[j] * phi[j];
+			for (int j = 0; j < Q; j++) {
+				float pj = phi[j];
+				b[i * Q + j] = j == 0 ? pj : pj - dt * (-pj - pj * pj * pj + 2.0f * (pj * sum - lap[j]));
+			}
+		}
+		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(float));
+	}
+	free(lap);
+}
+
+/* ---- Ostwald ripening: ostwald.cpp:68-122 ----------------------------------
+ * p = {dt, L, D, Calpha, Cbeta, A, delta, epsilon, kappa}; field 0 = C.
+ * ORC_CUBE_POW evaluates pow(x,3) and pow(x,2) with libm as the reference does;
+ * ORC_CUBE_CR uses the correctly rounded cube and x*x, as the device does. */
+static double sq64(double x, int mode) { return mode == ORC_CUBE_POW ? pow(x, 2.0) : x * x; }
+
+void orc_ostwald_steps_f64(const orc_grid* g, int Q, double* a, double* b, double* w, const double* p, int steps,
+                           int cube)
+{
+	const double dt = p[0], L = p[1], D = p[2], Calpha = p[3], Cbeta = p[4], A = p[5], delta = p[6];
+	const double epsilon = p[7], kappa = p[8];
+	const double Cmatrix = 0.5 * (Calpha + Cbeta);
+	const double B = A / pow(Cbeta - Cmatrix, 2);
+	const double gamma = 2.0 / pow(Cbeta - Calpha, 2);
+	const double Dalpha = gamma / pow(delta, 2);
+	const double Dbeta = gamma / pow(delta, 2);
+	const int64_t n = nodes_of(g);
+	double* vlap = (double*)malloc(sizeof(double) * (size_t)Q);
+	orc_grid g1 = *g;
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			double sum = 0.0;
+			for (int j = 1; j < Q; j++) sum += sq64(a[i * Q + j], cube);
+			const double C = a[i * Q];
+			gg_lap_f64(g, Q, a, i, vlap); /* field 0's laplacian: the single-field form, same terms */
+			const double lap = vlap[0];
+			w[i] = -A * (C - Cmatrix) + B * cube64(C - Cmatrix, cube) + Dalpha * cube64(C - Calpha, cube) +
+			       Dbeta * cube64(C - Cbeta, cube) - gamma * (C - Calpha) * sum - kappa * lap;
+		}
+		for (int64_t i = 0; i < n; i++) {
+			const double lap = orc_laplacian_f64(&g1, w, i);
+			const double C = a[i * Q];
+			b[i * Q] = C + dt * D * lap;
+			double sum = 0.0;
+			for (int j = 1; j < Q; j++) sum += sq64(a[i * Q + j], cube);
+			gg_lap_f64(g, Q, a, i, vlap);
+			for (int j = 1; j < Q; j++) {
+				const double value = a[i * Q + j];
+				b[i * Q + j] = value - dt * L * (-gamma * sq64(C - Calpha, cube) + delta * cube64(value, cube) +
+				                                 epsilon * value * (sum - sq64(value, cube)) - kappa * vlap[j]);
+			}
+		}
+		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(double));
+	}
+	free(vlap);
+}
+
+/* the float form: constants derived in double and rounded, float arithmetic */
+void orc_ostwald_steps_f32(const orc_grid* g, int Q, float* a, float* b, float* w, const double* p, int steps)
+{
+	const double Cm64 = 0.5 * (p[3] + p[4]);
+	const double gamma64 = 2.0 / pow(p[4] - p[3], 2);
+	const float A = (float)p[5], B = (float)(p[5] / pow(p[4] - Cm64, 2)), gamma = (float)gamma64;
+	const float Dalpha = (float)(gamma64 / pow(p[6], 2)), Dbeta = Dalpha, delta = (float)p[6];
+	const float epsilon = (float)p[7], kappa = (float)p[8], Calpha = (float)p[3], Cbeta = (float)p[4];
+	const float Cmatrix = (float)Cm64, dtD = (float)(p[0] * p[2]), dtL = (float)(p[0] * p[1]);
+	const int64_t n = nodes_of(g);
+	float* lap = (float*)malloc(sizeof(float) * (size_t)Q);
+	int x[3] = {0, 0, 0};
+	for (int s = 0; s < steps; s++) {
+		for (int64_t i = 0; i < n; i++) {
+			const float* phi = a + i * Q;
+			position(g, i, x);
+			for (int j = 0; j < Q; j++) lap[j] = 0.0f;
+			for (int d = 0; d < g->dim; d++) {
+				const float* yh = a + neighbour(g, x, d, +1) * Q;
+				const float* yl = a + neighbour(g, x, d, -1) * Q;
+				float wt = (float)weight_of(g, d);
+				for (int j = 0; j < Q; j++) lap[j] += wt * (yh[j] - 2.0f * phi[j] + yl[j]);
+			}
+			float sum = 0.0f;
+			for (int j = 1; j < Q; j++) sum += phi[j] * phi[j];
+			const float C = phi[0];
+			const float ca = C - Cmatrix, cb = C - Calpha, cc = C - Cbeta;
+			w[i] = -A * ca + B * (ca * ca * ca) + Dalpha * (cb * cb * cb) + Dbeta * (cc * cc * cc) - gamma * cb * sum -
+			       kappa * lap[0];
+			for (int j = 1; j < Q; j++) {
+				const float v = phi[j];
+				b[i * Q + j] = v - dtL * (-gamma * (cb * cb) + delta * (v * v * v) + epsilon * v * (sum - v * v) -
+				                          kappa * lap[j]);
+			}
+		}
+		for (int64_t i = 0; i < n; i++) b[i * Q] = a[i * Q] + dtD * orc_laplacian_f32(g, w, i);
 		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(float));
 	}
 	free(lap);

@@ -10,6 +10,8 @@
 //   -DHARNESS_GG  includes examples/coarsening/grain_growth/isotropic/phase_field/graingrowth.cpp
 //   -DHARNESS_MA  includes examples/phase_transitions/model_A/model_A.cpp (stock only)
 //   -DHARNESS_MB  includes examples/phase_transitions/model_B/model_B.cpp (stock only)
+//   -DHARNESS_ZN  includes examples/coarsening/zener_pinning/isotropic/phase_field/zener.cpp (stock only)
+//   -DHARNESS_OW  includes examples/coarsening/ostwald_ripening/isotropic/phase_field/ostwald.cpp (stock only)
 // MMSP_MAIN is predefined so that the reference's main() (MMSP.main.hpp:38) is
 // skipped; this file supplies a small main() instead.
 //
@@ -41,8 +43,12 @@
 #include "model_A.cpp"
 #elif defined(HARNESS_MB)
 #include "model_B.cpp"
+#elif defined(HARNESS_ZN)
+#include "zener.cpp"
+#elif defined(HARNESS_OW)
+#include "ostwald.cpp"
 #else
-#error "define one of HARNESS_AC, HARNESS_CH, HARNESS_GG, HARNESS_MA, HARNESS_MB"
+#error "define one of HARNESS_AC, HARNESS_CH, HARNESS_GG, HARNESS_MA, HARNESS_MB, HARNESS_ZN, HARNESS_OW"
 #endif
 
 namespace harness {
@@ -116,12 +122,13 @@ template <int dim, typename T> void update_config(grid<dim, MMSP::vector<T> >& o
 }
 #endif
 
-#if defined(HARNESS_MA) || defined(HARNESS_MB)
-// model A / model B: only the stock update (hard-coded parameters); the noise
-// comes from rand(), seeded by srand(HARNESS_SEED) when that is set
+#if defined(HARNESS_MA) || defined(HARNESS_MB) || defined(HARNESS_ZN) || defined(HARNESS_OW)
+// model A / model B, Zener pinning, Ostwald ripening: only the stock update
+// (hard-coded parameters); model A/B's noise comes from rand(), seeded by
+// srand(HARNESS_SEED) when that is set
 template <class G> void update_config(G&, int)
 {
-	std::fprintf(stderr, "ref harness: model A/B run the stock update only\n");
+	std::fprintf(stderr, "ref harness: this example runs the stock update only\n");
 	std::exit(2);
 }
 #endif
@@ -146,7 +153,7 @@ template <class G> int run(const char* in, const char* out, int steps, bool stoc
 
 template <int dim> int dispatch_type(const std::string& type, const char* in, const char* out, int steps, bool stock)
 {
-#if defined(HARNESS_GG)
+#if defined(HARNESS_GG) || defined(HARNESS_ZN) || defined(HARNESS_OW)
 	if (type == "grid:vector:double") return run<grid<dim, MMSP::vector<double> > >(in, out, steps, stock);
 	if (type == "grid:vector:float") return run<grid<dim, MMSP::vector<float> > >(in, out, steps, stock);
 #else

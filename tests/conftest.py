@@ -37,6 +37,10 @@ def params_of(case):
         return p if v == "config" else [1.0, 1.0, 1.0, 1.0, 0.01]  # cahn-hilliard.cpp:59-63
     if m == "gg":
         return p if v == "config" else [0.01]  # graingrowth.cpp:84
+    if m == "zn":
+        return [0.01]  # zener.cpp:115
+    if m == "ow":
+        return [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]  # ostwald.cpp:68-82
     raise ValueError(m)
 
 
@@ -44,7 +48,7 @@ def model_of(case):
     """Product model id for a golden case."""
     if case["model"] == "ch":
         return "ch_dw" if case["variant"] == "config" else "ch"
-    return case["model"]
+    return {"zn": "zener", "ow": "ostwald"}.get(case["model"], case["model"])
 
 
 @pytest.fixture(scope="session")

@@ -48,6 +48,12 @@ CASES = [
     ("gg3d_f64", "gg", (10, 8, 12), 3, "f64", None, None, 173, (0.0, 1.0), 30, "stock", []),
     ("gg2d_f32ref", "gg", (24, 20), 6, "f32", None, None, 179, (0.0, 0.01), 60, "config", [0.05]),
     ("gg2d_bc_f64", "gg", (14, 18), 4, "f64", [(N, M), (D, N)], [1.0, 0.5], 181, (0.0, 1.0), 20, "stock", []),
+    ("zn2d_f64", "zn", (20, 70), 4, "f64", [(N, M), (P, D)], None, 191, (0.0, 1.0), 30, "stock", []),
+    ("zn3d_f64", "zn", (8, 10, 12), 3, "f64", None, None, 193, (0.0, 1.0), 20, "stock", []),
+    ("ow1d_f64", "ow", (64,), 2, "f64", [(N, D)], None, 197, (0.0, 1.0), 60, "stock", []),
+    ("ow2d_f64", "ow", (24, 20), 3, "f64", None, None, 199, (0.0, 1.0), 60, "stock", []),
+    ("ow3d_f64", "ow", (8, 9, 10), 3, "f64", [(M, N), (P, P), (D, P)], [0.5, 1.0, 2.0], 211, (0.0, 1.0), 30, "stock",
+     []),
 ]
 
 

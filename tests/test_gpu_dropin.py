@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 mx = pytest.importorskip("mmsp_amd")
 
 PROGRAMS = {"ac": ("allen-cahn", "allen-cahn-gpu"), "ch": ("cahn-hilliard", "cahn-hilliard-gpu"),
-            "gg": ("graingrowth", "graingrowth-gpu")}
+            "gg": ("graingrowth", "graingrowth-gpu"), "zn": ("zener", "zener-gpu"), "ow": ("ostwald", "ostwald-gpu")}
 
 
 def say(*a):
@@ -50,7 +50,8 @@ def progress_shape(out):
 # stock-parameter golden cases: the drop-in programs use the examples' own parameters
 # (the stock programs' typedefs are double grids, so the float fixtures are not used here)
 CASES = [("ac2d_f64", "ac"), ("ac3d_bc_f64", "ac"), ("ch2d_stock_f64", "ch"),
-         ("gg3d_f64", "gg"), ("gg2d_bc_f64", "gg"), ("gg1d_f64", "gg")]
+         ("gg3d_f64", "gg"), ("gg2d_bc_f64", "gg"), ("gg1d_f64", "gg"),
+         ("zn2d_f64", "zn"), ("zn3d_f64", "zn"), ("ow2d_f64", "ow"), ("ow3d_f64", "ow")]
 
 
 @pytest.mark.parametrize("name,model", CASES)
@@ -93,6 +94,10 @@ def test_dropin_program_matches_stock_program(name, model, tmp_path):
         exp = O.ac(gin.data, p, steps, form=form, bc=bc, dx=gin.dx)
     elif model == "ch":
         exp = O.ch(gin.data, p, steps, mu=O.MU_POLY, form=form, bc=bc, dx=gin.dx)
+    elif model == "zn":
+        exp = O.zener(gin.data, p[0], steps, form=form, bc=bc, dx=gin.dx)
+    elif model == "ow":
+        exp = O.ostwald(gin.data, p, steps, form=form, bc=bc, dx=gin.dx)
     else:
         exp = O.gg(gin.data, p[0], steps, form=form, bc=bc, dx=gin.dx)
     last = dat.read_dat(str(dg / files_g[-1])).data

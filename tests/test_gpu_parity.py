@@ -43,6 +43,10 @@ def oracle_run(model, a, params, steps, form, bc=None, dx=None, cube=O.CUBE_CR):
         return O.ch(a, params, steps, mu=O.MU_POLY, form=form, cube=cube, bc=bc, dx=dx)
     if model == "ch_dw":
         return O.ch(a, params, steps, mu=O.MU_DW, form=form, cube=cube, bc=bc, dx=dx)
+    if model == "zener":
+        return O.zener(a, params[0], steps, form=form, cube=cube, bc=bc, dx=dx)
+    if model == "ostwald":
+        return O.ostwald(a, params, steps, form=form, cube=cube, bc=bc, dx=dx)
     return O.gg(a, params[0], steps, form=form, cube=cube, bc=bc, dx=dx)
 
 
@@ -389,3 +393,48 @@ def test_c2_cahn_hilliard_full_size_mass():
             g.step("ch_dw", p, 50)
             outs.append(g.download())
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_zener_and_ostwald(prec):
+    """Zener pinning on the generic kernel and the wave strips (every compiled
+    field count, partial strips, several boundaries), Ostwald ripening on the
+    generic two-pass kernels in 1-D, 2-D and 3-D: bitwise against the oracle."""
+    zcases = [((70, 300), 4, [(N, M), (D, P)]), ((40, 130), 2, None), ((9, 70), 8, [(M, M), (N, D)]),
+              ((33, 64), 16, [(P, P), (M, N)])]
+    if prec == "f32":
+        zcases.append(((20, 100), 32, [(P, P), (N, P)]))
+    for shape, q, bc in zcases:
+        say("[gpu] zener %s q=%d %s" % (shape, q, prec))
+        a = O.uniform(int(np.prod(shape)) * q, 600 + q, 0.0, 1.0).reshape(shape + (q,))
+        if prec == "f32":
+            a = a.astype(np.float32)
+        exp = O.zener(a, 0.01, 9, form=prec, bc=bc)
+        for variant, tile in [(1, None), (2, (1, 64, 8)), (2, (1, 64, 24)), (0, None)]:
+            with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+                g.set_variant(variant)
+                if tile:
+                    g.set_tile(*tile)
+                    g.set_tile_options(body=2)
+                g.upload(a)
+                g.step("zener", [0.01], 9)
+                assert np.array_equal(g.download(), exp), (shape, q, variant, tile)
+    p = [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]
+    for shape, q, bc in [((90,), 2, [(N, D)]), ((40, 36), 3, None), ((10, 9, 12), 4, [(M, N), (P, P), (D, P)])]:
+        say("[gpu] ostwald %s q=%d %s" % (shape, q, prec))
+        a = O.uniform(int(np.prod(shape)) * q, 700 + q, 0.0, 1.0).reshape(shape + (q,))
+        if prec == "f32":
+            a = a.astype(np.float32)
+        exp = O.ostwald(a, p, 12, form=prec, bc=bc)
+        with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+            g.upload(a)
+            g.step("ostwald", p, 5)
+            g.step("ostwald", p, 7)
+            assert np.array_equal(g.download(), exp), (shape, q)
+    with mx.Grid((8, 8), fields=3, vector=True, precision="f32ref") as g:
+        with pytest.raises(mx.MmspxError) as e:
+            g.step("ostwald", p, 1)
+        assert e.value.code == -8
+    with mx.Grid((8, 8), fields=1, vector=True, precision=prec) as g:
+        with pytest.raises(mx.MmspxError):
+            g.step("ostwald", p, 1)  # no order parameter

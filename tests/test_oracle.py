@@ -33,6 +33,10 @@ def run_oracle(case, a, cube, bc=None, dx=None):
     if case["model"] == "ch":
         mu = O.MU_DW if case["variant"] == "config" else O.MU_POLY
         return O.ch(a, p, case["steps"], mu=mu, form=form, cube=cube, bc=bc, dx=dx)
+    if case["model"] == "zn":
+        return O.zener(a, p[0], case["steps"], form=form, cube=cube, bc=bc, dx=dx)
+    if case["model"] == "ow":
+        return O.ostwald(a, p, case["steps"], form=form, cube=cube, bc=bc, dx=dx)
     return O.gg(a, p[0], case["steps"], form=form, cube=cube, bc=bc, dx=dx)
 
 
@@ -331,3 +335,56 @@ def test_dat_readable_by_reference_reader(tmp_path):
         subprocess.run([os.path.join(O.REF_DIR, "mmsp2tsv"), str(f), str(out)], check=True, capture_output=True)
         vals = [float(line.split()[-1]) for line in open(out) if line.strip() and not line.startswith("#")]
         assert np.allclose(sorted(vals), sorted(a.reshape(-1)), atol=1e-12)
+
+
+# ---- Zener pinning and Ostwald ripening --------------------------------------------
+@pytest.mark.parametrize("form", ["f64", "f32"])
+def test_zener_holds_the_particles_and_is_grain_growth_without_them(form):
+    """Field 0 keeps its values bitwise; with field 0 zero everywhere the other
+    fields follow grain growth exactly (field 0 then adds nothing to the sum)."""
+    shape, q = (14, 17), 4
+    a = O.uniform(int(np.prod(shape)) * q, 301, 0.0, 1.0).reshape(shape + (q,))
+    if form == "f32":
+        a = a.astype(np.float32)
+    bc = [(O.NEUMANN, O.MIRROR), (O.PERIODIC, O.DIRICHLET)]
+    z = O.zener(a, 0.01, 9, form=form, bc=bc)
+    assert np.array_equal(z[..., 0], a[..., 0])
+    a0 = a.copy()
+    a0[..., 0] = 0
+    assert np.array_equal(O.zener(a0, 0.01, 9, form=form, bc=bc), O.gg(a0, 0.01, 9, form=form, bc=bc))
+
+
+@pytest.mark.parametrize("form", ["f64", "f32"])
+def test_ostwald_conserves_the_concentration(form):
+    """C' = C + dt*D*lap(w): on a periodic grid the concentration's sum is
+    invariant (to rounding), while the order parameters are not."""
+    shape, q = (20, 18, 16), 3
+    a = O.uniform(int(np.prod(shape)) * q, 303, 0.0, 1.0).reshape(shape + (q,))
+    if form == "f32":
+        a = a.astype(np.float32)
+    p = [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]
+    out = O.ostwald(a, p, 40, form=form)
+    m0, m1 = O.mass(a, vector=True), O.mass(out, vector=True)
+    assert abs(m1[0] - m0[0]) / abs(m0[0]) < (1e-13 if form == "f64" else 1e-5)
+    assert np.all(np.abs(m1[1:] - m0[1:]) / np.abs(m0[1:]) > 1e-4)
+
+
+def test_ostwald_uniform_state_follows_the_local_map():
+    """A spatially uniform state has zero laplacians: C stays and each order
+    parameter follows phi' = phi - dt*L*(-gamma (C-Ca)^2 + delta phi^3 +
+    epsilon phi (sum - phi^2)), evaluated here in Python floats from
+    the model's definition (sum over the order parameters)."""
+    q, steps = 3, 5
+    vals = [0.3, 0.62, 0.17]
+    a = np.broadcast_to(np.array(vals), (6, 5, q)).copy()
+    p = [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]
+    out = O.ostwald(a, p, steps)
+    dt, L, Ca, Cb, delta, eps = 0.001, 1.0, 0.05, 0.95, 1.0, 3.0
+    gamma = 2.0 / (Cb - Ca) ** 2
+    C, phi = vals[0], list(vals[1:])
+    for _ in range(steps):
+        s = sum(v * v for v in phi)
+        phi = [v - dt * L * (-gamma * (C - Ca) ** 2 + delta * v ** 3 + eps * v * (s - v * v)) for v in phi]
+    assert np.all(out[..., 0] == C)
+    for j in range(1, q):
+        assert np.allclose(out[..., j], phi[j - 1], rtol=1e-13, atol=0)
