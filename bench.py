@@ -1,7 +1,7 @@
 """Benchmark of the explicit update path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload ch3d|ch3d_f32|ac2d|gg2d|model_b3d|batch_ac|serve|ckpt]
+                    [--workload ch3d|ch3d_f32|ac2d|gg2d|zener2d|ostwald2d|model_b3d|batch_ac|serve|ckpt]
                     [--no-cpu-baseline] [--tune | --tile TY,TZ,CHUNK[,OPT]]
                     [--requests N] [--hbm-budget-gb G]
                     [--graph-steps G] [--no-sequential] [--ckpt-threads T]
@@ -59,6 +59,13 @@ WORKLOADS = {
                  lo=0.0, hi=0.01, fields=32, bytes_per_cell=256, name="grain-growth-2d-q32", cpu_steps=2),
     # model B with the example's parameters (model_B.cpp:86-92: r = u = K = M = 1,
     # dt = 0.01, kT = 0.01, dV = 1, so sigma = sqrt(2 kT / (dt dV))) on the C2 grid
+    # Zener pinning (zener.cpp:115: dt = 0.01), 2048^2 with 16 fields (field 0 the particles), FP32
+    "zener2d": dict(model="zener", shape=(2048, 2048), prec="f32", params=[0.01], steps=500, seed=20260109,
+                    lo=0.0, hi=1.0, fields=16, bytes_per_cell=2 * 16 * 4, name="zener-2d-q16", cpu_steps=2),
+    # Ostwald ripening with the example's constants (ostwald.cpp:68-82), 1024^2, C and 3 order parameters, FP64
+    "ostwald2d": dict(model="ostwald", shape=(1024, 1024), prec="f64",
+                      params=[0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0], steps=500, seed=20260110,
+                      lo=0.0, hi=1.0, fields=4, bytes_per_cell=2 * 4 * 8, name="ostwald-2d-q4", cpu_steps=2),
     "model_b3d": dict(model="ch_noise", shape=(256, 256, 256), prec="f64",
                       params=[1.0, 1.0, 1.0, 1.0, 0.01, math.sqrt(2.0 * 0.01 / (0.01 * 1.0)), 20260107], steps=2000,
                       seed=20260108, lo=-0.01, hi=0.01, fields=1, bytes_per_cell=16, name="model-b-3d", cpu_steps=2),
@@ -76,7 +83,7 @@ def cpu_baseline(w, init):
 
     shape, steps = w["shape"], w["cpu_steps"]
     nodes = int(np.prod(shape))
-    ref_bin = {"ch_dw": "ch", "ac": "ac", "gg": "gg", "ch_noise": "mb"}[w["model"]]
+    ref_bin = {"ch_dw": "ch", "ac": "ac", "gg": "gg", "ch_noise": "mb", "zener": "zn", "ostwald": "ow"}[w["model"]]
     if O.ref_available() and os.path.exists(os.path.join(O.REF_DIR, "ref_" + ref_bin)):
         with tempfile.TemporaryDirectory() as td:
             fin = os.path.join(td, "in.dat")
@@ -85,9 +92,9 @@ def cpu_baseline(w, init):
             if w["prec"] != "f64":
                 data = data.astype(np.float32)
             dat.write_dat(fin, dat.make_grid(data, vector=vec), compress=False)
-            model = {"ch_dw": "ch", "ac": "ac", "gg": "gg", "ch_noise": "mb"}[w["model"]]
-            # model B: the example's own update (its parameters are the workload's)
-            variant = "stock" if model == "mb" else "config"
+            model = ref_bin
+            # model B, Zener, Ostwald: the example's own update (its parameters are the workload's)
+            variant = "stock" if model in ("mb", "zn", "ow") else "config"
             sec = O.ref_run(model, fin, "-", steps, variant, w["params"], timeout=600)
         kind, what = "reference", "oracle/_ref reference update() compiled from the reference sources"
     else:
@@ -98,6 +105,10 @@ def cpu_baseline(w, init):
             O.ac(init.reshape(shape), w["params"], steps)
         elif w["model"] == "ch_noise":
             O.ch_noise(init.reshape(shape), w["params"], steps)
+        elif w["model"] == "zener":
+            O.zener(init.reshape(shape + (w["fields"],)), w["params"][0], steps)
+        elif w["model"] == "ostwald":
+            O.ostwald(init.reshape(shape + (w["fields"],)), w["params"], steps)
         else:
             O.gg(init.reshape(shape + (w["fields"],)), w["params"][0], steps)
         sec = time.perf_counter() - t0
@@ -535,7 +546,7 @@ def main():
     per_launch = alg_bytes / lps
     achieved = per_launch / mean_launch_s / 1e9
     value = nodes * K / dev_s
-    base = {"ch_dw": "ch", "ch": "ch", "ch_noise": "ch", "ac": "ac", "gg": "gg"}[w["model"]]
+    base = {"ch_dw": "ch", "ch": "ch", "ch_noise": "ch", "ac": "ac", "gg": "gg", "zener": "gg", "ostwald": "ow"}[w["model"]]
     if tile["variant"] != 2:
         kname = base + "_generic"
     elif base == "ch":
