@@ -82,7 +82,7 @@ enum {
 	/* Vector-laplacian models (vector grids; forms MMSPX_F64 and MMSPX_F32): */
 	MMSPX_ZENER = 6,    /* grain growth with Zener pinning, zener_pinning/isotropic/phase_field/zener.cpp:104-141.
 	                       p = {dt}; field 0 (the particles) keeps its values, fields j >= 1 as MMSPX_GG */
-	MMSPX_OSTWALD = 7   /* Ostwald ripening, ostwald_ripening/isotropic/phase_field/ostwald.cpp:56-123.
+	MMSPX_OSTWALD = 7,  /* Ostwald ripening, ostwald_ripening/isotropic/phase_field/ostwald.cpp:56-123.
 	                       p = {dt, L, D, Calpha, Cbeta, A, delta, epsilon, kappa}; field 0 is the
 	                       concentration C, fields j >= 1 the order parameters; Cmatrix = (Calpha+Cbeta)/2,
 	                       B = A/(Cbeta-Cmatrix)^2, gamma = 2/(Cbeta-Calpha)^2, Dalpha = Dbeta = gamma/delta^2;
@@ -90,6 +90,12 @@ enum {
 	                           - gamma(C-Calpha) sum_j phi_j^2 - kappa lap C;  C' = C + dt*D*lap(w);
 	                       phi_j' = phi_j - dt*L*(-gamma(C-Calpha)^2 + delta phi_j^3
 	                                + epsilon phi_j (sum - phi_j^2) - kappa lap phi_j) */
+	MMSPX_GG_ANISO = 8  /* multi-phase grain growth, grain_growth/anisotropic/phase_field/graingrowth.cpp:94-191.
+	                       p = {dt, width, gamma, mu}: every pair of grains with boundary energy gamma and
+	                       mobility mu (the example's anisotropy.hpp gives 1 and 1, as float). Per node: the
+	                       fields present in the node's 2*dim+1 neighbourhood; if fewer than two, the node is
+	                       copied; else pairwise variational derivatives and rates, phi' = phi + dt*(2/S)*dphi/dt
+	                       clamped to [0, 1], then divided by the sum (Gibbs simplex). At most 32 fields. */
 };
 /* The noise: g(seed, step, node) is a standard normal deviate (float), the
  * Box-Muller transform of the first two words of one Philox4x32-10 block

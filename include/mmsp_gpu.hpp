@@ -253,6 +253,16 @@ template <int dim, typename T> void zener(MMSP::grid<dim, MMSP::vector<T> >& gri
 	run_vector(grid, MMSPX_ZENER, p, 1, steps);
 }
 
+// Multi-phase grain growth, grain_growth/anisotropic/phase_field/graingrowth.cpp:94-191;
+// gamma and mu: every pair's boundary energy and mobility (anisotropy.hpp gives 1).
+template <int dim, typename T>
+void grain_growth_multiphase(MMSP::grid<dim, MMSP::vector<T> >& grid, int steps, double dt = 0.01, double width = 8.0,
+                             double gamma = 1.0, double mu = 1.0)
+{
+	const double p[4] = {dt, width, gamma, mu};
+	run_vector(grid, MMSPX_GG_ANISO, p, 4, steps);
+}
+
 // Ostwald ripening, ostwald_ripening/isotropic/phase_field/ostwald.cpp:56-123;
 // field 0 is the concentration. Defaults: the example's constants (:66-84).
 template <int dim, typename T>

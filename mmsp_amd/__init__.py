@@ -25,14 +25,14 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "mmsp_amd.h")
 
 MIRROR, NEUMANN, PERIODIC, PARALLEL, DIRICHLET = 0, 1, 2, 3, 4
 F64, F32, F32REF = 0, 1, 2
-AC, CH, CH_DW, GG, AC_NOISE, CH_NOISE, ZENER, OSTWALD = 0, 1, 2, 3, 4, 5, 6, 7
+AC, CH, CH_DW, GG, AC_NOISE, CH_NOISE, ZENER, OSTWALD, GG_ANISO = 0, 1, 2, 3, 4, 5, 6, 7, 8
 NODE_MAJOR, FIELD_MAJOR = 0, 1
 MASS, ENERGY = 0, 1
 
 PRECISIONS = {"f64": F64, "f32": F32, "f32ref": F32REF}
 MODELS = {"ac": AC, "ch": CH, "ch_dw": CH_DW, "gg": GG, "ac_noise": AC_NOISE, "ch_noise": CH_NOISE, "zener": ZENER,
-          "ostwald": OSTWALD}
-NPARAMS = {AC: 5, CH: 5, CH_DW: 3, GG: 1, AC_NOISE: 7, CH_NOISE: 7, ZENER: 1, OSTWALD: 9}
+          "ostwald": OSTWALD, "gg_aniso": GG_ANISO}
+NPARAMS = {AC: 5, CH: 5, CH_DW: 3, GG: 1, AC_NOISE: 7, CH_NOISE: 7, ZENER: 1, OSTWALD: 9, GG_ANISO: 4}
 
 
 class MmspxError(RuntimeError):

@@ -89,6 +89,7 @@ int model_nparams(int model)
 	case MMSPX_CH_NOISE: return 7;
 	case MMSPX_ZENER: return 1;
 	case MMSPX_OSTWALD: return 9;
+	case MMSPX_GG_ANISO: return 4;
 	}
 	return -1;
 }
@@ -148,6 +149,18 @@ int make_params(int model, const double* p, int np, ModelParams* out)
 		m.ow[OW_DTD] = dt * D;
 		m.ow[OW_DTL] = dt * L;
 		for (int k = 0; k < 13; k++) m.owf[k] = (float)m.ow[k];
+	} else if (model == MMSPX_GG_ANISO) {
+		// p = {dt, width, gamma, mu}; energy() and mobility() return float
+		// (anisotropy.hpp), then eps and w as graingrowth.cpp:146-148 form them
+		const double dt = p[0], width = p[1];
+		const double gamma = (double)(float)p[2], mu = (double)(float)p[3];
+		if (!(width > 0.0) || !(gamma >= 0.0)) return fail(MMSPX_EINVAL, "width must be > 0 and gamma >= 0");
+		const double eps = 4.0 / std::acos(-1.0) * std::sqrt(0.5 * gamma * width);
+		m.an[AN_C2] = 0.5 * eps * eps;
+		m.an[AN_W] = 4.0 * gamma / width;
+		m.an[AN_MU] = mu;
+		m.an[AN_DT] = dt;
+		for (int k = 0; k < 4; k++) m.anf[k] = (float)m.an[k];
 	} else {
 		m.dt = p[0];
 		m.dtf = (float)p[0];
@@ -320,8 +333,11 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 		if (!s->desc.vector) return fail(MMSPX_EINVAL, "grain growth, Zener pinning and Ostwald ripening need a vector grid");
 		if (model == MMSPX_OSTWALD && s->geom.fields < 2)
 			return fail(MMSPX_EINVAL, "Ostwald ripening needs a concentration field and at least one order parameter");
-		if ((model == MMSPX_ZENER || model == MMSPX_OSTWALD) && s->desc.precision == MMSPX_F32REF)
-			return fail(MMSPX_EUNSUPPORTED, "Zener pinning and Ostwald ripening run the MMSPX_F64 and MMSPX_F32 forms");
+		if ((model == MMSPX_ZENER || model == MMSPX_OSTWALD || model == MMSPX_GG_ANISO) &&
+		    s->desc.precision == MMSPX_F32REF)
+			return fail(MMSPX_EUNSUPPORTED, "this model runs the MMSPX_F64 and MMSPX_F32 forms");
+		if (model == MMSPX_GG_ANISO && s->geom.fields > kAnisoMaxFields)
+			return fail(MMSPX_EUNSUPPORTED, "anisotropic grain growth takes at most 32 fields");
 	} else if (s->desc.vector) {
 		return fail(MMSPX_EINVAL, "Allen-Cahn and Cahn-Hilliard need a scalar grid");
 	}

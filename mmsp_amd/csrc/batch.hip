@@ -161,7 +161,8 @@ bool batch_model_ok(int model, const Geom& g, std::string* why)
 		return no("batches do not run the stochastic forms (each member would need its own step count)");
 	case MMSPX_ZENER:
 	case MMSPX_OSTWALD:
-		return no("batches do not run Zener pinning or Ostwald ripening");
+	case MMSPX_GG_ANISO:
+		return no("batches do not run Zener pinning, Ostwald ripening or multi-phase grain growth");
 	}
 	return no("unknown model");
 }

@@ -56,10 +56,19 @@ struct ModelParams {
 	// Ostwald ripening constants, derived as ostwald.cpp:68-82 derives them
 	double ow[13];
 	float owf[13];
+	// anisotropic grain growth: 0.5*eps*eps, w = 4 gamma/width, mu, dt
+	// (graingrowth.cpp:146-148, 165, 105)
+	double an[4];
+	float anf[4];
 };
+enum { AN_C2, AN_W, AN_MU, AN_DT };
+constexpr int kAnisoMaxFields = 32;
 enum { OW_A, OW_B, OW_G, OW_DA, OW_DB, OW_DELTA, OW_EPS, OW_K, OW_CA, OW_CB, OW_CM, OW_DTD, OW_DTL };
 
-inline bool vector_model(int model) { return model == MMSPX_GG || model == MMSPX_ZENER || model == MMSPX_OSTWALD; }
+inline bool vector_model(int model)
+{
+	return model == MMSPX_GG || model == MMSPX_ZENER || model == MMSPX_OSTWALD || model == MMSPX_GG_ANISO;
+}
 
 inline bool noisy_model(int model) { return model == MMSPX_AC_NOISE || model == MMSPX_CH_NOISE; }
 inline bool ch_family(int model) { return model == MMSPX_CH || model == MMSPX_CH_DW || model == MMSPX_CH_NOISE; }

@@ -175,6 +175,100 @@ __global__ void __launch_bounds__(256) ow_pass1(const typename Form<PREC>::S* __
 	}
 }
 
+// Multi-phase (anisotropic) grain growth, grain_growth/anisotropic/phase_field/
+// graingrowth.cpp:108-188, one thread per node, at most kAnisoMaxFields
+// fields. The fields present in the node's 2*dim+1 neighbourhood (mapped as
+// check_boundary maps) form the active set; below two the node is copied.
+// Otherwise the pairwise variational derivatives (:141-157) and rates
+// (:160-167), the clamped update and the projection onto the simplex
+// (:170-185), in the reference's loop and operation order.
+template <int DIM, int PREC>
+__global__ void __launch_bounds__(256) gga_generic(const typename Form<PREC>::S* __restrict__ a,
+                                                   typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	constexpr bool dbl = PREC == MMSPX_F64;
+	const S c2 = dbl ? (S)p.an[AN_C2] : (S)p.anf[AN_C2];
+	const S w = dbl ? (S)p.an[AN_W] : (S)p.anf[AN_W];
+	const S mu = dbl ? (S)p.an[AN_MU] : (S)p.anf[AN_MU];
+	const S dt = dbl ? (S)p.an[AN_DT] : (S)p.anf[AN_DT];
+	const int64_t N = g.nodes;
+	const int Q = g.fields;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.nodes; i += gridDim.x * blockDim.x) {
+		int x[3];
+		coords<DIM>(g, i, x);
+		int ih[3], il[3];
+#pragma unroll
+		for (int d = 0; d < DIM; d++) {
+			ih[d] = nbr(g, i, x, d, +1);
+			il[d] = nbr(g, i, x, d, -1);
+		}
+		uint32_t act = 0;
+		S cnt = 0;
+		for (int h = 0; h < Q; h++) {
+			const S* __restrict__ pl = a + h * N;
+			bool on = pl[i] > S(0);
+#pragma unroll
+			for (int d = 0; d < DIM; d++) on = on || pl[ih[d]] > S(0) || pl[il[d]] > S(0);
+			if (on) {
+				act |= 1u << h;
+				cnt += S(1);
+			}
+		}
+		if (cnt < S(2)) {
+			for (int h = 0; h < Q; h++) b[h * N + i] = a[h * N + i];
+			continue;
+		}
+		S phi[kAnisoMaxFields], lap[kAnisoMaxFields], dF[kAnisoMaxFields], dp[kAnisoMaxFields];
+		for (int h = 0; h < Q; h++) {
+			const S* __restrict__ pl = a + h * N;
+			phi[h] = pl[i];
+			typename F::L l = F::lap_zero();
+			if (act >> h & 1u) {
+#pragma unroll
+				for (int d = 0; d < DIM; d++) F::lap_add(l, g.w[d], g.wf[d], pl[ih[d]], phi[h], pl[il[d]]);
+			}
+			lap[h] = l;
+			dF[h] = 0;
+			dp[h] = 0;
+		}
+		for (int h = 0; h < Q; h++) {
+			if (!(act >> h & 1u)) continue;
+			for (int j = h + 1; j < Q; j++) {
+				if (!(act >> j & 1u)) continue;
+				dF[h] += c2 * lap[j] + w * phi[j];
+				dF[j] += c2 * lap[h] + w * phi[h];
+				for (int k = j + 1; k < Q; k++) {
+					if (!(act >> k & 1u)) continue;
+					dF[h] += S(3) * phi[j] * phi[k];
+					dF[j] += S(3) * phi[k] * phi[h];
+					dF[k] += S(3) * phi[h] * phi[j];
+				}
+			}
+		}
+		for (int h = 0; h < Q; h++) {
+			if (!(act >> h & 1u)) continue;
+			for (int j = h + 1; j < Q; j++) {
+				if (!(act >> j & 1u)) continue;
+				dp[h] -= mu * (dF[h] - dF[j]);
+				dp[j] -= mu * (dF[j] - dF[h]);
+			}
+		}
+		S sum = 0;
+		const S rate = dt * (S(2) / cnt);
+		for (int h = 0; h < Q; h++) {
+			S v = phi[h] + rate * dp[h];
+			if (v > S(1)) v = S(1);
+			if (v < S(0)) v = S(0);
+			phi[h] = v;
+			sum += v;
+		}
+		const S rsum = fabs(sum) > S(0) ? S(1) / sum : S(0);
+		for (int h = 0; h < Q; h++) b[h * N + i] = phi[h] * rsum;
+	}
+}
+
 template <int DIM, int PREC>
 __global__ void __launch_bounds__(256) ow_pass2(const typename Form<PREC>::S* __restrict__ a,
                                                 const typename Form<PREC>::S* __restrict__ w,
@@ -236,6 +330,14 @@ static int launch_dim_prec(const Geom& g, const ModelParams& p, const void* av, 
 	case MMSPX_ZENER:
 		hipLaunchKernelGGL((gg_generic<DIM, PREC, true>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
 		*launches += 1;
+		break;
+	case MMSPX_GG_ANISO:
+		if constexpr (PREC == MMSPX_F32REF) {
+			return fail(MMSPX_EUNSUPPORTED, "anisotropic grain growth runs the MMSPX_F64 and MMSPX_F32 forms");
+		} else {
+			hipLaunchKernelGGL((gga_generic<DIM, PREC>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
+			*launches += 1;
+		}
 		break;
 	case MMSPX_OSTWALD:
 		if constexpr (PREC == MMSPX_F32REF) {

@@ -86,6 +86,8 @@ def lib():
         L.orc_zener_steps_f32.argtypes = [G, C.c_int, fp, fp, C.c_double, C.c_int]
         L.orc_ostwald_steps_f64.argtypes = [G, C.c_int, dp, dp, dp, dp, C.c_int, C.c_int]
         L.orc_ostwald_steps_f32.argtypes = [G, C.c_int, fp, fp, fp, dp, C.c_int]
+        L.orc_gga_steps_f64.argtypes = [G, C.c_int, dp, dp, dp, C.c_int]
+        L.orc_gga_steps_f32.argtypes = [G, C.c_int, fp, fp, dp, C.c_int]
         L.orc_ch_steps_f32ref.argtypes = [G, fp, fp, fp, dp, C.c_int, C.c_int, C.c_int]
         L.orc_ch_steps_f32.argtypes = [G, fp, fp, fp, dp, C.c_int, C.c_int]
         L.orc_gg_steps_f64.argtypes = [G, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int]
@@ -286,6 +288,24 @@ def ostwald(a, params, steps, form="f64", cube=CUBE_CR, bc=None, dx=None):
                                     _p(p, C.c_double), steps)
     else:
         raise ValueError("Ostwald ripening: forms f64 and f32")
+    return x
+
+
+def gg_aniso(a, params, steps, form="f64", bc=None, dx=None):
+    """Multi-phase grain growth; `a` node-major; params = {dt, width, gamma, mu}."""
+    shape, q = a.shape[:-1], a.shape[-1]
+    g = grid_desc(shape, bc, dx)
+    p = _params(params)
+    if form == "f64":
+        x = np.array(a, dtype=np.float64, order="C")
+        y = np.empty_like(x)
+        lib().orc_gga_steps_f64(C.byref(g), q, _p(x, C.c_double), _p(y, C.c_double), _p(p, C.c_double), steps)
+    elif form == "f32":
+        x = np.array(a, dtype=np.float32, order="C")
+        y = np.empty_like(x)
+        lib().orc_gga_steps_f32(C.byref(g), q, _p(x, C.c_float), _p(y, C.c_float), _p(p, C.c_double), steps)
+    else:
+        raise ValueError("multi-phase grain growth: forms f64 and f32")
     return x
 
 

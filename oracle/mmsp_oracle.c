@@ -16,6 +16,7 @@
  *   grain growth update   examples/coarsening/grain_growth/isotropic/phase_field/graingrowth.cpp:84-110
  *   Zener pinning         examples/coarsening/zener_pinning/isotropic/phase_field/zener.cpp:104-141
  *   Ostwald ripening      examples/coarsening/ostwald_ripening/isotropic/phase_field/ostwald.cpp:56-123
+ *   multi-phase GG        examples/coarsening/grain_growth/anisotropic/phase_field/graingrowth.cpp:94-191
  *   model A / model B     examples/phase_transitions/model_A/model_A.cpp:74-105,
  *                         examples/phase_transitions/model_B/model_B.cpp:74-111 (noise: normal
  *                         deviates as gaussian(), model_A.cpp:13-39, from Philox4x32-10 and
@@ -733,6 +734,149 @@ void orc_ostwald_steps_f32(const orc_grid* g, int Q, float* a, float* b, float* 
 		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(float));
 	}
 	free(lap);
+}
+
+/* ---- anisotropic (multi-phase) grain growth ---------------------------------
+ * grain_growth/anisotropic/phase_field/graingrowth.cpp:94-191, p = {dt, width,
+ * gamma, mu}; energy() and mobility() (anisotropy.hpp) return float. No pow:
+ * the restatement is bitwise the reference. */
+#define ORC_GGA_MAXQ 64
+void orc_gga_steps_f64(const orc_grid* g, int Q, double* a, double* b, const double* p, int steps)
+{
+	const double dt = p[0], width = p[1];
+	const double gamma = (double)(float)p[2], mu = (double)(float)p[3];
+	const double eps = 4.0 / acos(-1.0) * sqrt(0.5 * gamma * width);
+	const double w = 4.0 * gamma / width;
+	const int64_t n = nodes_of(g);
+	double lap[ORC_GGA_MAXQ], dFdp[ORC_GGA_MAXQ], dpdt[ORC_GGA_MAXQ];
+	int s[ORC_GGA_MAXQ];
+	int x[3] = {0, 0, 0};
+	for (int st = 0; st < steps; st++) {
+		for (int64_t i = 0; i < n; i++) {
+			const double* phi = a + i * Q;
+			position(g, i, x);
+			double S = 0.0;
+			for (int h = 0; h < Q; h++) {
+				s[h] = 0;
+				for (int j = 0; j < g->dim && !s[h]; j++)
+					for (int k = -1; k <= 1 && !s[h]; k++)
+						if (a[neighbour(g, x, j, k) * Q + h] > 0.0) s[h] = 1;
+				S += s[h];
+			}
+			if (S < 2.0) {
+				for (int h = 0; h < Q; h++) b[i * Q + h] = phi[h];
+				continue;
+			}
+			gg_lap_f64(g, Q, a, i, lap);
+			for (int h = 0; h < Q; h++) dFdp[h] = dpdt[h] = 0.0;
+			for (int h = 0; h < Q; h++)
+				if (s[h] > 0.0)
+					for (int j = h + 1; j < Q; j++)
+						if (s[j] > 0.0) {
+							dFdp[h] += 0.5 * eps * eps * lap[j] + w * phi[j];
+							dFdp[j] += 0.5 * eps * eps * lap[h] + w * phi[h];
+							for (int k = j + 1; k < Q; k++)
+								if (s[k] > 0.0) {
+									dFdp[h] += 3.0 * phi[j] * phi[k];
+									dFdp[j] += 3.0 * phi[k] * phi[h];
+									dFdp[k] += 3.0 * phi[h] * phi[j];
+								}
+						}
+			for (int h = 0; h < Q; h++)
+				if (s[h] > 0.0)
+					for (int j = h + 1; j < Q; j++)
+						if (s[j] > 0.0) {
+							dpdt[h] -= mu * (dFdp[h] - dFdp[j]);
+							dpdt[j] -= mu * (dFdp[j] - dFdp[h]);
+						}
+			double sum = 0.0;
+			for (int h = 0; h < Q; h++) {
+				double value = phi[h] + dt * (2.0 / S) * dpdt[h];
+				if (value > 1.0) value = 1.0;
+				if (value < 0.0) value = 0.0;
+				b[i * Q + h] = value;
+				sum += value;
+			}
+			double rsum = 0.0;
+			if (fabs(sum) > 0.0) rsum = 1.0 / sum;
+			for (int h = 0; h < Q; h++) b[i * Q + h] *= rsum;
+		}
+		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(double));
+	}
+}
+
+/* the float form: constants formed in double and rounded, float arithmetic */
+void orc_gga_steps_f32(const orc_grid* g, int Q, float* a, float* b, const double* p, int steps)
+{
+	const double gamma64 = (double)(float)p[2];
+	const double eps64 = 4.0 / acos(-1.0) * sqrt(0.5 * gamma64 * p[1]);
+	const float c2 = (float)(0.5 * eps64 * eps64), w = (float)(4.0 * gamma64 / p[1]);
+	const float mu = (float)(double)(float)p[3], dt = (float)p[0];
+	const int64_t n = nodes_of(g);
+	float lap[ORC_GGA_MAXQ], dF[ORC_GGA_MAXQ], dp[ORC_GGA_MAXQ];
+	int s[ORC_GGA_MAXQ];
+	int x[3] = {0, 0, 0};
+	for (int st = 0; st < steps; st++) {
+		for (int64_t i = 0; i < n; i++) {
+			const float* phi = a + i * Q;
+			position(g, i, x);
+			float S = 0.0f;
+			for (int h = 0; h < Q; h++) {
+				s[h] = 0;
+				for (int j = 0; j < g->dim && !s[h]; j++)
+					for (int k = -1; k <= 1 && !s[h]; k++)
+						if (a[neighbour(g, x, j, k) * Q + h] > 0.0f) s[h] = 1;
+				S += (float)s[h];
+			}
+			if (S < 2.0f) {
+				for (int h = 0; h < Q; h++) b[i * Q + h] = phi[h];
+				continue;
+			}
+			for (int h = 0; h < Q; h++) {
+				lap[h] = 0.0f;
+				dF[h] = dp[h] = 0.0f;
+			}
+			for (int d = 0; d < g->dim; d++) {
+				const float* yh = a + neighbour(g, x, d, +1) * Q;
+				const float* yl = a + neighbour(g, x, d, -1) * Q;
+				const float wt = (float)weight_of(g, d);
+				for (int h = 0; h < Q; h++)
+					if (s[h]) lap[h] += wt * (yh[h] - 2.0f * phi[h] + yl[h]);
+			}
+			for (int h = 0; h < Q; h++)
+				if (s[h])
+					for (int j = h + 1; j < Q; j++)
+						if (s[j]) {
+							dF[h] += c2 * lap[j] + w * phi[j];
+							dF[j] += c2 * lap[h] + w * phi[h];
+							for (int k = j + 1; k < Q; k++)
+								if (s[k]) {
+									dF[h] += 3.0f * phi[j] * phi[k];
+									dF[j] += 3.0f * phi[k] * phi[h];
+									dF[k] += 3.0f * phi[h] * phi[j];
+								}
+						}
+			for (int h = 0; h < Q; h++)
+				if (s[h])
+					for (int j = h + 1; j < Q; j++)
+						if (s[j]) {
+							dp[h] -= mu * (dF[h] - dF[j]);
+							dp[j] -= mu * (dF[j] - dF[h]);
+						}
+			float sum = 0.0f;
+			const float rate = dt * (2.0f / S);
+			for (int h = 0; h < Q; h++) {
+				float v = phi[h] + rate * dp[h];
+				if (v > 1.0f) v = 1.0f;
+				if (v < 0.0f) v = 0.0f;
+				b[i * Q + h] = v;
+				sum += v;
+			}
+			const float rsum = fabsf(sum) > 0.0f ? 1.0f / sum : 0.0f;
+			for (int h = 0; h < Q; h++) b[i * Q + h] *= rsum;
+		}
+		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(float));
+	}
 }
 
 /* ---- reductions ----------------------------------------------------------- */

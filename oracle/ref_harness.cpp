@@ -12,6 +12,7 @@
 //   -DHARNESS_MB  includes examples/phase_transitions/model_B/model_B.cpp (stock only)
 //   -DHARNESS_ZN  includes examples/coarsening/zener_pinning/isotropic/phase_field/zener.cpp (stock only)
 //   -DHARNESS_OW  includes examples/coarsening/ostwald_ripening/isotropic/phase_field/ostwald.cpp (stock only)
+//   -DHARNESS_GA  includes examples/coarsening/grain_growth/anisotropic/phase_field/graingrowth.cpp (stock only)
 // MMSP_MAIN is predefined so that the reference's main() (MMSP.main.hpp:38) is
 // skipped; this file supplies a small main() instead.
 //
@@ -47,8 +48,10 @@
 #include "zener.cpp"
 #elif defined(HARNESS_OW)
 #include "ostwald.cpp"
+#elif defined(HARNESS_GA)
+#include "graingrowth.cpp"
 #else
-#error "define one of HARNESS_AC, HARNESS_CH, HARNESS_GG, HARNESS_MA, HARNESS_MB, HARNESS_ZN, HARNESS_OW"
+#error "define one of HARNESS_AC, HARNESS_CH, HARNESS_GG, HARNESS_MA, HARNESS_MB, HARNESS_ZN, HARNESS_OW, HARNESS_GA"
 #endif
 
 namespace harness {
@@ -122,7 +125,7 @@ template <int dim, typename T> void update_config(grid<dim, MMSP::vector<T> >& o
 }
 #endif
 
-#if defined(HARNESS_MA) || defined(HARNESS_MB) || defined(HARNESS_ZN) || defined(HARNESS_OW)
+#if defined(HARNESS_MA) || defined(HARNESS_MB) || defined(HARNESS_ZN) || defined(HARNESS_OW) || defined(HARNESS_GA)
 // model A / model B, Zener pinning, Ostwald ripening: only the stock update
 // (hard-coded parameters); model A/B's noise comes from rand(), seeded by
 // srand(HARNESS_SEED) when that is set
@@ -153,7 +156,7 @@ template <class G> int run(const char* in, const char* out, int steps, bool stoc
 
 template <int dim> int dispatch_type(const std::string& type, const char* in, const char* out, int steps, bool stock)
 {
-#if defined(HARNESS_GG) || defined(HARNESS_ZN) || defined(HARNESS_OW)
+#if defined(HARNESS_GG) || defined(HARNESS_ZN) || defined(HARNESS_OW) || defined(HARNESS_GA)
 	if (type == "grid:vector:double") return run<grid<dim, MMSP::vector<double> > >(in, out, steps, stock);
 	if (type == "grid:vector:float") return run<grid<dim, MMSP::vector<float> > >(in, out, steps, stock);
 #else

@@ -41,6 +41,8 @@ def params_of(case):
         return [0.01]  # zener.cpp:115
     if m == "ow":
         return [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]  # ostwald.cpp:68-82
+    if m == "ga":
+        return [0.01, 8.0, 1.0, 1.0]  # anisotropic graingrowth.cpp:105-106, anisotropy.hpp energy/mobility 1
     raise ValueError(m)
 
 
@@ -48,7 +50,7 @@ def model_of(case):
     """Product model id for a golden case."""
     if case["model"] == "ch":
         return "ch_dw" if case["variant"] == "config" else "ch"
-    return {"zn": "zener", "ow": "ostwald"}.get(case["model"], case["model"])
+    return {"zn": "zener", "ow": "ostwald", "ga": "gg_aniso"}.get(case["model"], case["model"])
 
 
 @pytest.fixture(scope="session")

@@ -54,6 +54,12 @@ CASES = [
     ("ow2d_f64", "ow", (24, 20), 3, "f64", None, None, 199, (0.0, 1.0), 60, "stock", []),
     ("ow3d_f64", "ow", (8, 9, 10), 3, "f64", [(M, N), (P, P), (D, P)], [0.5, 1.0, 2.0], 211, (0.0, 1.0), 30, "stock",
      []),
+    # multi-phase grain growth from one-hot grains (blocks of 4 nodes per axis): nodes inside a grain
+    # keep their values, nodes near boundaries evolve
+    ("ga1d_f64", "ga", (64,), 3, "f64", [(N, D)], None, 223, (0.0, 1.0), 40, "stock", []),
+    ("ga2d_f64", "ga", (24, 28), 4, "f64", None, None, 227, (0.0, 1.0), 40, "stock", []),
+    ("ga3d_f64", "ga", (8, 10, 12), 5, "f64", [(M, N), (P, P), (D, P)], [1.0, 0.5, 2.0], 229, (0.0, 1.0), 20,
+     "stock", []),
 ]
 
 
@@ -62,7 +68,19 @@ def sha256(path):
         return hashlib.sha256(f.read()).hexdigest()
 
 
-def make_input(shape, q, dtype, seed, lo, hi):
+def one_hot_grains(shape, q, seed, block=4):
+    """phi = 1 for the grain of each node, 0 for the others; grain ids drawn per block of nodes."""
+    blocks = tuple((n + block - 1) // block for n in shape)
+    ids = np.floor(O.uniform(int(np.prod(blocks)), seed, 0.0, q)).astype(int).reshape(blocks)
+    for ax in range(len(shape)):
+        ids = np.repeat(ids, block, axis=ax)
+    ids = ids[tuple(slice(0, n) for n in shape)]
+    return (ids[..., None] == np.arange(q)).astype(np.float64)
+
+
+def make_input(shape, q, dtype, seed, lo, hi, model=None):
+    if model == "ga":
+        return one_hot_grains(shape, q, seed)
     n = int(np.prod(shape)) * max(q, 1)
     a = O.uniform(n, seed, lo, hi)
     a = a.reshape(tuple(shape) + ((q,) if q else ()))
@@ -74,7 +92,7 @@ def main():
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` where /root/reference exists")
     manifest = []
     for (name, model, shape, q, dtype, bc, dx, seed, (lo, hi), steps, variant, params) in CASES:
-        a = make_input(shape, q, dtype, seed, lo, hi)
+        a = make_input(shape, q, dtype, seed, lo, hi, model)
         g = dat.make_grid(a, vector=q > 0, bc=bc, dx=dx)
         fin = os.path.join(HERE, name + ".in.dat")
         fout = os.path.join(HERE, name + ".out.dat")

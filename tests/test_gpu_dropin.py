@@ -23,7 +23,8 @@ pytestmark = pytest.mark.gpu
 mx = pytest.importorskip("mmsp_amd")
 
 PROGRAMS = {"ac": ("allen-cahn", "allen-cahn-gpu"), "ch": ("cahn-hilliard", "cahn-hilliard-gpu"),
-            "gg": ("graingrowth", "graingrowth-gpu"), "zn": ("zener", "zener-gpu"), "ow": ("ostwald", "ostwald-gpu")}
+            "gg": ("graingrowth", "graingrowth-gpu"), "zn": ("zener", "zener-gpu"), "ow": ("ostwald", "ostwald-gpu"),
+            "ga": ("graingrowth-aniso", "graingrowth-aniso-gpu")}
 
 
 def say(*a):
@@ -51,7 +52,8 @@ def progress_shape(out):
 # (the stock programs' typedefs are double grids, so the float fixtures are not used here)
 CASES = [("ac2d_f64", "ac"), ("ac3d_bc_f64", "ac"), ("ch2d_stock_f64", "ch"),
          ("gg3d_f64", "gg"), ("gg2d_bc_f64", "gg"), ("gg1d_f64", "gg"),
-         ("zn2d_f64", "zn"), ("zn3d_f64", "zn"), ("ow2d_f64", "ow"), ("ow3d_f64", "ow")]
+         ("zn2d_f64", "zn"), ("zn3d_f64", "zn"), ("ow2d_f64", "ow"), ("ow3d_f64", "ow"), ("ga2d_f64", "ga"),
+         ("ga3d_f64", "ga")]
 
 
 @pytest.mark.parametrize("name,model", CASES)
@@ -98,6 +100,8 @@ def test_dropin_program_matches_stock_program(name, model, tmp_path):
         exp = O.zener(gin.data, p[0], steps, form=form, bc=bc, dx=gin.dx)
     elif model == "ow":
         exp = O.ostwald(gin.data, p, steps, form=form, bc=bc, dx=gin.dx)
+    elif model == "ga":
+        exp = O.gg_aniso(gin.data, p, steps, form=form, bc=bc, dx=gin.dx)
     else:
         exp = O.gg(gin.data, p[0], steps, form=form, bc=bc, dx=gin.dx)
     last = dat.read_dat(str(dg / files_g[-1])).data

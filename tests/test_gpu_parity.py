@@ -47,6 +47,8 @@ def oracle_run(model, a, params, steps, form, bc=None, dx=None, cube=O.CUBE_CR):
         return O.zener(a, params[0], steps, form=form, cube=cube, bc=bc, dx=dx)
     if model == "ostwald":
         return O.ostwald(a, params, steps, form=form, cube=cube, bc=bc, dx=dx)
+    if model == "gg_aniso":
+        return O.gg_aniso(a, params, steps, form=form, bc=bc, dx=dx)
     return O.gg(a, params[0], steps, form=form, cube=cube, bc=bc, dx=dx)
 
 
@@ -438,3 +440,31 @@ def test_zener_and_ostwald(prec):
     with mx.Grid((8, 8), fields=1, vector=True, precision=prec) as g:
         with pytest.raises(mx.MmspxError):
             g.step("ostwald", p, 1)  # no order parameter
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_multiphase_grain_growth(prec):
+    """Multi-phase grain growth from one-hot grains: bitwise against the oracle
+    in 1-D, 2-D and 3-D, several boundaries, up to 32 fields."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(os.path.dirname(__file__), "golden",
+                                                                              "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    p = [0.01, 8.0, 1.0, 1.0]
+    for shape, q, bc in [((300,), 3, [(N, D)]), ((70, 90), 6, [(N, M), (D, P)]), ((64, 64), 32, None),
+                         ((10, 12, 14), 4, [(M, N), (P, P), (D, P)])]:
+        say("[gpu] multi-phase grain growth %s q=%d %s" % (shape, q, prec))
+        a = mg.one_hot_grains(shape, q, 900 + q)
+        if prec == "f32":
+            a = a.astype(np.float32)
+        exp = O.gg_aniso(a, p, 25, form=prec, bc=bc)
+        with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+            g.upload(a)
+            g.step("gg_aniso", p, 10)
+            g.step("gg_aniso", p, 15)
+            assert np.array_equal(g.download(), exp), (shape, q)
+    with mx.Grid((8, 8), fields=33, vector=True, precision=prec) as g:
+        with pytest.raises(mx.MmspxError):
+            g.step("gg_aniso", p, 1)  # more than 32 fields

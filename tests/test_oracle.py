@@ -37,6 +37,8 @@ def run_oracle(case, a, cube, bc=None, dx=None):
         return O.zener(a, p[0], case["steps"], form=form, cube=cube, bc=bc, dx=dx)
     if case["model"] == "ow":
         return O.ostwald(a, p, case["steps"], form=form, cube=cube, bc=bc, dx=dx)
+    if case["model"] == "ga":
+        return O.gg_aniso(a, p, case["steps"], form=form, bc=bc, dx=dx)
     return O.gg(a, p[0], case["steps"], form=form, cube=cube, bc=bc, dx=dx)
 
 
@@ -388,3 +390,37 @@ def test_ostwald_uniform_state_follows_the_local_map():
     assert np.all(out[..., 0] == C)
     for j in range(1, q):
         assert np.allclose(out[..., j], phi[j - 1], rtol=1e-13, atol=0)
+
+
+def _grains(shape, q, seed):
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(os.path.dirname(__file__), "golden",
+                                                                              "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    return mg.one_hot_grains(shape, q, seed)
+
+
+@pytest.mark.parametrize("form", ["f64", "f32"])
+def test_multiphase_grain_growth_stays_on_the_simplex(form):
+    """After every step each node's fields lie in [0, 1] and sum to 1 (the
+    projection), nodes whose neighbourhood holds one grain keep their values,
+    and relabelling the grains relabels the result (uniform energies)."""
+    shape, q = (30, 26), 5
+    a = _grains(shape, q, 41)
+    if form == "f32":
+        a = a.astype(np.float32)
+    p = [0.01, 8.0, 1.0, 1.0]
+    out = O.gg_aniso(a, p, 1, form=form)
+    tol = 4e-16 if form == "f64" else 4e-7
+    assert out.min() >= 0 and out.max() <= 1 and np.abs(out.sum(-1) - 1).max() <= tol
+    # one grain in the whole 5-point neighbourhood: copied
+    lone = np.ones(shape, dtype=bool)
+    for ax in range(2):
+        for sh in (-1, 1):
+            lone &= (np.roll(a, sh, axis=ax) == a).all(-1)
+    assert np.array_equal(out[lone], a[lone]) and lone.any() and (~lone).any()
+    later = O.gg_aniso(a, p, 15, form=form)
+    perm = [3, 0, 4, 1, 2]
+    assert np.allclose(O.gg_aniso(a[..., perm], p, 15, form=form), later[..., perm], atol=50 * tol)
