@@ -1,7 +1,7 @@
 """Benchmark of the explicit update path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload ch3d|ch3d_f32|ac2d|gg2d|zener2d|ostwald2d|model_b3d|batch_ac|serve|ckpt]
+                    [--workload ch3d|ch3d_f32|ac2d|gg2d|zener2d|ostwald2d|gganiso2d|model_b3d|batch_ac|serve|ckpt]
                     [--no-cpu-baseline] [--tune | --tile TY,TZ,CHUNK[,OPT]]
                     [--requests N] [--hbm-budget-gb G]
                     [--graph-steps G] [--no-sequential] [--ckpt-threads T]
@@ -66,6 +66,11 @@ WORKLOADS = {
     "ostwald2d": dict(model="ostwald", shape=(1024, 1024), prec="f64",
                       params=[0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0], steps=500, seed=20260110,
                       lo=0.0, hi=1.0, fields=4, bytes_per_cell=2 * 4 * 8, name="ostwald-2d-q4", cpu_steps=2),
+    # multi-phase grain growth (anisotropic/phase_field/graingrowth.cpp:105-106: dt = 0.01, width = 8;
+    # energy and mobility 1), 1024^2 with 8 grains as one-hot blocks of 16 nodes, FP64
+    "gganiso2d": dict(model="gg_aniso", shape=(1024, 1024), prec="f64", params=[0.01, 8.0, 1.0, 1.0], steps=500,
+                      seed=20260111, lo=0.0, hi=1.0, fields=8, bytes_per_cell=2 * 8 * 8, name="multiphase-gg-2d-q8",
+                      cpu_steps=2, init="grains"),
     "model_b3d": dict(model="ch_noise", shape=(256, 256, 256), prec="f64",
                       params=[1.0, 1.0, 1.0, 1.0, 0.01, math.sqrt(2.0 * 0.01 / (0.01 * 1.0)), 20260107], steps=2000,
                       seed=20260108, lo=-0.01, hi=0.01, fields=1, bytes_per_cell=16, name="model-b-3d", cpu_steps=2),
@@ -83,7 +88,8 @@ def cpu_baseline(w, init):
 
     shape, steps = w["shape"], w["cpu_steps"]
     nodes = int(np.prod(shape))
-    ref_bin = {"ch_dw": "ch", "ac": "ac", "gg": "gg", "ch_noise": "mb", "zener": "zn", "ostwald": "ow"}[w["model"]]
+    ref_bin = {"ch_dw": "ch", "ac": "ac", "gg": "gg", "ch_noise": "mb", "zener": "zn", "ostwald": "ow",
+               "gg_aniso": "ga"}[w["model"]]
     if O.ref_available() and os.path.exists(os.path.join(O.REF_DIR, "ref_" + ref_bin)):
         with tempfile.TemporaryDirectory() as td:
             fin = os.path.join(td, "in.dat")
@@ -94,7 +100,7 @@ def cpu_baseline(w, init):
             dat.write_dat(fin, dat.make_grid(data, vector=vec), compress=False)
             model = ref_bin
             # model B, Zener, Ostwald: the example's own update (its parameters are the workload's)
-            variant = "stock" if model in ("mb", "zn", "ow") else "config"
+            variant = "stock" if model in ("mb", "zn", "ow", "ga") else "config"
             sec = O.ref_run(model, fin, "-", steps, variant, w["params"], timeout=600)
         kind, what = "reference", "oracle/_ref reference update() compiled from the reference sources"
     else:
@@ -109,6 +115,8 @@ def cpu_baseline(w, init):
             O.zener(init.reshape(shape + (w["fields"],)), w["params"][0], steps)
         elif w["model"] == "ostwald":
             O.ostwald(init.reshape(shape + (w["fields"],)), w["params"], steps)
+        elif w["model"] == "gg_aniso":
+            O.gg_aniso(init.reshape(shape + (w["fields"],)), w["params"], steps)
         else:
             O.gg(init.reshape(shape + (w["fields"],)), w["params"][0], steps)
         sec = time.perf_counter() - t0
@@ -121,6 +129,19 @@ def cpu_baseline(w, init):
         "sample": "%s, %s, %d steps of the full %s grid, serial, update() only (I/O excluded)"
         % (what, w["name"], steps, "x".join(map(str, shape))),
     }
+
+
+def grains(shape, q, seed, block=16):
+    """One-hot grains: each block of `block` nodes per axis belongs to a grain
+    drawn from the seed (phi = 1 for it, 0 for the others)."""
+    import mmsp_amd as mx
+
+    blocks = tuple((n + block - 1) // block for n in shape)
+    ids = np.floor(mx.fill_uniform(int(np.prod(blocks)), seed, 0.0, q)).astype(int).reshape(blocks)
+    for ax in range(len(shape)):
+        ids = np.repeat(ids, block, axis=ax)
+    ids = ids[tuple(slice(0, n) for n in shape)]
+    return (ids[..., None] == np.arange(q)).astype(np.float64)
 
 
 def c4_members(seed=20260104, count=64):
@@ -493,6 +514,8 @@ def main():
     nodes = int(np.prod(shape))
     q = w["fields"]
     init = mx.fill_uniform(nodes * q, w["seed"], w["lo"], w["hi"])
+    if w.get("init") == "grains":
+        init = grains(shape, q, w["seed"]).reshape(-1)
 
     g = mx.Grid(shape, fields=q, vector=q > 1, precision=w["prec"])
     g.upload(init.astype(g.dtype).reshape(g.host_shape))
@@ -546,7 +569,8 @@ def main():
     per_launch = alg_bytes / lps
     achieved = per_launch / mean_launch_s / 1e9
     value = nodes * K / dev_s
-    base = {"ch_dw": "ch", "ch": "ch", "ch_noise": "ch", "ac": "ac", "gg": "gg", "zener": "gg", "ostwald": "ow"}[w["model"]]
+    base = {"ch_dw": "ch", "ch": "ch", "ch_noise": "ch", "ac": "ac", "gg": "gg", "zener": "gg", "ostwald": "ow",
+            "gg_aniso": "gga"}[w["model"]]
     if tile["variant"] != 2:
         kname = base + "_generic"
     elif base == "ch":
