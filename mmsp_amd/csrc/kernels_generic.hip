@@ -182,7 +182,11 @@ __global__ void __launch_bounds__(256) ow_pass1(const typename Form<PREC>::S* __
 // Otherwise the pairwise variational derivatives (:141-157) and rates
 // (:160-167), the clamped update and the projection onto the simplex
 // (:170-185), in the reference's loop and operation order.
-template <int DIM, int PREC>
+// QMAX: the field count the per-node arrays are compiled for; every loop over
+// fields runs to QMAX with the fields past g.fields skipped, so for QMAX up to
+// 8 the loops unroll, the arrays are indexed by constants and live in
+// registers (the reference's example has 4 fields).
+template <int DIM, int PREC, int QMAX>
 __global__ void __launch_bounds__(256) gga_generic(const typename Form<PREC>::S* __restrict__ a,
                                                    typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p)
 {
@@ -195,6 +199,8 @@ __global__ void __launch_bounds__(256) gga_generic(const typename Form<PREC>::S*
 	const S dt = dbl ? (S)p.an[AN_DT] : (S)p.anf[AN_DT];
 	const int64_t N = g.nodes;
 	const int Q = g.fields;
+	// full unrolling (register arrays) up to 8 fields; more keep their loops
+	constexpr int UN = QMAX <= 8 ? QMAX : 1;
 	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.nodes; i += gridDim.x * blockDim.x) {
 		int x[3];
 		coords<DIM>(g, i, x);
@@ -204,28 +210,37 @@ __global__ void __launch_bounds__(256) gga_generic(const typename Form<PREC>::S*
 			ih[d] = nbr(g, i, x, d, +1);
 			il[d] = nbr(g, i, x, d, -1);
 		}
+		// the fields present in the neighbourhood (graingrowth.cpp:116-131)
 		uint32_t act = 0;
 		S cnt = 0;
-		for (int h = 0; h < Q; h++) {
-			const S* __restrict__ pl = a + h * N;
-			bool on = pl[i] > S(0);
+		S phi[QMAX];
+#pragma unroll UN
+		for (int h = 0; h < QMAX; h++) {
+			phi[h] = 0;
+			if (h < Q) {
+				const S* __restrict__ pl = a + h * N;
+				phi[h] = pl[i];
+				bool on = phi[h] > S(0);
 #pragma unroll
-			for (int d = 0; d < DIM; d++) on = on || pl[ih[d]] > S(0) || pl[il[d]] > S(0);
-			if (on) {
-				act |= 1u << h;
-				cnt += S(1);
+				for (int d = 0; d < DIM; d++) on = on || pl[ih[d]] > S(0) || pl[il[d]] > S(0);
+				if (on) {
+					act |= 1u << h;
+					cnt += S(1);
+				}
 			}
 		}
 		if (cnt < S(2)) {
-			for (int h = 0; h < Q; h++) b[h * N + i] = a[h * N + i];
+#pragma unroll UN
+			for (int h = 0; h < QMAX; h++)
+				if (h < Q) b[h * N + i] = phi[h];
 			continue;
 		}
-		S phi[kAnisoMaxFields], lap[kAnisoMaxFields], dF[kAnisoMaxFields], dp[kAnisoMaxFields];
-		for (int h = 0; h < Q; h++) {
-			const S* __restrict__ pl = a + h * N;
-			phi[h] = pl[i];
+		S lap[QMAX], dF[QMAX], dp[QMAX];
+#pragma unroll UN
+		for (int h = 0; h < QMAX; h++) {
 			typename F::L l = F::lap_zero();
 			if (act >> h & 1u) {
+				const S* __restrict__ pl = a + h * N;
 #pragma unroll
 				for (int d = 0; d < DIM; d++) F::lap_add(l, g.w[d], g.wf[d], pl[ih[d]], phi[h], pl[il[d]]);
 			}
@@ -233,13 +248,17 @@ __global__ void __launch_bounds__(256) gga_generic(const typename Form<PREC>::S*
 			dF[h] = 0;
 			dp[h] = 0;
 		}
-		for (int h = 0; h < Q; h++) {
+		// variational derivatives (:141-157)
+#pragma unroll UN
+		for (int h = 0; h < QMAX; h++) {
 			if (!(act >> h & 1u)) continue;
-			for (int j = h + 1; j < Q; j++) {
+#pragma unroll UN
+			for (int j = h + 1; j < QMAX; j++) {
 				if (!(act >> j & 1u)) continue;
 				dF[h] += c2 * lap[j] + w * phi[j];
 				dF[j] += c2 * lap[h] + w * phi[h];
-				for (int k = j + 1; k < Q; k++) {
+#pragma unroll UN
+				for (int k = j + 1; k < QMAX; k++) {
 					if (!(act >> k & 1u)) continue;
 					dF[h] += S(3) * phi[j] * phi[k];
 					dF[j] += S(3) * phi[k] * phi[h];
@@ -247,17 +266,23 @@ __global__ void __launch_bounds__(256) gga_generic(const typename Form<PREC>::S*
 				}
 			}
 		}
-		for (int h = 0; h < Q; h++) {
+		// rates (:160-167)
+#pragma unroll UN
+		for (int h = 0; h < QMAX; h++) {
 			if (!(act >> h & 1u)) continue;
-			for (int j = h + 1; j < Q; j++) {
+#pragma unroll UN
+			for (int j = h + 1; j < QMAX; j++) {
 				if (!(act >> j & 1u)) continue;
 				dp[h] -= mu * (dF[h] - dF[j]);
 				dp[j] -= mu * (dF[j] - dF[h]);
 			}
 		}
+		// clamped update and projection onto the simplex (:170-185)
 		S sum = 0;
 		const S rate = dt * (S(2) / cnt);
-		for (int h = 0; h < Q; h++) {
+#pragma unroll UN
+		for (int h = 0; h < QMAX; h++) {
+			if (h >= Q) continue;
 			S v = phi[h] + rate * dp[h];
 			if (v > S(1)) v = S(1);
 			if (v < S(0)) v = S(0);
@@ -265,7 +290,9 @@ __global__ void __launch_bounds__(256) gga_generic(const typename Form<PREC>::S*
 			sum += v;
 		}
 		const S rsum = fabs(sum) > S(0) ? S(1) / sum : S(0);
-		for (int h = 0; h < Q; h++) b[h * N + i] = phi[h] * rsum;
+#pragma unroll UN
+		for (int h = 0; h < QMAX; h++)
+			if (h < Q) b[h * N + i] = phi[h] * rsum;
 	}
 }
 
@@ -335,7 +362,14 @@ static int launch_dim_prec(const Geom& g, const ModelParams& p, const void* av, 
 		if constexpr (PREC == MMSPX_F32REF) {
 			return fail(MMSPX_EUNSUPPORTED, "anisotropic grain growth runs the MMSPX_F64 and MMSPX_F32 forms");
 		} else {
-			hipLaunchKernelGGL((gga_generic<DIM, PREC>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
+			if (g.fields <= 4)
+				hipLaunchKernelGGL((gga_generic<DIM, PREC, 4>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
+			else if (g.fields <= 8)
+				hipLaunchKernelGGL((gga_generic<DIM, PREC, 8>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
+			else if (g.fields <= 16)
+				hipLaunchKernelGGL((gga_generic<DIM, PREC, 16>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
+			else
+				hipLaunchKernelGGL((gga_generic<DIM, PREC, kAnisoMaxFields>), dim3(blocks), dim3(256), 0, st, a, b, g, p);
 			*launches += 1;
 		}
 		break;
