@@ -375,6 +375,101 @@ __device__ __forceinline__ void ch_wave(const typename Form<PREC>::S* __restrict
 		ch_wave_dir<PREC, MU, false>(a, b, g, p, c0, xb, xe);
 }
 
+// ---- 2-D Ostwald ripening, wave strips (periodic): one wave per (strip, chunk)
+// Lane l holds column c0 - 2 + l, as the Cahn-Hilliard wave strips: w is
+// formed on lanes 1..62 and the new values written on lanes 2..61. Rows: the
+// concentration C at x .. x+2 (x+3 in flight) for w(x+1), the order
+// parameters at x-1 .. x+1 (x+2 in flight), w at x-1 .. x+1, and the sums
+// sum_j phi_j^2 of rows x and x+1. Row x's new C is C + dt*D*lap(w), its order
+// parameters follow their local update: the arithmetic of ow_pass1/ow_pass2
+// (ostwald.cpp:87-122), value by value, so the same bits.
+template <int PREC, int Q>
+__device__ __forceinline__ void ow_wave(const typename Form<PREC>::S* __restrict__ a,
+                                        typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                        int c0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	constexpr int R = Q - 1;  // order parameters
+	const int lane = threadIdx.x & 63;
+	const int nx = g.n[0], ny = g.n[1];
+	const size_t N = (size_t)nx * ny;
+	const int c = c0 - 2 + lane;
+	const unsigned cm = (unsigned)wrap(c, ny);
+	const bool out = lane >= 2 && lane < 2 + kChWaveOut && c < ny;
+	auto row = [&](int x) { return (size_t)wrap(x, nx) * ny + cm; };
+	auto sum_of = [&](const S* ph) {
+		typename F::OSum s = 0;
+#pragma unroll
+		for (int j = 0; j < R; j++) s += ph[j] * ph[j];
+		return s;
+	};
+	// w at row r from C at r-1 (cb), r (cc), r+1 (ca) and the sum at r
+	auto w_at = [&](S cb, S cc, S ca, typename F::OSum sum) {
+		const S zl = from_prev(cc, cc), zh = from_next(cc, cc);
+		typename F::L lap = F::lap_zero();
+		F::lap_add(lap, g.w[0], g.wf[0], ca, cc, cb);
+		F::lap_add(lap, g.w[1], g.wf[1], zh, cc, zl);
+		return F::ow_w(cc, sum, lap, p);
+	};
+	S c0r = a[row(xb - 2)], c1r = a[row(xb - 1)];
+	S cx = a[row(xb)], cx1 = a[row(xb + 1)], cx2 = a[row(xb + 2)];
+	S cq = (xb + 1 < xe) ? a[row(xb + 3)] : S(0);  // C(x+3) in flight
+	S fm[R], fc[R], fp[R], fq[R];                     // phi rows x-1, x, x+1; x+2 in flight
+#pragma unroll
+	for (int j = 0; j < R; j++) {
+		const S* __restrict__ aj = a + (size_t)(j + 1) * N;
+		fm[j] = aj[row(xb - 1)];
+		fc[j] = aj[row(xb)];
+		fp[j] = aj[row(xb + 1)];
+		fq[j] = (xb + 1 < xe) ? aj[row(xb + 2)] : S(0);
+	}
+	typename F::OSum sm = sum_of(fm), sc = sum_of(fc);
+	S wm = w_at(c0r, c1r, cx, sm);  // w(x-1)
+	S wc = w_at(c1r, cx, cx1, sc);  // w(x)
+	for (int x = xb; x < xe; x++) {
+		const bool more = x + 1 < xe;
+		// loads for the next row go out first
+		S cn = 0, fn[R];
+		if (more && x + 2 < xe) cn = a[row(x + 4)];
+#pragma unroll
+		for (int j = 0; j < R; j++) fn[j] = (more && x + 2 < xe) ? (a + (size_t)(j + 1) * N)[row(x + 3)] : S(0);
+		const typename F::OSum sp = sum_of(fp);
+		const S wp = w_at(cx, cx1, cx2, sp);  // w(x+1)
+		// C'(x) = C + dt*D*lap(w)
+		{
+			const S zl = from_prev(wc, wc), zh = from_next(wc, wc);
+			typename F::L lap = F::lap_zero();
+			F::lap_add(lap, g.w[0], g.wf[0], wp, wc, wm);
+			F::lap_add(lap, g.w[1], g.wf[1], zh, wc, zl);
+			if (out) b[(size_t)x * ny + c] = F::ow_c(cx, lap, p);
+		}
+#pragma unroll
+		for (int j = 0; j < R; j++) {
+			const S y = fc[j];
+			const S yl = from_prev(y, y), yh = from_next(y, y);
+			typename F::L lap = F::lap_zero();
+			F::lap_add(lap, g.w[0], g.wf[0], fp[j], y, fm[j]);
+			F::lap_add(lap, g.w[1], g.wf[1], yh, y, yl);
+			if (out) b[(size_t)(j + 1) * N + (size_t)x * ny + c] = F::ow_phi(y, cx, sc, lap, p);
+		}
+		wm = wc;
+		wc = wp;
+		sc = sp;
+		cx = cx1;
+		cx1 = cx2;
+		cx2 = cq;
+		cq = cn;
+#pragma unroll
+		for (int j = 0; j < R; j++) {
+			fm[j] = fc[j];
+			fc[j] = fp[j];
+			fp[j] = fq[j];
+			fq[j] = fn[j];
+		}
+	}
+}
+
 // ---- 2-D Allen-Cahn, temporal blocks (periodic): K <= KMAX steps per pass ----
 // A TX x kTbCols tile and its K-cell halo (at most 64 columns: one wave per
 // row) are read once into LDS; the block then takes K steps in LDS, the valid

@@ -468,3 +468,24 @@ def test_multiphase_grain_growth(prec):
     with mx.Grid((8, 8), fields=33, vector=True, precision=prec) as g:
         with pytest.raises(mx.MmspxError):
             g.step("gg_aniso", p, 1)  # more than 32 fields
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_ostwald_wave_strips(prec):
+    """The 2-D Ostwald wave strips (periodic): partial strips, grids narrower
+    than a strip, chunks of 2, 3, 8 and 64 rows, 2 to 8 fields; bitwise
+    against the oracle (and so against the generic two-pass kernels)."""
+    p = [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]
+    for shape, q in [((40, 300), 4), ((17, 5), 2), ((70, 130), 3), ((33, 64), 8), ((9, 61), 5)]:
+        say("[gpu] ostwald wave strips %s q=%d %s" % (shape, q, prec))
+        a = O.uniform(int(np.prod(shape)) * q, 800 + q, 0.0, 1.0).reshape(shape + (q,))
+        if prec == "f32":
+            a = a.astype(np.float32)
+        exp = O.ostwald(a, p, 9, form=prec)
+        for chunk in (2, 3, 8, 64):
+            with mx.Grid(shape, fields=q, vector=True, precision=prec) as g:
+                g.set_variant(2)
+                g.set_tile(1, 60, chunk)
+                g.upload(a)
+                g.step("ostwald", p, 9)
+                assert np.array_equal(g.download(), exp), (shape, q, chunk)
