@@ -207,8 +207,11 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	if (model == MMSPX_GG) return TuneChoice{1, 0, 0, 0, 0};
 	// 2-D Cahn-Hilliard: wave strips (1024^2 FP64 7.3 us per step against 13.5
 	// us for the LDS row strips; 4096^2 FP64 53 us against 69 us, FP32 34 us
-	// against 56 us); shorter chunks for smaller grids
-	if (g.dim == 2) return TuneChoice{2, 1, kChWaveOut, nodes < ((int64_t)1 << 22) ? 8 : 16, 0};
+	// against 56 us). Chunks from a sweep of 3..16 rows: 4 below 2^21 nodes
+	// (256^2 FP64 4.6 us against 5.5 with 8 rows, FP32 3.9 against 4.8), 8 up
+	// to 2^23 (2048^2 FP64 17.9 us, FP32 12.9), 16 above (4096^2 55.5 / 38.3)
+	if (g.dim == 2)
+		return TuneChoice{2, 1, kChWaveOut, nodes < ((int64_t)1 << 21) ? 4 : nodes < ((int64_t)1 << 23) ? 8 : 16, 0};
 	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16, 0};
 	// 256^3: register queues on 32 x 64 tiles, 16-plane chunks. FP64 with 768
 	// threads per tile (54.1-55.0 us per step, against 55.9-57.1 us with 512
