@@ -97,16 +97,19 @@ enum {
 	                       copied; else pairwise variational derivatives and rates, phi' = phi + dt*(2/S)*dphi/dt
 	                       clamped to [0, 1], then divided by the sum (Gibbs simplex). At most 32 fields. */
 };
-/* The noise: g(seed, step, node) is a standard normal deviate (float), the
- * Box-Muller transform of the first two words of one Philox4x32-10 block
- * (Salmon et al., SC'11) with key {seed low, seed high} and counter {node low,
- * node high, step low, step high}. `step` counts the state's steps since its
- * creation (mmspx_step calls continue it; mmspx_autotune leaves it as it
- * was); `node` is the node's index in the reference's order. The noise does
- * not depend on the kernel, its tiles or the launch: every path gives the
- * same bits. The reference's gaussian() (model_A.cpp:13-39) draws from rand()
- * in node order by the polar method, so trajectories agree in distribution
- * only. Every precision form; single grids only (not batches or servers). */
+/* The noise: g(seed, step, node) is a standard normal deviate (float). For a
+ * node at axis-0 index x and index r within its x-plane (the node's position
+ * in the reference's order is x * plane + r), one Philox4x32-10 block
+ * (Salmon et al., SC'11) with key {seed low, seed high} and counter {r, x/2,
+ * step low, step high} gives, by the Box-Muller transform of its first two
+ * words, a pair of independent deviates: the cosine one for even x, the sine
+ * one for odd x. `step` counts the state's steps since its creation
+ * (mmspx_step calls continue it; mmspx_autotune leaves it as it was). The
+ * noise does not depend on the kernel, its tiles or the launch: every path
+ * gives the same bits. The reference's gaussian() (model_A.cpp:13-39) draws
+ * from rand() in node order by the polar method, so trajectories agree in
+ * distribution only. Every precision form; single grids only (not batches
+ * or servers). */
 
 /* ---- host array layouts for upload/download ---- */
 enum {

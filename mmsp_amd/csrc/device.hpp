@@ -130,40 +130,72 @@ __device__ __forceinline__ float ln_unit(float u)
 	return (float)e * 0.693147182f + 2.0f * s * q;
 }
 
-// cos(2 pi t) for t in [0, 1) a multiple of 2^-24: quadrant and octant
-// symmetries reduce it exactly to sin or cos of theta in [0, pi/4], where the
-// Taylor series to theta^10 (cos) or theta^9 (sin) is below float rounding.
-__device__ __forceinline__ float cos_2pi(float t)
+// cos(2 pi t) and sin(2 pi t) for t in [0, 1) a multiple of 2^-24: the
+// quadrant q and f = 4t - q are exact; with theta = f pi/2 folded into
+// [0, pi/4] (cos(f pi/2) = sin((1-f) pi/2)), Taylor series to theta^10 (cos)
+// and theta^9 (sin), below float rounding there, give A = cos(f pi/2) and
+// B = sin(f pi/2); the quadrant then permutes and signs them.
+struct CosSin {
+	float c, s;
+};
+__device__ __forceinline__ CosSin cos_sin_2pi(float t)
 {
 	const float v = t * 4.0f;
 	const int q = (int)v;
 	const float f = v - (float)q;
-	const float gq = (q & 1) ? 1.0f - f : f;  // cos(2 pi t) = +-cos(gq pi/2)
-	const bool hi = gq > 0.5f;
-	const float h = hi ? 1.0f - gq : gq;      // cos(gq pi/2) = sin(h pi/2) when hi
+	const bool hi = f > 0.5f;
+	const float h = hi ? 1.0f - f : f;
 	const float th = h * 1.57079633f, z = th * th;
 	const float cs = 1.0f + z * (-1.0f / 2.0f + z * (1.0f / 24.0f + z * (-1.0f / 720.0f +
 	                 z * (1.0f / 40320.0f + z * (-1.0f / 3628800.0f)))));
 	const float sn = th * (1.0f + z * (-1.0f / 6.0f + z * (1.0f / 120.0f + z * (-1.0f / 5040.0f +
 	                 z * (1.0f / 362880.0f)))));
-	const float c = hi ? sn : cs;
-	return (q == 1 || q == 2) ? -c : c;
+	const float A = hi ? sn : cs, B = hi ? cs : sn;
+	CosSin r;
+	r.c = q == 0 ? A : q == 1 ? -B : q == 2 ? -A : B;
+	r.s = q == 0 ? B : q == 1 ? A : q == 2 ? -B : -A;
+	return r;
 }
 
-// Standard normal deviate for (seed, step, node): one Philox4x32-10 block,
-// key {seed low, seed high}, counter {node low, node high, step low, step
-// high}; Box-Muller on its first two words, u1 = (w0 + 1/2) 2^-32 in (0, 1],
-// u2 = (w1 >> 8) 2^-24 in [0, 1): g = sqrt(-2 ln u1) cos(2 pi u2), in float.
-// The reference's gaussian() (model_A.cpp:13-39) transforms uniforms by
-// Marsaglia's polar method, whose rejection loop would leave the lanes of a
-// wave waiting on each other; both give exact normal deviates.
-__device__ __forceinline__ float normal_deviate(uint64_t seed, uint64_t step, uint64_t node)
+// The pair of standard normal deviates of one Philox4x32-10 block, key {seed
+// low, seed high}, counter {r, xh, step low, step high}: Box-Muller on its
+// first two words, u1 = (w0 + 1/2) 2^-32 in (0, 1], u2 = (w1 >> 8) 2^-24 in
+// [0, 1), R = sqrt(-2 ln u1), the pair {R cos(2 pi u2), R sin(2 pi u2)}, in
+// float. A node with axis-0 index x and index r within its axis-0 plane draws
+// the first of pair (r, x >> 1) when x is even and the second when x is odd,
+// so a kernel marching along axis 0 forms one pair per two planes. (The
+// reference's gaussian(), model_A.cpp:13-39, keeps the second value of its
+// polar pair for the next call in the same way; its rejection loop would
+// leave the lanes of a wave waiting on each other, so Box-Muller here.)
+__device__ __forceinline__ CosSin normal_pair(uint64_t seed, uint64_t step, uint32_t xh, uint32_t r)
 {
-	uint32_t c[4] = {(uint32_t)node, (uint32_t)(node >> 32), (uint32_t)step, (uint32_t)(step >> 32)};
+	uint32_t c[4] = {r, xh, (uint32_t)step, (uint32_t)(step >> 32)};
 	philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
 	const float u1 = ((float)c[0] + 0.5f) * 0x1p-32f;
 	const float u2 = (float)(c[1] >> 8) * 0x1p-24f;
-	return sqrtf(-2.0f * ln_unit(u1)) * cos_2pi(u2);
+	const float R = sqrtf(-2.0f * ln_unit(u1));
+	const CosSin cs = cos_sin_2pi(u2);
+	CosSin g;
+	g.c = R * cs.c;
+	g.s = R * cs.s;
+	return g;
+}
+
+__device__ __forceinline__ float normal_deviate(uint64_t seed, uint64_t step, int x, uint32_t r)
+{
+	const CosSin g = normal_pair(seed, step, (uint32_t)x >> 1, r);
+	return (x & 1) ? g.s : g.c;
+}
+
+// The same deviate for a kernel that visits the planes of a column in order:
+// the partner plane's value (x ^ 1) is kept in `keep`, valid for plane `kx`.
+__device__ __forceinline__ float normal_deviate_cached(uint64_t seed, uint64_t step, int x, uint32_t r, int kx,
+                                                       float& keep)
+{
+	if (x == kx) return keep;
+	const CosSin g = normal_pair(seed, step, (uint32_t)x >> 1, r);
+	keep = (x & 1) ? g.c : g.s;
+	return (x & 1) ? g.s : g.c;
 }
 
 // ---- precision forms -----------------------------------------------------
@@ -338,19 +370,19 @@ template <> struct Form<MMSPX_F32REF> {
 };
 
 // mu of the Cahn-Hilliard family: MU 0 the reference's polynomial, 1 the
-// double well, 2 the polynomial plus noise (node = the node's index)
+// double well, 2 the polynomial plus noise (gd the node's deviate)
 template <int PREC, int MU>
 __device__ __forceinline__ typename Form<PREC>::S mu_of(typename Form<PREC>::S c, typename Form<PREC>::L lap,
-                                                        const ModelParams& p, uint64_t node)
+                                                        const ModelParams& p, float gd)
 {
 	using F = Form<PREC>;
 	if constexpr (MU == 2) {
-		return F::mu_noise(c, lap, normal_deviate(p.seed, p.step, node), p);
+		return F::mu_noise(c, lap, gd, p);
 	} else if constexpr (MU == 1) {
-		(void)node;
+		(void)gd;
 		return F::mu_dw(c, lap, p);
 	} else {
-		(void)node;
+		(void)gd;
 		return F::mu_poly(c, lap, p);
 	}
 }

@@ -66,7 +66,8 @@ __global__ void __launch_bounds__(256) ac_generic(const typename Form<PREC>::S* 
 		int x[3];
 		coords<DIM>(g, i, x);
 		if constexpr (NOISE)
-			b[i] = F::ac_noise(a[i], lap_at<DIM, PREC>(g, a, i, x), normal_deviate(p.seed, p.step, (uint64_t)i), p);
+			b[i] = F::ac_noise(a[i], lap_at<DIM, PREC>(g, a, i, x),
+			                   normal_deviate(p.seed, p.step, x[0], (uint32_t)(i - x[0] * g.s[0])), p);
 		else
 			b[i] = F::ac(a[i], lap_at<DIM, PREC>(g, a, i, x), p);
 	}
@@ -82,7 +83,9 @@ __global__ void __launch_bounds__(256) ch_mu_generic(const typename Form<PREC>::
 		int x[3];
 		coords<DIM>(g, i, x);
 		typename F::L lap = lap_at<DIM, PREC>(g, a, i, x);
-		mu[i] = mu_of<PREC, MU>(a[i], lap, p, (uint64_t)i);
+		float gd = 0.0f;
+		if constexpr (MU == 2) gd = normal_deviate(p.seed, p.step, x[0], (uint32_t)(i - x[0] * g.s[0]));
+		mu[i] = mu_of<PREC, MU>(a[i], lap, p, gd);
 	}
 }
 

@@ -109,11 +109,12 @@ __device__ __forceinline__ void ch_tile(const typename Form<PREC>::S* __restrict
 					F::lap_add(lap, g.w[0], g.wf[0], fp[q], y, fm[q]);
 					if (DIM == 3) F::lap_add(lap, g.w[1], g.wf[1], fc[q + PZ], y, fc[q - PZ]);
 					F::lap_add(lap, g.w[DIM - 1], g.wf[DIM - 1], fc[q + 1], y, fc[q - 1]);
-					const uint64_t node = MU == 2 ? (uint64_t)wrap(P, nx) * plane +
-					                                    (DIM == 3 ? wrap(y0 - HY / 2 + r, ny) * nz : 0) +
-					                                    wrap(z0 - 1 + c, nz)
-					                              : 0;
-					mu[e] = mu_of<PREC, MU>(y, lap, p, node);
+					float gd = 0.0f;
+					if constexpr (MU == 2)
+						gd = normal_deviate(p.seed, p.step, wrap(P, nx),
+						                    (uint32_t)((DIM == 3 ? wrap(y0 - HY / 2 + r, ny) * nz : 0) +
+						                               wrap(z0 - 1 + c, nz)));
+					mu[e] = mu_of<PREC, MU>(y, lap, p, gd);
 				}
 			}
 		}
@@ -242,6 +243,10 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 	load_own(xb - 1, fc);
 	load_own(xb, fp);
 	load_ring(xb - 1, rc);
+	// noise (MU 2): the own points' deviates of the partner plane nkx
+	int nkx = -1;
+	float nkeep[MU == 2 ? MREG : 1];
+	(void)nkeep;
 
 	for (int P = xb - 1; P <= xe; P++) {
 		// loads for the next iteration go out first
@@ -257,6 +262,9 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 		__syncthreads();
 		// mu(P) at the own points: axis-0 neighbours from the queue, the rest from LDS
 		S* __restrict__ lmu = smu[P & 1];
+		const int nx_w = wrap(P, nx);
+		const bool nhit = nx_w == nkx;
+		(void)nhit;
 #pragma unroll
 		for (int k = 0; k < MREG; k++) {
 			if (threadIdx.x + k * NT < MN) {
@@ -266,10 +274,13 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 				F::lap_add(lap, g.w[0], g.wf[0], fp[k], y, fm[k]);
 				if (DIM == 3) F::lap_add(lap, g.w[1], g.wf[1], lphi[q + PZ], y, lphi[q - PZ]);
 				F::lap_add(lap, g.w[DIM - 1], g.wf[DIM - 1], lphi[q + 1], y, lphi[q - 1]);
-				mp[k] = mu_of<PREC, MU>(y, lap, p, MU == 2 ? (uint64_t)wrap(P, nx) * plane + ooff[k] : 0);
+				float gd = 0.0f;
+				if constexpr (MU == 2) gd = normal_deviate_cached(p.seed, p.step, nx_w, (uint32_t)ooff[k], nkx, nkeep[k]);
+				mp[k] = mu_of<PREC, MU>(y, lap, p, gd);
 				lmu[threadIdx.x + k * NT] = mp[k];
 			}
 		}
+		if constexpr (MU == 2) nkx = nhit ? -1 : (nx_w ^ 1);
 		__syncthreads();
 		// c'(P-1) at the own interior points
 		if (P - 1 >= xb) {
@@ -325,13 +336,23 @@ __device__ __forceinline__ void ch_wave_dir(const typename Form<PREC>::S* __rest
 	const unsigned cm = (unsigned)wrap(c, ny);
 	const bool out = lane >= 2 && lane < 2 + kChWaveOut && c < ny;
 	auto ld = [&](int x) { return a[(size_t)wrap(x, nx) * ny + cm]; };
-	// mu at row r from phi at r-d (pb), r (pc), r+d (pa)
+	// mu at row r from phi at r-d (pb), r (pc), r+d (pa); rows come in
+	// marching order, so noise (MU 2) keeps the partner row's deviate
+	int nkx = -1;
+	float nkeep = 0.0f;
 	auto mu = [&](int r, S pb, S pc, S pa) {
 		const S zl = from_prev(pc, pc), zh = from_next(pc, pc);
 		typename F::L lap = F::lap_zero();
 		F::lap_add(lap, g.w[0], g.wf[0], DOWN ? pb : pa, pc, DOWN ? pa : pb);
 		F::lap_add(lap, g.w[1], g.wf[1], zh, pc, zl);
-		return mu_of<PREC, MU>(pc, lap, p, MU == 2 ? (uint64_t)wrap(r, nx) * ny + cm : 0);
+		float gd = 0.0f;
+		if constexpr (MU == 2) {
+			const int xw = wrap(r, nx);
+			const bool hit = xw == nkx;
+			gd = normal_deviate_cached(p.seed, p.step, xw, cm, nkx, nkeep);
+			nkx = hit ? -1 : (xw ^ 1);
+		}
+		return mu_of<PREC, MU>(pc, lap, p, gd);
 	};
 	const int x0 = DOWN ? xe - 1 : xb, x1 = DOWN ? xb - 1 : xe;
 	const int n = xe - xb;  // rows written
@@ -800,6 +821,10 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 	auto rrow = [&](int x) { return (unsigned)edge_map(x, nx, g.b0[0], g.b1[0]) * (unsigned)ny; };
 	S w0[Q], w1[Q], w2[Q], w3[Q];  // rows x-1, x, x+1; x+2 in flight
 	S h1 = 0, h2 = 0;              // beside-the-strip columns of rows x and x+1
+	int nkx = -1;                  // model A: the partner row of the kept deviate
+	float nkeep = 0.0f;
+	(void)nkx;
+	(void)nkeep;
 	{
 		const unsigned r0 = rrow(xb - 1), r1 = rrow(xb), r2 = rrow(xb + 1);
 #pragma unroll
@@ -831,7 +856,12 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 			for (int j = 0; j < Q; j++) sum += w1[j] * w1[j];
 		}
 		const unsigned o = (unsigned)x * (unsigned)ny + (unsigned)c;
-		const float gd = KIND == 2 && out ? normal_deviate(p.seed, p.step, o) : 0.0f;
+		float gd = 0.0f;
+		if constexpr (KIND == 2) {
+			const bool hit = x == nkx;
+			if (out) gd = normal_deviate_cached(p.seed, p.step, x, (uint32_t)c, nkx, nkeep);
+			nkx = hit ? -1 : (x ^ 1);
+		}
 #pragma unroll
 		for (int j = 0; j < Q; j++) {
 			const S y = w1[j];

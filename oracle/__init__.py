@@ -72,10 +72,9 @@ def lib():
         L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
         L.orc_ln_unit.restype = C.c_float
         L.orc_ln_unit.argtypes = [C.c_float]
-        L.orc_cos_2pi.restype = C.c_float
-        L.orc_cos_2pi.argtypes = [C.c_float]
+        L.orc_cos_sin_2pi.argtypes = [C.c_float, P(C.c_float), P(C.c_float)]
         L.orc_normal.restype = C.c_float
-        L.orc_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        L.orc_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_ac_noise_steps_f64.argtypes = [G, dp, dp, dp, C.c_int, C.c_uint64, C.c_int]
         L.orc_ac_noise_steps_f32ref.argtypes = [G, fp, fp, dp, C.c_int, C.c_uint64, C.c_int]
         L.orc_ac_noise_steps_f32.argtypes = [G, fp, fp, dp, C.c_int, C.c_uint64]
@@ -201,12 +200,15 @@ def ln_unit(u: float) -> float:
     return lib().orc_ln_unit(u)
 
 
-def cos_2pi(t: float) -> float:
-    return lib().orc_cos_2pi(t)
+def cos_sin_2pi(t: float):
+    c, s = C.c_float(), C.c_float()
+    lib().orc_cos_sin_2pi(t, C.byref(c), C.byref(s))
+    return c.value, s.value
 
 
-def normal(seed: int, step: int, node: int) -> float:
-    return lib().orc_normal(seed, step, node)
+def normal(seed: int, step: int, x: int, r: int = 0) -> float:
+    """The deviate of the node with axis-0 index x and index r within its axis-0 plane."""
+    return lib().orc_normal(seed, step, x, r)
 
 
 def ac_noise(a, params, steps, form="f64", step0=0, cube=CUBE_CR, bc=None, dx=None):

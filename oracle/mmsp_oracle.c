@@ -354,39 +354,52 @@ float orc_ln_unit(float u)
 	return (float)e * 0.693147182f + 2.0f * s * q;
 }
 
-/* cos(2 pi t), t in [0, 1) a multiple of 2^-24: exact reduction by quadrant
- * and octant to theta in [0, pi/4], Taylor series to theta^10 / theta^9. */
-float orc_cos_2pi(float t)
+/* cos(2 pi t) and sin(2 pi t), t in [0, 1) a multiple of 2^-24: quadrant q and
+ * f = 4t - q exact; theta = f pi/2 folded into [0, pi/4]; Taylor series to
+ * theta^10 (cos) and theta^9 (sin) give A = cos(f pi/2), B = sin(f pi/2). */
+void orc_cos_sin_2pi(float t, float* cosv, float* sinv)
 {
 	const float v = t * 4.0f;
 	const int q = (int)v;
 	const float f = v - (float)q;
-	const float gq = (q & 1) ? 1.0f - f : f;
-	const int hi = gq > 0.5f;
-	const float h = hi ? 1.0f - gq : gq;
+	const int hi = f > 0.5f;
+	const float h = hi ? 1.0f - f : f;
 	const float th = h * 1.57079633f, z = th * th;
 	const float cs = 1.0f + z * (-1.0f / 2.0f + z * (1.0f / 24.0f + z * (-1.0f / 720.0f +
 	                 z * (1.0f / 40320.0f + z * (-1.0f / 3628800.0f)))));
 	const float sn = th * (1.0f + z * (-1.0f / 6.0f + z * (1.0f / 120.0f + z * (-1.0f / 5040.0f +
 	                 z * (1.0f / 362880.0f)))));
-	const float c = hi ? sn : cs;
-	return (q == 1 || q == 2) ? -c : c;
+	const float A = hi ? sn : cs, B = hi ? cs : sn;
+	*cosv = q == 0 ? A : q == 1 ? -B : q == 2 ? -A : B;
+	*sinv = q == 0 ? B : q == 1 ? A : q == 2 ? -B : -A;
 }
 
-/* Standard normal deviate for (seed, step, node): Box-Muller on the first two
- * words of Philox4x32-10 with key {seed lo, seed hi} and counter {node lo,
- * node hi, step lo, step hi}: u1 = (w0 + 1/2) 2^-32, u2 = (w1 >> 8) 2^-24,
- * g = sqrt(-2 ln u1) cos(2 pi u2) in float. (The reference's gaussian(),
- * model_A.cpp:13-39, uses the polar method on rand(): the same distribution.) */
-float orc_normal(uint64_t seed, uint64_t step, uint64_t node)
+/* Standard normal deviate of the node with axis-0 index x and index r within
+ * its axis-0 plane: Box-Muller on the first two words of Philox4x32-10 with key
+ * {seed lo, seed hi} and counter {r, x >> 1, step lo, step hi}: u1 = (w0 + 1/2)
+ * 2^-32, u2 = (w1 >> 8) 2^-24, R = sqrt(-2 ln u1); R cos(2 pi u2) for even x,
+ * R sin(2 pi u2) for odd x, in float. (The reference's gaussian(),
+ * model_A.cpp:13-39, uses the polar method on rand() and likewise hands out the
+ * two values of each pair in turn: the same distribution.) */
+float orc_normal(uint64_t seed, uint64_t step, uint64_t x, uint64_t r)
 {
 	const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-	const uint32_t ctr[4] = {(uint32_t)node, (uint32_t)(node >> 32), (uint32_t)step, (uint32_t)(step >> 32)};
+	const uint32_t ctr[4] = {(uint32_t)r, (uint32_t)(x >> 1), (uint32_t)step, (uint32_t)(step >> 32)};
 	uint32_t w[4];
 	orc_philox4x32_10(ctr, key, w);
 	const float u1 = ((float)w[0] + 0.5f) * 0x1p-32f;
 	const float u2 = (float)(w[1] >> 8) * 0x1p-24f;
-	return sqrtf(-2.0f * orc_ln_unit(u1)) * orc_cos_2pi(u2);
+	float c, sn;
+	orc_cos_sin_2pi(u2, &c, &sn);
+	const float R = sqrtf(-2.0f * orc_ln_unit(u1));
+	return (x & 1) ? R * sn : R * c;
+}
+
+/* the deviate of node i: x = i / plane, r = i mod plane (plane = nodes per axis-0 index) */
+static float node_normal(const orc_grid* g, uint64_t seed, uint64_t step, int64_t i)
+{
+	const int64_t plane = nodes_of(g) / g->n[0];
+	return orc_normal(seed, step, (uint64_t)(i / plane), (uint64_t)(i % plane));
 }
 
 /* ---- model A: p = {r, u, K, M, dt, sigma, seed}; step0 = the state's step count
@@ -401,7 +414,7 @@ void orc_ac_noise_steps_f64(const orc_grid* g, double* a, double* b, const doubl
 	for (int s = 0; s < steps; s++) {
 		for (int64_t i = 0; i < n; i++) {
 			double phi = a[i];
-			double noise = 0.0 + (double)orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma;
+			double noise = 0.0 + (double)node_normal(g, seed, step0 + (uint64_t)s, i) * sigma;
 			b[i] = phi - dt * M * (-r * phi + u * cube64(phi, cube) - K * orc_laplacian_f64(g, a, i) + noise);
 		}
 		memcpy(a, b, (size_t)n * sizeof(double));
@@ -419,7 +432,7 @@ void orc_ac_noise_steps_f32ref(const orc_grid* g, float* a, float* b, const doub
 		for (int64_t i = 0; i < n; i++) {
 			float phi = a[i];
 			float lap = orc_laplacian_f32ref(g, a, i);
-			float noise = (float)(0.0 + (double)orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma);
+			float noise = (float)(0.0 + (double)node_normal(g, seed, step0 + (uint64_t)s, i) * sigma);
 			b[i] = (float)((double)phi -
 			               dt * M * (-r * (double)phi + u * cube64((double)phi, cube) - K * (double)lap + (double)noise));
 		}
@@ -437,7 +450,7 @@ void orc_ac_noise_steps_f32(const orc_grid* g, float* a, float* b, const double*
 		for (int64_t i = 0; i < n; i++) {
 			float phi = a[i];
 			float c3 = phi * phi * phi;
-			float noise = orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma;
+			float noise = node_normal(g, seed, step0 + (uint64_t)s, i) * sigma;
 			b[i] = phi - dt * M * (-r * phi + u * c3 - K * orc_laplacian_f32(g, a, i) + noise);
 		}
 		memcpy(a, b, (size_t)n * sizeof(float));
@@ -456,7 +469,7 @@ void orc_ch_noise_steps_f64(const orc_grid* g, double* a, double* b, double* mu,
 	for (int s = 0; s < steps; s++) {
 		for (int64_t i = 0; i < n; i++) {
 			double c = a[i];
-			double noise = 0.0 + (double)orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma;
+			double noise = 0.0 + (double)node_normal(g, seed, step0 + (uint64_t)s, i) * sigma;
 			mu[i] = -p[0] * c + p[1] * cube64(c, cube) - p[2] * orc_laplacian_f64(g, a, i) + noise;
 		}
 		for (int64_t i = 0; i < n; i++) b[i] = a[i] + p[4] * p[3] * orc_laplacian_f64(g, mu, i);
@@ -474,7 +487,7 @@ void orc_ch_noise_steps_f32ref(const orc_grid* g, float* a, float* b, float* mu,
 		for (int64_t i = 0; i < n; i++) {
 			double c = a[i];
 			double lap = orc_laplacian_f32ref(g, a, i);
-			float noise = (float)(0.0 + (double)orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma);
+			float noise = (float)(0.0 + (double)node_normal(g, seed, step0 + (uint64_t)s, i) * sigma);
 			mu[i] = (float)(-p[0] * c + p[1] * cube64(c, cube) - p[2] * lap + (double)noise);
 		}
 		for (int64_t i = 0; i < n; i++)
@@ -493,7 +506,7 @@ void orc_ch_noise_steps_f32(const orc_grid* g, float* a, float* b, float* mu, co
 		for (int64_t i = 0; i < n; i++) {
 			float c = a[i];
 			float lap = orc_laplacian_f32(g, a, i);
-			float noise = orc_normal(seed, step0 + (uint64_t)s, (uint64_t)i) * sigma;
+			float noise = node_normal(g, seed, step0 + (uint64_t)s, i) * sigma;
 			mu[i] = -(float)p[0] * c + (float)p[1] * (c * c * c) - (float)p[2] * lap + noise;
 		}
 		const float M = (float)p[3], dt = (float)p[4];

@@ -49,27 +49,36 @@ def test_ln_and_cos_against_libm():
     assert O.ln_unit(1.0) == 0.0
     ts = np.concatenate([np.arange(0, 2 ** 24, 4099), [0, 2 ** 22, 2 ** 23, 3 * 2 ** 22, 2 ** 24 - 1]]) * 2.0 ** -24
     for t in ts:
-        got, want = O.cos_2pi(float(t)), math.cos(2 * math.pi * float(t))
-        assert abs(got - want) <= 3e-7, (t, got, want)
-    assert O.cos_2pi(0.0) == 1.0 and O.cos_2pi(0.5) == -1.0
+        c, s = O.cos_sin_2pi(float(t))
+        assert abs(c - math.cos(2 * math.pi * float(t))) <= 3e-7, (t, c)
+        assert abs(s - math.sin(2 * math.pi * float(t))) <= 3e-7, (t, s)
+    assert O.cos_sin_2pi(0.0) == (1.0, 0.0) and O.cos_sin_2pi(0.5) == (-1.0, 0.0)
+    assert O.cos_sin_2pi(0.25) == (0.0, 1.0) and O.cos_sin_2pi(0.75) == (0.0, -1.0)
 
 
 def test_deviates_are_standard_normal_and_independent():
-    n = 200000
-    g = np.array([O.normal(11, 3, i) for i in range(n)])
-    assert abs(g.mean()) < 5 / math.sqrt(n)
-    assert abs(g.var() - 1.0) < 5 * math.sqrt(2.0 / n)
-    assert abs(scipy_stats.skew(g)) < 5 * math.sqrt(6.0 / n)
-    assert abs(scipy_stats.kurtosis(g)) < 5 * math.sqrt(24.0 / n)
+    # node (x, r): x the axis-0 index, r the index within the plane; the two
+    # planes x = 2k and 2k+1 share a Philox block (the cos and sin of one pair)
+    n = 100000
+    ge = np.array([O.normal(11, 3, 2 * (i % 50), i // 50) for i in range(n)])
+    go = np.array([O.normal(11, 3, 2 * (i % 50) + 1, i // 50) for i in range(n)])
+    g = np.concatenate([ge, go])
+    m = g.size
+    assert abs(g.mean()) < 5 / math.sqrt(m)
+    assert abs(g.var() - 1.0) < 5 * math.sqrt(2.0 / m)
+    assert abs(scipy_stats.skew(g)) < 5 * math.sqrt(6.0 / m)
+    assert abs(scipy_stats.kurtosis(g)) < 5 * math.sqrt(24.0 / m)
     assert scipy_stats.kstest(g, "norm").pvalue > 1e-4
-    # neighbouring nodes, the next step, another seed
-    h = np.array([O.normal(11, 4, i) for i in range(20000)])
-    k = np.array([O.normal(12, 3, i) for i in range(20000)])
-    lim = 5 / math.sqrt(20000)
-    assert abs(np.corrcoef(g[:19999], g[1:20000])[0, 1]) < lim
-    assert abs(np.corrcoef(g[:20000], h)[0, 1]) < lim
-    assert abs(np.corrcoef(g[:20000], k)[0, 1]) < lim
-    assert O.normal(11, 3, 17) == g[17]  # a function of (seed, step, node) only
+    lim = 5 / math.sqrt(n)
+    # the two values of one pair, neighbouring r, the next step, another seed
+    assert abs(np.corrcoef(ge, go)[0, 1]) < lim
+    assert abs(np.corrcoef(ge ** 2, go ** 2)[0, 1]) < lim  # independent, not merely uncorrelated
+    h = np.array([O.normal(11, 4, 2 * (i % 50), i // 50) for i in range(n)])
+    k = np.array([O.normal(12, 3, 2 * (i % 50), i // 50) for i in range(n)])
+    r1 = np.array([O.normal(11, 3, 2 * (i % 50), i // 50 + 7919) for i in range(n)])
+    for other in (h, k, r1):
+        assert abs(np.corrcoef(ge, other)[0, 1]) < lim
+    assert O.normal(11, 3, 2 * 17, 0) == ge[17]  # a function of (seed, step, x, r) only
     assert np.abs(g).max() < 6.8  # u1 >= 2^-33 bounds |g| by sqrt(66 ln 2)
 
 
