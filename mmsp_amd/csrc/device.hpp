@@ -387,4 +387,125 @@ __device__ __forceinline__ typename Form<PREC>::S mu_of(typename Form<PREC>::S c
 	}
 }
 
+// Pairwise variational derivatives (grain_growth/anisotropic/phase_field/
+// graingrowth.cpp:141-157) and rates (:160-167) over the field slots set in
+// `act`, in the reference's loop and operation order (ascending slots); dp
+// gets the rates, zero in the other slots. Up to 8 slots the loops unroll and
+// the arrays live in registers.
+template <int PREC, int QMAX>
+__device__ __forceinline__ void gga_rates(const typename Form<PREC>::S (&phi)[QMAX],
+                                          const typename Form<PREC>::S (&lap)[QMAX], uint32_t act,
+                                          const ModelParams& p, typename Form<PREC>::S (&dp)[QMAX])
+{
+	using S = typename Form<PREC>::S;
+	constexpr bool dbl = PREC == MMSPX_F64;
+	constexpr int UN = QMAX <= 8 ? QMAX : 1;
+	const S c2 = dbl ? (S)p.an[AN_C2] : (S)p.anf[AN_C2];
+	const S w = dbl ? (S)p.an[AN_W] : (S)p.anf[AN_W];
+	const S mu = dbl ? (S)p.an[AN_MU] : (S)p.anf[AN_MU];
+	S dF[QMAX];
+#pragma unroll UN
+	for (int h = 0; h < QMAX; h++) {
+		dF[h] = 0;
+		dp[h] = 0;
+	}
+#pragma unroll UN
+	for (int h = 0; h < QMAX; h++) {
+		if (!(act >> h & 1u)) continue;
+#pragma unroll UN
+		for (int j = h + 1; j < QMAX; j++) {
+			if (!(act >> j & 1u)) continue;
+			dF[h] += c2 * lap[j] + w * phi[j];
+			dF[j] += c2 * lap[h] + w * phi[h];
+#pragma unroll UN
+			for (int k = j + 1; k < QMAX; k++) {
+				if (!(act >> k & 1u)) continue;
+				dF[h] += S(3) * phi[j] * phi[k];
+				dF[j] += S(3) * phi[k] * phi[h];
+				dF[k] += S(3) * phi[h] * phi[j];
+			}
+		}
+	}
+#pragma unroll UN
+	for (int h = 0; h < QMAX; h++) {
+		if (!(act >> h & 1u)) continue;
+#pragma unroll UN
+		for (int j = h + 1; j < QMAX; j++) {
+			if (!(act >> j & 1u)) continue;
+			dp[h] -= mu * (dF[h] - dF[j]);
+			dp[j] -= mu * (dF[j] - dF[h]);
+		}
+	}
+}
+
+// Multi-phase grain growth at one node with at least two fields present
+// (graingrowth.cpp:141-185): `act` holds the present fields (cnt of them) and
+// lap their laplacians. gga_rates, then the clamped update and the projection
+// onto the simplex (:170-185); phi holds the node's values on entry and the
+// new ones on return. Fields at or past Q are skipped. For 5 to 8 field slots,
+// a node with at most 4 fields present packs them, in ascending order, into 4
+// slots first: the same additions in the same order as the full loops (which
+// skip the absent fields), on a quarter of the field triples.
+template <int PREC, int QMAX>
+__device__ __forceinline__ void gga_node(typename Form<PREC>::S (&phi)[QMAX], const typename Form<PREC>::S (&lap)[QMAX],
+                                         uint32_t act, typename Form<PREC>::S cnt, int Q, const ModelParams& p)
+{
+	using S = typename Form<PREC>::S;
+	constexpr bool dbl = PREC == MMSPX_F64;
+	constexpr int UN = QMAX <= 8 ? QMAX : 1;
+	constexpr int K = 4;
+	const S dt = dbl ? (S)p.an[AN_DT] : (S)p.anf[AN_DT];
+	S dp[QMAX];
+	if constexpr (QMAX > K && QMAX <= 8) {
+		if (cnt <= S(K)) {
+			S cphi[K], clap[K], cdp[K];
+#pragma unroll
+			for (int s = 0; s < K; s++) cphi[s] = clap[s] = 0;
+#pragma unroll
+			for (int f = 0; f < QMAX; f++) {
+				const bool on = act >> f & 1u;
+				const int r = __popc(act & ((1u << f) - 1u));  // the field's rank among the present ones
+#pragma unroll
+				for (int s = 0; s < K; s++) {
+					if (on && r == s) {
+						cphi[s] = phi[f];
+						clap[s] = lap[f];
+					}
+				}
+			}
+			gga_rates<PREC, K>(cphi, clap, (1u << (int)cnt) - 1u, p, cdp);
+#pragma unroll
+			for (int f = 0; f < QMAX; f++) {
+				const bool on = act >> f & 1u;
+				const int r = __popc(act & ((1u << f) - 1u));
+				S v = 0;
+#pragma unroll
+				for (int s = 0; s < K; s++)
+					if (on && r == s) v = cdp[s];
+				dp[f] = v;
+			}
+		} else {
+			gga_rates<PREC, QMAX>(phi, lap, act, p, dp);
+		}
+	} else {
+		gga_rates<PREC, QMAX>(phi, lap, act, p, dp);
+	}
+	// clamped update and projection onto the simplex (:170-185)
+	S sum = 0;
+	const S rate = dt * (S(2) / cnt);
+#pragma unroll UN
+	for (int h = 0; h < QMAX; h++) {
+		if (h >= Q) continue;
+		S v = phi[h] + rate * dp[h];
+		if (v > S(1)) v = S(1);
+		if (v < S(0)) v = S(0);
+		phi[h] = v;
+		sum += v;
+	}
+	const S rsum = fabs(sum) > S(0) ? S(1) / sum : S(0);
+#pragma unroll UN
+	for (int h = 0; h < QMAX; h++)
+		if (h < Q) phi[h] = phi[h] * rsum;
+}
+
 }  // namespace mmspx

@@ -182,9 +182,7 @@ __global__ void __launch_bounds__(256) ow_pass1(const typename Form<PREC>::S* __
 // graingrowth.cpp:108-188, one thread per node, at most kAnisoMaxFields
 // fields. The fields present in the node's 2*dim+1 neighbourhood (mapped as
 // check_boundary maps) form the active set; below two the node is copied.
-// Otherwise the pairwise variational derivatives (:141-157) and rates
-// (:160-167), the clamped update and the projection onto the simplex
-// (:170-185), in the reference's loop and operation order.
+// Otherwise gga_node (device.hpp) updates it.
 // QMAX: the field count the per-node arrays are compiled for; every loop over
 // fields runs to QMAX with the fields past g.fields skipped, so for QMAX up to
 // 8 the loops unroll, the arrays are indexed by constants and live in
@@ -195,11 +193,6 @@ __global__ void __launch_bounds__(256) gga_generic(const typename Form<PREC>::S*
 {
 	using F = Form<PREC>;
 	using S = typename F::S;
-	constexpr bool dbl = PREC == MMSPX_F64;
-	const S c2 = dbl ? (S)p.an[AN_C2] : (S)p.anf[AN_C2];
-	const S w = dbl ? (S)p.an[AN_W] : (S)p.anf[AN_W];
-	const S mu = dbl ? (S)p.an[AN_MU] : (S)p.anf[AN_MU];
-	const S dt = dbl ? (S)p.an[AN_DT] : (S)p.anf[AN_DT];
 	const int64_t N = g.nodes;
 	const int Q = g.fields;
 	// full unrolling (register arrays) up to 8 fields; more keep their loops
@@ -232,70 +225,23 @@ __global__ void __launch_bounds__(256) gga_generic(const typename Form<PREC>::S*
 				}
 			}
 		}
-		if (cnt < S(2)) {
+		if (cnt >= S(2)) {
+			S lap[QMAX];
 #pragma unroll UN
-			for (int h = 0; h < QMAX; h++)
-				if (h < Q) b[h * N + i] = phi[h];
-			continue;
-		}
-		S lap[QMAX], dF[QMAX], dp[QMAX];
-#pragma unroll UN
-		for (int h = 0; h < QMAX; h++) {
-			typename F::L l = F::lap_zero();
-			if (act >> h & 1u) {
-				const S* __restrict__ pl = a + h * N;
+			for (int h = 0; h < QMAX; h++) {
+				typename F::L l = F::lap_zero();
+				if (act >> h & 1u) {
+					const S* __restrict__ pl = a + h * N;
 #pragma unroll
-				for (int d = 0; d < DIM; d++) F::lap_add(l, g.w[d], g.wf[d], pl[ih[d]], phi[h], pl[il[d]]);
-			}
-			lap[h] = l;
-			dF[h] = 0;
-			dp[h] = 0;
-		}
-		// variational derivatives (:141-157)
-#pragma unroll UN
-		for (int h = 0; h < QMAX; h++) {
-			if (!(act >> h & 1u)) continue;
-#pragma unroll UN
-			for (int j = h + 1; j < QMAX; j++) {
-				if (!(act >> j & 1u)) continue;
-				dF[h] += c2 * lap[j] + w * phi[j];
-				dF[j] += c2 * lap[h] + w * phi[h];
-#pragma unroll UN
-				for (int k = j + 1; k < QMAX; k++) {
-					if (!(act >> k & 1u)) continue;
-					dF[h] += S(3) * phi[j] * phi[k];
-					dF[j] += S(3) * phi[k] * phi[h];
-					dF[k] += S(3) * phi[h] * phi[j];
+					for (int d = 0; d < DIM; d++) F::lap_add(l, g.w[d], g.wf[d], pl[ih[d]], phi[h], pl[il[d]]);
 				}
+				lap[h] = l;
 			}
+			gga_node<PREC, QMAX>(phi, lap, act, cnt, Q, p);
 		}
-		// rates (:160-167)
-#pragma unroll UN
-		for (int h = 0; h < QMAX; h++) {
-			if (!(act >> h & 1u)) continue;
-#pragma unroll UN
-			for (int j = h + 1; j < QMAX; j++) {
-				if (!(act >> j & 1u)) continue;
-				dp[h] -= mu * (dF[h] - dF[j]);
-				dp[j] -= mu * (dF[j] - dF[h]);
-			}
-		}
-		// clamped update and projection onto the simplex (:170-185)
-		S sum = 0;
-		const S rate = dt * (S(2) / cnt);
-#pragma unroll UN
-		for (int h = 0; h < QMAX; h++) {
-			if (h >= Q) continue;
-			S v = phi[h] + rate * dp[h];
-			if (v > S(1)) v = S(1);
-			if (v < S(0)) v = S(0);
-			phi[h] = v;
-			sum += v;
-		}
-		const S rsum = fabs(sum) > S(0) ? S(1) / sum : S(0);
 #pragma unroll UN
 		for (int h = 0; h < QMAX; h++)
-			if (h < Q) b[h * N + i] = phi[h] * rsum;
+			if (h < Q) b[h * N + i] = phi[h];
 	}
 }
 

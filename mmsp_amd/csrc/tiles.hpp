@@ -796,7 +796,10 @@ __device__ __forceinline__ void gg_tile(const typename Form<PREC>::S* __restrict
 // formed in the reference's order before the field updates
 // (graingrowth.cpp:91-107); KIND 0, Allen-Cahn (Q = 1, allen-cahn.cpp:68-71);
 // KIND 2, model A (Allen-Cahn plus noise, model_A.cpp:97-101); KIND 3, Zener
-// pinning (grain growth with field 0 held, zener.cpp:117-135).
+// pinning (grain growth with field 0 held, zener.cpp:117-135); KIND 4,
+// multi-phase grain growth (grain_growth/anisotropic/phase_field/
+// graingrowth.cpp:108-188: the fields present among the node and its four
+// neighbours, then gga_node; fewer than two present, the node is copied).
 template <int PREC, int Q, int KIND>
 __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restrict__ a,
                                            typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
@@ -862,18 +865,44 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 			if (out) gd = normal_deviate_cached(p.seed, p.step, x, (uint32_t)c, nkx, nkeep);
 			nkx = hit ? -1 : (x ^ 1);
 		}
+		if constexpr (KIND == 4) {
+			S phi[Q], lap[Q];
+			uint32_t act = 0;
+			S cnt = 0;
 #pragma unroll
-		for (int j = 0; j < Q; j++) {
-			const S y = w1[j];
-			const S yl = from_prev(y, lane_value(h1, j)), yh = from_next(y, lane_value(h1, Q + j));
-			typename F::L lap = F::lap_zero();
-			F::lap_add(lap, g.w[0], g.wf[0], w2[j], y, w0[j]);
-			F::lap_add(lap, g.w[1], g.wf[1], yh, y, yl);
-			if (out)
-				(b + j * N)[o] = KIND == 1 ? F::gg(y, sum, lap, p)
-				                 : KIND == 2 ? F::ac_noise(y, lap, gd, p)
-				                 : KIND == 3 ? (j == 0 ? y : F::gg(y, sum, lap, p))
-				                             : F::ac(y, lap, p);
+			for (int j = 0; j < Q; j++) {
+				const S y = w1[j];
+				const S yl = from_prev(y, lane_value(h1, j)), yh = from_next(y, lane_value(h1, Q + j));
+				phi[j] = y;
+				const bool on = y > S(0) || w2[j] > S(0) || w0[j] > S(0) || yh > S(0) || yl > S(0);
+				typename F::L l = F::lap_zero();
+				if (on) {
+					act |= 1u << j;
+					cnt += S(1);
+					F::lap_add(l, g.w[0], g.wf[0], w2[j], y, w0[j]);
+					F::lap_add(l, g.w[1], g.wf[1], yh, y, yl);
+				}
+				lap[j] = l;
+			}
+			if (cnt >= S(2)) gga_node<PREC, Q>(phi, lap, act, cnt, Q, p);
+			if (out) {
+#pragma unroll
+				for (int j = 0; j < Q; j++) (b + j * N)[o] = phi[j];
+			}
+		} else {
+#pragma unroll
+			for (int j = 0; j < Q; j++) {
+				const S y = w1[j];
+				const S yl = from_prev(y, lane_value(h1, j)), yh = from_next(y, lane_value(h1, Q + j));
+				typename F::L lap = F::lap_zero();
+				F::lap_add(lap, g.w[0], g.wf[0], w2[j], y, w0[j]);
+				F::lap_add(lap, g.w[1], g.wf[1], yh, y, yl);
+				if (out)
+					(b + j * N)[o] = KIND == 1 ? F::gg(y, sum, lap, p)
+					                 : KIND == 2 ? F::ac_noise(y, lap, gd, p)
+					                 : KIND == 3 ? (j == 0 ? y : F::gg(y, sum, lap, p))
+					                             : F::ac(y, lap, p);
+			}
 		}
 #pragma unroll
 		for (int j = 0; j < Q; j++) {

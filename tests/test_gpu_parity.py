@@ -489,3 +489,64 @@ def test_ostwald_wave_strips(prec):
                 g.upload(a)
                 g.step("ostwald", p, 9)
                 assert np.array_equal(g.download(), exp), (shape, q, chunk)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_multiphase_wave_strips(prec):
+    """The 2-D multi-phase wave strips (2, 4 and 8 fields, any boundary):
+    one-hot grains and sparse random fields (every active-set size), partial
+    strips, grids narrower than a strip, chunks of 2 to 32 rows; bitwise
+    against the oracle, and the default choice against the generic kernel."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(os.path.dirname(__file__), "golden",
+                                                                              "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    p = [0.01, 8.0, 1.0, 1.0]
+    cases = [((40, 300), 4, None, "grains"), ((17, 5), 2, [(N, D), (M, P)], "sparse"),
+             ((70, 130), 8, [(P, P), (N, M)], "sparse"), ((33, 64), 8, None, "grains"),
+             ((9, 61), 2, [(D, N), (P, D)], "grains"), ((50, 129), 4, [(M, M), (D, D)], "sparse"),
+             ((40, 200), 8, [(P, P), (P, P)], "thin")]
+    for shape, q, bc, kind in cases:
+        n = int(np.prod(shape))
+        if kind == "grains":
+            a = mg.one_hot_grains(shape, q, 700 + q)
+        else:
+            v = O.uniform(n * q, 710 + q, 0.0, 1.0).reshape(shape + (q,))
+            # "thin": 2 to 7 fields present at most nodes (both paths of the 8-field kernels)
+            keep = O.uniform(n * q, 720 + q, 0.0, 1.0).reshape(shape + (q,)) < (0.1 if kind == "thin" else 0.35)
+            a = np.where(keep, v, 0.0)
+        if prec == "f32":
+            a = a.astype(np.float32)
+        exp = O.gg_aniso(a, p, 9, form=prec, bc=bc)
+        for chunk in (2, 3, 8, 32):
+            say("[gpu] multi-phase wave strips %s q=%d %s chunk %d %s" % (shape, q, kind, chunk, prec))
+            with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+                g.set_variant(2)
+                g.set_tile(1, 64, chunk)
+                g.set_tile_options(body=2)
+                g.upload(a)
+                g.step("gg_aniso", p, 9)
+                assert np.array_equal(g.download(), exp), (shape, q, chunk)
+        with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+            g.upload(a)
+            g.step("gg_aniso", p, 4)
+            g.step("gg_aniso", p, 5)
+            assert np.array_equal(g.download(), exp), (shape, q, "default")
+        with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+            g.set_variant(1)
+            g.upload(a)
+            g.step("gg_aniso", p, 9)
+            assert np.array_equal(g.download(), exp), (shape, q, "generic")
+    a = mg.one_hot_grains((20, 70), 6, 7)
+    if prec == "f32":
+        a = a.astype(np.float32)
+    with mx.Grid((20, 70), fields=6, vector=True, precision=prec) as g:
+        g.upload(a)
+        g.set_variant(2)  # 6 fields: no strips compiled
+        with pytest.raises(mx.MmspxError):
+            g.step("gg_aniso", p, 1)
+        g.set_variant(0)  # the default falls back to the generic kernel
+        g.step("gg_aniso", p, 2)
+        assert np.array_equal(g.download(), O.gg_aniso(a, p, 2, form=prec))
