@@ -217,14 +217,14 @@ int mmspx_set_variant(mmspx_state* s, int variant);
 int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
 /* Tile body, all with the same bits. Cahn-Hilliard: 0 keeps every neighbour
  * in LDS rings; 1 keeps the axis-0 neighbours in per-thread register queues;
- * 3-D 32 x 64 tiles only: 2 is the register queues with 768 threads per tile
- * (fewer points, fewer registers each). 2-D Allen-Cahn and grain growth with
+ * 3-D 32 x 64 tiles only: 2 and 3 are the register queues with 768 and 1024
+ * threads per tile (fewer points, fewer registers each). 2-D Allen-Cahn and grain growth with
  * 1 x 64 strips: 2 selects the wave strips (one 64-column strip per wave,
  * neighbours across lanes by DPP, no LDS). */
 int mmspx_set_tile_options(mmspx_state* s, int body);
 /* Returns variant + 16 * tile body (1: register queues, or the 2-D
  * grain-growth tile; 2: wave strips, or 3-D register queues on 768 threads;
- * 3: temporal blocks). Before any tile is set, reports what the last step ran
+ * 3: temporal blocks, or 3-D register queues on 1024 threads). Before any tile is set, reports what the last step ran
  * (the Cahn-Hilliard default before any step). */
 int mmspx_get_tile(const mmspx_state* s, int* ty, int* tz, int* chunk);
 /* Project autotuner: times every candidate tile for `model` on this state's

@@ -222,13 +222,16 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	if (g.dim == 2)
 		return TuneChoice{2, 1, kChWaveOut, nodes < ((int64_t)1 << 21) ? 4 : nodes < ((int64_t)1 << 23) ? 8 : 16, 0};
 	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16, 0};
-	// 256^3: register queues on 32 x 64 tiles, 16-plane chunks. FP64 with 768
-	// threads per tile (54.1-55.0 us per step, against 55.9-57.1 us with 512
-	// threads, 57.6 us for 16 x 64 tiles, 61.5 us with 1024 threads and 64 us
-	// for the LDS-ring body's best tile; neither a second plane of loads in
-	// flight nor odd chunks marching downward gained); FP32 with 512 threads
-	// (37.0-37.8 us, 36.4-38.6 us with 768: within the noise)
-	return TuneChoice{2, 32, 64, 16, prec == MMSPX_F64 ? 2 : 1};
+	// 256^3: register queues on 32 x 64 tiles with 768 threads. FP64 with
+	// 32-plane chunks, one workgroup per CU (52.3 us per step, against 53.0
+	// with 512 or 1024 threads, 55.9 with 16-plane chunks, 54.4 for 16 x 64
+	// tiles with 24-plane chunks, 64-75 us for chunks of 12, 20 and 24 that
+	// leave CUs with uneven work); FP32 with 16-plane chunks, two per CU
+	// (31.3 us, against 31.7 with 512 threads, 35.0 with 32-plane chunks)
+	// model B (noise, FP64): 32-plane chunks on 1024 threads (95.9 us against
+	// 97.6 with 768)
+	if (model == MMSPX_CH_NOISE && prec == MMSPX_F64) return TuneChoice{2, 32, 64, 32, 3};
+	return TuneChoice{2, 32, 64, prec == MMSPX_F64 ? 32 : 16, 2};
 }
 
 bool tile_known(const Geom& g, const TuneChoice& t)
@@ -272,7 +275,7 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		// 32 x 64: register queues only (the LDS rings would leave one
 		// workgroup per CU), on 512 and 768 threads
 		for (int c : {12, 16, 22, 32})
-			for (int body = 1; body <= 2; body++) add(32, 64, c, body);
+			for (int body = 1; body <= 3; body++) add(32, 64, c, body);
 		for (int c : {20, 22, 24}) add(16, 64, c, 1);
 	} else if (g.dim == 2) {
 		// LDS-row strips (Cahn-Hilliard: both tile bodies)
@@ -346,9 +349,10 @@ static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, vo
                           const TuneChoice& t)
 {
 	const int chunk = t.chunk > 0 ? t.chunk : 32;
-	if (g.dim == 3 && t.opt == 2) {
-		// register queues on 32 x 64 tiles with 768 threads (fewer own points each)
-		if (t.ty != 32 || t.tz != 64) return fail(MMSPX_EINVAL, "tile body 2 runs 32 x 64 tiles");
+	if (g.dim == 3 && (t.opt == 2 || t.opt == 3)) {
+		// register queues on 32 x 64 tiles with 768 or 1024 threads (fewer own points each)
+		if (t.ty != 32 || t.tz != 64) return fail(MMSPX_EINVAL, "tile bodies 2 and 3 run 32 x 64 tiles");
+		if (t.opt == 3) return launch_ch_body<PREC, MU, 3, 32, 64, 1024, 1>(g, p, a, b, st, chunk);
 		return launch_ch_body<PREC, MU, 3, 32, 64, 768, 1>(g, p, a, b, st, chunk);
 	}
 	if (g.dim == 3) {

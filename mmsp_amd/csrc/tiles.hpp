@@ -217,41 +217,49 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 			rq[j] = pr * PZ + pc;
 		}
 	}
+	// Loads are unconditional: lanes past the region load node 0 of the plane
+	// (offset 0) and planes wrap, so every address is inside the grid. With a
+	// fixed count of loads per plane and no register copies of a plane in
+	// flight, the compiler waits for a plane's values only where they are
+	// first used, one plane after the load.
 	auto load_own = [&](int x, S* v) {
 		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
 #pragma unroll
-		for (int k = 0; k < MREG; k++)
-			if (threadIdx.x + k * NT < MN) v[k] = src[ooff[k]];
+		for (int k = 0; k < MREG; k++) v[k] = src[ooff[k]];
 	};
 	auto load_ring = [&](int x, S* v) {
 		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
 #pragma unroll
-		for (int j = 0; j < RREG; j++)
-			if (threadIdx.x + j * NT < RN) v[j] = src[roff[j]];
+		for (int j = 0; j < RREG; j++) v[j] = src[roff[j]];
 	};
 
-	S fm[MREG], fc[MREG], fp[MREG], fn[MREG];  // phi at P-1, P, P+1; P+2 in flight
-	S mm[MREG], mc[MREG], mp[MREG];            // mu at P-2, P-1, P
-	S rc[RREG], rn[RREG];                      // ring phi at P; P+1 in flight
+	// phi queue: four register planes whose roles (P-1, P, P+1, P+2 in flight)
+	// rotate with the unrolled steps; ring phi: two planes (P, P+1 in flight)
+	S f0[MREG], f1[MREG], f2[MREG], f3[MREG];
+	S mm[MREG], mc[MREG], mp[MREG];  // mu at P-2, P-1, P
+	S r0[RREG], r1[RREG];
 #pragma unroll
-	for (int k = 0; k < MREG; k++) fn[k] = mm[k] = mc[k] = mp[k] = 0;
+	for (int k = 0; k < MREG; k++) f3[k] = mm[k] = mc[k] = mp[k] = 0;
 #pragma unroll
-	for (int j = 0; j < RREG; j++) rn[j] = 0;
+	for (int j = 0; j < RREG; j++) r1[j] = 0;
 	// the previous tile of a persistent loop may still read the planes
 	__syncthreads();
-	load_own(xb - 2, fm);
-	load_own(xb - 1, fc);
-	load_own(xb, fp);
-	load_ring(xb - 1, rc);
+	load_own(xb - 2, f0);
+	load_own(xb - 1, f1);
+	load_own(xb, f2);
+	load_ring(xb - 1, r0);
 	// noise (MU 2): the own points' deviates of the partner plane nkx
 	int nkx = -1;
 	float nkeep[MU == 2 ? MREG : 1];
 	(void)nkeep;
 
-	for (int P = xb - 1; P <= xe; P++) {
-		// loads for the next iteration go out first
-		if (P + 2 <= xe + 1) load_own(P + 2, fn);
-		if (P + 1 <= xe) load_ring(P + 1, rn);
+	// one plane: fm, fc, fp hold phi at P-1, P, P+1; fn receives P+2; rc holds
+	// the ring at P and rn receives P+1
+	auto step = [&](int P, const S* fm, const S* fc, const S* fp, S* fn, const S* rc, S* rn) {
+		// loads for the next planes go out first (past the chunk they fetch
+		// wrapped planes that are not used)
+		load_own(P + 2, fn);
+		load_ring(P + 1, rn);
 		S* __restrict__ lphi = sphi[P & 1];
 #pragma unroll
 		for (int k = 0; k < MREG; k++)
@@ -301,14 +309,18 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 		}
 #pragma unroll
 		for (int k = 0; k < MREG; k++) {
-			fm[k] = fc[k];
-			fc[k] = fp[k];
-			fp[k] = fn[k];
 			mm[k] = mc[k];
 			mc[k] = mp[k];
 		}
-#pragma unroll
-		for (int j = 0; j < RREG; j++) rc[j] = rn[j];
+	};
+	for (int P = xb - 1; P <= xe; P += 4) {
+		step(P, f0, f1, f2, f3, r0, r1);
+		if (P + 1 > xe) break;
+		step(P + 1, f1, f2, f3, f0, r1, r0);
+		if (P + 2 > xe) break;
+		step(P + 2, f2, f3, f0, f1, r0, r1);
+		if (P + 3 > xe) break;
+		step(P + 3, f3, f0, f1, f2, r1, r0);
 	}
 }
 

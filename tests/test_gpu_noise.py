@@ -80,14 +80,15 @@ def test_model_b_generic(prec):
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
 def test_model_b_fused_bodies(prec):
-    """3-D: the LDS-ring body, register queues on 512 and 768 threads, partial
+    """3-D: the LDS-ring body, register queues on 512, 768 and 1024 threads, partial
     tiles and short chunks; 2-D: row strips of both bodies and the wave strips
     (both marching directions)."""
     p = [1.0, 1.0, 1.0, 1.0, 0.005, 0.3, 6]
     shape = (40, 36, 70)
     a = seeded(shape, 43, 0.45, 0.55, prec)
     exp = O.ch_noise(a, p, 7, form=prec)
-    for tile, body in [((16, 64, 16), 0), ((16, 64, 16), 1), ((32, 64, 12), 1), ((32, 64, 16), 2), ((8, 32, 7), 0)]:
+    for tile, body in [((16, 64, 16), 0), ((16, 64, 16), 1), ((32, 64, 12), 1), ((32, 64, 16), 2), ((8, 32, 7), 0),
+                       ((32, 64, 5), 3)]:
         say("[gpu] model B fused %s body %d %s" % (tile, body, prec))
         got = run(shape, prec, "ch_noise", p, 7, a, variant=2, tile=tile, body=body)
         assert np.array_equal(got, exp), (tile, body)

@@ -161,15 +161,17 @@ def test_fused_ch_options_and_directions(prec):
                 g.upload(a)
                 g.step("ch_dw", p, 13)
                 assert np.array_equal(g.download(), exp13), (tile, rq)
-    # register queues on 32 x 64 tiles with 768 threads: partial tiles in both
-    # axes, a short last chunk, a grid smaller than one tile
+    # register queues on 32 x 64 tiles with 768 and 1024 threads: partial
+    # tiles in both axes, short last chunks, a grid smaller than one tile;
+    # chunks of 2, 5, 7 and 16 planes end the four-plane unrolled march at
+    # each of its four steps
     small = (12, 10, 20)
     a_s = O.uniform(int(np.prod(small)), 79, 0.45, 0.55).reshape(small)
     if prec == "f32":
         a_s = a_s.astype(np.float32)
     exp_s = O.ch(a_s, p, 13, mu=O.MU_DW, form=prec)
-    for body in (2,):
-        for chunk in (7, 16):
+    for body in (2, 3):
+        for chunk in (2, 5, 7, 16):
             for arr, ex in ((a, exp13), (a_s, exp_s)):
                 with mx.Grid(arr.shape, precision=prec) as g:
                     g.set_variant(2)
