@@ -575,8 +575,10 @@ def main():
         kname = base + "_generic"
     elif base == "ch":
         kname = "ch_fused"
-    elif tile["opt"] == 2 and base != "ch":
-        kname = "wave_strips2" if tile["ty"] == 2 else "wave_strips"
+    elif base == "ow":
+        kname = "ow_waves"
+    elif tile["opt"] == 2:
+        kname = "wave_strips"
     elif tile["opt"] == 3 and base == "ac":
         kname = "ac_tblocks"
     elif base == "gg" and tile["opt"] == 1:
