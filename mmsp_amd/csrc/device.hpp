@@ -96,8 +96,10 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 #pragma unroll
 	for (int r = 0; r < 10; r++) {
 		const uint32_t m0 = 0xD2511F53u, m1 = 0xCD9E8D57u;
-		const uint32_t lo0 = m0 * c[0], hi0 = __umulhi(m0, c[0]);
-		const uint32_t lo1 = m1 * c[2], hi1 = __umulhi(m1, c[2]);
+		// one 32 x 32 -> 64-bit product gives both halves (v_mad_u64_u32)
+		const uint64_t p0 = (uint64_t)m0 * c[0], p1 = (uint64_t)m1 * c[2];
+		const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+		const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
 		const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
 		c[0] = n0;
 		c[1] = lo1;

@@ -229,7 +229,8 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// leave CUs with uneven work); FP32 with 16-plane chunks, two per CU
 	// (31.3 us, against 31.7 with 512 threads, 35.0 with 32-plane chunks)
 	// model B (noise, FP64): 32-plane chunks on 1024 threads (95.9 us against
-	// 97.6 with 768)
+	// 97.6 with 768 in one sweep, 92.6 against 91.6 in another: the two are
+	// within the noise; 16-plane chunks take 99-107 us)
 	if (model == MMSPX_CH_NOISE && prec == MMSPX_F64) return TuneChoice{2, 32, 64, 32, 3};
 	return TuneChoice{2, 32, 64, prec == MMSPX_F64 ? 32 : 16, 2};
 }
