@@ -189,12 +189,14 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	const int64_t nodes = g.nodes;
 	// Allen-Cahn: periodic grids take temporal blocks of 16 x 48 nodes and 8
 	// steps per pass (512^2: 2.05 us per step in FP64 against 4.58 us for one
-	// thread per node, FP32 1.71 against 3.37); other boundaries keep the
-	// one-thread-per-node kernel, which beats the strips at every size swept.
-	if (model == MMSPX_AC) return all_periodic(g) ? TuneChoice{2, 16, kTbCols, 8, 3} : TuneChoice{1, 0, 0, 0, 0};
-	// model A: the noise costs arithmetic per node and step, so the wave
-	// strips (one load and one store per node) rather than one thread per node
-	if (model == MMSPX_AC_NOISE) return TuneChoice{2, 1, kWaveCols, 16, 2};
+	// thread per node, FP32 1.71 against 3.37).
+	// Other boundaries and model A: the wave strips, 4-row chunks below 2^21
+	// nodes, 8 below 2^23, 16 above (Neumann FP64: 1024^2 5.5 us per step
+	// against 8.6 for one thread per node, 2048^2 12.6 against 16.5, 4096^2
+	// 43.3 against 61.1; model A FP64 1024^2 7.7 us, 4096^2 60.8)
+	const int wave_chunk = nodes < ((int64_t)1 << 21) ? 4 : nodes < ((int64_t)1 << 23) ? 8 : 16;
+	if (model == MMSPX_AC) return all_periodic(g) ? TuneChoice{2, 16, kTbCols, 8, 3} : TuneChoice{2, 1, kWaveCols, wave_chunk, 2};
+	if (model == MMSPX_AC_NOISE) return TuneChoice{2, 1, kWaveCols, wave_chunk, 2};
 	if (model == MMSPX_ZENER) return TuneChoice{2, 1, kWaveCols, 32, 2};
 	// multi-phase grain growth (one-hot grains, 8 fields): 2-row chunks below
 	// 2^21 nodes (1024^2 FP64 36.7 us per step against 39.1 with 4 rows, 40.0

@@ -834,37 +834,36 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 	const S* __restrict__ ah = a + (size_t)(hl ? lane % Q : 0) * N;
 	// offsets inside one field plane (fewer than 2^31 nodes)
 	auto rrow = [&](int x) { return (unsigned)edge_map(x, nx, g.b0[0], g.b1[0]) * (unsigned)ny; };
-	S w0[Q], w1[Q], w2[Q], w3[Q];  // rows x-1, x, x+1; x+2 in flight
-	S h1 = 0, h2 = 0;              // beside-the-strip columns of rows x and x+1
-	int nkx = -1;                  // model A: the partner row of the kept deviate
+	// four row slots (all Q fields of a row) whose roles (x-1, x, x+1, x+2 in
+	// flight) rotate over a four-row unrolled march, so no row is copied
+	// while its loads are in flight and each field's loads are waited for
+	// only where that field is first used; the beside-the-strip register
+	// likewise
+	S wa[Q], wb[Q], wc[Q], wd[Q];
+	S ha = 0, hb = 0, hc = 0, hd = 0;
+	int nkx = -1;  // model A: the partner row of the kept deviate
 	float nkeep = 0.0f;
 	(void)nkx;
 	(void)nkeep;
-	{
-		const unsigned r0 = rrow(xb - 1), r1 = rrow(xb), r2 = rrow(xb + 1);
+	// Row loads are unconditional: past the chunk's last needed row (xe) that
+	// row is loaded again, and lanes at or past 2Q load field 0 beside the
+	// strip, unused; every address is a mapped node of the grid.
+	auto load_row = [&](int x, S* w, S& h) {
+		const unsigned r = rrow(min(x, xe));
 #pragma unroll
-		for (int j = 0; j < Q; j++) {
-			const S* __restrict__ aj = a + j * N;
-			w0[j] = aj[r0 + cm];
-			w1[j] = aj[r1 + cm];
-			w2[j] = aj[r2 + cm];
-			w3[j] = 0;
-		}
-		if (hl) {
-			h1 = ah[r1 + ch];
-			h2 = ah[r2 + ch];
-		}
-	}
-	for (int x = xb; x < xe; x++) {
-		// row x+2 only when row x+1 is computed here (then x+2 <= nx, at most one past the edge)
-		const bool more = x + 1 < xe;
-		if (more) {
-			const unsigned r3 = rrow(x + 2);
+		for (int j = 0; j < Q; j++) w[j] = (a + j * N)[r + cm];
+		h = ah[r + ch];
+	};
 #pragma unroll
-			for (int j = 0; j < Q; j++) w3[j] = (a + j * N)[r3 + cm];
-		}
-		S h3 = 0;
-		if (more && hl) h3 = ah[rrow(x + 2) + ch];
+	for (int j = 0; j < Q; j++) wd[j] = 0;
+	load_row(xb - 1, wa, ha);
+	load_row(xb, wb, hb);
+	load_row(xb + 1, wc, hc);
+	// row x from rows x-1 (w0), x (w1, beside-the-strip h1) and x+1 (w2);
+	// row x+2 goes to w3 and h3
+	auto row = [&](int x, const S* w0, const S* w1, const S* w2, S* w3, const S h1, S& h3) {
+		// (the multi-phase march, with short chunks, skips the load it does not need)
+		if (KIND != 4 || x + 1 < xe) load_row(x + 2, w3, h3);
 		typename F::Sum sum = 0;
 		if (KIND == 1 || KIND == 3) {
 #pragma unroll
@@ -916,14 +915,32 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 					                             : F::ac(y, lap, p);
 			}
 		}
+	};
+	if constexpr (KIND == 4) {
+		// the multi-phase body is large: four copies of it overflow the
+		// instruction cache, so one row per iteration and the slots advance
+		// by copies
+		for (int x = xb; x < xe; x++) {
+			row(x, wa, wb, wc, wd, hb, hd);
 #pragma unroll
-		for (int j = 0; j < Q; j++) {
-			w0[j] = w1[j];
-			w1[j] = w2[j];
-			w2[j] = w3[j];
+			for (int j = 0; j < Q; j++) {
+				wa[j] = wb[j];
+				wb[j] = wc[j];
+				wc[j] = wd[j];
+			}
+			hb = hc;
+			hc = hd;
 		}
-		h1 = h2;
-		h2 = h3;
+	} else {
+		for (int x = xb; x < xe; x += 4) {
+			row(x, wa, wb, wc, wd, hb, hd);
+			if (x + 1 >= xe) break;
+			row(x + 1, wb, wc, wd, wa, hc, ha);
+			if (x + 2 >= xe) break;
+			row(x + 2, wc, wd, wa, wb, hd, hb);
+			if (x + 3 >= xe) break;
+			row(x + 3, wd, wa, wb, wc, ha, hc);
+		}
 	}
 }
 
