@@ -197,6 +197,8 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	const int wave_chunk = nodes < ((int64_t)1 << 21) ? 4 : nodes < ((int64_t)1 << 23) ? 8 : 16;
 	if (model == MMSPX_AC) return all_periodic(g) ? TuneChoice{2, 16, kTbCols, 8, 3} : TuneChoice{2, 1, kWaveCols, wave_chunk, 2};
 	if (model == MMSPX_AC_NOISE) return TuneChoice{2, 1, kWaveCols, wave_chunk, 2};
+	// Zener pinning (2048^2 x 16 FP32): 32-row chunks, 119 us per step against
+	// 124 with 16 rows and 130 with 24
 	if (model == MMSPX_ZENER) return TuneChoice{2, 1, kWaveCols, 32, 2};
 	// multi-phase grain growth (one-hot grains, 8 fields): 2-row chunks below
 	// 2^21 nodes (1024^2 FP64 36.7 us per step against 39.1 with 4 rows, 40.0
@@ -208,9 +210,10 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// chunks 2..16 at every size (the generic two passes: 28 us at 1024^2)
 	if (model == MMSPX_OSTWALD) return TuneChoice{2, 1, kChWaveOut, 4, 0};
 	// grain growth with 32 fields (2048^2 FP32): 64-column wave strips with
-	// 32-row chunks take 260-264 us per step against 272 us for 62-column
-	// strips (halo lanes, misaligned rows), 331-338 us for the 8 x 64 tile,
-	// 388 us for one thread per node and 530 us for the LDS-row strips. 32
+	// 32-row chunks take 249 us per step (255 with 64-row chunks, 259 with
+	// 16; before the strips' rotating march, 260-264 us against 272 us for
+	// 62-column strips with halo lanes, 331-338 us for the 8 x 64 tile, 388
+	// us for one thread per node and 530 us for the LDS-row strips). 32
 	// FP64 fields do not fit the strips' registers: the 8 x 64 tile. Other
 	// field counts keep the generic kernel.
 	if (model == MMSPX_GG && g.fields == 32)
