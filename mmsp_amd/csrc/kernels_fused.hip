@@ -187,15 +187,21 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 {
 	if (!fused_supported(g, prec, model)) return TuneChoice{1, 0, 0, 0, 0};
 	const int64_t nodes = g.nodes;
-	// Allen-Cahn: periodic grids take temporal blocks of 16 x 48 nodes and 8
+	// Allen-Cahn: periodic grids take temporal blocks of 48 columns and 8
 	// steps per pass (512^2: 2.05 us per step in FP64 against 4.58 us for one
-	// thread per node, FP32 1.71 against 3.37).
+	// thread per node). Block heights from a sweep of 8, 16, 32 rows x 4, 8
+	// steps: 8 rows up to 2^19 nodes, where more blocks fill more CUs (512^2
+	// FP64 1.71 us against 2.00 with 16 rows, FP32 1.35 against 1.64); 32
+	// rows in FP64 and 16 in FP32 below 2^21 (1024^2 3.33 and 2.26 us); 16
+	// above (2048^2 10.8 and 5.6 us).
 	// Other boundaries and model A: the wave strips, 4-row chunks below 2^21
 	// nodes, 8 below 2^23, 16 above (Neumann FP64: 1024^2 5.5 us per step
 	// against 8.6 for one thread per node, 2048^2 12.6 against 16.5, 4096^2
 	// 43.3 against 61.1; model A FP64 1024^2 7.7 us, 4096^2 60.8)
 	const int wave_chunk = nodes < ((int64_t)1 << 21) ? 4 : nodes < ((int64_t)1 << 23) ? 8 : 16;
-	if (model == MMSPX_AC) return all_periodic(g) ? TuneChoice{2, 16, kTbCols, 8, 3} : TuneChoice{2, 1, kWaveCols, wave_chunk, 2};
+	const int tb_rows = nodes <= ((int64_t)1 << 19) ? 8 : nodes < ((int64_t)1 << 21) && prec == MMSPX_F64 ? 32 : 16;
+	if (model == MMSPX_AC)
+		return all_periodic(g) ? TuneChoice{2, tb_rows, kTbCols, 8, 3} : TuneChoice{2, 1, kWaveCols, wave_chunk, 2};
 	if (model == MMSPX_AC_NOISE) return TuneChoice{2, 1, kWaveCols, wave_chunk, 2};
 	// Zener pinning (2048^2 x 16 FP32): 32-row chunks, 119 us per step against
 	// 124 with 16 rows and 130 with 24
@@ -248,7 +254,7 @@ bool tile_known(const Geom& g, const TuneChoice& t)
 		       (t.ty == 8 && t.tz == 32) || (t.ty == 4 && t.tz == 128);
 	if (g.dim == 2)
 		return (t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256 || t.tz == kChWaveOut)) ||
-		       ((t.ty == 16 || t.ty == 32) && t.tz == kTbCols && t.chunk <= kTbMaxSteps) ||
+		       ((t.ty == 8 || t.ty == 16 || t.ty == 32) && t.tz == kTbCols && t.chunk <= kTbMaxSteps) ||
 		       (t.opt == 1 && (t.ty == 8 || t.ty == 16) && t.tz == 64);
 	return false;
 }
@@ -302,7 +308,7 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 			for (int c : {8, 12, 16, 24, 32}) add(1, kWaveCols, c, 2);
 		// periodic Allen-Cahn: temporal blocks (chunk = steps per pass)
 		if (model == MMSPX_AC && all_periodic(g))
-			for (int tx : {16, 32})
+			for (int tx : {8, 16, 32})
 				for (int k : {4, 8}) add(tx, kTbCols, k, 3);
 	}
 	return n;
@@ -571,6 +577,7 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		if (t.tz == kTbCols) {
 			if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "temporal blocks need periodic boundaries");
 			if (nsteps < 1 || nsteps > kTbMaxSteps) return fail(MMSPX_EINVAL, "temporal blocks take 1..8 steps per pass");
+			if (t.ty == 8) return launch_tb<PREC, 8>(g, p, a, b, st, nsteps);
 			return t.ty == 32 ? launch_tb<PREC, 32>(g, p, a, b, st, nsteps) : launch_tb<PREC, 16>(g, p, a, b, st, nsteps);
 		}
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);

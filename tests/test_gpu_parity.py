@@ -297,7 +297,7 @@ def test_wave_strips(prec):
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
 def test_ac_temporal_blocks(prec):
-    """Periodic Allen-Cahn temporal blocks (16 or 32 rows x 48 columns, up to
+    """Periodic Allen-Cahn temporal blocks (8, 16 or 32 rows x 48 columns, up to
     8 steps per pass): partial tiles, a grid smaller than one tile, step counts
     that leave a short last pass, and graph-replayed runs; bitwise against the
     oracle's single steps."""
@@ -309,7 +309,7 @@ def test_ac_temporal_blocks(prec):
             a = a.astype(np.float32)
         steps = 13 if shape[0] < 512 else 107
         exp = O.ac(a, p, steps, form=prec)
-        for tx in (16, 32):
+        for tx in (8, 16, 32):
             for k in (4, 8):
                 with mx.Grid(shape, precision=prec) as g:
                     g.set_variant(2)
