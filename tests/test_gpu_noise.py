@@ -61,7 +61,7 @@ def test_model_a_wave_strips(prec):
         say("[gpu] model A wave strips %s %s" % (shape, prec))
         a = seeded(shape, 37, -0.2, 0.2, prec)
         exp = O.ac_noise(a, p, 11, form=prec, bc=bc)
-        for chunk in (8, 24):
+        for chunk in (7, 8, 24):
             got = run(shape, prec, "ac_noise", p, 11, a, bc=bc, variant=2, tile=(1, 64, chunk), body=2)
             assert np.array_equal(got, exp), (shape, chunk)
         # the default for 2-D model A is the wave strips; steps split over calls continue the noise

@@ -259,7 +259,9 @@ def test_wave_strips(prec):
             a = a.astype(np.float32)
         p = [1.0, 1.0, 1.0, 1.0, 0.01]
         exp = O.ac(a, p, 11, form=prec, bc=bc)
-        for chunk in (8, 24):
+        # chunks of 7, 8 and 24 rows (and the short last ones) end the
+        # four-row unrolled march at each of its four rows
+        for chunk in (7, 8, 24):
             with mx.Grid(shape, precision=prec, bc=bc) as g:
                 g.set_variant(2)
                 g.set_tile(1, 64, chunk)
@@ -277,7 +279,7 @@ def test_wave_strips(prec):
         if prec != "f64":
             a = a.astype(np.float32)
         exp = O.gg(a, 0.01, 9, form=prec, bc=bc)
-        for chunk in (8, 16):
+        for chunk in (7, 8, 16):
             with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
                 g.set_variant(2)
                 g.set_tile(1, 64, chunk)
@@ -414,7 +416,7 @@ def test_zener_and_ostwald(prec):
         if prec == "f32":
             a = a.astype(np.float32)
         exp = O.zener(a, 0.01, 9, form=prec, bc=bc)
-        for variant, tile in [(1, None), (2, (1, 64, 8)), (2, (1, 64, 24)), (0, None)]:
+        for variant, tile in [(1, None), (2, (1, 64, 7)), (2, (1, 64, 8)), (2, (1, 64, 24)), (0, None)]:
             with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
                 g.set_variant(variant)
                 if tile:
