@@ -131,6 +131,39 @@ def cpu_baseline(w, init):
     }
 
 
+def cpu_multicore(w, init):
+    """Time the oracle's C restatement of the same update on all host cores
+    (OpenMP node loops; Allen-Cahn and Cahn-Hilliard forms only) on the
+    bounded sample cpu_baseline uses, or None for the vector models."""
+    import oracle as O
+
+    if w["model"] not in ("ac", "ch_dw", "ch_noise"):
+        return None
+    shape, steps = w["shape"], max(4, w["cpu_steps"])
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    before = O.set_threads(0)
+    try:
+        used = O.set_threads(threads)
+        a = init.reshape(shape)
+        form = "f64" if w["prec"] == "f64" else "f32"
+        if form == "f32":
+            a = a.astype(np.float32)
+        t0 = time.perf_counter()
+        if w["model"] == "ch_dw":
+            O.ch(a, w["params"], steps, mu=O.MU_DW, form=form)
+        elif w["model"] == "ac":
+            O.ac(a, w["params"], steps, form=form)
+        else:
+            O.ch_noise(a, w["params"], steps, form=form)
+        sec = time.perf_counter() - t0
+    finally:
+        O.set_threads(before)
+    nodes = int(np.prod(shape))
+    return {"value": nodes * steps / sec, "unit": "cell-updates/s", "cores": used, "kind": "port",
+            "sample": "oracle/liboracle C restatement (%s form), %s, %d steps of the full %s grid, %d OpenMP threads"
+            % (form, w["name"], steps, "x".join(map(str, shape)), used)}
+
+
 def grains(shape, q, seed, block=16):
     """One-hot grains: each block of `block` nodes per axis belongs to a grain
     drawn from the seed (phi = 1 for it, 0 for the others)."""
@@ -637,6 +670,12 @@ def main():
             out["cpu_baseline"] = cpu_baseline(w, init)
         except Exception as e:  # the baseline is reported, never the measured value
             out["cpu_baseline"] = {"value": None, "error": str(e)}
+        try:
+            mc = cpu_multicore(w, init)
+            if mc:
+                out["config"]["cpu_multicore"] = mc
+        except Exception as e:
+            out["config"]["cpu_multicore"] = {"value": None, "error": str(e)}
     print(json.dumps(out), flush=True)
 
 

@@ -68,6 +68,8 @@ def lib():
         L.orc_ac_steps_f32ref.argtypes = [G, fp, fp, dp, C.c_int, C.c_int]
         L.orc_ac_steps_f32.argtypes = [G, fp, fp, dp, C.c_int]
         L.orc_ch_steps_f64.argtypes = [G, dp, dp, dp, dp, C.c_int, C.c_int, C.c_int]
+        L.orc_set_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
         u32p = P(C.c_uint32)
         L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
         L.orc_ln_unit.restype = C.c_float
@@ -188,6 +190,14 @@ def ch(a, params, steps, mu=MU_DW, form="f64", cube=CUBE_CR, bc=None, dx=None):
 
 
 # ---- stochastic forms: p = {r, u, K, M, dt, sigma, seed} ----------------------
+def set_threads(n: int) -> int:
+    """Threads for the node loops of the scalar steppers (Allen-Cahn,
+    Cahn-Hilliard and their stochastic forms) on grids of 2^16 nodes or more;
+    n <= 0 keeps the current count. Returns the count in use. The results do
+    not depend on it."""
+    return int(lib().orc_set_threads(int(n)))
+
+
 def philox4x32_10(counter, key):
     c = np.ascontiguousarray(np.asarray(counter, dtype=np.uint32))
     k = np.ascontiguousarray(np.asarray(key, dtype=np.uint32))

@@ -45,6 +45,9 @@
  * scalar map, fixed points, symmetry, mass invariance).
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -197,6 +200,22 @@ float orc_laplacian_f32(const orc_grid* g, const float* a, int64_t i)
 	return lap;
 }
 
+/* Node loops of the scalar steppers run on OpenMP threads for grids of 2^16
+ * nodes or more. Each node's arithmetic is unchanged, so the results do not
+ * depend on the thread count (orc_set_threads). */
+#define ORC_PAR_NODES _Pragma("omp parallel for schedule(static) if (n >= 65536)")
+
+int orc_set_threads(int t)
+{
+#ifdef _OPENMP
+	if (t > 0) omp_set_num_threads(t);
+	return omp_get_max_threads();
+#else
+	(void)t;
+	return 1;
+#endif
+}
+
 /* ---- Allen-Cahn: p = {r, u, K, M, dt} ----------------------------------- */
 /* new = phi - dt*M*(-r*phi + u*phi^3 - K*lap)   (allen-cahn.cpp:70) */
 void orc_ac_steps_f64(const orc_grid* g, double* a, double* b, const double* p, int steps, int cube)
@@ -204,6 +223,7 @@ void orc_ac_steps_f64(const orc_grid* g, double* a, double* b, const double* p, 
 	const double r = p[0], u = p[1], K = p[2], M = p[3], dt = p[4];
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			double phi = a[i];
 			b[i] = phi - dt * M * (-r * phi + u * cube64(phi, cube) - K * orc_laplacian_f64(g, a, i));
@@ -217,6 +237,7 @@ void orc_ac_steps_f32ref(const orc_grid* g, float* a, float* b, const double* p,
 	const double r = p[0], u = p[1], K = p[2], M = p[3], dt = p[4];
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			float phi = a[i];
 			float lap = orc_laplacian_f32ref(g, a, i);
@@ -231,6 +252,7 @@ void orc_ac_steps_f32(const orc_grid* g, float* a, float* b, const double* p, in
 	const float r = (float)p[0], u = (float)p[1], K = (float)p[2], M = (float)p[3], dt = (float)p[4];
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			float phi = a[i];
 			float c3 = phi * phi * phi;
@@ -249,6 +271,7 @@ void orc_ch_steps_f64(const orc_grid* g, double* a, double* b, double* mu, const
 {
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			double c = a[i];
 			double lap = orc_laplacian_f64(g, a, i);
@@ -259,6 +282,7 @@ void orc_ch_steps_f64(const orc_grid* g, double* a, double* b, double* mu, const
 		}
 		const double M = form == ORC_MU_POLY ? p[3] : p[1];
 		const double dt = form == ORC_MU_POLY ? p[4] : p[2];
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++)
 			b[i] = a[i] + dt * M * orc_laplacian_f64(g, mu, i);
 		memcpy(a, b, (size_t)n * sizeof(double));
@@ -270,6 +294,7 @@ void orc_ch_steps_f32ref(const orc_grid* g, float* a, float* b, float* mu, const
 {
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			double c = a[i];
 			double lap = orc_laplacian_f32ref(g, a, i);
@@ -280,6 +305,7 @@ void orc_ch_steps_f32ref(const orc_grid* g, float* a, float* b, float* mu, const
 		}
 		const double M = form == ORC_MU_POLY ? p[3] : p[1];
 		const double dt = form == ORC_MU_POLY ? p[4] : p[2];
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++)
 			b[i] = (float)((double)a[i] + dt * M * (double)orc_laplacian_f32ref(g, mu, i));
 		memcpy(a, b, (size_t)n * sizeof(float));
@@ -290,6 +316,7 @@ void orc_ch_steps_f32(const orc_grid* g, float* a, float* b, float* mu, const do
 {
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			float c = a[i];
 			float lap = orc_laplacian_f32(g, a, i);
@@ -300,6 +327,7 @@ void orc_ch_steps_f32(const orc_grid* g, float* a, float* b, float* mu, const do
 		}
 		const float M = (float)(form == ORC_MU_POLY ? p[3] : p[1]);
 		const float dt = (float)(form == ORC_MU_POLY ? p[4] : p[2]);
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++)
 			b[i] = a[i] + dt * M * orc_laplacian_f32(g, mu, i);
 		memcpy(a, b, (size_t)n * sizeof(float));
@@ -412,6 +440,7 @@ void orc_ac_noise_steps_f64(const orc_grid* g, double* a, double* b, const doubl
 	const uint64_t seed = (uint64_t)p[6];
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			double phi = a[i];
 			double noise = 0.0 + (double)node_normal(g, seed, step0 + (uint64_t)s, i) * sigma;
@@ -429,6 +458,7 @@ void orc_ac_noise_steps_f32ref(const orc_grid* g, float* a, float* b, const doub
 	const uint64_t seed = (uint64_t)p[6];
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			float phi = a[i];
 			float lap = orc_laplacian_f32ref(g, a, i);
@@ -447,6 +477,7 @@ void orc_ac_noise_steps_f32(const orc_grid* g, float* a, float* b, const double*
 	const uint64_t seed = (uint64_t)p[6];
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			float phi = a[i];
 			float c3 = phi * phi * phi;
@@ -467,11 +498,13 @@ void orc_ch_noise_steps_f64(const orc_grid* g, double* a, double* b, double* mu,
 	const uint64_t seed = (uint64_t)p[6];
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			double c = a[i];
 			double noise = 0.0 + (double)node_normal(g, seed, step0 + (uint64_t)s, i) * sigma;
 			mu[i] = -p[0] * c + p[1] * cube64(c, cube) - p[2] * orc_laplacian_f64(g, a, i) + noise;
 		}
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) b[i] = a[i] + p[4] * p[3] * orc_laplacian_f64(g, mu, i);
 		memcpy(a, b, (size_t)n * sizeof(double));
 	}
@@ -484,12 +517,14 @@ void orc_ch_noise_steps_f32ref(const orc_grid* g, float* a, float* b, float* mu,
 	const uint64_t seed = (uint64_t)p[6];
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			double c = a[i];
 			double lap = orc_laplacian_f32ref(g, a, i);
 			float noise = (float)(0.0 + (double)node_normal(g, seed, step0 + (uint64_t)s, i) * sigma);
 			mu[i] = (float)(-p[0] * c + p[1] * cube64(c, cube) - p[2] * lap + (double)noise);
 		}
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++)
 			b[i] = (float)((double)a[i] + p[4] * p[3] * (double)orc_laplacian_f32ref(g, mu, i));
 		memcpy(a, b, (size_t)n * sizeof(float));
@@ -503,6 +538,7 @@ void orc_ch_noise_steps_f32(const orc_grid* g, float* a, float* b, float* mu, co
 	const uint64_t seed = (uint64_t)p[6];
 	const int64_t n = nodes_of(g);
 	for (int s = 0; s < steps; s++) {
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) {
 			float c = a[i];
 			float lap = orc_laplacian_f32(g, a, i);
@@ -510,6 +546,7 @@ void orc_ch_noise_steps_f32(const orc_grid* g, float* a, float* b, float* mu, co
 			mu[i] = -(float)p[0] * c + (float)p[1] * (c * c * c) - (float)p[2] * lap + noise;
 		}
 		const float M = (float)p[3], dt = (float)p[4];
+		ORC_PAR_NODES
 		for (int64_t i = 0; i < n; i++) b[i] = a[i] + dt * M * orc_laplacian_f32(g, mu, i);
 		memcpy(a, b, (size_t)n * sizeof(float));
 	}

@@ -424,3 +424,26 @@ def test_multiphase_grain_growth_stays_on_the_simplex(form):
     later = O.gg_aniso(a, p, 15, form=form)
     perm = [3, 0, 4, 1, 2]
     assert np.allclose(O.gg_aniso(a[..., perm], p, 15, form=form), later[..., perm], atol=50 * tol)
+
+
+def test_thread_count_does_not_change_the_bits():
+    """The scalar steppers' node loops run on OpenMP threads from 2^16 nodes;
+    one thread and several give the same bits in every form."""
+    a3 = O.uniform(48 * 40 * 40, 91, 0.45, 0.55).reshape(48, 40, 40)
+    a2 = O.uniform(260 * 256, 92, -0.3, 0.3).reshape(260, 256)
+    runs = [lambda f: O.ch(a3, [1.0, 1.0, 0.005], 3, mu=O.MU_DW, form=f),
+            lambda f: O.ch(a2, [1.0, 1.0, 1.0, 1.0, 0.005], 3, mu=O.MU_POLY, form=f),
+            lambda f: O.ac(a2, [1.0, 1.0, 1.0, 1.0, 0.01], 3, form=f),
+            lambda f: O.ac_noise(a2, [1.0, 1.0, 1.0, 1.0, 0.01, 0.7, 5], 3, form=f),
+            lambda f: O.ch_noise(a3, [1.0, 1.0, 1.0, 1.0, 0.005, 0.3, 6], 3, form=f)]
+    before = O.set_threads(0)
+    try:
+        for form in ("f64", "f32", "f32ref"):
+            O.set_threads(1)
+            one = [r(form) for r in runs]
+            O.set_threads(4)
+            four = [r(form) for r in runs]
+            for k, (x, y) in enumerate(zip(one, four)):
+                assert np.array_equal(x, y), (form, k)
+    finally:
+        O.set_threads(before)
