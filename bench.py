@@ -612,6 +612,8 @@ def main():
         kname = "ow_waves"
     elif tile["opt"] == 2:
         kname = "wave_strips"
+    elif tile["opt"] == 4:
+        kname = "wave_strips_split"
     elif tile["opt"] == 3 and base == "ac":
         kname = "ac_tblocks"
     elif base == "gg" and tile["opt"] == 1:

@@ -102,6 +102,21 @@ __global__ void __launch_bounds__(256) wave_strips(const typename Form<PREC>::S*
 	strip_wave<PREC, Q, KIND>(a, b, g, p, s * kWaveCols, xb, min(xb + chunk_len, g.n[0]));
 }
 
+// Split wave strips (2-D grain growth, Zener pinning): one 128-thread
+// workgroup per (strip, chunk) item, its two waves holding half the fields
+// each. Items run strip-fastest, as the wave strips.
+template <int PREC, int Q, int KIND>
+__global__ void __launch_bounds__(128) wave_strips_split(const typename Form<PREC>::S* __restrict__ a,
+                                                         typename Form<PREC>::S* __restrict__ b, Geom g,
+                                                         ModelParams p, int chunk_len, int strips, int nitems)
+{
+	const int item = xcd_tile(nitems);
+	if (item < 0) return;  // the whole workgroup
+	const int s = item % strips, chunk = item / strips;
+	const int xb = chunk * chunk_len;
+	strip_wave_split<PREC, Q, KIND>(a, b, g, p, s * kWaveCols, xb, min(xb + chunk_len, g.n[0]));
+}
+
 // Temporal blocks (2-D periodic Allen-Cahn): one workgroup per TX x kTbCols
 // tile, `steps` (<= kTbMaxSteps) steps per launch.
 template <int PREC, int TX>
@@ -203,9 +218,6 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	if (model == MMSPX_AC)
 		return all_periodic(g) ? TuneChoice{2, tb_rows, kTbCols, 8, 3} : TuneChoice{2, 1, kWaveCols, wave_chunk, 2};
 	if (model == MMSPX_AC_NOISE) return TuneChoice{2, 1, kWaveCols, wave_chunk, 2};
-	// Zener pinning (2048^2 x 16 FP32): 32-row chunks, 119 us per step against
-	// 124 with 16 rows and 130 with 24
-	if (model == MMSPX_ZENER) return TuneChoice{2, 1, kWaveCols, 32, 2};
 	// multi-phase grain growth (one-hot grains, 8 fields): 2-row chunks below
 	// 2^21 nodes (1024^2 FP64 36.7 us per step against 39.1 with 4 rows, 40.0
 	// with 8 and 77.3 for one thread per node), 8 rows above (2048^2 FP64 139
@@ -215,15 +227,24 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// 236.8 us per step at 512^2 .. 4096^2, at or within 4 % of the best of
 	// chunks 2..16 at every size (the generic two passes: 28 us at 1024^2)
 	if (model == MMSPX_OSTWALD) return TuneChoice{2, 1, kChWaveOut, 4, 0};
-	// grain growth with 32 fields (2048^2 FP32): 64-column wave strips with
-	// 32-row chunks take 249 us per step (255 with 64-row chunks, 259 with
-	// 16; before the strips' rotating march, 260-264 us against 272 us for
-	// 62-column strips with halo lanes, 331-338 us for the 8 x 64 tile, 388
-	// us for one thread per node and 530 us for the LDS-row strips). 32
-	// FP64 fields do not fit the strips' registers: the 8 x 64 tile. Other
-	// field counts keep the generic kernel.
-	if (model == MMSPX_GG && g.fields == 32)
-		return prec == MMSPX_F64 ? TuneChoice{2, 8, 64, 8, 1} : TuneChoice{2, 1, kWaveCols, 32, 2};
+	// Grain growth and Zener pinning with 8, 16 or 32 fields: the split strips
+	// (two waves per strip, half the fields each). Chunks of 8 rows below 2^19
+	// nodes, 16 below 2^21, 64 above (16 for 32 FP64 fields). Measured per
+	// step: 2048^2 x 32 FP32 233.6 us against 248.2 for the wave strips
+	// (32-row chunks) and 388 for one thread per node; 2048^2 x 32 FP64 474.6
+	// against 835.9 for the 8 x 64 tile; 2048^2 x 16 Zener FP32 115.4 against
+	// 118.3; 512^2 x 8 FP32 8.2 against 10.0 (strips) and 9.4 (one thread per
+	// node); 1024^2 x 16 FP64 46.4 against 48.7 and 82.0. Other field counts
+	// keep the generic kernel.
+	if ((model == MMSPX_GG || model == MMSPX_ZENER) && (g.fields == 8 || g.fields == 16 || g.fields == 32)) {
+		const int c = nodes < ((int64_t)1 << 19)                    ? 8
+		              : nodes < ((int64_t)1 << 21)                  ? 16
+		              : (prec == MMSPX_F64 && g.fields == 32) ? 16
+		                                                            : 64;
+		return TuneChoice{2, 1, kWaveCols, c, 4};
+	}
+	// Zener pinning with 2 or 4 fields: the wave strips, 32-row chunks
+	if (model == MMSPX_ZENER) return TuneChoice{2, 1, kWaveCols, 32, 2};
 	if (model == MMSPX_GG) return TuneChoice{1, 0, 0, 0, 0};
 	// 2-D Cahn-Hilliard: wave strips (1024^2 FP64 7.3 us per step against 13.5
 	// us for the LDS row strips; 4096^2 FP64 53 us against 69 us, FP32 34 us
@@ -274,9 +295,12 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		for (int c : {2, 4, 8, 16, 32}) add(1, kWaveCols, c, 2);
 		return n;
 	}
+	const bool split = (model == MMSPX_GG || model == MMSPX_ZENER) && (g.fields == 8 || g.fields == 16 || g.fields == 32);
 	if (g.dim == 2 && (model == MMSPX_AC_NOISE || model == MMSPX_ZENER)) {
-		// model A and Zener pinning: wave strips only
+		// model A and Zener pinning: wave strips, and the split strips
 		for (int c : {8, 12, 16, 24, 32}) add(1, kWaveCols, c, 2);
+		if (split)
+			for (int c : {8, 16, 32}) add(1, kWaveCols, c, 4);
 		return n;
 	}
 	if (g.dim == 3) {
@@ -306,6 +330,8 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		const bool wave = model == MMSPX_AC || (model == MMSPX_GG && (g.fields < 32 || prec != MMSPX_F64));
 		if (wave)
 			for (int c : {8, 12, 16, 24, 32}) add(1, kWaveCols, c, 2);
+		if (split)
+			for (int c : {8, 16, 32}) add(1, kWaveCols, c, 4);
 		// periodic Allen-Cahn: temporal blocks (chunk = steps per pass)
 		if (model == MMSPX_AC && all_periodic(g))
 			for (int tx : {8, 16, 32})
@@ -517,6 +543,41 @@ static int launch_wave(const Geom& g, const ModelParams& p, const void* a, void*
 	return fail(MMSPX_EUNSUPPORTED, "no wave-strip kernel for this field count and precision");
 }
 
+template <int PREC, int Q, int KIND>
+static int launch_split_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	using S = typename Form<PREC>::S;
+	const int strips = (g.n[1] + kWaveCols - 1) / kWaveCols;
+	const int nitems = strips * ((g.n[0] + chunk - 1) / chunk);
+	const int blocks = (nitems + 7) / 8 * 8;
+	hipLaunchKernelGGL((wave_strips_split<PREC, Q, KIND>), dim3(blocks), dim3(128), 0, st, (const S*)a, (S*)b, g, p,
+	                   chunk, strips, nitems);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+// The split strips: grain growth and Zener pinning with 8, 16 or 32 fields
+template <int PREC>
+static int launch_split(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	if (p.model == MMSPX_ZENER) {
+		if constexpr (PREC != MMSPX_F32REF) {
+			switch (g.fields) {
+			case 8: return launch_split_q<PREC, 8, 3>(g, p, a, b, st, chunk);
+			case 16: return launch_split_q<PREC, 16, 3>(g, p, a, b, st, chunk);
+			case 32: return launch_split_q<PREC, 32, 3>(g, p, a, b, st, chunk);
+			}
+		}
+	} else {
+		switch (g.fields) {
+		case 8: return launch_split_q<PREC, 8, 1>(g, p, a, b, st, chunk);
+		case 16: return launch_split_q<PREC, 16, 1>(g, p, a, b, st, chunk);
+		case 32: return launch_split_q<PREC, 32, 1>(g, p, a, b, st, chunk);
+		}
+	}
+	return fail(MMSPX_EUNSUPPORTED, "split wave strips take 8, 16 or 32 fields");
+}
+
 template <int PREC, int Q>
 static int launch_ow_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
@@ -564,6 +625,7 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		}
 		return fail(MMSPX_EUNSUPPORTED, "Ostwald ripening runs the wave strips (1 x 60) or the generic kernels");
 	case MMSPX_ZENER:
+		if (t.opt == 4 && t.ty == 1 && t.tz == kWaveCols) return launch_split<PREC>(g, p, a, b, st, chunk);
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		return fail(MMSPX_EUNSUPPORTED, "Zener pinning runs the wave strips (1 x 64, body 2) or the generic kernel");
 	case MMSPX_GG_ANISO:
@@ -587,6 +649,7 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 	case MMSPX_GG:
 		// ty > 1: 2-D tiles of ty (8 or 16) rows x 64 columns; body 2 with 64
 		// columns: wave strips; otherwise LDS-row marching strips
+		if (t.opt == 4 && t.ty == 1 && t.tz == kWaveCols) return launch_split<PREC>(g, p, a, b, st, chunk);
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		if (t.ty > 1) return t.ty == 16 ? launch_gg_tile<PREC, 16>(g, p, a, b, st) : launch_gg_tile<PREC, 8>(g, p, a, b, st);
 		if (t.tz == 64) return launch_gg_tz<PREC, 64>(g, p, a, b, st, chunk);

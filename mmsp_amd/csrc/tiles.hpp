@@ -944,4 +944,94 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 	}
 }
 
+// ---- 2-D grain growth, split wave strips: two waves per (strip, chunk) -------
+// The strip_wave layout (lane l on column c0 + l of a 64-column strip, row
+// window in registers, column neighbours by DPP), with the Q fields split
+// between the two waves of a 128-thread workgroup: wave h holds fields
+// h*Q/2 .. (h+1)*Q/2 - 1, half the registers, so more waves fit a SIMD. The
+// sum of phi_j^2 runs in the reference's field order across the pair: wave 0
+// sums its fields from zero and passes the partial sum through LDS, wave 1
+// continues it over its fields and passes the total back (two barriers per
+// row). KIND 1: grain growth (graingrowth.cpp:91-107); KIND 3: Zener pinning
+// (field 0 held, zener.cpp:117-135).
+template <int PREC, int Q, int KIND>
+__device__ __forceinline__ void strip_wave_split(const typename Form<PREC>::S* __restrict__ a,
+                                                 typename Form<PREC>::S* __restrict__ b, const Geom& g,
+                                                 const ModelParams& p, int c0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	using Sum = typename F::Sum;
+	constexpr int H = Q / 2;
+	static_assert(Q % 2 == 0 && Q <= 64, "the split strips hold Q/2 fields per wave");
+	__shared__ Sum part[64], total[64];
+	const int lane = threadIdx.x & 63;
+	const int half = (int)(threadIdx.x >> 6) & 1;
+	const int f0 = half * H;
+	const int nx = g.n[0], ny = g.n[1];
+	const size_t N = (size_t)nx * ny;
+	const int c = c0 + lane;
+	const bool out = c < ny;
+	const unsigned cm = (unsigned)edge_map(c, ny, g.b0[1], g.b1[1]);
+	// beside-the-strip loads: lane k < H field f0 + k left of the strip, lane
+	// H + k the same field right of it; other lanes load field f0, unused
+	const unsigned ch = (unsigned)edge_map(lane < H ? c0 - 1 : c0 + 64, ny, g.b0[1], g.b1[1]);
+	const S* __restrict__ ah = a + (size_t)(f0 + (lane < 2 * H ? lane % H : 0)) * N;
+	const S* __restrict__ af = a + (size_t)f0 * N;
+	S* __restrict__ bf = b + (size_t)f0 * N;
+	auto rrow = [&](int x) { return (unsigned)edge_map(x, nx, g.b0[0], g.b1[0]) * (unsigned)ny; };
+	S wa[H], wb[H], wc[H], wd[H];
+	S ha = 0, hb = 0, hc = 0, hd = 0;
+	auto load_row = [&](int x, S* w, S& h) {
+		const unsigned r = rrow(min(x, xe));
+#pragma unroll
+		for (int j = 0; j < H; j++) w[j] = (af + j * N)[r + cm];
+		h = ah[r + ch];
+	};
+#pragma unroll
+	for (int j = 0; j < H; j++) wd[j] = 0;
+	load_row(xb - 1, wa, ha);
+	load_row(xb, wb, hb);
+	load_row(xb + 1, wc, hc);
+	auto row = [&](int x, const S* w0, const S* w1, const S* w2, S* w3, const S h1, S& h3) {
+		load_row(x + 2, w3, h3);
+		// sum_j phi_j^2 of row x in field order: wave 0 from zero over its
+		// fields, wave 1 on from there; both then hold the total
+		Sum acc = 0;
+		if (half == 0) {
+#pragma unroll
+			for (int j = 0; j < H; j++) acc += w1[j] * w1[j];
+			part[lane] = acc;
+		}
+		__syncthreads();
+		if (half == 1) {
+			acc = part[lane];
+#pragma unroll
+			for (int j = 0; j < H; j++) acc += w1[j] * w1[j];
+			total[lane] = acc;
+		}
+		__syncthreads();
+		if (half == 0) acc = total[lane];
+		const unsigned o = (unsigned)x * (unsigned)ny + (unsigned)c;
+#pragma unroll
+		for (int j = 0; j < H; j++) {
+			const S y = w1[j];
+			const S yl = from_prev(y, lane_value(h1, j)), yh = from_next(y, lane_value(h1, H + j));
+			typename F::L lap = F::lap_zero();
+			F::lap_add(lap, g.w[0], g.wf[0], w2[j], y, w0[j]);
+			F::lap_add(lap, g.w[1], g.wf[1], yh, y, yl);
+			if (out) (bf + j * N)[o] = (KIND == 3 && f0 + j == 0) ? y : F::gg(y, acc, lap, p);
+		}
+	};
+	for (int x = xb; x < xe; x += 4) {
+		row(x, wa, wb, wc, wd, hb, hd);
+		if (x + 1 >= xe) break;
+		row(x + 1, wb, wc, wd, wa, hc, ha);
+		if (x + 2 >= xe) break;
+		row(x + 2, wc, wd, wa, wb, hd, hb);
+		if (x + 3 >= xe) break;
+		row(x + 3, wd, wa, wb, wc, ha, hc);
+	}
+}
+
 }  // namespace mmspx

@@ -279,15 +279,26 @@ def test_wave_strips(prec):
         if prec != "f64":
             a = a.astype(np.float32)
         exp = O.gg(a, 0.01, 9, form=prec, bc=bc)
-        for chunk in (7, 8, 16):
-            with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
-                g.set_variant(2)
-                g.set_tile(1, 64, chunk)
-                g.set_tile_options(body=2)  # wave strips
-                g.upload(a)
-                g.step("gg", [0.01], 9)
-                assert np.array_equal(g.download(), exp), (shape, q, chunk)
-    if prec == "f64":  # 32 FP64 fields do not fit the strip's registers: refused, not run
+        # body 2: wave strips; body 4: split strips (8, 16, 32 fields)
+        for body in ((2, 4) if q in (8, 16, 32) else (2,)):
+            for chunk in (7, 8, 16):
+                with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+                    g.set_variant(2)
+                    g.set_tile(1, 64, chunk)
+                    g.set_tile_options(body=body)
+                    g.upload(a)
+                    g.step("gg", [0.01], 9)
+                    assert np.array_equal(g.download(), exp), (shape, q, chunk, body)
+    if prec == "f64":  # 32 FP64 fields: the split strips hold them, the wave strips are refused
+        q, shape = 32, (23, 70)
+        a = O.uniform(int(np.prod(shape)) * q, 532, 0.0, 1.0).reshape(shape + (q,))
+        with mx.Grid(shape, fields=q, vector=True, precision="f64") as g:
+            g.set_variant(2)
+            g.set_tile(1, 64, 5)
+            g.set_tile_options(body=4)
+            g.upload(a)
+            g.step("gg", [0.01], 6)
+            assert np.array_equal(g.download(), O.gg(a, 0.01, 6, form="f64"))
         with mx.Grid((16, 70), fields=32, vector=True, precision="f64") as g:
             g.set_variant(2)
             g.set_tile(1, 64, 8)
@@ -403,8 +414,9 @@ def test_c2_cahn_hilliard_full_size_mass():
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_zener_and_ostwald(prec):
-    """Zener pinning on the generic kernel and the wave strips (every compiled
-    field count, partial strips, several boundaries), Ostwald ripening on the
+    """Zener pinning on the generic kernel, the wave strips and the split
+    strips (every compiled field count, partial strips, several boundaries),
+    Ostwald ripening on the
     generic two-pass kernels in 1-D, 2-D and 3-D: bitwise against the oracle."""
     zcases = [((70, 300), 4, [(N, M), (D, P)]), ((40, 130), 2, None), ((9, 70), 8, [(M, M), (N, D)]),
               ((33, 64), 16, [(P, P), (M, N)])]
@@ -416,15 +428,18 @@ def test_zener_and_ostwald(prec):
         if prec == "f32":
             a = a.astype(np.float32)
         exp = O.zener(a, 0.01, 9, form=prec, bc=bc)
-        for variant, tile in [(1, None), (2, (1, 64, 7)), (2, (1, 64, 8)), (2, (1, 64, 24)), (0, None)]:
+        runs = [(1, None, 0), (2, (1, 64, 7), 2), (2, (1, 64, 8), 2), (2, (1, 64, 24), 2), (0, None, 0)]
+        if q in (8, 16, 32):
+            runs += [(2, (1, 64, 6), 4), (2, (1, 64, 9), 4)]  # the split strips
+        for variant, tile, body in runs:
             with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
                 g.set_variant(variant)
                 if tile:
                     g.set_tile(*tile)
-                    g.set_tile_options(body=2)
+                    g.set_tile_options(body=body)
                 g.upload(a)
                 g.step("zener", [0.01], 9)
-                assert np.array_equal(g.download(), exp), (shape, q, variant, tile)
+                assert np.array_equal(g.download(), exp), (shape, q, variant, tile, body)
     p = [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]
     for shape, q, bc in [((90,), 2, [(N, D)]), ((40, 36), 3, None), ((10, 9, 12), 4, [(M, N), (P, P), (D, P)])]:
         say("[gpu] ostwald %s q=%d %s" % (shape, q, prec))
