@@ -221,8 +221,8 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
  * threads per tile (fewer points, fewer registers each). 2-D Allen-Cahn and grain growth with
  * 1 x 64 strips: 2 selects the wave strips (one 64-column strip per wave,
  * neighbours across lanes by DPP, no LDS); 2-D grain growth and Zener pinning
- * with 8, 16 or 32 fields: 4 selects the split strips (two waves per strip,
- * half the fields each). */
+ * with 8, 16 or 32 fields: 4 and 5 select the split strips (two or four
+ * waves per strip, each holding its share of the fields). */
 int mmspx_set_tile_options(mmspx_state* s, int body);
 /* Returns variant + 16 * tile body (1: register queues, or the 2-D
  * grain-growth tile; 2: wave strips, or 3-D register queues on 768 threads;

@@ -434,7 +434,7 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 	// tz == kTbCols, Allen-Cahn temporal blocks (body 3); otherwise the body
 	// chosen by mmspx_set_tile_options (register queues, wave strips) is kept
 	int opt = (s->tuned && (s->tile.opt == 1 || s->tile.opt == 2 || (s->tile.opt == 3 && s->geom.dim == 3) ||
-	                        (s->tile.opt == 4 && s->geom.dim == 2)))
+	                        (s->tile.opt >= 4 && s->geom.dim == 2)))
 	              ? s->tile.opt
 	              : 0;
 	if (s->geom.dim == 2 && ty > 1 && tz == kTbCols) opt = 3;
@@ -450,7 +450,7 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 int mmspx_set_tile_options(mmspx_state* s, int body)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
-	if (body < 0 || body > 4 || (body == 3 && s->geom.dim != 3) || (body == 4 && s->geom.dim != 2))
+	if (body < 0 || body > 5 || (body == 3 && s->geom.dim != 3) || (body >= 4 && s->geom.dim != 2))
 		return fail(MMSPX_EINVAL, "unknown tile body");
 	if (!s->tuned) {
 		s->tile = default_choice(s->geom, s->desc.precision, MMSPX_CH_DW);

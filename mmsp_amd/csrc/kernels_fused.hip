@@ -102,19 +102,19 @@ __global__ void __launch_bounds__(256) wave_strips(const typename Form<PREC>::S*
 	strip_wave<PREC, Q, KIND>(a, b, g, p, s * kWaveCols, xb, min(xb + chunk_len, g.n[0]));
 }
 
-// Split wave strips (2-D grain growth, Zener pinning): one 128-thread
-// workgroup per (strip, chunk) item, its two waves holding half the fields
-// each. Items run strip-fastest, as the wave strips.
-template <int PREC, int Q, int KIND>
-__global__ void __launch_bounds__(128) wave_strips_split(const typename Form<PREC>::S* __restrict__ a,
-                                                         typename Form<PREC>::S* __restrict__ b, Geom g,
-                                                         ModelParams p, int chunk_len, int strips, int nitems)
+// Split wave strips (2-D grain growth, Zener pinning): one workgroup of W
+// waves per (strip, chunk) item, each wave holding Q/W of the fields. Items
+// run strip-fastest, as the wave strips.
+template <int PREC, int Q, int KIND, int W>
+__global__ void __launch_bounds__(64 * W) wave_strips_split(const typename Form<PREC>::S* __restrict__ a,
+                                                            typename Form<PREC>::S* __restrict__ b, Geom g,
+                                                            ModelParams p, int chunk_len, int strips, int nitems)
 {
 	const int item = xcd_tile(nitems);
 	if (item < 0) return;  // the whole workgroup
 	const int s = item % strips, chunk = item / strips;
 	const int xb = chunk * chunk_len;
-	strip_wave_split<PREC, Q, KIND>(a, b, g, p, s * kWaveCols, xb, min(xb + chunk_len, g.n[0]));
+	strip_wave_split<PREC, Q, KIND, W>(a, b, g, p, s * kWaveCols, xb, min(xb + chunk_len, g.n[0]));
 }
 
 // Temporal blocks (2-D periodic Allen-Cahn): one workgroup per TX x kTbCols
@@ -227,21 +227,18 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// 236.8 us per step at 512^2 .. 4096^2, at or within 4 % of the best of
 	// chunks 2..16 at every size (the generic two passes: 28 us at 1024^2)
 	if (model == MMSPX_OSTWALD) return TuneChoice{2, 1, kChWaveOut, 4, 0};
-	// Grain growth and Zener pinning with 8, 16 or 32 fields: the split strips
-	// (two waves per strip, half the fields each). Chunks of 8 rows below 2^19
-	// nodes, 16 below 2^21, 64 above (16 for 32 FP64 fields). Measured per
-	// step: 2048^2 x 32 FP32 233.6 us against 248.2 for the wave strips
-	// (32-row chunks) and 388 for one thread per node; 2048^2 x 32 FP64 474.6
-	// against 835.9 for the 8 x 64 tile; 2048^2 x 16 Zener FP32 115.4 against
-	// 118.3; 512^2 x 8 FP32 8.2 against 10.0 (strips) and 9.4 (one thread per
-	// node); 1024^2 x 16 FP64 46.4 against 48.7 and 82.0. Other field counts
-	// keep the generic kernel.
+	// Grain growth and Zener pinning with 8, 16 or 32 fields: the split strips.
+	// Below 2^21 nodes four waves per strip (a quarter of the fields each)
+	// with 8-row chunks in FP32 and 16 in FP64; above, two waves with 64-row
+	// chunks. Measured per step: 2048^2 x 32 FP32 233.7 us against 248.2 for
+	// the wave strips (32-row chunks) and 388 for one thread per node; 2048^2
+	// x 32 FP64 459.5 against 835.9 for the 8 x 64 tile; 2048^2 x 16 Zener
+	// FP32 115.4 against 118.3; 512^2 x 32 FP32 15.7 against 20.3 on two
+	// waves; 512^2 x 8 FP32 7.9 against 9.4 for one thread per node; 1024^2 x
+	// 16 FP64 47.1 against 82.0. Other field counts keep the generic kernel.
 	if ((model == MMSPX_GG || model == MMSPX_ZENER) && (g.fields == 8 || g.fields == 16 || g.fields == 32)) {
-		const int c = nodes < ((int64_t)1 << 19)                    ? 8
-		              : nodes < ((int64_t)1 << 21)                  ? 16
-		              : (prec == MMSPX_F64 && g.fields == 32) ? 16
-		                                                            : 64;
-		return TuneChoice{2, 1, kWaveCols, c, 4};
+		if (nodes < ((int64_t)1 << 21)) return TuneChoice{2, 1, kWaveCols, prec == MMSPX_F64 ? 16 : 8, 5};
+		return TuneChoice{2, 1, kWaveCols, 64, 4};
 	}
 	// Zener pinning with 2 or 4 fields: the wave strips, 32-row chunks
 	if (model == MMSPX_ZENER) return TuneChoice{2, 1, kWaveCols, 32, 2};
@@ -300,7 +297,8 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		// model A and Zener pinning: wave strips, and the split strips
 		for (int c : {8, 12, 16, 24, 32}) add(1, kWaveCols, c, 2);
 		if (split)
-			for (int c : {8, 16, 32}) add(1, kWaveCols, c, 4);
+			for (int c : {8, 16, 32, 64})
+				for (int body = 4; body <= 5; body++) add(1, kWaveCols, c, body);
 		return n;
 	}
 	if (g.dim == 3) {
@@ -331,7 +329,8 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		if (wave)
 			for (int c : {8, 12, 16, 24, 32}) add(1, kWaveCols, c, 2);
 		if (split)
-			for (int c : {8, 16, 32}) add(1, kWaveCols, c, 4);
+			for (int c : {8, 16, 32, 64})
+				for (int body = 4; body <= 5; body++) add(1, kWaveCols, c, body);
 		// periodic Allen-Cahn: temporal blocks (chunk = steps per pass)
 		if (model == MMSPX_AC && all_periodic(g))
 			for (int tx : {8, 16, 32})
@@ -543,39 +542,44 @@ static int launch_wave(const Geom& g, const ModelParams& p, const void* a, void*
 	return fail(MMSPX_EUNSUPPORTED, "no wave-strip kernel for this field count and precision");
 }
 
-template <int PREC, int Q, int KIND>
+template <int PREC, int Q, int KIND, int W>
 static int launch_split_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
 	using S = typename Form<PREC>::S;
 	const int strips = (g.n[1] + kWaveCols - 1) / kWaveCols;
 	const int nitems = strips * ((g.n[0] + chunk - 1) / chunk);
 	const int blocks = (nitems + 7) / 8 * 8;
-	hipLaunchKernelGGL((wave_strips_split<PREC, Q, KIND>), dim3(blocks), dim3(128), 0, st, (const S*)a, (S*)b, g, p,
-	                   chunk, strips, nitems);
+	hipLaunchKernelGGL((wave_strips_split<PREC, Q, KIND, W>), dim3(blocks), dim3(64 * W), 0, st, (const S*)a, (S*)b, g,
+	                   p, chunk, strips, nitems);
 	MMSPX_HIP(hipGetLastError());
 	return 0;
 }
 
-// The split strips: grain growth and Zener pinning with 8, 16 or 32 fields
-template <int PREC>
-static int launch_split(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+template <int PREC, int KIND, int W>
+static int launch_split_w(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
-	if (p.model == MMSPX_ZENER) {
-		if constexpr (PREC != MMSPX_F32REF) {
-			switch (g.fields) {
-			case 8: return launch_split_q<PREC, 8, 3>(g, p, a, b, st, chunk);
-			case 16: return launch_split_q<PREC, 16, 3>(g, p, a, b, st, chunk);
-			case 32: return launch_split_q<PREC, 32, 3>(g, p, a, b, st, chunk);
-			}
-		}
-	} else {
-		switch (g.fields) {
-		case 8: return launch_split_q<PREC, 8, 1>(g, p, a, b, st, chunk);
-		case 16: return launch_split_q<PREC, 16, 1>(g, p, a, b, st, chunk);
-		case 32: return launch_split_q<PREC, 32, 1>(g, p, a, b, st, chunk);
-		}
+	switch (g.fields) {
+	case 8: return launch_split_q<PREC, 8, KIND, W>(g, p, a, b, st, chunk);
+	case 16: return launch_split_q<PREC, 16, KIND, W>(g, p, a, b, st, chunk);
+	case 32: return launch_split_q<PREC, 32, KIND, W>(g, p, a, b, st, chunk);
 	}
 	return fail(MMSPX_EUNSUPPORTED, "split wave strips take 8, 16 or 32 fields");
+}
+
+// The split strips: grain growth and Zener pinning with 8, 16 or 32 fields
+// on W = 2 (body 4) or 4 (body 5) waves per strip
+template <int PREC>
+static int launch_split(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk,
+                        int waves)
+{
+	if (p.model == MMSPX_ZENER) {
+		if constexpr (PREC != MMSPX_F32REF)
+			return waves == 4 ? launch_split_w<PREC, 3, 4>(g, p, a, b, st, chunk)
+			                  : launch_split_w<PREC, 3, 2>(g, p, a, b, st, chunk);
+		return fail(MMSPX_EUNSUPPORTED, "Zener pinning runs the MMSPX_F64 and MMSPX_F32 forms");
+	}
+	return waves == 4 ? launch_split_w<PREC, 1, 4>(g, p, a, b, st, chunk)
+	                  : launch_split_w<PREC, 1, 2>(g, p, a, b, st, chunk);
 }
 
 template <int PREC, int Q>
@@ -625,7 +629,8 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		}
 		return fail(MMSPX_EUNSUPPORTED, "Ostwald ripening runs the wave strips (1 x 60) or the generic kernels");
 	case MMSPX_ZENER:
-		if (t.opt == 4 && t.ty == 1 && t.tz == kWaveCols) return launch_split<PREC>(g, p, a, b, st, chunk);
+		if ((t.opt == 4 || t.opt == 5) && t.ty == 1 && t.tz == kWaveCols)
+			return launch_split<PREC>(g, p, a, b, st, chunk, t.opt == 5 ? 4 : 2);
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		return fail(MMSPX_EUNSUPPORTED, "Zener pinning runs the wave strips (1 x 64, body 2) or the generic kernel");
 	case MMSPX_GG_ANISO:
@@ -649,7 +654,8 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 	case MMSPX_GG:
 		// ty > 1: 2-D tiles of ty (8 or 16) rows x 64 columns; body 2 with 64
 		// columns: wave strips; otherwise LDS-row marching strips
-		if (t.opt == 4 && t.ty == 1 && t.tz == kWaveCols) return launch_split<PREC>(g, p, a, b, st, chunk);
+		if ((t.opt == 4 || t.opt == 5) && t.ty == 1 && t.tz == kWaveCols)
+			return launch_split<PREC>(g, p, a, b, st, chunk, t.opt == 5 ? 4 : 2);
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		if (t.ty > 1) return t.ty == 16 ? launch_gg_tile<PREC, 16>(g, p, a, b, st) : launch_gg_tile<PREC, 8>(g, p, a, b, st);
 		if (t.tz == 64) return launch_gg_tz<PREC, 64>(g, p, a, b, st, chunk);

@@ -944,17 +944,17 @@ __device__ __forceinline__ void strip_wave(const typename Form<PREC>::S* __restr
 	}
 }
 
-// ---- 2-D grain growth, split wave strips: two waves per (strip, chunk) -------
+// ---- 2-D grain growth, split wave strips: W waves per (strip, chunk) --------
 // The strip_wave layout (lane l on column c0 + l of a 64-column strip, row
 // window in registers, column neighbours by DPP), with the Q fields split
-// between the two waves of a 128-thread workgroup: wave h holds fields
-// h*Q/2 .. (h+1)*Q/2 - 1, half the registers, so more waves fit a SIMD. The
-// sum of phi_j^2 runs in the reference's field order across the pair: wave 0
-// sums its fields from zero and passes the partial sum through LDS, wave 1
-// continues it over its fields and passes the total back (two barriers per
-// row). KIND 1: grain growth (graingrowth.cpp:91-107); KIND 3: Zener pinning
-// (field 0 held, zener.cpp:117-135).
-template <int PREC, int Q, int KIND>
+// between the W (2 or 4) waves of a workgroup: wave h holds fields
+// h*Q/W .. (h+1)*Q/W - 1, a W-th of the registers, so more waves fit a SIMD.
+// The sum of phi_j^2 runs in the reference's field order across the waves:
+// wave 0 sums its fields from zero and passes the partial sum through LDS,
+// each next wave continues it over its fields, and the last passes the total
+// back (W barriers per row). KIND 1: grain growth (graingrowth.cpp:91-107);
+// KIND 3: Zener pinning (field 0 held, zener.cpp:117-135).
+template <int PREC, int Q, int KIND, int W>
 __device__ __forceinline__ void strip_wave_split(const typename Form<PREC>::S* __restrict__ a,
                                                  typename Form<PREC>::S* __restrict__ b, const Geom& g,
                                                  const ModelParams& p, int c0, int xb, int xe)
@@ -962,11 +962,11 @@ __device__ __forceinline__ void strip_wave_split(const typename Form<PREC>::S* _
 	using F = Form<PREC>;
 	using S = typename F::S;
 	using Sum = typename F::Sum;
-	constexpr int H = Q / 2;
-	static_assert(Q % 2 == 0 && Q <= 64, "the split strips hold Q/2 fields per wave");
+	constexpr int H = Q / W;
+	static_assert(Q % W == 0 && 2 * H <= 64, "the split strips hold Q/W fields per wave");
 	__shared__ Sum part[64], total[64];
 	const int lane = threadIdx.x & 63;
-	const int half = (int)(threadIdx.x >> 6) & 1;
+	const int half = (int)(threadIdx.x >> 6);  // the wave's part of the fields, 0 .. W-1
 	const int f0 = half * H;
 	const int nx = g.n[0], ny = g.n[1];
 	const size_t N = (size_t)nx * ny;
@@ -996,22 +996,28 @@ __device__ __forceinline__ void strip_wave_split(const typename Form<PREC>::S* _
 	auto row = [&](int x, const S* w0, const S* w1, const S* w2, S* w3, const S h1, S& h3) {
 		load_row(x + 2, w3, h3);
 		// sum_j phi_j^2 of row x in field order: wave 0 from zero over its
-		// fields, wave 1 on from there; both then hold the total
+		// fields, each next wave on from there; all then hold the total
 		Sum acc = 0;
 		if (half == 0) {
 #pragma unroll
 			for (int j = 0; j < H; j++) acc += w1[j] * w1[j];
 			part[lane] = acc;
 		}
-		__syncthreads();
-		if (half == 1) {
-			acc = part[lane];
 #pragma unroll
-			for (int j = 0; j < H; j++) acc += w1[j] * w1[j];
-			total[lane] = acc;
+		for (int k = 1; k < W; k++) {
+			__syncthreads();
+			if (half == k) {
+				acc = part[lane];
+#pragma unroll
+				for (int j = 0; j < H; j++) acc += w1[j] * w1[j];
+				if (k + 1 < W)
+					part[lane] = acc;
+				else
+					total[lane] = acc;
+			}
 		}
 		__syncthreads();
-		if (half == 0) acc = total[lane];
+		if (half + 1 < W) acc = total[lane];
 		const unsigned o = (unsigned)x * (unsigned)ny + (unsigned)c;
 #pragma unroll
 		for (int j = 0; j < H; j++) {

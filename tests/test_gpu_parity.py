@@ -279,8 +279,8 @@ def test_wave_strips(prec):
         if prec != "f64":
             a = a.astype(np.float32)
         exp = O.gg(a, 0.01, 9, form=prec, bc=bc)
-        # body 2: wave strips; body 4: split strips (8, 16, 32 fields)
-        for body in ((2, 4) if q in (8, 16, 32) else (2,)):
+        # body 2: wave strips; bodies 4 and 5: split strips on 2 and 4 waves (8, 16, 32 fields)
+        for body in ((2, 4, 5) if q in (8, 16, 32) else (2,)):
             for chunk in (7, 8, 16):
                 with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
                     g.set_variant(2)
@@ -298,7 +298,13 @@ def test_wave_strips(prec):
             g.set_tile_options(body=4)
             g.upload(a)
             g.step("gg", [0.01], 6)
-            assert np.array_equal(g.download(), O.gg(a, 0.01, 6, form="f64"))
+            exp = O.gg(a, 0.01, 6, form="f64")
+            assert np.array_equal(g.download(), exp)
+            g.set_tile_options(body=5)
+            g.set_tile(1, 64, 7)
+            g.upload(a)
+            g.step("gg", [0.01], 6)
+            assert np.array_equal(g.download(), exp)
         with mx.Grid((16, 70), fields=32, vector=True, precision="f64") as g:
             g.set_variant(2)
             g.set_tile(1, 64, 8)
@@ -430,7 +436,7 @@ def test_zener_and_ostwald(prec):
         exp = O.zener(a, 0.01, 9, form=prec, bc=bc)
         runs = [(1, None, 0), (2, (1, 64, 7), 2), (2, (1, 64, 8), 2), (2, (1, 64, 24), 2), (0, None, 0)]
         if q in (8, 16, 32):
-            runs += [(2, (1, 64, 6), 4), (2, (1, 64, 9), 4)]  # the split strips
+            runs += [(2, (1, 64, 6), 4), (2, (1, 64, 9), 4), (2, (1, 64, 5), 5)]  # the split strips
         for variant, tile, body in runs:
             with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
                 g.set_variant(variant)
