@@ -575,3 +575,21 @@ def test_multiphase_wave_strips(prec):
         g.set_variant(0)  # the default falls back to the generic kernel
         g.step("gg_aniso", p, 2)
         assert np.array_equal(g.download(), O.gg_aniso(a, p, 2, form=prec))
+
+
+def test_zero_steps_leave_the_state():
+    """step(model, p, 0) launches nothing and keeps the state, also for the
+    stochastic forms, whose step count (the noise counter) stays where it was."""
+    a = O.uniform(24 * 70, 31, 0.3, 0.7).reshape(24, 70)
+    q = O.uniform(24 * 70 * 8, 32, 0.0, 1.0).reshape(24, 70, 8)
+    for model, params, arr, kw in [("ac", [1.0, 1.0, 1.0, 1.0, 0.01], a, {}), ("ch_dw", [1.0, 1.0, 0.005], a, {}),
+                                   ("gg", [0.01], q, dict(fields=8, vector=True)),
+                                   ("ch_noise", [1.0, 1.0, 1.0, 1.0, 0.005, 0.3, 5], a, {})]:
+        say("[gpu] zero steps %s" % model)
+        with mx.Grid(arr.shape[:2], **kw) as g:
+            g.upload(arr)
+            g.step(model, params, 0)
+            assert np.array_equal(g.download(), arr), model
+            if model == "ch_noise":
+                g.step(model, params, 3)
+                assert np.array_equal(g.download(), O.ch_noise(arr, params, 3)), model
