@@ -525,8 +525,8 @@ def main():
     ap.add_argument("--tile", default=None, help="ty,tz,chunk[,opt] for the tiled kernel (opt 1: register queues)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the path is the per-GPU kernel; "
-              "multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print("bench.py: measures one GPU only: the path is the per-GPU kernel; sharding over several GPUs "
+              "(slabs along axis 0, one halo exchange per stencil pass) is not built", file=sys.stderr)
         sys.exit(2)
 
     import mmsp_amd as mx
