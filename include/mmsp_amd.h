@@ -226,7 +226,8 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
 int mmspx_set_tile_options(mmspx_state* s, int body);
 /* Returns variant + 16 * tile body (1: register queues, or the 2-D
  * grain-growth tile; 2: wave strips, or 3-D register queues on 768 threads;
- * 3: temporal blocks, or 3-D register queues on 1024 threads). Before any tile is set, reports what the last step ran
+ * 3: temporal blocks, or 3-D register queues on 1024 threads; 4 and 5: the
+ * split strips on two and four waves). Before any tile is set, reports what the last step ran
  * (the Cahn-Hilliard default before any step). */
 int mmspx_get_tile(const mmspx_state* s, int* ty, int* tz, int* chunk);
 /* Project autotuner: times every candidate tile for `model` on this state's
