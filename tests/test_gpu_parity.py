@@ -593,3 +593,56 @@ def test_zero_steps_leave_the_state():
             if model == "ch_noise":
                 g.step(model, params, 3)
                 assert np.array_equal(g.download(), O.ch_noise(arr, params, 3)), model
+
+
+@pytest.mark.parametrize("case", ["gg32", "zener16", "gga8", "ostwald4", "ac_neumann", "ch3d", "gg4"])
+def test_autotune_runs_every_candidate_and_keeps_the_state(case):
+    """mmspx_autotune times every compiled candidate of the model on the state
+    (the wave strips, the split strips on two and four waves, the tiles, the
+    generic kernel), restores the state, and the tile it keeps gives the
+    oracle's bits."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(os.path.dirname(__file__), "golden",
+                                                                              "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    say("[gpu] autotune %s" % case)
+    if case == "gg32":
+        shape, q, prec, model, p = (70, 200), 32, "f32", "gg", [0.01]
+        a = O.uniform(int(np.prod(shape)) * q, 801, 0.0, 1.0).reshape(shape + (q,)).astype(np.float32)
+        exp = O.gg(a, 0.01, 7, form=prec)
+    elif case == "gg4":
+        shape, q, prec, model, p = (66, 130), 4, "f64", "gg", [0.01]
+        a = O.uniform(int(np.prod(shape)) * q, 806, 0.0, 1.0).reshape(shape + (q,))
+        exp = O.gg(a, 0.01, 7, form=prec)
+    elif case == "zener16":
+        shape, q, prec, model, p = (50, 140), 16, "f32", "zener", [0.01]
+        a = O.uniform(int(np.prod(shape)) * q, 802, 0.0, 1.0).reshape(shape + (q,)).astype(np.float32)
+        exp = O.zener(a, 0.01, 7, form=prec)
+    elif case == "gga8":
+        shape, q, prec, model, p = (64, 200), 8, "f64", "gg_aniso", [0.01, 8.0, 1.0, 1.0]
+        a = mg.one_hot_grains(shape, q, 803)
+        exp = O.gg_aniso(a, p, 7, form=prec)
+    elif case == "ostwald4":
+        shape, q, prec, model, p = (64, 150), 4, "f64", "ostwald", [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]
+        a = O.uniform(int(np.prod(shape)) * q, 804, 0.0, 1.0).reshape(shape + (q,))
+        exp = O.ostwald(a, p, 7, form=prec)
+    elif case == "ac_neumann":
+        shape, q, prec, model, p = (70, 200), 0, "f64", "ac", [1.0, 1.0, 1.0, 1.0, 0.01]
+        a = O.uniform(int(np.prod(shape)), 805, -1.0, 1.0).reshape(shape)
+        exp = O.ac(a, p, 7, form=prec, bc=[(N, N), (N, N)])
+    else:
+        shape, q, prec, model, p = (40, 36, 70), 0, "f64", "ch_dw", [1.0, 1.0, 0.005]
+        a = O.uniform(int(np.prod(shape)), 807, 0.45, 0.55).reshape(shape)
+        exp = O.ch(a, p, 7, mu=O.MU_DW, form=prec)
+    kw = dict(fields=q, vector=True) if q else {}
+    bc = [(N, N), (N, N)] if case == "ac_neumann" else None
+    with mx.Grid(shape, precision=prec, bc=bc, **kw) as g:
+        g.upload(a)
+        rep = g.autotune(model, p, trial_steps=2)
+        lines = [ln for ln in rep.splitlines() if ln.startswith("variant=")]
+        assert len(lines) >= 2, rep  # the generic kernel and at least one tiled candidate
+        assert np.array_equal(g.download(), a)  # restored
+        g.step(model, p, 7)
+        assert np.array_equal(g.download(), exp), (case, g.get_tile())
