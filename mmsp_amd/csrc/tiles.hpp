@@ -966,8 +966,8 @@ __device__ __forceinline__ void strip_wave_split(const typename Form<PREC>::S* _
 	static_assert(Q % W == 0 && 2 * H <= 64, "the split strips hold Q/W fields per wave");
 	__shared__ Sum part[64], total[64];
 	const int lane = threadIdx.x & 63;
-	const int half = (int)(threadIdx.x >> 6);  // the wave's part of the fields, 0 .. W-1
-	const int f0 = half * H;
+	const int wv = (int)(threadIdx.x >> 6);  // the wave's share of the fields, 0 .. W-1
+	const int f0 = wv * H;
 	const int nx = g.n[0], ny = g.n[1];
 	const size_t N = (size_t)nx * ny;
 	const int c = c0 + lane;
@@ -998,7 +998,7 @@ __device__ __forceinline__ void strip_wave_split(const typename Form<PREC>::S* _
 		// sum_j phi_j^2 of row x in field order: wave 0 from zero over its
 		// fields, each next wave on from there; all then hold the total
 		Sum acc = 0;
-		if (half == 0) {
+		if (wv == 0) {
 #pragma unroll
 			for (int j = 0; j < H; j++) acc += w1[j] * w1[j];
 			part[lane] = acc;
@@ -1006,7 +1006,7 @@ __device__ __forceinline__ void strip_wave_split(const typename Form<PREC>::S* _
 #pragma unroll
 		for (int k = 1; k < W; k++) {
 			__syncthreads();
-			if (half == k) {
+			if (wv == k) {
 				acc = part[lane];
 #pragma unroll
 				for (int j = 0; j < H; j++) acc += w1[j] * w1[j];
@@ -1017,7 +1017,7 @@ __device__ __forceinline__ void strip_wave_split(const typename Form<PREC>::S* _
 			}
 		}
 		__syncthreads();
-		if (half + 1 < W) acc = total[lane];
+		if (wv + 1 < W) acc = total[lane];
 		const unsigned o = (unsigned)x * (unsigned)ny + (unsigned)c;
 #pragma unroll
 		for (int j = 0; j < H; j++) {
