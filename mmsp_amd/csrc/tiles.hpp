@@ -447,35 +447,42 @@ __device__ __forceinline__ void ow_wave(const typename Form<PREC>::S* __restrict
 	};
 	S c0r = a[row(xb - 2)], c1r = a[row(xb - 1)];
 	S cx = a[row(xb)], cx1 = a[row(xb + 1)], cx2 = a[row(xb + 2)];
-	S cq = (xb + 1 < xe) ? a[row(xb + 3)] : S(0);  // C(x+3) in flight
-	S fm[R], fc[R], fp[R], fq[R];                     // phi rows x-1, x, x+1; x+2 in flight
+	// five-slot rings for C (rows x .. x+2, x+3 in flight, x+4 loading) and
+	// phi (rows x-1 .. x+1, x+2 in flight, x+3 loading) whose roles rotate over
+	// a five-row unrolled march, so no row is copied while its loads are in
+	// flight
+	S C3 = (xb + 1 < xe) ? a[row(xb + 3)] : S(0), C4 = 0;
+	S F0[R], F1[R], F2[R], F3[R], F4[R];
 #pragma unroll
 	for (int j = 0; j < R; j++) {
 		const S* __restrict__ aj = a + (size_t)(j + 1) * N;
-		fm[j] = aj[row(xb - 1)];
-		fc[j] = aj[row(xb)];
-		fp[j] = aj[row(xb + 1)];
-		fq[j] = (xb + 1 < xe) ? aj[row(xb + 2)] : S(0);
+		F0[j] = aj[row(xb - 1)];
+		F1[j] = aj[row(xb)];
+		F2[j] = aj[row(xb + 1)];
+		F3[j] = (xb + 1 < xe) ? aj[row(xb + 2)] : S(0);
+		F4[j] = 0;
 	}
-	typename F::OSum sm = sum_of(fm), sc = sum_of(fc);
-	S wm = w_at(c0r, c1r, cx, sm);  // w(x-1)
-	S wc = w_at(c1r, cx, cx1, sc);  // w(x)
-	for (int x = xb; x < xe; x++) {
-		const bool more = x + 1 < xe;
-		// loads for the next row go out first
-		S cn = 0, fn[R];
-		if (more && x + 2 < xe) cn = a[row(x + 4)];
+	typename F::OSum sc = sum_of(F1);
+	S wm = w_at(c0r, c1r, cx, sum_of(F0));  // w(x-1)
+	S wc = w_at(c1r, cx, cx1, sc);          // w(x)
+	// row x: C at x, x+1, x+2 (cx0, cx1, cx2), C(x+4) to cn; phi at x-1, x,
+	// x+1 (fm, fc, fp), phi(x+3) to fn
+	auto step = [&](int x, S cx0, S cx1_, S cx2_, S& cn, const S* fm, const S* fc, const S* fp, S* fn) {
+		// loads for the next rows go out first
+		if (x + 2 < xe) {
+			cn = a[row(x + 4)];
 #pragma unroll
-		for (int j = 0; j < R; j++) fn[j] = (more && x + 2 < xe) ? (a + (size_t)(j + 1) * N)[row(x + 3)] : S(0);
+			for (int j = 0; j < R; j++) fn[j] = (a + (size_t)(j + 1) * N)[row(x + 3)];
+		}
 		const typename F::OSum sp = sum_of(fp);
-		const S wp = w_at(cx, cx1, cx2, sp);  // w(x+1)
+		const S wp = w_at(cx0, cx1_, cx2_, sp);  // w(x+1)
 		// C'(x) = C + dt*D*lap(w)
 		{
 			const S zl = from_prev(wc, wc), zh = from_next(wc, wc);
 			typename F::L lap = F::lap_zero();
 			F::lap_add(lap, g.w[0], g.wf[0], wp, wc, wm);
 			F::lap_add(lap, g.w[1], g.wf[1], zh, wc, zl);
-			if (out) b[(size_t)x * ny + c] = F::ow_c(cx, lap, p);
+			if (out) b[(size_t)x * ny + c] = F::ow_c(cx0, lap, p);
 		}
 #pragma unroll
 		for (int j = 0; j < R; j++) {
@@ -484,22 +491,23 @@ __device__ __forceinline__ void ow_wave(const typename Form<PREC>::S* __restrict
 			typename F::L lap = F::lap_zero();
 			F::lap_add(lap, g.w[0], g.wf[0], fp[j], y, fm[j]);
 			F::lap_add(lap, g.w[1], g.wf[1], yh, y, yl);
-			if (out) b[(size_t)(j + 1) * N + (size_t)x * ny + c] = F::ow_phi(y, cx, sc, lap, p);
+			if (out) b[(size_t)(j + 1) * N + (size_t)x * ny + c] = F::ow_phi(y, cx0, sc, lap, p);
 		}
 		wm = wc;
 		wc = wp;
 		sc = sp;
-		cx = cx1;
-		cx1 = cx2;
-		cx2 = cq;
-		cq = cn;
-#pragma unroll
-		for (int j = 0; j < R; j++) {
-			fm[j] = fc[j];
-			fc[j] = fp[j];
-			fp[j] = fq[j];
-			fq[j] = fn[j];
-		}
+	};
+	S C0 = cx, C1 = cx1, C2 = cx2;
+	for (int x = xb; x < xe; x += 5) {
+		step(x, C0, C1, C2, C4, F0, F1, F2, F4);
+		if (x + 1 >= xe) break;
+		step(x + 1, C1, C2, C3, C0, F1, F2, F3, F0);
+		if (x + 2 >= xe) break;
+		step(x + 2, C2, C3, C4, C1, F2, F3, F4, F1);
+		if (x + 3 >= xe) break;
+		step(x + 3, C3, C4, C0, C2, F3, F4, F0, F2);
+		if (x + 4 >= xe) break;
+		step(x + 4, C4, C0, C1, C3, F4, F0, F1, F3);
 	}
 }
 
