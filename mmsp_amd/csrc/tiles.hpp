@@ -341,7 +341,6 @@ __device__ __forceinline__ void ch_wave_dir(const typename Form<PREC>::S* __rest
 	using F = Form<PREC>;
 	using S = typename F::S;
 	constexpr int d = DOWN ? -1 : 1;
-	constexpr int PD = 4;  // rows in flight beyond the window
 	const int lane = threadIdx.x & 63;
 	const int nx = g.n[0], ny = g.n[1];
 	const int c = c0 - 2 + lane;
@@ -368,32 +367,39 @@ __device__ __forceinline__ void ch_wave_dir(const typename Form<PREC>::S* __rest
 	};
 	const int x0 = DOWN ? xe - 1 : xb, x1 = DOWN ? xb - 1 : xe;
 	const int n = xe - xb;  // rows written
-	// phi at x0-2d .. x0+2d, then mu at x0-d and x0
+	// phi rows in a ring of eight registers: at the row x written at position
+	// u of the unrolled march, slot (u + k) & 7 holds phi(x + k d) for k = 0..6
+	// (rows 3..6 still in flight) and receives row x + 7d; the slots change
+	// roles instead of being copied, so a row's loads are waited for only
+	// where the row is first used. A row is loaded only if the chunk reads it.
+	constexpr int RING = 8;
+	S f[RING];
 	const S fm2 = ld(x0 - 2 * d), fm1 = ld(x0 - d);
-	S f0 = ld(x0), f1 = ld(x0 + d), f2 = ld(x0 + 2 * d);
-	// q[k] = phi(x + (3+k)d) in flight; it becomes f2 when row x+(k+1)d is
-	// written, so it is read only if that row is in the chunk
-	S q[PD];
+	f[0] = ld(x0);
+	f[1] = ld(x0 + d);
+	f[2] = ld(x0 + 2 * d);
 #pragma unroll
-	for (int k = 0; k < PD; k++) q[k] = (k + 1 < n) ? ld(x0 + (3 + k) * d) : S(0);
-	S mb = mu(x0 - d, fm2, fm1, f0);  // mu(x0 - d)
-	S mc = mu(x0, fm1, f0, f1);       // mu(x0)
-	int i = 0;
-	for (int x = x0; x != x1; x += d, i++) {
-		const S ma = mu(x + d, f0, f1, f2);  // mu(x+d)
-		const S zl = from_prev(mc, mc), zh = from_next(mc, mc);
-		typename F::L lap = F::lap_zero();
-		F::lap_add(lap, g.w[0], g.wf[0], DOWN ? mb : ma, mc, DOWN ? ma : mb);
-		F::lap_add(lap, g.w[1], g.wf[1], zh, mc, zl);
-		if (out) b[(size_t)x * ny + c] = F::ch(f0, lap, p);
-		mb = mc;
-		mc = ma;
-		f0 = f1;
-		f1 = f2;
-		f2 = q[0];
+	for (int k = 0; k < 4; k++) f[3 + k] = (k + 1 < n) ? ld(x0 + (3 + k) * d) : S(0);
+	f[RING - 1] = 0;
+	S mb = mu(x0 - d, fm2, fm1, f[0]);  // mu(x0 - d)
+	S mc = mu(x0, fm1, f[0], f[1]);     // mu(x0)
+	for (int x = x0, i = 0;;) {
 #pragma unroll
-		for (int k = 0; k + 1 < PD; k++) q[k] = q[k + 1];
-		q[PD - 1] = (i + PD + 1 < n) ? ld(x + (3 + PD) * d) : S(0);
+		for (int u = 0; u < RING; u++, i++) {
+			if (x == x1) return;
+			// row x + 7d becomes phi(x'+2d) at row x' = x + 5d
+			if (i + 5 < n) f[(u + RING - 1) % RING] = ld(x + (RING - 1) * d);
+			const S f0 = f[u], f1 = f[(u + 1) % RING], f2 = f[(u + 2) % RING];
+			const S ma = mu(x + d, f0, f1, f2);  // mu(x+d)
+			const S zl = from_prev(mc, mc), zh = from_next(mc, mc);
+			typename F::L lap = F::lap_zero();
+			F::lap_add(lap, g.w[0], g.wf[0], DOWN ? mb : ma, mc, DOWN ? ma : mb);
+			F::lap_add(lap, g.w[1], g.wf[1], zh, mc, zl);
+			if (out) b[(size_t)x * ny + c] = F::ch(f0, lap, p);
+			mb = mc;
+			mc = ma;
+			x += d;
+		}
 	}
 }
 
