@@ -246,13 +246,19 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// x 4 FP32 8.0 us against 12.0, 4096^2 x 4 FP64 219.7 against 354.1)
 	if (model == MMSPX_GG || model == MMSPX_ZENER)
 		return TuneChoice{2, 1, kWaveCols, nodes < ((int64_t)1 << 21) ? 4 : 16, 2};
-	// 2-D Cahn-Hilliard: wave strips (1024^2 FP64 7.3 us per step against 13.5
-	// us for the LDS row strips; 4096^2 FP64 53 us against 69 us, FP32 34 us
-	// against 56 us). Chunks from a sweep of 3..16 rows: 4 below 2^21 nodes
-	// (256^2 FP64 4.6 us against 5.5 with 8 rows, FP32 3.9 against 4.8), 8 up
-	// to 2^23 (2048^2 FP64 17.9 us, FP32 12.9), 16 above (4096^2 55.5 / 38.3)
-	if (g.dim == 2)
-		return TuneChoice{2, 1, kChWaveOut, nodes < ((int64_t)1 << 21) ? 4 : nodes < ((int64_t)1 << 23) ? 8 : 16, 0};
+	// 2-D Cahn-Hilliard: wave strips (1024^2 FP64 6.8 us per step against 13.5
+	// us for the LDS row strips; 4096^2 FP64 42.4 us against 69 us). Chunks
+	// from a sweep of 3..32 rows with the row ring: 3 below 2^17 nodes (256^2
+	// FP64 3.0 us against 3.2 with 4 rows), 4 below 2^21 (1024^2 6.8, FP32
+	// 5.2), 8 below 2^23 (2048^2 FP64 16.5, FP32 11.3), 16 above (4096^2 42.4
+	// and 30.7)
+	if (g.dim == 2) {
+		const int c = nodes < ((int64_t)1 << 17)   ? 3
+		              : nodes < ((int64_t)1 << 21) ? 4
+		              : nodes < ((int64_t)1 << 23) ? 8
+		                                           : 16;
+		return TuneChoice{2, 1, kChWaveOut, c, 0};
+	}
 	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16, 0};
 	// 256^3: register queues on 32 x 64 tiles with 768 threads. FP64 with
 	// 32-plane chunks, one workgroup per CU (52.3 us per step, against 53.0
