@@ -223,10 +223,11 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// with 8 and 77.3 for one thread per node), 8 rows above (2048^2 FP64 139
 	// us, FP32 70; 4096^2 FP64 500, FP32 265: within 2 % of the best of 2..16)
 	if (model == MMSPX_GG_ANISO) return TuneChoice{2, 1, kWaveCols, nodes < ((int64_t)1 << 21) ? 2 : 8, 2};
-	// Ostwald ripening (4 fields FP64): 4-row chunks take 8.7, 18.8, 58.8 and
-	// 236.8 us per step at 512^2 .. 4096^2, at or within 4 % of the best of
-	// chunks 2..16 at every size (the generic two passes: 28 us at 1024^2)
-	if (model == MMSPX_OSTWALD) return TuneChoice{2, 1, kChWaveOut, 4, 0};
+	// Ostwald ripening (4 fields): with the row rings, 4-row chunks below 2^21
+	// nodes (1024^2 FP64 17.0 us per step against 18.0 with 8 rows; the
+	// generic two passes: 28 us), 8 above (2048^2 FP64 51.1 against 52.2, FP32
+	// 27.2 against 28.5)
+	if (model == MMSPX_OSTWALD) return TuneChoice{2, 1, kChWaveOut, nodes < ((int64_t)1 << 21) ? 4 : 8, 0};
 	// Grain growth and Zener pinning with 8, 16 or 32 fields: the split strips.
 	// Below 2^21 nodes four waves per strip (a quarter of the fields each)
 	// with 8-row chunks in FP32 and 16 in FP64; above, two waves with 64-row
