@@ -251,13 +251,13 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// us for the LDS row strips; 4096^2 FP64 42.4 us against 69 us). Chunks
 	// from a sweep of 3..32 rows with the row ring: 3 below 2^17 nodes (256^2
 	// FP64 3.0 us against 3.2 with 4 rows), 4 below 2^21 (1024^2 6.8, FP32
-	// 5.2), 8 below 2^23 (2048^2 FP64 16.5, FP32 11.3), 16 above (4096^2 42.4
-	// and 30.7)
+	// 5.2), then 16 in FP64 (2048^2 15.6 against 16.5 with 8; 4096^2 42.4) and
+	// 8 in FP32 below 2^23 (2048^2 11.3 against 11.5), 16 above (4096^2 30.7)
 	if (g.dim == 2) {
-		const int c = nodes < ((int64_t)1 << 17)   ? 3
-		              : nodes < ((int64_t)1 << 21) ? 4
-		              : nodes < ((int64_t)1 << 23) ? 8
-		                                           : 16;
+		const int c = nodes < ((int64_t)1 << 17)                          ? 3
+		              : nodes < ((int64_t)1 << 21)                        ? 4
+		              : nodes < ((int64_t)1 << 23) && prec != MMSPX_F64 ? 8
+		                                                                  : 16;
 		return TuneChoice{2, 1, kChWaveOut, c, 0};
 	}
 	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16, 0};
