@@ -51,6 +51,23 @@ int validate_desc(const mmspx_desc* d)
 	return 0;
 }
 
+int validate_model(const mmspx_desc& d, const Geom& g, int model)
+{
+	if (vector_model(model)) {
+		if (!d.vector) return fail(MMSPX_EINVAL, "grain growth, Zener pinning and Ostwald ripening need a vector grid");
+		if (model == MMSPX_OSTWALD && g.fields < 2)
+			return fail(MMSPX_EINVAL, "Ostwald ripening needs a concentration field and at least one order parameter");
+		if ((model == MMSPX_ZENER || model == MMSPX_OSTWALD || model == MMSPX_GG_ANISO) &&
+		    d.precision == MMSPX_F32REF)
+			return fail(MMSPX_EUNSUPPORTED, "this model runs the MMSPX_F64 and MMSPX_F32 forms");
+		if (model == MMSPX_GG_ANISO && g.fields > kAnisoMaxFields)
+			return fail(MMSPX_EUNSUPPORTED, "anisotropic grain growth takes at most 32 fields");
+	} else if (d.vector) {
+		return fail(MMSPX_EINVAL, "Allen-Cahn and Cahn-Hilliard need a scalar grid");
+	}
+	return 0;
+}
+
 Geom make_geom(const mmspx_desc& d)
 {
 	Geom g;
@@ -322,6 +339,21 @@ static std::vector<int> plan_passes(int n, int spl, bool even)
 	return v;
 }
 
+// Every constant a launch bakes into its arguments (all but the per-launch
+// step count): a captured graph is replayed only when all of them match.
+static bool same_baked_params(const ModelParams& a, const ModelParams& b)
+{
+	if (a.model != b.model || a.nr != b.nr || a.u != b.u || a.K != b.K || a.dtM != b.dtM || a.kappa != b.kappa ||
+	    a.dt != b.dt || a.nrf != b.nrf || a.uf != b.uf || a.Kf != b.Kf || a.dtMf != b.dtMf || a.kappaf != b.kappaf ||
+	    a.dtf != b.dtf || a.sigma != b.sigma || a.sigmaf != b.sigmaf || a.seed != b.seed)
+		return false;
+	for (int k = 0; k < 13; k++)
+		if (a.ow[k] != b.ow[k] || a.owf[k] != b.owf[k]) return false;
+	for (int k = 0; k < 4; k++)
+		if (a.an[k] != b.an[k] || a.anf[k] != b.anf[k]) return false;
+	return true;
+}
+
 static int step_impl(mmspx_state* s, int model, const double* params, int np, int steps)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
@@ -329,18 +361,8 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	ModelParams p;
 	int rc = make_params(model, params, np, &p);
 	if (rc) return rc;
-	if (vector_model(model)) {
-		if (!s->desc.vector) return fail(MMSPX_EINVAL, "grain growth, Zener pinning and Ostwald ripening need a vector grid");
-		if (model == MMSPX_OSTWALD && s->geom.fields < 2)
-			return fail(MMSPX_EINVAL, "Ostwald ripening needs a concentration field and at least one order parameter");
-		if ((model == MMSPX_ZENER || model == MMSPX_OSTWALD || model == MMSPX_GG_ANISO) &&
-		    s->desc.precision == MMSPX_F32REF)
-			return fail(MMSPX_EUNSUPPORTED, "this model runs the MMSPX_F64 and MMSPX_F32 forms");
-		if (model == MMSPX_GG_ANISO && s->geom.fields > kAnisoMaxFields)
-			return fail(MMSPX_EUNSUPPORTED, "anisotropic grain growth takes at most 32 fields");
-	} else if (s->desc.vector) {
-		return fail(MMSPX_EINVAL, "Allen-Cahn and Cahn-Hilliard need a scalar grid");
-	}
+	rc = validate_model(s->desc, s->geom, model);
+	if (rc) return rc;
 	const Geom& g = s->geom;
 	bool fused = false;
 	TuneChoice tc{};
@@ -363,10 +385,7 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	// the stochastic forms: each launch carries its step number.
 	const size_t small_bytes = (size_t)64 << 20;
 	if (steps >= 2 * kGraphSteps && s->bytes <= small_bytes && !noisy_model(model)) {
-		const ModelParams& q = s->graph_params;
-		const bool same_params = q.nr == p.nr && q.u == p.u && q.K == p.K && q.dtM == p.dtM && q.kappa == p.kappa &&
-		                         q.dt == p.dt && q.nrf == p.nrf && q.uf == p.uf && q.Kf == p.Kf && q.dtMf == p.dtMf &&
-		                         q.kappaf == p.kappaf && q.dtf == p.dtf;
+		const bool same_params = same_baked_params(s->graph_params, p);
 		const bool same_tile = s->graph_tile.variant == tc.variant && s->graph_tile.ty == tc.ty &&
 		                       s->graph_tile.tz == tc.tz && s->graph_tile.chunk == tc.chunk &&
 		                       s->graph_tile.opt == tc.opt;

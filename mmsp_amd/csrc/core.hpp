@@ -75,6 +75,8 @@ inline bool ch_family(int model) { return model == MMSPX_CH || model == MMSPX_CH
 inline bool ac_family(int model) { return model == MMSPX_AC || model == MMSPX_AC_NOISE; }
 int make_params(int model, const double* p, int np, ModelParams* out);
 int model_nparams(int model);
+// the model runs on a grid of this kind (scalar/vector, fields, precision form)
+int validate_model(const mmspx_desc& d, const Geom& g, int model);
 
 // ---- the grid state ------------------------------------------------------------
 struct TuneChoice {

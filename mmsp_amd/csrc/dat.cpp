@@ -177,8 +177,12 @@ int dat_read(const char* filename, const mmspx_desc& want, int file_f64, void* h
 	int f64 = 0;
 	int rc = parse_header(in, filename, &d, &f64);
 	if (rc) return rc;
-	if (d.dim != want.dim || f64 != file_f64 || d.vector != want.vector)
-		return fail(MMSPX_EFORMAT, std::string(filename) + ": header changed while reading");
+	// `host` was sized from `want`: every extent, origin and the field count
+	// must still match, or a block could be scattered past its end
+	bool same = d.dim == want.dim && f64 == file_f64 && d.vector == want.vector &&
+	            (d.vector ? d.fields : 1) == (want.vector ? want.fields : 1);
+	for (int k = 0; k < d.dim && same; k++) same = d.n[k] == want.n[k] && d.g0[k] == want.g0[k];
+	if (!same) return fail(MMSPX_EFORMAT, std::string(filename) + ": header changed while reading");
 	const size_t elem = f64 ? 8 : 4;
 	const int q = d.vector ? d.fields : 1;
 	const size_t node_bytes = d.vector ? 4 + q * elem : elem;

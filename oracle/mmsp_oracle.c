@@ -572,19 +572,23 @@ static void gg_lap_f64(const orc_grid* g, int Q, const double* a, int64_t i, dou
 void orc_gg_steps_f64(const orc_grid* g, int Q, double* a, double* b, double dt, int steps, int cube)
 {
 	const int64_t n = nodes_of(g);
-	double* lap = (double*)malloc(sizeof(double) * (size_t)Q);
 	for (int s = 0; s < steps; s++) {
-		for (int64_t i = 0; i < n; i++) {
-			const double* phi = a + i * Q;
-			gg_lap_f64(g, Q, a, i, lap);
-			double sum = 0.0;
-			for (int j = 0; j < Q; j++) sum += phi[j] * phi[j];
-			for (int j = 0; j < Q; j++)
-				b[i * Q + j] = phi[j] - dt * (-phi[j] - cube64(phi[j], cube) + 2.0 * (phi[j] * sum - lap[j]));
+#pragma omp parallel if (n >= 65536)
+		{
+			double* lap = (double*)malloc(sizeof(double) * (size_t)Q);
+#pragma omp for schedule(static)
+			for (int64_t i = 0; i < n; i++) {
+				const double* phi = a + i * Q;
+				gg_lap_f64(g, Q, a, i, lap);
+				double sum = 0.0;
+				for (int j = 0; j < Q; j++) sum += phi[j] * phi[j];
+				for (int j = 0; j < Q; j++)
+					b[i * Q + j] = phi[j] - dt * (-phi[j] - cube64(phi[j], cube) + 2.0 * (phi[j] * sum - lap[j]));
+			}
+			free(lap);
 		}
 		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(double));
 	}
-	free(lap);
 }
 
 void orc_gg_steps_f32ref(const orc_grid* g, int Q, float* a, float* b, double dt, int steps, int cube)
@@ -622,29 +626,33 @@ void orc_gg_steps_f32(const orc_grid* g, int Q, float* a, float* b, double dt64,
 {
 	const int64_t n = nodes_of(g);
 	const float dt = (float)dt64;
-	float* lap = (float*)malloc(sizeof(float) * (size_t)Q);
-	int x[3] = {0, 0, 0};
 	for (int s = 0; s < steps; s++) {
-		for (int64_t i = 0; i < n; i++) {
-			const float* phi = a + i * Q;
-			position(g, i, x);
-			for (int j = 0; j < Q; j++) lap[j] = 0.0f;
-			for (int d = 0; d < g->dim; d++) {
-				const float* yh = a + neighbour(g, x, d, +1) * Q;
-				const float* yl = a + neighbour(g, x, d, -1) * Q;
-				float w = (float)weight_of(g, d);
-				for (int j = 0; j < Q; j++) lap[j] += w * (yh[j] - 2.0f * phi[j] + yl[j]);
+#pragma omp parallel if (n >= 65536)
+		{
+			float* lap = (float*)malloc(sizeof(float) * (size_t)Q);
+			int x[3] = {0, 0, 0};
+#pragma omp for schedule(static)
+			for (int64_t i = 0; i < n; i++) {
+				const float* phi = a + i * Q;
+				position(g, i, x);
+				for (int j = 0; j < Q; j++) lap[j] = 0.0f;
+				for (int d = 0; d < g->dim; d++) {
+					const float* yh = a + neighbour(g, x, d, +1) * Q;
+					const float* yl = a + neighbour(g, x, d, -1) * Q;
+					float w = (float)weight_of(g, d);
+					for (int j = 0; j < Q; j++) lap[j] += w * (yh[j] - 2.0f * phi[j] + yl[j]);
+				}
+				float sum = 0.0f;
+				for (int j = 0; j < Q; j++) sum += phi[j] * phi[j];
+				for (int j = 0; j < Q; j++) {
+					float pj = phi[j];
+					b[i * Q + j] = pj - dt * (-pj - pj * pj * pj + 2.0f * (pj * sum - lap[j]));
+				}
 			}
-			float sum = 0.0f;
-			for (int j = 0; j < Q; j++) sum += phi[j] * phi[j];
-			for (int j = 0; j < Q; j++) {
-				float pj = phi[j];
-				b[i * Q + j] = pj - dt * (-pj - pj * pj * pj + 2.0f * (pj * sum - lap[j]));
-			}
+			free(lap);
 		}
 		memcpy(a, b, (size_t)n * (size_t)Q * sizeof(float));
 	}
-	free(lap);
 }
 
 /* ---- Zener pinning: zener.cpp:117-135, p = {dt} -------------------------- */

@@ -71,7 +71,10 @@ def test_batch_allen_cahn_boundaries_and_alone_vs_in_mix():
         assert np.array_equal(got[m], O.ac(init[m], params[m], 20, form="f32", bc=bcs[m]))
 
 
-@pytest.mark.parametrize("tile", [None, (60, 3, 4, 0), (64, 16, 8, 0)])
+# 2-D members by wave strips with chunks of 3, 5, 7, 13 and 16 rows (the
+# eight-row unrolled march ends at every ring position), or by LDS row strips
+@pytest.mark.parametrize("tile", [None, (60, 3, 4, 0), (60, 5, 4, 0), (60, 7, 4, 0), (60, 13, 4, 0), (60, 16, 4, 0),
+                                  (64, 16, 8, 0)])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_batch_cahn_hilliard_2d_and_3d_members(prec, tile):
     shapes = [(64, 64), (20, 36, 40), (100, 70), (16, 16, 16), (33, 65, 70), (40, 600), (9, 20, 300), (70, 257)]

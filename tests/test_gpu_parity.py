@@ -205,9 +205,11 @@ def test_fused_ch_options_and_directions(prec):
                     g.upload(a2)
                     g.step("ch_dw", p, 11)
                     assert np.array_equal(g.download(), exp2), (shape2, tz, rq)
-        # wave strips (60 columns; odd chunks march downward; chunks of 2, 3, 8
-        # rows and one chunk of the whole grid)
-        for chunk in (2, 3, 8, 64):
+        # wave strips (60 columns; odd chunks march downward): the eight-row
+        # unrolled march ends at every ring position (chunks of 2, 3, 5, 7, 8,
+        # 13 and 16 rows, 16 being the FP64 default above 2^21 nodes) and one
+        # chunk takes the whole grid
+        for chunk in (2, 3, 5, 7, 8, 13, 16, 64):
             with mx.Grid(shape2, precision=prec) as g:
                 g.set_variant(2)
                 g.set_tile(1, 60, chunk)
@@ -575,6 +577,53 @@ def test_multiphase_wave_strips(prec):
         g.set_variant(0)  # the default falls back to the generic kernel
         g.step("gg_aniso", p, 2)
         assert np.array_equal(g.download(), O.gg_aniso(a, p, 2, form=prec))
+
+
+def test_graph_replay_follows_new_parameters():
+    """Runs of 100 steps replay a captured graph whose launches bake the
+    model's constants in; a second run with other constants on the same grid
+    must capture again. Ostwald ripening and multi-phase grain growth (their
+    constants live outside the Allen-Cahn/Cahn-Hilliard fields), and Allen-Cahn
+    with another K: set A for 100 steps, then set B for 100, bitwise against
+    the oracle."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(os.path.dirname(__file__), "golden",
+                                                                              "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    ow_a = [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]
+    ow_b = [0.002, 0.5, 2.0, 0.1, 0.9, 1.5, 0.5, 2.0, 1.0]
+    ga_a = [0.01, 8.0, 1.0, 1.0]
+    ga_b = [0.02, 6.0, 0.5, 2.0]
+    ac_a = [1.0, 1.0, 1.0, 1.0, 0.01]
+    ac_b = [1.0, 1.0, 1.6, 1.0, 0.01]
+    for prec in ("f64", "f32"):
+        cases = [("ostwald", (40, 36), 3, ow_a, ow_b, None), ("ostwald", (10, 9, 12), 3, ow_a, ow_b, None),
+                 ("gg_aniso", (40, 70), 4, ga_a, ga_b, "grains"), ("gg_aniso", (8, 10, 12), 4, ga_a, ga_b, "grains"),
+                 ("ac", (64, 64), 0, ac_a, ac_b, None)]
+        for model, shape, q, pa, pb, kind in cases:
+            say("[gpu] graph replay with new parameters %s %s %s" % (model, shape, prec))
+            n = int(np.prod(shape))
+            if kind == "grains":
+                a = mg.one_hot_grains(shape, q, 950 + q)
+            elif q:
+                a = O.uniform(n * q, 960 + q, 0.0, 1.0).reshape(shape + (q,))
+            else:
+                a = O.uniform(n, 970, -0.01, 0.01).reshape(shape)
+            if prec == "f32":
+                a = a.astype(np.float32)
+            kw = dict(fields=q, vector=True) if q else {}
+            with mx.Grid(shape, precision=prec, **kw) as g:
+                g.upload(a)
+                g.step(model, pa, 100)
+                mid = g.download()
+                g.step(model, pb, 100)
+                got = g.download()
+            exp_mid = oracle_run(model, a, pa, 100, prec)
+            exp = oracle_run(model, exp_mid, pb, 100, prec)
+            assert np.array_equal(mid, exp_mid), (model, shape, prec, "set A")
+            assert np.array_equal(got, exp), (model, shape, prec, "set B")
 
 
 def test_zero_steps_leave_the_state():
