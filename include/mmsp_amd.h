@@ -156,6 +156,29 @@ int mmspx_download(mmspx_state* s, void* host, int layout);
 int mmspx_step(mmspx_state* s, int model, const double* params, int nparams, int steps);
 int mmspx_sync(mmspx_state* s);
 
+/* Explicit steps on caller-owned device buffers, allocating nothing.
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. The same kernels as mmspx_step with its default choice
+ * (no tuned tile, no captured graph), enqueued on `stream` (a hipStream_t;
+ * NULL: the null stream); the call returns without waiting.
+ *   buf0 holds the state in the device layout (scalar grids: the nodes in the
+ *   reference's order; vector grids: one contiguous plane per field), buf1 is
+ *   a second buffer of the same size; both belong to the caller, on the
+ *   current device. On return *result_buf (0 or 1) names the buffer that will
+ *   hold the state after `steps` steps; the other holds intermediate data.
+ *   first_step numbers the first step (the stochastic forms' noise counter).
+ * The two-pass Cahn-Hilliard and Ostwald kernels (non-periodic Cahn-Hilliard,
+ * 1-D and 3-D Ostwald ripening ...) keep one scalar plane (nodes x element
+ * size) in a fixed device array of MMSPX_NOALLOC_SCRATCH_BYTES that each GPU
+ * holds until the process ends; larger planes are refused (MMSPX_EINVAL).
+ * Calls on one GPU share that array: calls whose work overlaps on one GPU
+ * race. The first call on each device looks the array's address up (steps = 0
+ * is enough); make that call before capturing later ones into a graph. */
+#define MMSPX_NOALLOC_SCRATCH_BYTES (32u << 20)
+int mmspx_step_noalloc(const mmspx_desc* desc, void* buf0, void* buf1, int model, const double* params,
+                       int nparams, int steps, int64_t first_step, void* stream, int* result_buf);
+
 /* Same as mmspx_step, bracketed by HIP events on the state's stream; waits and
  * returns the elapsed seconds and the kernel launches enqueued. */
 int mmspx_step_timed(mmspx_state* s, int model, const double* params, int nparams, int steps,

@@ -98,6 +98,8 @@ def _load():
         "mmspx_upload": (C.c_int, [vp, vp, C.c_int]),
         "mmspx_download": (C.c_int, [vp, vp, C.c_int]),
         "mmspx_step": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int]),
+        "mmspx_step_noalloc": (C.c_int, [P(Desc), vp, vp, C.c_int, dp, C.c_int, C.c_int, C.c_int64, vp,
+                                         P(C.c_int)]),
         "mmspx_sync": (C.c_int, [vp]),
         "mmspx_step_timed": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, dp, i64p]),
         "mmspx_reduce": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, dp, C.c_int]),
@@ -201,6 +203,22 @@ def fill_uniform(n, seed, lo, hi):
 def footprint(shape, model, fields=1, vector=False, precision="f64"):
     d = make_desc(shape, fields, vector, precision)
     return lib.mmspx_footprint(C.byref(d), _model(model))
+
+
+NOALLOC_SCRATCH_BYTES = 32 << 20  # MMSPX_NOALLOC_SCRATCH_BYTES
+
+
+def step_noalloc(desc: Desc, buf0: int, buf1: int, model, params, steps: int, first_step: int = 0,
+                 stream: int = 0) -> int:
+    """mmspx_step_noalloc on caller-owned device buffers (addresses as ints,
+    the device layout); enqueued on `stream` (a hipStream_t as an int, 0: the
+    null stream). Returns 0 or 1: the buffer holding the state after `steps`."""
+    m = _model(model)
+    p = _params(m, params)
+    res = C.c_int(-1)
+    check(lib.mmspx_step_noalloc(C.byref(desc), C.c_void_p(buf0), C.c_void_p(buf1), m, _dp(p), len(p), int(steps),
+                                 int(first_step), C.c_void_p(stream), C.byref(res)))
+    return res.value
 
 
 class Grid:
