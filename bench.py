@@ -262,7 +262,7 @@ def bench_batch_ac(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": value * 8 / 1e9 / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles("ac_batch_batch_ac"),
+            "traffic": traffic_from_profiles("ac_batch_batch_ac", tile),
             "kernel": "ac_batch",
             "algorithmic_bytes_per_launch": 8 * cells,
         },
@@ -394,8 +394,13 @@ def serve_trace(n, seed=20260106, lam=200.0):
 
 
 def bench_serve(args):
-    """Served Cahn-Hilliard stream through the continuous-batching server (FP32
-    state, FP64 mass check); latency is completion minus submission."""
+    """Served Cahn-Hilliard stream through the continuous-batching server: FP32
+    state with FP64 mass checks, requests of more than --fp64-above steps run
+    in FP64 in the same rounds; latency is completion minus submission. Then
+    every request runs alone, one after another, on the plain FP64
+    single-grid path from the start the server used (float-rounded for the
+    FP32 requests): that measures the FP64 one-after-another time and each
+    served result's error against the FP64 form."""
     import threading
 
     import mmsp_amd as mx
@@ -405,7 +410,9 @@ def bench_serve(args):
     params = [1.0, 1.0, 0.005]
     inits = [mx.fill_uniform(int(np.prod(r["shape"])), r["seed"], 0.45, 0.55).reshape(r["shape"]) for r in trace]
     cells = sum(int(np.prod(r["shape"])) * r["steps"] for r in trace)
-    sv = mx.Server("ch_dw", "f32", hbm_budget=args.hbm_budget_gb << 30, max_slots=512, chunk_steps=100)
+    thr = args.fp64_above
+    sv = mx.Server("ch_dw", "f32", hbm_budget=args.hbm_budget_gb << 30, max_slots=512, chunk_steps=100,
+                   fp64_above_steps=thr)
     ids = [None] * n
     late = [0.0] * n
 
@@ -423,29 +430,32 @@ def bench_serve(args):
     th.join()
     infos = [sv.wait(i, 600) for i in ids]
     st = sv.stats()
-    # every 10th request: its served FP32 result against the same solve run
-    # alone on the plain FP64 single-grid path, which also times that path
-    sample = [k for k in range(n) if k % 10 == 0 and infos[k]["status"] == mx.DONE]
-    served = {k: sv.fetch(ids[k]) for k in sample}
-    sv.close()
-    done = [i for i in infos if i["status"] == mx.DONE]
-    lat = np.array([i["t_done"] - i["t_submit"] for i in done])
-    span = max(i["t_done"] for i in done) - min(i["t_submit"] for i in infos)
-    mass = [abs(i["mass1"] - i["mass0"]) / abs(i["mass0"]) for i in done]
+    done = [k for k in range(n) if infos[k]["status"] == mx.DONE]
+    lat = np.array([infos[k]["t_done"] - infos[k]["t_submit"] for k in done])
+    span = max(infos[k]["t_done"] for k in done) - min(i["t_submit"] for i in infos)
     value = len(done) / span
     cu = cells / span
-    errs, seq_s, seq_cells = [], 0.0, 0
-    for k in sample:
+    alg_bytes = sum(int(np.prod(trace[k]["shape"])) * trace[k]["steps"] * (16 if infos[k]["precision"] == "f64" else 8)
+                    for k in done)
+    # every request alone on the FP64 single-grid path, one after another
+    errs32, errs64, mass32, mass64 = [], [], [], []
+    seq_s = 0.0
+    for k in done:
         r = trace[k]
+        served = sv.fetch(ids[k])
+        f64 = infos[k]["precision"] == "f64"
+        start = inits[k] if f64 else inits[k].astype(np.float32).astype(np.float64)
         with mx.Grid(r["shape"], precision="f64") as g:
-            g.upload(inits[k].astype(np.float32).astype(np.float64))  # the served start, rounded to float
+            g.upload(start)
             sec, _ = g.step_timed("ch_dw", params, r["steps"])
             r64 = g.download()
         seq_s += sec
-        seq_cells += int(np.prod(r["shape"])) * r["steps"]
-        errs.append((r["steps"], float(np.abs(served[k] - r64).max() / np.abs(r64).max())))
-    seq_est = seq_s * cells / seq_cells if seq_cells else None
-    short = [e for st_, e in errs if st_ <= 1000]
+        e = float(np.abs(served - r64).max() / np.abs(r64).max())
+        m = abs(infos[k]["mass1"] - infos[k]["mass0"]) / abs(infos[k]["mass0"])
+        (errs64 if f64 else errs32).append((r["steps"], e))
+        (mass64 if f64 else mass32).append(m)
+    sv.close()
+    all_err = [e for _, e in errs32 + errs64]
     return {
         "metric": "solves/s",
         "value": value,
@@ -457,7 +467,7 @@ def bench_serve(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": "served-cahn-hilliard-stream",
@@ -465,20 +475,26 @@ def bench_serve(args):
             "arrival_rate_per_s": 200.0,
             "completed": len(done),
             "rejected": st["rejected"],
+            "fp64_above_steps": thr,
+            "requests_fp32": len(errs32),
+            "requests_fp64": len(errs64),
             "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
             "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_latency_ms": float(lat.mean() * 1e3),
             "makespan_s": span,
             "cell_updates_per_s": cu,
             "total_cell_updates": cells,
-            "max_rel_mass_drift_fp32": float(max(mass)),
-            "fp64_check_sample": len(sample),
-            "max_normwise_err_vs_fp64_sample": max(e for _, e in errs) if errs else None,
-            "median_normwise_err_vs_fp64_sample": float(np.median([e for _, e in errs])) if errs else None,
-            "max_normwise_err_vs_fp64_sample_le_1000_steps": max(short) if short else None,
-            "normwise_err_vs_fp64_by_steps": sorted(errs),
-            "fp64_sequential_s_estimated": seq_est,
-            "speedup_vs_fp64_sequential_estimated": seq_est / span if seq_est else None,
+            "max_normwise_err_vs_fp64": max(all_err) if all_err else None,
+            "max_normwise_err_vs_fp64_fp32_requests": max(e for _, e in errs32) if errs32 else None,
+            "max_normwise_err_vs_fp64_fp64_requests": max(e for _, e in errs64) if errs64 else None,
+            "normwise_err_vs_fp64_fp32_requests_by_steps": sorted(errs32)[-10:],
+            "max_rel_mass_drift_fp32_requests": float(max(mass32)) if mass32 else None,
+            "max_rel_mass_drift_fp64_requests": float(max(mass64)) if mass64 else None,
+            "fp64_sequential_s": seq_s,
+            "fp64_sequential_solves_per_s": len(done) / seq_s if seq_s else None,
+            "speedup_vs_fp64_sequential": seq_s / span if seq_s else None,
+            "fp64_sequential_note": "device time of every request run alone on the FP64 single-grid path, "
+                                    "one after another (uploads and downloads excluded)",
             "peak_concurrent": st["peak_concurrent"],
             "peak_bytes": st["peak_bytes"],
             "rounds": st["rounds"],
@@ -488,23 +504,29 @@ def bench_serve(args):
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": cu * 8 / 1e9,
+            "achieved": alg_bytes / span / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": cu * 8 / 1e9 / HBM_PEAK_GBS,
+            "frac": alg_bytes / span / 1e9 / HBM_PEAK_GBS,
             "traffic": None,
             "kernel": "ch_batch",
+            "note": "algorithmic bytes (8 per FP32 and 16 per FP64 cell-update) over the makespan",
         },
     }
 
 
-def traffic_from_profiles(kernel_tag):
-    """HBM bytes per launch from committed rocprofv3 --pmc summaries, if any."""
+def traffic_from_profiles(kernel_tag, tile=None):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary of this
+    kernel and workload, or None when there is none or it was measured on
+    another tile than `tile` (the one the timed run used)."""
     p = os.path.join(ROOT, "profiles", kernel_tag + ".traffic.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f).get("traffic_bytes_per_launch")
-    return None
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    if tile is not None and d.get("tile") != tile:
+        return None
+    return d.get("traffic_bytes_per_launch")
 
 
 def main():
@@ -517,6 +539,8 @@ def main():
     ap.add_argument("--ckpt-threads", type=int, default=16, help="ckpt: host threads deflating a checkpoint")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests in the trace")
     ap.add_argument("--hbm-budget-gb", type=int, default=128, help="serve: device memory the server may hold")
+    ap.add_argument("--fp64-above", type=int, default=1000,
+                    help="serve: requests of more steps run in FP64 (0: every request in FP32)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph-steps", type=int, default=100, help="batch_ac: steps per captured graph (0: direct launches)")
     ap.add_argument("--no-sequential", action="store_true",
@@ -659,7 +683,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(kernel_tag),
+            "traffic": traffic_from_profiles(kernel_tag, tile),
             "kernel": kernel_tag,
             "kernel_mean_us": mean_launch_s * 1e6,
             "kernel_isolated_mean_us": mean_s / lps * 1e6,

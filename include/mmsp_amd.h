@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MMSPX_ABI_VERSION 1
+#define MMSPX_ABI_VERSION 2
 
 /* ---- error codes ---- */
 enum {
@@ -312,6 +312,10 @@ typedef struct mmspx_server_config {
 	                        least remaining work (nodes x steps left), as many as fill the GPU */
 	int model;           /* MMSPX_CH_DW, MMSPX_CH or MMSPX_AC */
 	int precision;       /* MMSPX_F32 or MMSPX_F64 */
+	int fp64_above_steps; /* MMSPX_F32 servers: requests of more steps run in MMSPX_F64 (0: every request in
+	                         `precision`). The FP32 error of a Cahn-Hilliard solve against the FP64 form grows
+	                         with its length (the spinodal instability amplifies the rounding of the stored
+	                         state); 1000 keeps the served results of the benchmark stream within 1e-5 */
 } mmspx_server_config;
 
 typedef struct mmspx_request_info {
@@ -322,6 +326,7 @@ typedef struct mmspx_request_info {
 	double t_admit;
 	double t_done;
 	double mass0, mass1;  /* FP64 compensated mass at admission and at completion */
+	int precision;        /* the form the request runs in (MMSPX_F32 or MMSPX_F64) */
 } mmspx_request_info;
 
 int mmspx_server_create(const mmspx_server_config* cfg, mmspx_server** out);
@@ -340,7 +345,8 @@ int mmspx_server_fetch(mmspx_server* sv, int64_t id, double* out, size_t count);
 int mmspx_server_info(mmspx_server* sv, int64_t id, mmspx_request_info* info);
 double mmspx_server_clock(mmspx_server* sv);
 /* Counters: [0] admitted, [1] completed, [2] rejected, [3] rounds, [4] peak bytes,
- * [5] peak requests resident at once, [6] kernel launches. */
+ * [5] peak requests resident at once, [6] kernel launches, [7] requests run in
+ * MMSPX_F64 by an MMSPX_F32 server (fp64_above_steps). */
 int mmspx_server_stats(mmspx_server* sv, int64_t* out, int nout);
 
 #ifdef __cplusplus

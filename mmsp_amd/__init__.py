@@ -62,6 +62,7 @@ class ServerConfig(C.Structure):
         ("chunk_steps", C.c_int),
         ("model", C.c_int),
         ("precision", C.c_int),
+        ("fp64_above_steps", C.c_int),
     ]
 
 
@@ -75,6 +76,7 @@ class RequestInfo(C.Structure):
         ("t_done", C.c_double),
         ("mass0", C.c_double),
         ("mass1", C.c_double),
+        ("precision", C.c_int),
     ]
 
 
@@ -474,13 +476,16 @@ QUEUED, RUNNING, DONE, REJECTED, CANCELLED = 0, 1, 2, 3, 4
 class Server:
     """Continuous-batching solve server (a worker thread in the library owns the GPU)."""
 
-    def __init__(self, model="ch_dw", precision="f32", hbm_budget=64 << 30, max_slots=256, chunk_steps=100):
+    def __init__(self, model="ch_dw", precision="f32", hbm_budget=64 << 30, max_slots=256, chunk_steps=100,
+                 fp64_above_steps=0):
+        """fp64_above_steps > 0 (FP32 servers): requests of more steps run in FP64."""
         cfg = ServerConfig()
         cfg.hbm_budget = int(hbm_budget)
         cfg.max_slots = int(max_slots)
         cfg.chunk_steps = int(chunk_steps)
         cfg.model = _model(model)
         cfg.precision = PRECISIONS[precision]
+        cfg.fp64_above_steps = int(fp64_above_steps)
         h = C.c_void_p()
         check(lib.mmspx_server_create(C.byref(cfg), C.byref(h)))
         self._h = h
@@ -513,7 +518,8 @@ class Server:
     @staticmethod
     def _info(i):
         return dict(id=i.id, status=i.status, steps_done=i.steps_done, t_submit=i.t_submit, t_admit=i.t_admit,
-                    t_done=i.t_done, mass0=i.mass0, mass1=i.mass1)
+                    t_done=i.t_done, mass0=i.mass0, mass1=i.mass1,
+                    precision="f64" if i.precision == F64 else "f32")
 
     def wait(self, rid, timeout=-1.0):
         i = RequestInfo()
@@ -538,9 +544,9 @@ class Server:
         return lib.mmspx_server_clock(self._h)
 
     def stats(self):
-        out = (C.c_int64 * 7)()
-        err = lib.mmspx_server_stats(self._h, out, 7)
-        keys = ["admitted", "completed", "rejected", "rounds", "peak_bytes", "peak_concurrent", "launches"]
+        out = (C.c_int64 * 8)()
+        err = lib.mmspx_server_stats(self._h, out, 8)
+        keys = ["admitted", "completed", "rejected", "rounds", "peak_bytes", "peak_concurrent", "launches", "fp64_routed"]
         d = dict(zip(keys, list(out)))
         d["worker_error"] = err
         return d

@@ -4,7 +4,11 @@
 // here the library serves a stream of requests with three threads, each with
 // its own stream:
 //   * submitted requests wait in a FIFO queue, their initial state already in
-//     the server's precision (the submitting thread converts it);
+//     the request's precision (the submitting thread converts it). A request
+//     runs in the server's precision, except that an FP32 server runs the
+//     requests of more than `fp64_above_steps` steps in FP64: the FP32
+//     trajectory of a spinodal decomposition drifts from the FP64 one as the
+//     instability amplifies the rounding of the stored state;
 //   * the worker admits queued requests while slots are free and their device
 //     footprint (two state buffers) fits the HBM budget; a request that could
 //     never fit is rejected;
@@ -51,6 +55,7 @@ struct Request {
 	mmspx_desc desc{};
 	Geom g{};
 	ModelParams p{};
+	int prec = MMSPX_F32;  // the form the request runs in
 	int steps = 0;
 	int done = 0;     // steps of rounds the device has finished
 	int planned = 0;  // steps queued
@@ -58,7 +63,7 @@ struct Request {
 	bool cancel = false;
 	double t_submit = 0, t_admit = 0, t_done = 0;
 	double mass0 = 0, mass1 = 0;
-	size_t bytes = 0;     // one state buffer (server precision, padded to 256 B)
+	size_t bytes = 0;     // one state buffer (request precision, padded to 256 B)
 	char* dev = nullptr;  // buf[0] at dev, buf[1] at dev + bytes
 	int idle_rounds = 0;             // rounds planned since it last advanced
 	hipEvent_t ev_last = nullptr;    // recorded after its last round (or when dropped)
@@ -112,7 +117,7 @@ struct mmspx_server {
 	int device = 0;
 	int cus = 256;
 	size_t in_use = 0;
-	int64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
+	int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	int worker_error = 0;
 	// 2-D members by 60-column wave strips of 8 rows, 3-D members by 16-plane chunks
 	BatchTile tile{kChWaveOut, 8, 16, 8, 0};
@@ -133,7 +138,7 @@ struct mmspx_server {
 	double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
-static size_t elem_of(const mmspx_server* sv) { return sv->cfg.precision == MMSPX_F64 ? 8 : 4; }
+static size_t elem_of(const Request* r) { return r->prec == MMSPX_F64 ? 8 : 4; }
 
 static void finish(mmspx_server* sv, Request* r, int status)
 {
@@ -152,7 +157,7 @@ static int upload_one(mmspx_server* sv, Request* r, std::vector<double>& part_ho
 	MMSPX_HIP(hipMallocFromPoolAsync((void**)&r->dev, 2 * r->bytes, sv->pool, sv->ust));
 	// the data goes to buf[0], the state after 0 steps
 	MMSPX_HIP(hipMemcpyAsync(r->dev, r->init.data(), r->init.size(), hipMemcpyHostToDevice, sv->ust));
-	int rc = launch_reduce(r->g, sv->cfg.precision, MMSPX_MASS, r->p, r->dev, sv->upart, sv->ust);
+	int rc = launch_reduce(r->g, r->prec, MMSPX_MASS, r->p, r->dev, sv->upart, sv->ust);
 	if (rc) return rc;
 	MMSPX_HIP(hipMemcpyAsync(part_host.data(), sv->upart, part_host.size() * sizeof(double), hipMemcpyDeviceToHost,
 	                         sv->ust));
@@ -201,11 +206,11 @@ static int download_one(mmspx_server* sv, Request* r, std::vector<double>& part_
 {
 	MMSPX_HIP(hipStreamWaitEvent(sv->dst, r->ev_last, 0));
 	const void* s = r->dev + ((r->steps & 1) ? r->bytes : 0);
-	int rc = launch_reduce(r->g, sv->cfg.precision, MMSPX_MASS, r->p, s, sv->dpart, sv->dst);
+	int rc = launch_reduce(r->g, r->prec, MMSPX_MASS, r->p, s, sv->dpart, sv->dst);
 	if (rc) return rc;
 	MMSPX_HIP(hipMemcpyAsync(part_host.data(), sv->dpart, part_host.size() * sizeof(double), hipMemcpyDeviceToHost,
 	                         sv->dst));
-	r->out.resize((size_t)r->g.nodes * elem_of(sv));
+	r->out.resize((size_t)r->g.nodes * elem_of(r));
 	MMSPX_HIP(hipMemcpyAsync(r->out.data(), s, r->out.size(), hipMemcpyDeviceToHost, sv->dst));
 	MMSPX_HIP(hipStreamSynchronize(sv->dst));
 	r->mass1 = sum_partials(part_host.data(), kRedBlocks);
@@ -313,10 +318,21 @@ static int enqueue_round(mmspx_server* sv)
 	sv->next_slot = (sv->next_slot + 1) % kSlots;
 	// the pinned mirrors are rewritten below: the copy that read them last must be done
 	if (sl.used) MMSPX_HIP(hipEventSynchronize(sl.done));
-	std::vector<BItem> l2, l3;
-	for (size_t m = 0; m < act.size(); m++) member_items((int)m, act[m]->g, sv->cfg.model, sv->tile, l2, l3);
-	const int n2 = (int)l2.size(), n3 = (int)l3.size();
-	int rc = ensure_slot(sl, (int)act.size(), n2 + n3);
+	// work lists per form: group 0 the server's precision, group 1 the FP64
+	// requests of an FP32 server; each group is one launch per step and kind
+	std::vector<BItem> l2[2], l3[2];
+	for (size_t m = 0; m < act.size(); m++) {
+		const int grp = act[m]->prec != sv->cfg.precision;
+		member_items((int)m, act[m]->g, sv->cfg.model, sv->tile, l2[grp], l3[grp]);
+	}
+	int n2[2], n3[2], off[2], total = 0;
+	for (int k = 0; k < 2; k++) {
+		n2[k] = (int)l2[k].size();
+		n3[k] = (int)l3[k].size();
+		off[k] = total;
+		total += n2[k] + n3[k];
+	}
+	int rc = ensure_slot(sl, (int)act.size(), total);
 	if (rc) return rc;
 	for (size_t m = 0; m < act.size(); m++) {
 		const Request* r = act[m];
@@ -327,16 +343,23 @@ static int enqueue_round(mmspx_server* sv)
 		bm.p = r->p;
 		bm.phase = (int)((r->planned - sv->gstep) & 1);
 	}
-	const std::vector<BItem> o2 = xcd_order(l2), o3 = xcd_order(l3);
-	std::copy(o2.begin(), o2.end(), sl.h_items);
-	std::copy(o3.begin(), o3.end(), sl.h_items + n2);
+	for (int k = 0; k < 2; k++) {
+		const std::vector<BItem> o2 = xcd_order(l2[k]), o3 = xcd_order(l3[k]);
+		std::copy(o2.begin(), o2.end(), sl.h_items + off[k]);
+		std::copy(o3.begin(), o3.end(), sl.h_items + off[k] + n2[k]);
+	}
 	MMSPX_HIP(hipMemcpyAsync(sl.d_mem, sl.h_mem, sizeof(BMember) * act.size(), hipMemcpyHostToDevice, sv->st));
-	MMSPX_HIP(hipMemcpyAsync(sl.d_items, sl.h_items, sizeof(BItem) * (size_t)(n2 + n3), hipMemcpyHostToDevice, sv->st));
+	MMSPX_HIP(hipMemcpyAsync(sl.d_items, sl.h_items, sizeof(BItem) * (size_t)total, hipMemcpyHostToDevice, sv->st));
 	int64_t launches = 0;
+	const int form[2] = {sv->cfg.precision, MMSPX_F64};
 	for (int k = 0; k < steps; k++) {
-		rc = launch_batch_step(sv->cfg.model, sv->cfg.precision, 1, sv->tile, sl.d_mem, sl.d_items, n2,
-		                       sl.d_items + n2, n3, (int)((sv->gstep + k) & 1), sv->cus, sv->st, &launches);
-		if (rc) return rc;
+		for (int grp = 0; grp < 2; grp++) {
+			if (n2[grp] + n3[grp] == 0) continue;
+			const BItem* it = sl.d_items + off[grp];
+			rc = launch_batch_step(sv->cfg.model, form[grp], 1, sv->tile, sl.d_mem, it, n2[grp], it + n2[grp], n3[grp],
+			                       (int)((sv->gstep + k) & 1), sv->cus, sv->st, &launches);
+			if (rc) return rc;
+		}
 	}
 	Round rd;
 	rd.steps = steps;
@@ -520,6 +543,7 @@ int mmspx_server_create(const mmspx_server_config* cfg, mmspx_server** out)
 		return fail(MMSPX_EINVAL, "the server runs Cahn-Hilliard or Allen-Cahn");
 	if (cfg->precision != MMSPX_F32 && cfg->precision != MMSPX_F64)
 		return fail(MMSPX_EINVAL, "the server runs MMSPX_F32 or MMSPX_F64");
+	if (cfg->fp64_above_steps < 0) return fail(MMSPX_EINVAL, "fp64_above_steps must be >= 0");
 	auto* sv = new (std::nothrow) mmspx_server();
 	if (!sv) return fail(MMSPX_ENOMEM, "host allocation failed");
 	sv->cfg = *cfg;
@@ -617,8 +641,11 @@ int mmspx_server_submit(mmspx_server* sv, const mmspx_desc* desc, const double* 
 	if (desc->vector) return fail(MMSPX_EINVAL, "the server takes scalar grids");
 	auto* r = new (std::nothrow) Request();
 	if (!r) return fail(MMSPX_ENOMEM, "host allocation failed");
+	r->prec = sv->cfg.precision;
+	if (sv->cfg.precision == MMSPX_F32 && sv->cfg.fp64_above_steps > 0 && steps > sv->cfg.fp64_above_steps)
+		r->prec = MMSPX_F64;
 	r->desc = *desc;
-	r->desc.precision = sv->cfg.precision;
+	r->desc.precision = r->prec;
 	r->g = make_geom(r->desc);
 	std::string why;
 	if (!batch_model_ok(sv->cfg.model, r->g, &why)) {
@@ -632,9 +659,9 @@ int mmspx_server_submit(mmspx_server* sv, const mmspx_desc* desc, const double* 
 	}
 	r->steps = steps;
 	const size_t n = (size_t)r->g.nodes;
-	r->bytes = (n * elem_of(sv) + 255) / 256 * 256;
-	r->init.resize(n * elem_of(sv));
-	if (sv->cfg.precision == MMSPX_F64) {
+	r->bytes = (n * elem_of(r) + 255) / 256 * 256;
+	r->init.resize(n * elem_of(r));
+	if (r->prec == MMSPX_F64) {
 		std::copy(init, init + n, (double*)r->init.data());
 	} else {
 		float* f = (float*)r->init.data();
@@ -647,6 +674,7 @@ int mmspx_server_submit(mmspx_server* sv, const mmspx_desc* desc, const double* 
 	}
 	r->id = sv->next_id++;
 	r->t_submit = sv->now();
+	if (r->prec != sv->cfg.precision) sv->stats[7] += 1;
 	sv->all[r->id] = r;
 	sv->queue.push_back(r);
 	*id = r->id;
@@ -664,6 +692,7 @@ static void fill_info(const Request* r, mmspx_request_info* info)
 	info->t_done = r->t_done;
 	info->mass0 = r->mass0;
 	info->mass1 = r->mass1;
+	info->precision = r->prec;
 }
 
 int mmspx_server_cancel(mmspx_server* sv, int64_t id)
@@ -717,7 +746,7 @@ int mmspx_server_fetch(mmspx_server* sv, int64_t id, double* out, size_t count)
 		sv->all.erase(it);
 	}
 	// the fetching thread converts to double
-	if (sv->cfg.precision == MMSPX_F64) {
+	if (r->prec == MMSPX_F64) {
 		const double* d = (const double*)r->out.data();
 		std::copy(d, d + count, out);
 	} else {
@@ -734,7 +763,7 @@ int mmspx_server_stats(mmspx_server* sv, int64_t* out, int nout)
 {
 	if (!sv || !out) return fail(MMSPX_EINVAL, "null argument");
 	std::lock_guard<std::mutex> lk(sv->mu);
-	for (int k = 0; k < nout && k < 7; k++) out[k] = sv->stats[k];
+	for (int k = 0; k < nout && k < 8; k++) out[k] = sv->stats[k];
 	return sv->worker_error;
 }
 

@@ -17,7 +17,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 30
     for n in names:
         assert hasattr(mx.lib, n), n
-    assert mx.lib.mmspx_abi_version() == 1
+    assert mx.lib.mmspx_abi_version() == 2
 
 
 def test_strerror_names_every_code():

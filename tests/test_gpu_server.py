@@ -63,6 +63,56 @@ def test_mixed_stream_alone_vs_in_mix():
         assert np.array_equal(sv.fetch(i), mix[2])
 
 
+def test_fp32_server_runs_long_requests_in_fp64():
+    """An FP32 server with fp64_above_steps = 60 runs the longer requests in
+    FP64, in the same rounds as the FP32 ones: each result is bitwise the
+    oracle's run of it in its form (and the single-grid path's); the FP64 ones
+    conserve mass to 1e-12."""
+    specs = [((64, 64), 40, 21), ((24, 20, 28), 75, 22), ((100, 36), 130, 23), ((16, 16, 16), 50, 24),
+             ((33, 40, 18), 61, 25), ((30, 70), 60, 26), ((20, 18, 300), 90, 27)]
+    say("[gpu] server fp32 with fp64 routing")
+    with mx.Server("ch_dw", "f32", max_slots=8, chunk_steps=16, fp64_above_steps=60) as sv:
+        ids = []
+        for k, (shape, steps, seed) in enumerate(specs):
+            ids.append(sv.submit(req(shape, seed), P_DW, steps))
+            time.sleep(0.002 * (k % 2))
+        infos = [sv.wait(i, 120) for i in ids]
+        outs = [sv.fetch(i) for i in ids]
+        st = sv.stats()
+    assert st["completed"] == len(specs) and st["worker_error"] == 0
+    assert st["fp64_routed"] == sum(1 for _, steps, _ in specs if steps > 60)
+    for k, (shape, steps, seed) in enumerate(specs):
+        long = steps > 60
+        assert infos[k]["precision"] == ("f64" if long else "f32"), k
+        if long:
+            exp = O.ch(req(shape, seed), P_DW, steps, mu=O.MU_DW)
+            assert abs(infos[k]["mass1"] - infos[k]["mass0"]) <= 1e-12 * abs(infos[k]["mass0"])
+        else:
+            exp = oracle_f32(req(shape, seed), P_DW, steps)
+        assert np.array_equal(outs[k], exp), (k, shape, steps)
+
+
+def test_fp32_results_at_the_routing_threshold_hold_1e5():
+    """At the benchmark stream's threshold (1000 steps), FP32-served results
+    stay within 1e-5 normwise of the FP64 form from the same float-rounded
+    start (2-D 512^2 and 3-D 96^3)."""
+    say("[gpu] server fp32 error at 1000 steps")
+    specs = [((512, 512), 1000, 31), ((96, 96, 96), 1000, 32)]
+    with mx.Server("ch_dw", "f32", chunk_steps=100, fp64_above_steps=1000) as sv:
+        ids = [sv.submit(req(shape, seed), P_DW, steps) for shape, steps, seed in specs]
+        infos = [sv.wait(i, 120) for i in ids]
+        outs = [sv.fetch(i) for i in ids]
+    for k, (shape, steps, seed) in enumerate(specs):
+        assert infos[k]["precision"] == "f32"
+        with mx.Grid(shape, precision="f64") as g:
+            g.upload(req(shape, seed).astype(np.float32).astype(np.float64))
+            g.step("ch_dw", P_DW, steps)
+            r64 = g.download()
+        err = np.abs(outs[k] - r64).max() / np.abs(r64).max()
+        say("[gpu] %s %d steps: fp32 err vs fp64 %.3e" % (shape, steps, err))
+        assert err <= 1e-5  # the FP32 bar for served solves
+
+
 def test_fp64_server_conserves_mass():
     say("[gpu] server fp64 mass")
     with mx.Server("ch_dw", "f64", chunk_steps=50) as sv:
