@@ -1,7 +1,7 @@
 """Benchmark of the explicit update path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload ch3d|ch3d_f32|ac2d|gg2d|zener2d|ostwald2d|gganiso2d|model_b3d|batch_ac|serve|ckpt]
+                    [--workload ch3d|ch3d_f32|ac2d|gg2d|zener2d|ostwald2d|gganiso2d|model_b3d|batch_ac|serve|ckpt|dropin]
                     [--no-cpu-baseline] [--tune | --tile TY,TZ,CHUNK[,OPT]]
                     [--requests N] [--hbm-budget-gb G]
                     [--graph-steps G] [--no-sequential] [--ckpt-threads T]
@@ -12,6 +12,9 @@ the served stream. --graph-steps and --no-sequential shape batch_ac runs
 made for profiling (direct launches; no FP64 one-after-another comparison).
 ckpt runs the 256^3 CH loop with a .dat checkpoint every half of the steps,
 serial writer against overlapped checkpoints on --ckpt-threads threads.
+dropin runs the reference's own driver program with the GPU update() on the
+256^3 CH grid, --increment steps per update() call, end to end.
+serve routes requests of more than --fp64-above steps to the FP64 form.
 
 Default workload: 3-D Cahn-Hilliard, grid<3,double> 256^3 periodic,
 mu = 2c(1-c)(1-2c) - kappa*lap(c), dc/dt = M*lap(mu), kappa = M = dx = 1,
@@ -314,16 +317,16 @@ def bench_ckpt(args):
                 final = g.download()
             sizes = [os.path.getsize(f) for f in files]
             out[mode] = dict(wall_s=wall, write_s=write_s, file_bytes=sizes)
-        # both loops wrote the same states
-        from oracle import dat
+        # both loops wrote the same states (read back by the library's own reader)
+        def read_back(name):
+            with mx.Grid.from_dat(os.path.join(td, name)) as gg:
+                return gg.download()
 
         for a, b in zip(sorted(f for f in os.listdir(td) if f.startswith("serial")),
                         sorted(f for f in os.listdir(td) if f.startswith("overlapped"))):
-            same = np.array_equal(dat.read_dat(os.path.join(td, a)).data, dat.read_dat(os.path.join(td, b)).data)
-            if not same:
+            if not np.array_equal(read_back(a), read_back(b)):
                 raise RuntimeError("checkpoint %s differs from %s" % (b, a))
-        last = dat.read_dat(os.path.join(td, sorted(os.listdir(td))[-1])).data
-        if not np.array_equal(last, final):
+        if not np.array_equal(read_back(sorted(os.listdir(td))[-1]), final):
             raise RuntimeError("the last checkpoint is not the final state")
     payload = nodes * 8
     ser, ovl = out["serial"], out["overlapped"]
@@ -369,6 +372,89 @@ def bench_ckpt(args):
             "sample": "the serial writer (compress2 level 9 of the whole 128 MiB payload, the reference's "
                       "output()), %d checkpoints of the same run" % len(ser["write_s"]),
         },
+    }
+
+
+def bench_dropin(args):
+    """The reference's driver loop end to end on the 256^3 FP64 Cahn-Hilliard
+    grid: the reference's explicit Cahn-Hilliard program with its update()
+    replaced by include/mmsp_gpu.hpp (mmsp_amd/dropin/cahn_hilliard_dw_gpu.cpp,
+    the reference's main(), reader and output() compiled from its headers with
+    the RAW switch), run as `prog start.dat 2000 100`: 20 update(grid, 100)
+    calls, each followed by output() of a raw .dat. Wall clock of the whole
+    process; the drop-in's own counters split it."""
+    import re
+    import subprocess
+
+    import mmsp_amd as mx
+
+    w = WORKLOADS["ch3d"]
+    K = args.steps if args.steps is not None else w["steps"]
+    inc = args.increment
+    shape = w["shape"]
+    nodes = int(np.prod(shape))
+    prog = os.path.join(ROOT, "mmsp_amd", "dropin", "bin", "cahn-hilliard-dw-gpu-raw")
+    if not os.path.exists(prog):
+        raise SystemExit("bench.py: %s is not built (make -C mmsp_amd/dropin REF=<mmsp checkout>)" % prog)
+    init = mx.fill_uniform(nodes, w["seed"], w["lo"], w["hi"]).reshape(shape)
+    with mx.Grid(shape) as g:  # the expected final state, in-process
+        g.upload(init)
+        g.step(w["model"], w["params"], K)
+        want = g.digest()
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR")) as td:
+        mx.write_dat_host(os.path.join(td, "start.dat"), init, shape, compress=False)
+        env = dict(os.environ, MMSP_GPU_STATS="1")
+        t0 = time.perf_counter()
+        r = subprocess.run([prog, "start.dat", str(K), str(inc)], cwd=td, env=env, capture_output=True, text=True,
+                           timeout=900)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise RuntimeError("drop-in program failed: %s" % r.stderr[-2000:])
+        m = re.search(r"update\(\) calls (\d+), steps (\d+), uploads (\d+), downloads (\d+); seconds: steps "
+                      r"([0-9.]+), uploads ([0-9.]+), downloads ([0-9.]+), digests ([0-9.]+)", r.stderr)
+        if not m:
+            raise RuntimeError("no drop-in counters in: %s" % r.stderr[-2000:])
+        calls, steps_run, uploads, downloads = (int(x) for x in m.groups()[:4])
+        step_s, up_s, down_s, dig_s = (float(x) for x in m.groups()[4:])
+        files = sorted(f for f in os.listdir(td) if f != "start.dat")
+        with mx.Grid.from_dat(os.path.join(td, files[-1])) as g:
+            got = g.digest()
+        if got != want:
+            raise RuntimeError("the drop-in's last checkpoint differs from the in-process run")
+    value = nodes * K / wall
+    return {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": 0,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "cahn-hilliard-3d-dropin-driver-loop",
+            "grid": list(shape),
+            "increment": inc,
+            "update_calls": calls,
+            "uploads": uploads,
+            "downloads": downloads,
+            "checkpoints": len(files),
+            "wall_s": wall,
+            "update_steps_s": step_s,
+            "upload_s": up_s,
+            "download_s": down_s,
+            "digest_s": dig_s,
+            "rest_s": wall - step_s - up_s - down_s - dig_s,
+            "rest_note": "process start, the reference's reader and its output() of raw .dat files",
+            "device_only_cell_updates_per_s": nodes * K / step_s,
+            "final_checkpoint_equals_in_process_run": True,
+        },
+        "roofline": None,
+        "cpu_baseline": cpu_baseline(w, init) if not args.no_cpu_baseline else None,
     }
 
 
@@ -535,7 +621,9 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=100,
                     help="untimed steps first (100 or more also builds the captured graph that small grids replay)")
-    ap.add_argument("--workload", default="ch3d", choices=sorted(WORKLOADS) + ["batch_ac", "serve", "ckpt"])
+    ap.add_argument("--workload", default="ch3d",
+                    choices=sorted(WORKLOADS) + ["batch_ac", "serve", "ckpt", "dropin"])
+    ap.add_argument("--increment", type=int, default=100, help="dropin: steps per update() call")
     ap.add_argument("--ckpt-threads", type=int, default=16, help="ckpt: host threads deflating a checkpoint")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests in the trace")
     ap.add_argument("--hbm-budget-gb", type=int, default=128, help="serve: device memory the server may hold")
@@ -563,6 +651,9 @@ def main():
         return
     if args.workload == "ckpt":
         print(json.dumps(bench_ckpt(args)), flush=True)
+        return
+    if args.workload == "dropin":
+        print(json.dumps(bench_dropin(args)), flush=True)
         return
     w = WORKLOADS[args.workload]
     K = args.steps if args.steps is not None else w["steps"]

@@ -188,6 +188,17 @@ int mmspx_step_timed(mmspx_state* s, int model, const double* params, int nparam
  * mmspx_step (ENERGY uses the model's gradient coefficient and bulk terms). */
 int mmspx_reduce(mmspx_state* s, int what, int model, const double* params, int nparams, double* out, int nout);
 
+/* A 64-bit digest of the state, in the order of the node-major host array
+ * that mmspx_download(s, host, MMSPX_NODE_MAJOR) writes: the sum, modulo 2^64,
+ * over every element of a mix of its bits and its index (node * fields +
+ * field). mmspx_host_digest gives the same value for a host array (elem_bytes
+ * 8 for double, 4 for float), so a caller can tell whether a host copy still
+ * holds what the device holds without uploading it. No counterpart in the
+ * reference: include/mmsp_gpu.hpp uses it to keep a grid's device state
+ * across update() calls (MMSP.main.hpp:304-325). */
+int mmspx_digest(mmspx_state* s, uint64_t* out);
+int mmspx_host_digest(const void* node_major, int64_t elements, int elem_bytes, uint64_t* out);
+
 /* .dat output/input in the reference's serial layout. compress=1 writes a
  * zlib level-9 block (the reference's default), 0 a raw block (readable by the
  * reference: MMSP.grid.cpp:1306-1330). */
@@ -245,7 +256,10 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
  * 1 x 64 strips: 2 selects the wave strips (one 64-column strip per wave,
  * neighbours across lanes by DPP, no LDS); 2-D grain growth and Zener pinning
  * with 8, 16 or 32 fields: 4 and 5 select the split strips (two or four
- * waves per strip, each holding its share of the fields). */
+ * waves per strip, each holding its share of the fields). 3-D periodic
+ * Cahn-Hilliard (deterministic chemical potentials): 6 and 7 take two steps
+ * per pass over HBM (6 on 32 x 64 tiles, 7 on 32 x 32 tiles; an odd last
+ * step runs body 2). */
 int mmspx_set_tile_options(mmspx_state* s, int body);
 /* Returns variant + 16 * tile body (1: register queues, or the 2-D
  * grain-growth tile; 2: wave strips, or 3-D register queues on 768 threads;
@@ -258,8 +272,10 @@ int mmspx_get_tile(const mmspx_state* s, int* ty, int* tz, int* chunk);
  * writes a one-line-per-candidate report. */
 int mmspx_autotune(mmspx_state* s, int model, const double* params, int nparams, int trial_steps, char* report,
                    size_t report_len);
-/* Per-step device time: `reps` steps, each bracketed by its own HIP event
- * pair on the state's stream; mean and minimum seconds per step. */
+/* Per-step device time: `reps` launches of the current kernel choice (one
+ * step, or the steps one launch of a multi-step body advances), each bracketed
+ * by its own HIP event pair on the state's stream; mean and minimum seconds
+ * per step. */
 int mmspx_kernel_time(mmspx_state* s, int model, const double* params, int nparams, int reps, double* mean_s,
                       double* min_s);
 /* Device-to-device copy kernel over `bytes` (read + write), best of `reps`;

@@ -20,14 +20,27 @@
 //
 // Link with -lmmsp_amd (include/mmsp_amd.h is the C ABI underneath).
 //
-// What a call does: the grid's node data goes to the device once, all `steps`
-// explicit steps run there (the MMSP::laplacian, check_boundary, swap and
-// ghostswap of the loop become device work), and the result comes back into
-// the same grid, so the caller sees the same grid the reference's loop leaves
-// (the reference swaps a locally constructed grid into the caller's,
-// MMSP.grid.cpp:1913-1991; values are what matter to callers). The progress
-// bar of MMSP::print_progress (MMSP.utility.cpp:138-178) is printed at the
-// same steps as the reference prints it. grid<dim,double> runs the FP64 form
+// What a call does: all `steps` explicit steps run on the device (the
+// MMSP::laplacian, check_boundary, swap and ghostswap of the loop become
+// device work), and the result comes back into the same grid, so the caller
+// sees the same grid the reference's loop leaves (the reference swaps a
+// locally constructed grid into the caller's, MMSP.grid.cpp:1913-1991; values
+// are what matter to callers). The device state stays resident between calls:
+// MMSP.main.hpp:304-325 calls update(grid, increment) in a loop with only
+// output() in between, so each grid (keyed by the grid object, its node data
+// and its shape) keeps one library state, and a call uploads the host data
+// only when it no longer matches what the device holds: a 64-bit digest of the
+// device state is taken before each download (mmspx_digest) and compared with
+// the host data's (mmspx_host_digest) at the next call. A grid the caller
+// changed in between is therefore uploaded again; an unchanged one is not.
+// mmsp_gpu::release() frees the resident states (otherwise they live until
+// the process ends). The progress bar of MMSP::print_progress
+// (MMSP.utility.cpp:138-178) is printed with the reference's marks; the steps
+// between marks are batched into runs of at least 100 steps (a mark is
+// printed once the device has passed it), so small grids still replay the
+// library's captured graphs. MMSP_GPU_STATS=1 in the environment prints the
+// calls, uploads, downloads and their seconds to stderr at exit.
+// grid<dim,double> runs the FP64 form
 // (bitwise the reference's arithmetic except the cube, see the library);
 // grid<dim,float> and vector<float> run the reference's mixed float/double
 // semantics unless set_native_float(true) selects pure float arithmetic.
@@ -44,8 +57,10 @@
 #error "mmsp_gpu.hpp drives one serial grid per process; MPI builds of MMSP are not supported"
 #endif
 
+#include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -108,21 +123,148 @@ template <int dim, typename G> mmspx_desc describe(const G& grid, int fields, in
 	return d;
 }
 
-// Runs `steps` on the state, printing the reference's progress bar at the
-// steps where MMSP::print_progress prints (step 0, every steps/20, the last).
+// ---- resident device states -------------------------------------------------
+struct resident {
+	const void* grid = nullptr;  // the grid object
+	const void* data = nullptr;  // its node data (scalar grids: &grid(0))
+	mmspx_desc desc{};
+	mmspx_state* s = nullptr;
+	uint64_t digest = 0;  // of the device state when it was last downloaded
+	bool has_digest = false;
+};
+
+struct counters {
+	long calls = 0, uploads = 0, downloads = 0, steps = 0;
+	double upload_s = 0, download_s = 0, step_s = 0, digest_s = 0;
+};
+
+inline counters& stats()
+{
+	static counters c;
+	return c;
+}
+
+inline void print_stats()
+{
+	const counters& c = stats();
+	std::fprintf(stderr,
+	             "mmsp_gpu: update() calls %ld, steps %ld, uploads %ld, downloads %ld; seconds: steps %.6f, "
+	             "uploads %.6f, downloads %.6f, digests %.6f\n",
+	             c.calls, c.steps, c.uploads, c.downloads, c.step_s, c.upload_s, c.download_s, c.digest_s);
+}
+
+inline double now_s()
+{
+	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+inline std::vector<resident>& residents()
+{
+	static std::vector<resident> v;
+	static bool hooked = false;
+	if (!hooked) {
+		hooked = true;
+		const char* e = std::getenv("MMSP_GPU_STATS");
+		if (e && *e && std::strcmp(e, "0") != 0) std::atexit(print_stats);
+	}
+	return v;
+}
+
+// Frees every resident device state (the next update() uploads again).
+inline void release()
+{
+	for (resident& r : residents())
+		if (r.s) mmspx_destroy(r.s);
+	residents().clear();
+}
+
+inline bool same_desc(const mmspx_desc& a, const mmspx_desc& b) { return std::memcmp(&a, &b, sizeof a) == 0; }
+
+// The resident state of this grid, created (or created again when the grid's
+// data or shape changed); at most eight grids stay resident, the oldest leaves.
+inline resident& resident_for(const void* grid, const void* data, const mmspx_desc& d)
+{
+	const size_t kMaxResident = 8;
+	std::vector<resident>& v = residents();
+	for (size_t k = 0; k < v.size(); k++) {
+		if (v[k].grid != grid) continue;
+		if (v[k].data == data && same_desc(v[k].desc, d)) return v[k];
+		mmspx_destroy(v[k].s);
+		v.erase(v.begin() + (long)k);
+		break;
+	}
+	if (v.size() >= kMaxResident) {
+		mmspx_destroy(v.front().s);
+		v.erase(v.begin());
+	}
+	resident r;
+	r.grid = grid;
+	r.data = data;
+	r.desc = d;
+	check(mmspx_create(&d, &r.s), "mmspx_create");
+	v.push_back(r);
+	return v.back();
+}
+
+// Uploads `host` (node-major) unless the device already holds it.
+inline void upload_if_changed(resident& r, const void* host, int64_t elements, int elem_bytes)
+{
+	if (r.has_digest) {
+		const double t0 = now_s();
+		uint64_t h = 0;
+		check(mmspx_host_digest(host, elements, elem_bytes, &h), "mmspx_host_digest");
+		stats().digest_s += now_s() - t0;
+		if (h == r.digest) return;
+	}
+	const double t0 = now_s();
+	check(mmspx_upload(r.s, host, MMSPX_NODE_MAJOR), "mmspx_upload");
+	stats().upload_s += now_s() - t0;
+	stats().uploads += 1;
+	r.has_digest = false;
+}
+
+// Downloads the state into `host` (node-major), keeping its digest.
+inline void download_keep_digest(resident& r, void* host)
+{
+	double t0 = now_s();
+	check(mmspx_digest(r.s, &r.digest), "mmspx_digest");
+	stats().digest_s += now_s() - t0;
+	t0 = now_s();
+	check(mmspx_download(r.s, host, MMSPX_NODE_MAJOR), "mmspx_download");
+	stats().download_s += now_s() - t0;
+	stats().downloads += 1;
+	r.has_digest = true;
+}
+
+// Runs `steps` on the state with the reference's progress bar: print_progress
+// marks step 0, every step with (20 * step) % steps == 0 and the last step,
+// each printed once that many steps are done. The device advances in runs of
+// at least 100 steps between host synchronisations.
 inline void run_with_progress(mmspx_state* s, int model, const double* p, int np, int steps)
 {
+	const int kMinRun = 100;  // two of the library's captured 50-step graphs
+	const double t0 = now_s();
 	int done = 0;
+	std::vector<int> pending;
+	auto advance = [&](int to) {
+		if (to > done) {
+			check(mmspx_step(s, model, p, np, to - done), "mmspx_step");
+			check(mmspx_sync(s), "mmspx_sync");
+			done = to;
+		}
+		size_t k = 0;
+		for (; k < pending.size() && pending[k] <= done; k++) MMSP::print_progress(pending[k], steps);
+		pending.erase(pending.begin(), pending.begin() + (long)k);
+	};
 	for (int step = 0; step < steps; step++) {
 		const bool prints = step == 0 || step == steps - 1 || (20 * step) % steps == 0;
 		if (!prints) continue;
-		check(mmspx_step(s, model, p, np, step - done), "mmspx_step");
-		check(mmspx_sync(s), "mmspx_sync");
-		done = step;
-		MMSP::print_progress(step, steps);
+		pending.push_back(step);
+		if (step == 0 || step - done >= kMinRun) advance(step);
 	}
-	check(mmspx_step(s, model, p, np, steps - done), "mmspx_step");
-	check(mmspx_sync(s), "mmspx_sync");
+	advance(steps);
+	stats().step_s += now_s() - t0;
+	stats().steps += steps;
 }
 
 // Scalar grids: grid<dim,double> / grid<dim,float>; in a serial build the
@@ -130,14 +272,13 @@ inline void run_with_progress(mmspx_state* s, int model, const double* p, int np
 template <int dim, typename T> void run_scalar(MMSP::grid<dim, T>& grid, int model, const double* p, int np, int steps)
 {
 	if (nodes(grid) == 0 || steps <= 0) return;
+	stats().calls += 1;
 	mmspx_desc d = describe<dim>(grid, fields(grid), 0, precision_of<T>::get());
-	mmspx_state* s = nullptr;
-	check(mmspx_create(&d, &s), "mmspx_create");
 	T* data = &grid(0);
-	check(mmspx_upload(s, data, MMSPX_NODE_MAJOR), "mmspx_upload");
-	run_with_progress(s, model, p, np, steps);
-	check(mmspx_download(s, data, MMSPX_NODE_MAJOR), "mmspx_download");
-	mmspx_destroy(s);
+	resident& r = resident_for(&grid, data, d);
+	upload_if_changed(r, data, nodes(grid), (int)sizeof(T));
+	run_with_progress(r.s, model, p, np, steps);
+	download_keep_digest(r, data);
 }
 
 // Allen-Cahn, allen-cahn.cpp:47-75: new = phi - dt*M*(-r*phi + u*phi^3 - K*lap(phi)).
@@ -216,6 +357,7 @@ void run_vector(MMSP::grid<dim, MMSP::vector<T> >& grid, int model, const double
 	const int n = nodes(grid);
 	const int q = fields(grid);
 	if (n == 0 || steps <= 0) return;
+	stats().calls += 1;
 	mmspx_desc d = describe<dim>(grid, q, 1, precision_of<T>::get());
 	std::vector<T> host((size_t)n * q);
 	for (int i = 0; i < n; i++) {
@@ -226,12 +368,11 @@ void run_vector(MMSP::grid<dim, MMSP::vector<T> >& grid, int model, const double
 		}
 		for (int j = 0; j < q; j++) host[(size_t)i * q + j] = v[j];
 	}
-	mmspx_state* s = nullptr;
-	check(mmspx_create(&d, &s), "mmspx_create");
-	check(mmspx_upload(s, host.data(), MMSPX_NODE_MAJOR), "mmspx_upload");
-	run_with_progress(s, model, p, np, steps);
-	check(mmspx_download(s, host.data(), MMSPX_NODE_MAJOR), "mmspx_download");
-	mmspx_destroy(s);
+	// the per-node vectors live on the heap: the grid object keys the state
+	resident& r = resident_for(&grid, &grid, d);
+	upload_if_changed(r, host.data(), (int64_t)n * q, (int)sizeof(T));
+	run_with_progress(r.s, model, p, np, steps);
+	download_keep_digest(r, host.data());
 	for (int i = 0; i < n; i++) {
 		MMSP::vector<T>& v = grid(i);
 		for (int j = 0; j < q; j++) v[j] = host[(size_t)i * q + j];

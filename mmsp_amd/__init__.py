@@ -106,6 +106,8 @@ def _load():
         "mmspx_step_timed": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, dp, i64p]),
         "mmspx_reduce": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, dp, C.c_int]),
         "mmspx_write_dat": (C.c_int, [vp, C.c_char_p, C.c_int]),
+        "mmspx_digest": (C.c_int, [vp, P(C.c_uint64)]),
+        "mmspx_host_digest": (C.c_int, [vp, C.c_int64, C.c_int, P(C.c_uint64)]),
         "mmspx_read_dat": (C.c_int, [C.c_char_p, C.c_int, P(vp)]),
         "mmspx_peek_dat": (C.c_int, [C.c_char_p, P(Desc)]),
         "mmspx_checkpoint_begin": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, P(vp)]),
@@ -200,6 +202,16 @@ def fill_uniform(n, seed, lo, hi):
     out = np.empty(int(n), dtype=np.float64)
     lib.mmspx_fill_uniform(_dp(out), int(n), seed, lo, hi)
     return out
+
+
+def host_digest(a) -> int:
+    """The state digest of a host array (node-major, float64 or float32)."""
+    a = np.ascontiguousarray(a)
+    if a.dtype not in (np.float64, np.float32):
+        raise ValueError("float64 or float32 arrays only")
+    out = C.c_uint64()
+    check(lib.mmspx_host_digest(a.ctypes.data_as(C.c_void_p), a.size, a.itemsize, C.byref(out)))
+    return out.value
 
 
 def footprint(shape, model, fields=1, vector=False, precision="f64"):
@@ -328,6 +340,12 @@ class Grid:
         mean, mn = C.c_double(), C.c_double()
         check(lib.mmspx_kernel_time(self._h, m, _dp(p), p.size, int(reps), C.byref(mean), C.byref(mn)))
         return mean.value, mn.value
+
+    def digest(self):
+        """64-bit digest of the state (equals host_digest of download())."""
+        out = C.c_uint64()
+        check(lib.mmspx_digest(self._h, C.byref(out)))
+        return out.value
 
     def mass(self):
         out = np.zeros(self.fields, dtype=np.float64)

@@ -459,6 +459,9 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 	if (s->geom.dim == 2 && ty > 1 && tz == kTbCols) opt = 3;
 	else if (s->geom.dim == 2 && ty > 1) opt = 1;
 	else if (s->geom.dim == 2 && s->tuned && s->tile.ty > 1) opt = 0;
+	// 3-D two-step bodies: 32 x 32 tiles run body 7 only; body 6 runs 32 x 64
+	if (s->geom.dim == 3 && ty == 32 && tz == 32) opt = 7;
+	else if (s->geom.dim == 3 && s->tuned && s->tile.opt == 6 && ty == 32 && tz == 64) opt = 6;
 	TuneChoice t{2, ty, tz, chunk, opt};
 	if (!tile_known(s->geom, t)) return fail(MMSPX_EINVAL, "no compiled tile of that shape for this grid");
 	s->tile = t;
@@ -469,7 +472,8 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 int mmspx_set_tile_options(mmspx_state* s, int body)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
-	if (body < 0 || body > 5 || (body == 3 && s->geom.dim != 3) || (body >= 4 && s->geom.dim != 2))
+	if (body < 0 || body > 7 || (body == 3 && s->geom.dim != 3) || ((body == 4 || body == 5) && s->geom.dim != 2) ||
+	    (body >= 6 && s->geom.dim != 3))
 		return fail(MMSPX_EINVAL, "unknown tile body");
 	if (!s->tuned) {
 		s->tile = default_choice(s->geom, s->desc.precision, MMSPX_CH_DW);
@@ -489,10 +493,21 @@ int mmspx_get_tile(const mmspx_state* s, int* ty, int* tz, int* chunk)
 	return t.variant + 16 * t.opt;
 }
 
+// Steps one launch of the state's current kernel choice advances (the
+// temporal-block bodies take several steps per launch).
+static int steps_per_launch_now(const mmspx_state* s, int model)
+{
+	if (s->variant == 1 || !fused_supported(s->geom, s->desc.precision, model)) return 1;
+	TuneChoice tc = s->tuned ? s->tile : default_choice(s->geom, s->desc.precision, model);
+	if (s->variant == 2 && tc.variant != 2) tc = default_choice(s->geom, s->desc.precision, model);
+	return tc.variant == 2 ? fused_steps_per_launch(model, tc) : 1;
+}
+
 int mmspx_kernel_time(mmspx_state* s, int model, const double* params, int np, int reps, double* mean_s,
                       double* min_s)
 {
 	if (!s || reps < 1 || reps > 100000) return fail(MMSPX_EINVAL, "bad arguments");
+	const int spl = steps_per_launch_now(s, model);
 	std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
 	int rc = 0;
 	for (auto& e : ev) {
@@ -503,7 +518,7 @@ int mmspx_kernel_time(mmspx_state* s, int model, const double* params, int np, i
 	}
 	for (int k = 0; k < reps && rc == 0; k++) {
 		if (hipEventRecord(ev[2 * k], s->stream) != hipSuccess) rc = fail(MMSPX_EDEVICE, "hipEventRecord failed");
-		if (rc == 0) rc = step_impl(s, model, params, np, 1);
+		if (rc == 0) rc = step_impl(s, model, params, np, spl);
 		if (rc == 0 && hipEventRecord(ev[2 * k + 1], s->stream) != hipSuccess)
 			rc = fail(MMSPX_EDEVICE, "hipEventRecord failed");
 	}
@@ -521,8 +536,8 @@ int mmspx_kernel_time(mmspx_state* s, int model, const double* params, int np, i
 	for (auto& e : ev)
 		if (e) (void)hipEventDestroy(e);
 	if (rc) return rc;
-	if (mean_s) *mean_s = sum / reps;
-	if (min_s) *min_s = mn;
+	if (mean_s) *mean_s = sum / reps / spl;
+	if (min_s) *min_s = mn / spl;
 	return 0;
 }
 
@@ -555,7 +570,7 @@ int mmspx_autotune(mmspx_state* s, int model, const double* params, int np, int 
 			s->tuned = true;
 		}
 		double mean = 0.0, mn = 0.0;
-		rc = step_impl(s, model, params, np, 2);  // warm-up (and lazy buffers)
+		rc = step_impl(s, model, params, np, 4);  // warm-up (and lazy buffers)
 		if (rc == 0) rc = mmspx_kernel_time(s, model, params, np, trial_steps, &mean, &mn);
 		if (rc == 0 && hipMemcpyAsync(s->cur, saved, s->bytes, hipMemcpyDeviceToDevice, s->stream) != hipSuccess)
 			rc = fail(MMSPX_EDEVICE, "restore copy failed");
@@ -665,6 +680,29 @@ int mmspx_reduce(mmspx_state* s, int what, int model, const double* params, int 
 	MMSPX_HIP(hipStreamSynchronize(s->stream));
 	for (int j = 0; j < sets; j++) out[j] = sum_partials(s->part_host + (size_t)j * 2 * kRedBlocks, kRedBlocks);
 	if (what == MMSPX_ENERGY) out[0] *= s->geom.vol;
+	return 0;
+}
+
+int mmspx_digest(mmspx_state* s, uint64_t* out)
+{
+	if (!s || !out) return fail(MMSPX_EINVAL, "null argument");
+	uint64_t* part = (uint64_t*)s->part;  // kRedBlocks words: the partials' buffer holds 2 * kRedBlocks doubles
+	int rc = launch_digest(s->geom, s->desc.precision, s->cur, part, s->stream);
+	if (rc) return rc;
+	MMSPX_HIP(hipMemcpyAsync(s->part_host, part, kRedBlocks * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+	MMSPX_HIP(hipStreamSynchronize(s->stream));
+	const uint64_t* h = (const uint64_t*)s->part_host;
+	uint64_t total = 0;
+	for (int k = 0; k < kRedBlocks; k++) total += h[k];
+	*out = total;
+	return 0;
+}
+
+int mmspx_host_digest(const void* node_major, int64_t elements, int elem_bytes, uint64_t* out)
+{
+	if (!out || elements < 0 || (elements > 0 && !node_major) || (elem_bytes != 4 && elem_bytes != 8))
+		return fail(MMSPX_EINVAL, "bad arguments");
+	*out = host_digest(node_major, elements, elem_bytes);
 	return 0;
 }
 

@@ -155,5 +155,9 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 
 // host-side compensated sum of the (sum, comp) partial pairs
 double sum_partials(const double* part, int count);
+// state digests (reduce.hip): per-block partial sums of the device state, and
+// the same digest of a node-major host array
+int launch_digest(const Geom& g, int prec, const void* a, uint64_t* part, hipStream_t st);
+uint64_t host_digest(const void* data, int64_t count, int elem_bytes);
 
 }  // namespace mmspx

@@ -49,6 +49,23 @@ __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __r
 		ch_tile<PREC, MU, DIM, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
 }
 
+// Two steps per pass (3-D, periodic, deterministic chemical potentials).
+template <int PREC, int MU, int TY, int TZ, int NT, bool PHI2>
+__global__ void __launch_bounds__(NT) ch_fused2(const typename Form<PREC>::S* __restrict__ a,
+                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
+                                                int chunk_len, int tiles_y, int tiles_z, int ntiles)
+{
+	const int tile = xcd_tile(ntiles);
+	if (tile < 0) return;
+	const int tz = tile % tiles_z;
+	const int t2 = tile / tiles_z;
+	const int ty = t2 % tiles_y;
+	const int chunk = t2 / tiles_y;
+	const int xb = chunk * chunk_len;
+	const int xe = min(xb + chunk_len, g.n[0]);
+	ch_tile_rq2<PREC, MU, TY, TZ, NT, PHI2>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
+}
+
 template <int PREC, int TZ>
 __global__ void __launch_bounds__(TZ) ac_tiled(const typename Form<PREC>::S* __restrict__ a,
                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
@@ -279,7 +296,7 @@ bool tile_known(const Geom& g, const TuneChoice& t)
 	if (t.chunk < 2 || t.chunk > 4096) return false;
 	if (g.dim == 3)
 		return (t.ty == 8 && t.tz == 64) || (t.ty == 16 && t.tz == 64) || (t.ty == 32 && t.tz == 64) ||
-		       (t.ty == 8 && t.tz == 32) || (t.ty == 4 && t.tz == 128);
+		       (t.ty == 8 && t.tz == 32) || (t.ty == 4 && t.tz == 128) || (t.ty == 32 && t.tz == 32);
 	if (g.dim == 2)
 		return (t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256 || t.tz == kChWaveOut)) ||
 		       ((t.ty == 8 || t.ty == 16 || t.ty == 32) && t.tz == kTbCols && t.chunk <= kTbMaxSteps) ||
@@ -321,6 +338,12 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		for (int c : {12, 16, 22, 32})
 			for (int body = 1; body <= 3; body++) add(32, 64, c, body);
 		for (int c : {20, 22, 24}) add(16, 64, c, 1);
+		// two steps per pass (deterministic chemical potentials)
+		if (model == MMSPX_CH || model == MMSPX_CH_DW)
+			for (int c : {16, 32, 64}) {
+				add(32, 64, c, 6);
+				add(32, 32, c, 7);
+			}
 	} else if (g.dim == 2) {
 		// LDS-row strips (Cahn-Hilliard: both tile bodies)
 		for (int tz : {64, 128, 256})
@@ -368,6 +391,21 @@ static int launch_ch_body(const Geom& g, const ModelParams& p, const void* a, vo
 	return 0;
 }
 
+template <int PREC, int MU, int TY, int TZ, int NT, bool PHI2>
+static int launch_ch2_body(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	using S = typename Form<PREC>::S;
+	const int tiles_y = (g.n[1] + TY - 1) / TY;
+	const int tiles_z = (g.n[2] + TZ - 1) / TZ;
+	const int chunks = (g.n[0] + chunk - 1) / chunk;
+	const int ntiles = tiles_y * tiles_z * chunks;
+	const int blocks = (ntiles + 7) / 8 * 8;
+	hipLaunchKernelGGL((ch_fused2<PREC, MU, TY, TZ, NT, PHI2>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g, p,
+	                   chunk, tiles_y, tiles_z, ntiles);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
 // opt 1 selects the register-queue body (opt 2 and 3, register queues with
 // more threads per tile, are dispatched in launch_ch_prec)
 template <int PREC, int MU, int DIM, int TY, int TZ, int NT>
@@ -393,9 +431,29 @@ static int launch_ch_waves(const Geom& g, const ModelParams& p, const void* a, v
 
 template <int PREC, int MU>
 static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st,
-                          const TuneChoice& t)
+                          const TuneChoice& t, int nsteps)
 {
 	const int chunk = t.chunk > 0 ? t.chunk : 32;
+	if (g.dim == 3 && (t.opt == 6 || t.opt == 7)) {
+		// two steps per pass: body 6 on 32 x 64 tiles (896 threads; the c^n plane
+		// double-buffered in FP32, single in FP64 where the LDS would not hold
+		// two), body 7 on 32 x 32 tiles (768 threads, every plane
+		// double-buffered); an odd last step runs the one-step register queues
+		if constexpr (MU == 2) {
+			return fail(MMSPX_EUNSUPPORTED, "the two-step bodies take the deterministic chemical potentials");
+		} else {
+			if (nsteps == 1)
+				return launch_ch_body<PREC, MU, 3, 32, 64, 768, 1>(g, p, a, b, st, PREC == MMSPX_F64 ? 32 : 16);
+			if (nsteps != 2) return fail(MMSPX_EINVAL, "the two-step bodies take one or two steps per pass");
+			if (t.opt == 6) {
+				if (t.ty != 32 || t.tz != 64) return fail(MMSPX_EINVAL, "tile body 6 runs 32 x 64 tiles");
+				return launch_ch2_body<PREC, MU, 32, 64, 896, PREC != MMSPX_F64>(g, p, a, b, st, chunk);
+			}
+			if (t.ty != 32 || t.tz != 32) return fail(MMSPX_EINVAL, "tile body 7 runs 32 x 32 tiles");
+			return launch_ch2_body<PREC, MU, 32, 32, 768, true>(g, p, a, b, st, chunk);
+		}
+	}
+	if (g.dim == 3 && t.ty == 32 && t.tz == 32) return fail(MMSPX_EINVAL, "32 x 32 tiles run tile body 7");
 	if (g.dim == 3 && (t.opt == 2 || t.opt == 3)) {
 		// register queues on 32 x 64 tiles with 768 or 1024 threads (fewer own points each)
 		if (t.ty != 32 || t.tz != 64) return fail(MMSPX_EINVAL, "tile bodies 2 and 3 run 32 x 64 tiles");
@@ -629,9 +687,9 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 {
 	const int chunk = t.chunk > 0 ? t.chunk : 32;
 	switch (p.model) {
-	case MMSPX_CH: return launch_ch_prec<PREC, 0>(g, p, a, b, st, t);
-	case MMSPX_CH_DW: return launch_ch_prec<PREC, 1>(g, p, a, b, st, t);
-	case MMSPX_CH_NOISE: return launch_ch_prec<PREC, 2>(g, p, a, b, st, t);
+	case MMSPX_CH: return launch_ch_prec<PREC, 0>(g, p, a, b, st, t, nsteps);
+	case MMSPX_CH_DW: return launch_ch_prec<PREC, 1>(g, p, a, b, st, t, nsteps);
+	case MMSPX_CH_NOISE: return launch_ch_prec<PREC, 2>(g, p, a, b, st, t, nsteps);
 	case MMSPX_OSTWALD:
 		if (g.dim == 2 && t.ty == 1 && t.tz == kChWaveOut) {
 			if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "Ostwald wave strips need periodic boundaries");
@@ -678,6 +736,7 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 int fused_steps_per_launch(int model, const TuneChoice& t)
 {
 	if (model == MMSPX_AC && t.tz == kTbCols) return std::max(1, std::min(t.chunk, kTbMaxSteps));
+	if ((model == MMSPX_CH || model == MMSPX_CH_DW) && (t.opt == 6 || t.opt == 7)) return 2;
 	return 1;
 }
 

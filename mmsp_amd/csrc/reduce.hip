@@ -14,6 +14,10 @@
 //   CH_DW    : c^2 (1-c)^2 + kappa/2 |grad c|^2
 //   GG       : sum_j(-phi_j^2/2 + phi_j^4/4) + sum_{j<k} phi_j^2 phi_k^2 + 1 * sum_j |grad phi_j|^2
 //              (kappa = 2, the coefficient of lap in graingrowth.cpp:105)
+#include <algorithm>
+#include <thread>
+#include <vector>
+
 #include "device.hpp"
 
 namespace mmspx {
@@ -172,6 +176,88 @@ int launch_reduce(const Geom& g, int prec, int what, const ModelParams& p, const
 	}
 	if (f64) return launch_energy<double>(g, ep, (const double*)a, part, st);
 	return launch_energy<float>(g, ep, (const float*)a, part, st);
+}
+
+// ---- state digest ---------------------------------------------------------------
+// A 64-bit digest of a state in the host's node-major order: the sum, modulo
+// 2^64, over every element of mix(bits ^ mix(index * golden + c)), index =
+// node * fields + field, bits = the element's bit pattern (a float's 32 bits
+// zero-extended). A sum is independent of the order of its terms, so the
+// device (field-major) and the host (node-major, any number of threads) get
+// the same value; mix is the SplitMix64 finalizer.
+__host__ __device__ __forceinline__ uint64_t digest_mix(uint64_t z)
+{
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	return z ^ (z >> 31);
+}
+
+__host__ __device__ __forceinline__ uint64_t digest_term(uint64_t index, uint64_t bits)
+{
+	return digest_mix(bits ^ digest_mix(index * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull));
+}
+
+template <typename S, typename U>
+__global__ void __launch_bounds__(kRedThreads) digest_kernel(const S* __restrict__ a, int nodes, int fields,
+                                                             uint64_t* part)
+{
+	uint64_t acc = 0;
+	const int64_t total = (int64_t)nodes * fields;
+	for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+		const int64_t j = k / nodes, n = k - j * nodes;  // field-major device index
+		U u;
+		const S v = a[k];
+		memcpy(&u, &v, sizeof(U));
+		acc += digest_term((uint64_t)(n * fields + j), (uint64_t)u);
+	}
+	for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+	__shared__ uint64_t lw[kRedThreads / 64];
+	if ((threadIdx.x & 63) == 0) lw[threadIdx.x >> 6] = acc;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint64_t t = 0;
+		for (int w = 0; w < kRedThreads / 64; w++) t += lw[w];
+		part[blockIdx.x] = t;
+	}
+}
+
+int launch_digest(const Geom& g, int prec, const void* a, uint64_t* part, hipStream_t st)
+{
+	if (prec == MMSPX_F64)
+		hipLaunchKernelGGL((digest_kernel<double, uint64_t>), dim3(kRedBlocks), dim3(kRedThreads), 0, st,
+		                   (const double*)a, g.nodes, g.fields, part);
+	else
+		hipLaunchKernelGGL((digest_kernel<float, uint32_t>), dim3(kRedBlocks), dim3(kRedThreads), 0, st,
+		                   (const float*)a, g.nodes, g.fields, part);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+uint64_t host_digest(const void* data, int64_t count, int elem_bytes)
+{
+	const int64_t min_per_thread = (int64_t)1 << 20;
+	int threads = (int)std::min<int64_t>(16, std::max<int64_t>(1, count / min_per_thread));
+	threads = std::min<int>(threads, std::max(1u, std::thread::hardware_concurrency()));
+	std::vector<uint64_t> sums((size_t)threads, 0);
+	auto work = [&](int t) {
+		const int64_t b = count * t / threads, e = count * (t + 1) / threads;
+		uint64_t acc = 0;
+		if (elem_bytes == 8) {
+			const uint64_t* w = (const uint64_t*)data;
+			for (int64_t i = b; i < e; i++) acc += digest_term((uint64_t)i, w[i]);
+		} else {
+			const uint32_t* w = (const uint32_t*)data;
+			for (int64_t i = b; i < e; i++) acc += digest_term((uint64_t)i, (uint64_t)w[i]);
+		}
+		sums[(size_t)t] = acc;
+	};
+	std::vector<std::thread> pool;
+	for (int t = 1; t < threads; t++) pool.emplace_back(work, t);
+	work(0);
+	for (auto& th : pool) th.join();
+	uint64_t total = 0;
+	for (uint64_t v : sums) total += v;
+	return total;
 }
 
 double sum_partials(const double* part, int count)

@@ -324,6 +324,235 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 	}
 }
 
+// ---- 3-D Cahn-Hilliard, two steps per pass (temporal blocks along the march) --
+// Periodic grids. One pass over a (TY x TZ) tile and planes [xb, xe) advances
+// two explicit steps: c^{n+2} = c^{n+1} + dt*M*lap(mu^{n+1}), c^{n+1} = c^n +
+// dt*M*lap(mu^n), mu^k = mu(c^k) - kappa*lap(c^k) (cahn-hilliard.cpp:69-76,
+// twice). c^{n+2} on the tile needs mu^{n+1} on a 1-cell in-plane halo, c^{n+1}
+// on 2, mu^n on 3 and c^n on 4, each on the axis-0 planes either side. Each
+// thread owns fixed points of the mu^n region (the tile and a 3-cell halo); one
+// iteration of the march, at plane P, computes
+//   mu^n(P)          at the own points                (region R3)
+//   c^{n+1}(P-1)     at the own points in R2 (2-cell halo)
+//   mu^{n+1}(P-2)    at the own points in R1 (1-cell halo)
+//   c^{n+2}(P-3)     at the own points of the tile, written to HBM.
+// Axial neighbours come from per-point register queues (c^n at P-1, P, P+1 and
+// P+2 in flight; mu^n, c^{n+1}, mu^{n+1} at their three planes); in-plane
+// neighbours through LDS planes. The planes of mu^n, c^{n+1} and mu^{n+1} are
+// double-buffered: an iteration reads the planes the previous one wrote, so
+// one barrier per iteration orders everything; the c^n plane is read in the
+// iteration that writes it, so it is double-buffered too (PHI2) or takes a
+// second barrier. Every value is the one the one-step kernels compute for that
+// node and step (the halo recomputes what the neighbouring tile computes), so
+// the result is bitwise two single steps; c^{n+1} never goes to HBM.
+template <int PREC, int MU, int TY, int TZ, int NT, bool PHI2>
+__device__ __forceinline__ void ch_tile_rq2(const typename Form<PREC>::S* __restrict__ a,
+                                            typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                            int y0, int z0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	static_assert(MU == 0 || MU == 1, "the two-step pass takes the deterministic chemical potentials");
+	constexpr int P4Y = TY + 8, P4Z = TZ + 8;  // c^n plane tile (4-cell halo)
+	constexpr int R3Y = TY + 6, R3Z = TZ + 6;  // mu^n: the threads' own points
+	constexpr int R2Y = TY + 4, R2Z = TZ + 4;  // c^{n+1}
+	constexpr int R1Y = TY + 2, R1Z = TZ + 2;  // mu^{n+1}
+	constexpr int PN = P4Y * P4Z, N3 = R3Y * R3Z, N2 = R2Y * R2Z, N1 = R1Y * R1Z;
+	constexpr int RN = PN - N3;  // c^n tile points outside R3 (a one-cell ring)
+	constexpr int M = (N3 + NT - 1) / NT;
+	constexpr int MR = (RN + NT - 1) / NT;
+	constexpr int NPHI = PHI2 ? 2 : 1;
+	__shared__ S lphi[NPHI][PN];
+	__shared__ S lmu0[2][N3];
+	__shared__ S lc1[2][N2];
+	__shared__ S lmu1[2][N1];
+
+	const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+	const int64_t plane = (int64_t)ny * nz;
+
+	// own point k: R3 index e = tid + k*NT at (r, c) of R3, node (y0-3+r, z0-3+c);
+	// q4 its index in the c^n tile, q2 / q1 in R2 / R1 (valid where the flags say)
+	int ooff[M], q4[M], q2[M], q1[M], fl[M];  // fl: 1 in R3, 2 in R2, 4 in R1, 8 written
+#pragma unroll
+	for (int k = 0; k < M; k++) {
+		const int e = threadIdx.x + k * NT;
+		ooff[k] = q4[k] = q2[k] = q1[k] = fl[k] = 0;
+		if (e < N3) {
+			const int r = e / R3Z, c = e - (e / R3Z) * R3Z;
+			const int y = y0 - 3 + r, z = z0 - 3 + c;
+			ooff[k] = wrap(y, ny) * nz + wrap(z, nz);
+			q4[k] = (r + 1) * P4Z + (c + 1);
+			int f = 1;
+			if (r >= 1 && r < R2Y + 1 && c >= 1 && c < R2Z + 1) {
+				f |= 2;
+				q2[k] = (r - 1) * R2Z + (c - 1);
+			}
+			if (r >= 2 && r < R1Y + 2 && c >= 2 && c < R1Z + 2) {
+				f |= 4;
+				q1[k] = (r - 2) * R1Z + (c - 2);
+			}
+			if (r >= 3 && r < TY + 3 && c >= 3 && c < TZ + 3 && y < ny && z < nz) f |= 8;
+			fl[k] = f;
+		}
+	}
+	// ring points: rows 0 and P4Y-1 and columns 0 and P4Z-1 of the c^n tile
+	int roff[MR], rq[MR];
+#pragma unroll
+	for (int j = 0; j < MR; j++) {
+		const int t = threadIdx.x + j * NT;
+		roff[j] = rq[j] = 0;
+		if (t < RN) {
+			int pr, pc;
+			if (t < P4Z) {
+				pr = 0;
+				pc = t;
+			} else if (t < 2 * P4Z) {
+				pr = P4Y - 1;
+				pc = t - P4Z;
+			} else {
+				const int u = t - 2 * P4Z;
+				pr = 1 + u / 2;
+				pc = (u & 1) ? P4Z - 1 : 0;
+			}
+			roff[j] = wrap(y0 - 4 + pr, ny) * nz + wrap(z0 - 4 + pc, nz);
+			rq[j] = pr * P4Z + pc;
+		}
+	}
+	// unconditional loads (lanes past the region load offset 0; planes wrap)
+	auto load_own = [&](int x, S* v) {
+		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
+#pragma unroll
+		for (int k = 0; k < M; k++) v[k] = src[ooff[k]];
+	};
+	auto load_ring = [&](int x, S* v) {
+		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
+#pragma unroll
+		for (int j = 0; j < MR; j++) v[j] = src[roff[j]];
+	};
+
+	S f0[M], f1[M], f2[M], f3[M];   // c^n queue: roles rotate over the four unrolled iterations
+	S m0m[M], m0c[M], m0p[M];        // mu^n at P-2, P-1, P
+	S c1m[M], c1c[M], c1p[M];        // c^{n+1} at P-3, P-2, P-1
+	S m1m[M], m1c[M], m1p[M];        // mu^{n+1} at P-4, P-3, P-2
+	S r0[MR], r1[MR];
+#pragma unroll
+	for (int k = 0; k < M; k++) {
+		f3[k] = m0m[k] = m0c[k] = m0p[k] = 0;
+		c1m[k] = c1c[k] = c1p[k] = m1m[k] = m1c[k] = m1p[k] = 0;
+	}
+#pragma unroll
+	for (int j = 0; j < MR; j++) r1[j] = 0;
+	// the previous tile of a persistent loop may still read the planes
+	__syncthreads();
+	const int x0 = xb - 3;  // the first iteration: mu^n at xb-3 needs c^n from xb-4
+	load_own(x0 - 1, f0);
+	load_own(x0, f1);
+	load_own(x0 + 1, f2);
+	load_ring(x0, r0);
+
+	auto iter = [&](int P, const S* fm, const S* fc, const S* fp, S* fn, const S* rc, S* rn) {
+		load_own(P + 2, fn);
+		load_ring(P + 1, rn);
+		S* __restrict__ lp = lphi[PHI2 ? (P & 1) : 0];
+#pragma unroll
+		for (int k = 0; k < M; k++)
+			if (fl[k]) lp[q4[k]] = fc[k];
+#pragma unroll
+		for (int j = 0; j < MR; j++)
+			if (threadIdx.x + j * NT < RN) lp[rq[j]] = rc[j];
+		__syncthreads();
+		// mu^n(P) at every own point
+		S* __restrict__ w0 = lmu0[P & 1];
+#pragma unroll
+		for (int k = 0; k < M; k++) {
+			if (fl[k]) {
+				const int q = q4[k];
+				const S y = fc[k];
+				typename F::L lap = F::lap_zero();
+				F::lap_add(lap, g.w[0], g.wf[0], fp[k], y, fm[k]);
+				F::lap_add(lap, g.w[1], g.wf[1], lp[q + P4Z], y, lp[q - P4Z]);
+				F::lap_add(lap, g.w[2], g.wf[2], lp[q + 1], y, lp[q - 1]);
+				m0p[k] = mu_of<PREC, MU>(y, lap, p, 0.0f);
+				w0[threadIdx.x + k * NT] = m0p[k];
+			}
+		}
+		if (!PHI2) __syncthreads();  // the c^n plane is rewritten by the next iteration
+		// c^{n+1}(P-1) at the own points in R2: in-plane mu^n(P-1) from the
+		// plane the previous iteration wrote
+		if (P - 1 >= xb - 2) {
+			const S* __restrict__ r0p = lmu0[(P - 1) & 1];
+			S* __restrict__ w1 = lc1[(P - 1) & 1];
+#pragma unroll
+			for (int k = 0; k < M; k++) {
+				if (fl[k] & 2) {
+					const int e = threadIdx.x + k * NT;
+					const S m = m0c[k];
+					typename F::L lap = F::lap_zero();
+					F::lap_add(lap, g.w[0], g.wf[0], m0p[k], m, m0m[k]);
+					F::lap_add(lap, g.w[1], g.wf[1], r0p[e + R3Z], m, r0p[e - R3Z]);
+					F::lap_add(lap, g.w[2], g.wf[2], r0p[e + 1], m, r0p[e - 1]);
+					c1p[k] = F::ch(fm[k], lap, p);
+					w1[q2[k]] = c1p[k];
+				}
+			}
+		}
+		// mu^{n+1}(P-2) at the own points in R1: in-plane c^{n+1}(P-2)
+		if (P - 2 >= xb - 1) {
+			const S* __restrict__ r1p = lc1[(P - 2) & 1];
+			S* __restrict__ w2 = lmu1[(P - 2) & 1];
+#pragma unroll
+			for (int k = 0; k < M; k++) {
+				if (fl[k] & 4) {
+					const int q = q2[k];
+					const S y = c1c[k];
+					typename F::L lap = F::lap_zero();
+					F::lap_add(lap, g.w[0], g.wf[0], c1p[k], y, c1m[k]);
+					F::lap_add(lap, g.w[1], g.wf[1], r1p[q + R2Z], y, r1p[q - R2Z]);
+					F::lap_add(lap, g.w[2], g.wf[2], r1p[q + 1], y, r1p[q - 1]);
+					m1p[k] = mu_of<PREC, MU>(y, lap, p, 0.0f);
+					w2[q1[k]] = m1p[k];
+				}
+			}
+		}
+		// c^{n+2}(P-3) at the own points of the tile: in-plane mu^{n+1}(P-3)
+		if (P - 3 >= xb) {
+			const S* __restrict__ r2p = lmu1[(P - 3) & 1];
+			S* __restrict__ dst = b + (int64_t)(P - 3) * plane;
+#pragma unroll
+			for (int k = 0; k < M; k++) {
+				if (fl[k] & 8) {
+					const int q = q1[k];
+					const S m = m1c[k];
+					typename F::L lap = F::lap_zero();
+					F::lap_add(lap, g.w[0], g.wf[0], m1p[k], m, m1m[k]);
+					F::lap_add(lap, g.w[1], g.wf[1], r2p[q + R1Z], m, r2p[q - R1Z]);
+					F::lap_add(lap, g.w[2], g.wf[2], r2p[q + 1], m, r2p[q - 1]);
+					dst[ooff[k]] = F::ch(c1m[k], lap, p);
+				}
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < M; k++) {
+			m0m[k] = m0c[k];
+			m0c[k] = m0p[k];
+			c1m[k] = c1c[k];
+			c1c[k] = c1p[k];
+			m1m[k] = m1c[k];
+			m1c[k] = m1p[k];
+		}
+	};
+	const int xlast = xe + 2;  // c^{n+2} at xe-1 is written in the iteration at xe+2
+	for (int P = x0; P <= xlast; P += 4) {
+		iter(P, f0, f1, f2, f3, r0, r1);
+		if (P + 1 > xlast) break;
+		iter(P + 1, f1, f2, f3, f0, r1, r0);
+		if (P + 2 > xlast) break;
+		iter(P + 2, f2, f3, f0, f1, r0, r1);
+		if (P + 3 > xlast) break;
+		iter(P + 3, f3, f0, f1, f2, r1, r0);
+	}
+}
+
 // ---- 2-D Cahn-Hilliard, wave strips (periodic): one wave per (strip, chunk) --
 // Lane l holds column c0 - 2 + l: mu is formed on lanes 1..62 and c' written
 // on lanes 2..61 (kChWaveOut columns per strip); column neighbours come from

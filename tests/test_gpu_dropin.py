@@ -32,14 +32,22 @@ def say(*a):
     sys.stdout.flush()
 
 
+DROPIN_BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mmsp_amd", "dropin", "bin")
+
+
+def where(name):
+    """Stock reference programs in oracle/_ref, drop-in programs (-gpu) in mmsp_amd/dropin/bin."""
+    return os.path.join(DROPIN_BIN if "-gpu" in name else O.REF_DIR, name)
+
+
 def have(name):
-    return os.path.exists(os.path.join(O.REF_DIR, name))
+    return os.path.exists(where(name))
 
 
-def run(prog, cwd, args):
-    r = subprocess.run([os.path.join(O.REF_DIR, prog)] + args, cwd=cwd, capture_output=True, text=True, timeout=300)
+def run(prog, cwd, args, env=None, stderr=False):
+    r = subprocess.run([where(prog)] + args, cwd=cwd, capture_output=True, text=True, timeout=300, env=env)
     assert r.returncode == 0, (prog, r.stdout[-2000:], r.stderr[-2000:])
-    return r.stdout
+    return (r.stdout, r.stderr) if stderr else r.stdout
 
 
 def progress_shape(out):
@@ -149,3 +157,48 @@ def test_stochastic_dropin_programs_agree_in_distribution(prog, tmp_path):
         assert (a.dim, a.dx, a.g0) == (2, [1.0, 1.0], [0, 0])
         v = float((a.data ** 2).mean())
         assert abs(v / expect - 1) < 0.05, (dd, v, expect)
+
+
+def gpu_stats(err):
+    """The counters mmsp_gpu.hpp prints at exit under MMSP_GPU_STATS=1."""
+    m = re.search(r"update\(\) calls (\d+), steps (\d+), uploads (\d+), downloads (\d+)", err)
+    assert m, err[-2000:]
+    return dict(zip(("calls", "steps", "uploads", "downloads"), map(int, m.groups())))
+
+
+def test_driver_loop_keeps_the_state_resident(tmp_path):
+    """MMSP.main.hpp:304-325 calls update(grid, increment) in a loop with
+    output() in between: the drop-in uploads the grid once and keeps it on the
+    device (one download per call), and the checkpoints equal the oracle's
+    states bitwise. A grid the caller changes between calls is uploaded again
+    (in-process, through the library's digests)."""
+    prog = "cahn-hilliard-dw-gpu"
+    if not have(prog):
+        pytest.skip("drop-in programs not built here")
+    say("[gpu] drop-in driver loop, resident state")
+    shape, steps, inc = (24, 20, 32), 300, 50
+    c0 = O.uniform(int(np.prod(shape)), 41, 0.45, 0.55).reshape(shape)
+    mx.write_dat_host(str(tmp_path / "start.dat"), c0, shape)
+    env = dict(os.environ, MMSP_GPU_STATS="1")
+    out, err = run(prog, str(tmp_path), ["start.dat", str(steps), str(inc)], env=env, stderr=True)
+    st = gpu_stats(err)
+    assert st == dict(calls=steps // inc, steps=steps, uploads=1, downloads=steps // inc), st
+    files = sorted(p for p in os.listdir(tmp_path) if p != "start.dat")
+    assert len(files) == steps // inc
+    state = c0
+    for f in files:
+        state = O.ch(state, [1.0, 1.0, 0.005], inc, mu=O.MU_DW)
+        assert np.array_equal(dat.read_dat(str(tmp_path / f)).data, state), f
+    # the digests: equal for the state and its download, different after one changed bit
+    with mx.Grid(shape) as g:
+        g.upload(c0)
+        g.step("ch_dw", [1.0, 1.0, 0.005], 7)
+        h = g.digest()
+        x = g.download()
+        assert h == mx.host_digest(x)
+        x.reshape(-1)[1234] = np.nextafter(x.reshape(-1)[1234], 2.0)
+        assert mx.host_digest(x) != h
+    q = O.uniform(30 * 40 * 8, 42, 0.0, 1.0).reshape(30, 40, 8).astype(np.float32)
+    with mx.Grid((30, 40), fields=8, vector=True, precision="f32") as g:
+        g.upload(q)
+        assert g.digest() == mx.host_digest(q)  # node-major order for vector grids

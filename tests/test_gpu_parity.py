@@ -180,6 +180,34 @@ def test_fused_ch_options_and_directions(prec):
                     g.upload(arr)
                     g.step("ch_dw", p, 13)
                     assert np.array_equal(g.download(), ex), (arr.shape, body, chunk)
+    # two steps per pass (body 6 on 32 x 64 tiles, 7 on 32 x 32): 13 steps are
+    # six two-step passes and one single step; chunks of 2, 5, 7, 16 and 32
+    # planes, partial tiles, a grid smaller than one tile
+    for body, tile in ((6, (32, 64)), (7, (32, 32))):
+        for chunk in (2, 5, 7, 16, 32):
+            for arr, ex in ((a, exp13), (a_s, exp_s)):
+                with mx.Grid(arr.shape, precision=prec) as g:
+                    g.set_variant(2)
+                    g.set_tile(tile[0], tile[1], chunk)
+                    g.set_tile_options(body=body)
+                    g.upload(arr)
+                    g.step("ch_dw", p, 13)
+                    assert g.get_tile()["opt"] == body
+                    assert np.array_equal(g.download(), ex), (arr.shape, body, chunk)
+        # the polynomial mu and unequal spacing
+        pp = [1.0, 1.0, 1.0, 1.0, 0.002]
+        b0 = O.uniform(int(np.prod(shape)), 81, -1.0, 1.0).reshape(shape)
+        if prec == "f32":
+            b0 = b0.astype(np.float32)
+        dx = [1.0, 0.5, 2.0]
+        exp_b = O.ch(b0, pp, 10, mu=O.MU_POLY, form=prec, dx=dx)
+        with mx.Grid(shape, precision=prec, dx=dx) as g:
+            g.set_variant(2)
+            g.set_tile(tile[0], tile[1], 16)
+            g.set_tile_options(body=body)
+            g.upload(b0)
+            g.step("ch", pp, 10)
+            assert np.array_equal(g.download(), exp_b), (body, "poly")
     with mx.Grid(shape, precision=prec) as g:  # the polynomial mu form too
         g.set_variant(2)
         g.set_tile(32, 64, 16)
@@ -224,6 +252,20 @@ def test_fused_ch_options_and_directions(prec):
         g.step("ch_dw", p, 7)
         g.step("ch_dw", p, 100)  # graph replay starting at odd parity
         assert np.array_equal(g.download(), exp)
+    with mx.Grid(shape, precision=prec) as g:  # two-step passes in a captured graph
+        g.set_variant(2)
+        g.set_tile(32, 32, 16)
+        g.set_tile_options(body=7)
+        g.upload(a)
+        g.step("ch_dw", p, 7)
+        g.step("ch_dw", p, 100)
+        assert np.array_equal(g.download(), exp)
+    with mx.Grid((8, 8, 8), precision=prec) as g:  # noise: the two-step bodies refuse it
+        g.set_variant(2)
+        g.set_tile(32, 32, 16)
+        g.set_tile_options(body=7)
+        with pytest.raises(mx.MmspxError):
+            g.step("ch_noise", [1.0, 1.0, 1.0, 1.0, 0.005, 0.3, 5], 2)
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
