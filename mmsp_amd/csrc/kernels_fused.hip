@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __r
 }
 
 // Two steps per pass (3-D, periodic, deterministic chemical potentials).
-template <int PREC, int MU, int TY, int TZ, int NT, bool PHI2>
+template <int PREC, int MU, int TY, int TZ, int NT, bool PHI2, bool UNIT>
 __global__ void __launch_bounds__(NT) ch_fused2(const typename Form<PREC>::S* __restrict__ a,
                                                 typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
                                                 int chunk_len, int tiles_y, int tiles_z, int ntiles)
@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(NT) ch_fused2(const typename Form<PREC>::S* __
 	const int chunk = t2 / tiles_y;
 	const int xb = chunk * chunk_len;
 	const int xe = min(xb + chunk_len, g.n[0]);
-	ch_tile_rq2<PREC, MU, TY, TZ, NT, PHI2>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
+	ch_tile_rq2<PREC, MU, TY, TZ, NT, PHI2, UNIT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
 }
 
 template <int PREC, int TZ>
@@ -288,6 +288,14 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// 97.6 with 768 in one sweep, 92.6 against 91.6 in another: the two are
 	// within the noise; 16-plane chunks take 99-107 us)
 	if (model == MMSPX_CH_NOISE && prec == MMSPX_F64) return TuneChoice{2, 32, 64, 32, 3};
+	// Two steps per pass where the chemical potential is deterministic: 256^3
+	// FP64 on 32 x 32 tiles with 64-plane chunks (52.8 us per step against 59.9
+	// for the one-step register queues on the same box; 32-plane chunks 56.0,
+	// 16-plane 62.5; 32 x 64 tiles spill in FP64, 70.1), FP32 on 32 x 64 tiles
+	// with 32-plane chunks (32.9 against 38.0; 16-plane 37.1, 64-plane 50.4;
+	// 32 x 32 tiles 38.2)
+	if (model != MMSPX_CH_NOISE && prec == MMSPX_F64) return TuneChoice{2, 32, 32, 64, 7};
+	if (model != MMSPX_CH_NOISE && prec == MMSPX_F32) return TuneChoice{2, 32, 64, 32, 6};
 	return TuneChoice{2, 32, 64, prec == MMSPX_F64 ? 32 : 16, 2};
 }
 
@@ -391,6 +399,15 @@ static int launch_ch_body(const Geom& g, const ModelParams& p, const void* a, vo
 	return 0;
 }
 
+// The grid weights and the gradient coefficient are all exactly 1 (dx = 1,
+// kappa or K = 1): the two-step bodies then leave out the multiplications by them.
+static bool unit_weights(const Geom& g, const ModelParams& p, int mu)
+{
+	for (int k = 0; k < 3; k++)
+		if (g.w[k] != 1.0 || g.wf[k] != 1.0f) return false;
+	return mu == 1 ? (p.kappa == 1.0 && p.kappaf == 1.0f) : (p.K == 1.0 && p.Kf == 1.0f);
+}
+
 template <int PREC, int MU, int TY, int TZ, int NT, bool PHI2>
 static int launch_ch2_body(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
@@ -400,8 +417,12 @@ static int launch_ch2_body(const Geom& g, const ModelParams& p, const void* a, v
 	const int chunks = (g.n[0] + chunk - 1) / chunk;
 	const int ntiles = tiles_y * tiles_z * chunks;
 	const int blocks = (ntiles + 7) / 8 * 8;
-	hipLaunchKernelGGL((ch_fused2<PREC, MU, TY, TZ, NT, PHI2>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g, p,
-	                   chunk, tiles_y, tiles_z, ntiles);
+	if (unit_weights(g, p, MU))
+		hipLaunchKernelGGL((ch_fused2<PREC, MU, TY, TZ, NT, PHI2, true>), dim3(blocks), dim3(NT), 0, st, (const S*)a,
+		                   (S*)b, g, p, chunk, tiles_y, tiles_z, ntiles);
+	else
+		hipLaunchKernelGGL((ch_fused2<PREC, MU, TY, TZ, NT, PHI2, false>), dim3(blocks), dim3(NT), 0, st, (const S*)a,
+		                   (S*)b, g, p, chunk, tiles_y, tiles_z, ntiles);
 	MMSPX_HIP(hipGetLastError());
 	return 0;
 }

@@ -345,7 +345,9 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 // second barrier. Every value is the one the one-step kernels compute for that
 // node and step (the halo recomputes what the neighbouring tile computes), so
 // the result is bitwise two single steps; c^{n+1} never goes to HBM.
-template <int PREC, int MU, int TY, int TZ, int NT, bool PHI2>
+// UNIT: every grid weight 1/dx^2 and the gradient coefficient are 1, so the
+// multiplications by them are left out (they are identities: same bits).
+template <int PREC, int MU, int TY, int TZ, int NT, bool PHI2, bool UNIT>
 __device__ __forceinline__ void ch_tile_rq2(const typename Form<PREC>::S* __restrict__ a,
                                             typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
                                             int y0, int z0, int xb, int xe)
@@ -418,6 +420,17 @@ __device__ __forceinline__ void ch_tile_rq2(const typename Form<PREC>::S* __rest
 			rq[j] = pr * P4Z + pc;
 		}
 	}
+	auto lapadd = [&](typename F::L& lap, int ax, S yh, S y, S yl) {
+		if constexpr (UNIT)
+			F::lap_add1(lap, yh, y, yl);
+		else
+			F::lap_add(lap, g.w[ax], g.wf[ax], yh, y, yl);
+	};
+	auto mu = [&](S c, typename F::L lap) -> S {
+		if constexpr (UNIT && MU == 1) return F::mu_dw1(c, lap);
+		else if constexpr (UNIT) return F::mu_poly1(c, lap, p);
+		else return mu_of<PREC, MU>(c, lap, p, 0.0f);
+	};
 	// unconditional loads (lanes past the region load offset 0; planes wrap)
 	auto load_own = [&](int x, S* v) {
 		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
@@ -469,10 +482,10 @@ __device__ __forceinline__ void ch_tile_rq2(const typename Form<PREC>::S* __rest
 				const int q = q4[k];
 				const S y = fc[k];
 				typename F::L lap = F::lap_zero();
-				F::lap_add(lap, g.w[0], g.wf[0], fp[k], y, fm[k]);
-				F::lap_add(lap, g.w[1], g.wf[1], lp[q + P4Z], y, lp[q - P4Z]);
-				F::lap_add(lap, g.w[2], g.wf[2], lp[q + 1], y, lp[q - 1]);
-				m0p[k] = mu_of<PREC, MU>(y, lap, p, 0.0f);
+				lapadd(lap, 0, fp[k], y, fm[k]);
+				lapadd(lap, 1, lp[q + P4Z], y, lp[q - P4Z]);
+				lapadd(lap, 2, lp[q + 1], y, lp[q - 1]);
+				m0p[k] = mu(y, lap);
 				w0[threadIdx.x + k * NT] = m0p[k];
 			}
 		}
@@ -488,9 +501,9 @@ __device__ __forceinline__ void ch_tile_rq2(const typename Form<PREC>::S* __rest
 					const int e = threadIdx.x + k * NT;
 					const S m = m0c[k];
 					typename F::L lap = F::lap_zero();
-					F::lap_add(lap, g.w[0], g.wf[0], m0p[k], m, m0m[k]);
-					F::lap_add(lap, g.w[1], g.wf[1], r0p[e + R3Z], m, r0p[e - R3Z]);
-					F::lap_add(lap, g.w[2], g.wf[2], r0p[e + 1], m, r0p[e - 1]);
+					lapadd(lap, 0, m0p[k], m, m0m[k]);
+					lapadd(lap, 1, r0p[e + R3Z], m, r0p[e - R3Z]);
+					lapadd(lap, 2, r0p[e + 1], m, r0p[e - 1]);
 					c1p[k] = F::ch(fm[k], lap, p);
 					w1[q2[k]] = c1p[k];
 				}
@@ -506,10 +519,10 @@ __device__ __forceinline__ void ch_tile_rq2(const typename Form<PREC>::S* __rest
 					const int q = q2[k];
 					const S y = c1c[k];
 					typename F::L lap = F::lap_zero();
-					F::lap_add(lap, g.w[0], g.wf[0], c1p[k], y, c1m[k]);
-					F::lap_add(lap, g.w[1], g.wf[1], r1p[q + R2Z], y, r1p[q - R2Z]);
-					F::lap_add(lap, g.w[2], g.wf[2], r1p[q + 1], y, r1p[q - 1]);
-					m1p[k] = mu_of<PREC, MU>(y, lap, p, 0.0f);
+					lapadd(lap, 0, c1p[k], y, c1m[k]);
+					lapadd(lap, 1, r1p[q + R2Z], y, r1p[q - R2Z]);
+					lapadd(lap, 2, r1p[q + 1], y, r1p[q - 1]);
+					m1p[k] = mu(y, lap);
 					w2[q1[k]] = m1p[k];
 				}
 			}
@@ -524,9 +537,9 @@ __device__ __forceinline__ void ch_tile_rq2(const typename Form<PREC>::S* __rest
 					const int q = q1[k];
 					const S m = m1c[k];
 					typename F::L lap = F::lap_zero();
-					F::lap_add(lap, g.w[0], g.wf[0], m1p[k], m, m1m[k]);
-					F::lap_add(lap, g.w[1], g.wf[1], r2p[q + R1Z], m, r2p[q - R1Z]);
-					F::lap_add(lap, g.w[2], g.wf[2], r2p[q + 1], m, r2p[q - 1]);
+					lapadd(lap, 0, m1p[k], m, m1m[k]);
+					lapadd(lap, 1, r2p[q + R1Z], m, r2p[q - R1Z]);
+					lapadd(lap, 2, r2p[q + 1], m, r2p[q - 1]);
 					dst[ooff[k]] = F::ch(c1m[k], lap, p);
 				}
 			}

@@ -208,6 +208,16 @@ def test_fused_ch_options_and_directions(prec):
             g.upload(b0)
             g.step("ch", pp, 10)
             assert np.array_equal(g.download(), exp_b), (body, "poly")
+        # the double well with kappa != 1 (the weighted form of the body)
+        pk = [0.8, 1.0, 0.005]
+        exp_k = O.ch(a, pk, 9, mu=O.MU_DW, form=prec)
+        with mx.Grid(shape, precision=prec) as g:
+            g.set_variant(2)
+            g.set_tile(tile[0], tile[1], 16)
+            g.set_tile_options(body=body)
+            g.upload(a)
+            g.step("ch_dw", pk, 9)
+            assert np.array_equal(g.download(), exp_k), (body, "kappa")
     with mx.Grid(shape, precision=prec) as g:  # the polynomial mu form too
         g.set_variant(2)
         g.set_tile(32, 64, 16)
