@@ -1,7 +1,8 @@
 """Benchmark of the explicit update path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload ch3d|ch3d_f32|ac2d|gg2d|zener2d|ostwald2d|gganiso2d|model_b3d|batch_ac|serve|ckpt|dropin]
+                    [--workload ch3d|ch3d_f32|ac2d|ac3d|gg2d|gg3d|zener2d|ostwald2d|gganiso2d|model_b3d|batch_ac|serve|
+                                ckpt|dropin]
                     [--no-cpu-baseline] [--tune | --tile TY,TZ,CHUNK[,OPT]]
                     [--requests N] [--hbm-budget-gb G]
                     [--graph-steps G] [--no-sequential] [--ckpt-threads T]
@@ -74,6 +75,12 @@ WORKLOADS = {
     "gganiso2d": dict(model="gg_aniso", shape=(1024, 1024), prec="f64", params=[0.01, 8.0, 1.0, 1.0], steps=500,
                       seed=20260111, lo=0.0, hi=1.0, fields=8, bytes_per_cell=2 * 8 * 8, name="multiphase-gg-2d-q8",
                       cpu_steps=2, init="grains"),
+    # 3-D Allen-Cahn (allen-cahn.cpp:58-71 parameters) on a 256^3 periodic grid, FP64
+    "ac3d": dict(model="ac", shape=(256, 256, 256), prec="f64", params=[1.0, 1.0, 1.0, 1.0, 0.01], steps=1000,
+                 seed=20260112, lo=-0.01, hi=0.01, fields=1, bytes_per_cell=16, name="allen-cahn-3d", cpu_steps=2),
+    # 3-D grain growth (graingrowth.cpp:84-105 with dt = 0.05), 256^3 with 8 fields, FP32
+    "gg3d": dict(model="gg", shape=(256, 256, 256), prec="f32", params=[0.05], steps=200, seed=20260113,
+                 lo=0.0, hi=0.01, fields=8, bytes_per_cell=2 * 8 * 4, name="grain-growth-3d-q8", cpu_steps=1),
     "model_b3d": dict(model="ch_noise", shape=(256, 256, 256), prec="f64",
                       params=[1.0, 1.0, 1.0, 1.0, 0.01, math.sqrt(2.0 * 0.01 / (0.01 * 1.0)), 20260107], steps=2000,
                       seed=20260108, lo=-0.01, hi=0.01, fields=1, bytes_per_cell=16, name="model-b-3d", cpu_steps=2),
@@ -721,6 +728,8 @@ def main():
             "gg_aniso": "gga"}[w["model"]]
     if tile["variant"] != 2:
         kname = base + "_generic"
+    elif len(shape) == 3 and tile["opt"] == 8:
+        kname = "march3d"
     elif base == "ch":
         kname = "ch_fused"
     elif base == "ow":

@@ -259,7 +259,10 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
  * waves per strip, each holding its share of the fields). 3-D periodic
  * Cahn-Hilliard (deterministic chemical potentials): 6 and 7 take two steps
  * per pass over HBM (6 on 32 x 64 tiles, 7 on 32 x 32 tiles; an odd last
- * step runs body 2). */
+ * step runs body 2). 3-D Allen-Cahn, model A, grain growth and Zener pinning
+ * (2, 4 or 8 fields), any boundary: 8 marches the tile along axis 0 (8, 16,
+ * 32 x 64 and 16 x 32 tiles for Allen-Cahn; 8 x 64 and 16 x 32 for the
+ * vector models). */
 int mmspx_set_tile_options(mmspx_state* s, int body);
 /* Returns variant + 16 * tile body (1: register queues, or the 2-D
  * grain-growth tile; 2: wave strips, or 3-D register queues on 768 threads;

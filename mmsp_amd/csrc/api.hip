@@ -462,6 +462,7 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 	// 3-D two-step bodies: 32 x 32 tiles run body 7 only; body 6 runs 32 x 64
 	if (s->geom.dim == 3 && ty == 32 && tz == 32) opt = 7;
 	else if (s->geom.dim == 3 && s->tuned && s->tile.opt == 6 && ty == 32 && tz == 64) opt = 6;
+	else if (s->geom.dim == 3 && s->tuned && s->tile.opt == 8) opt = 8;
 	TuneChoice t{2, ty, tz, chunk, opt};
 	if (!tile_known(s->geom, t)) return fail(MMSPX_EINVAL, "no compiled tile of that shape for this grid");
 	s->tile = t;
@@ -472,7 +473,7 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 int mmspx_set_tile_options(mmspx_state* s, int body)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
-	if (body < 0 || body > 7 || (body == 3 && s->geom.dim != 3) || ((body == 4 || body == 5) && s->geom.dim != 2) ||
+	if (body < 0 || body > 8 || (body == 3 && s->geom.dim != 3) || ((body == 4 || body == 5) && s->geom.dim != 2) ||
 	    (body >= 6 && s->geom.dim != 3))
 		return fail(MMSPX_EINVAL, "unknown tile body");
 	if (!s->tuned) {

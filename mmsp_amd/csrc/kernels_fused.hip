@@ -66,6 +66,23 @@ __global__ void __launch_bounds__(NT) ch_fused2(const typename Form<PREC>::S* __
 	ch_tile_rq2<PREC, MU, TY, TZ, NT, PHI2, UNIT>(a, b, g, p, ty * TY, tz * TZ, xb, xe);
 }
 
+// 3-D plane march (Allen-Cahn, model A, grain growth, Zener pinning): one
+// workgroup per (tile, chunk), XCD-contiguous runs as the Cahn-Hilliard tiles.
+template <int PREC, int Q, int KIND, int TY, int TZ, int NT>
+__global__ void __launch_bounds__(NT) march3d_k(const typename Form<PREC>::S* __restrict__ a,
+                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
+                                                int chunk_len, int tiles_y, int tiles_z, int ntiles)
+{
+	const int tile = xcd_tile(ntiles);
+	if (tile < 0) return;
+	const int tz = tile % tiles_z;
+	const int t2 = tile / tiles_z;
+	const int ty = t2 % tiles_y;
+	const int chunk = t2 / tiles_y;
+	const int xb = chunk * chunk_len;
+	march3d<PREC, Q, KIND, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, min(xb + chunk_len, g.n[0]));
+}
+
 template <int PREC, int TZ>
 __global__ void __launch_bounds__(TZ) ac_tiled(const typename Form<PREC>::S* __restrict__ a,
                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
@@ -189,9 +206,17 @@ static bool all_periodic(const Geom& g)
 
 static bool gg_q_compiled(int q) { return q == 2 || q == 4 || q == 8 || q == 16 || q == 32; }
 
+// field counts of the 3-D march for grain growth and Zener pinning
+static bool march_q_compiled(int q) { return q == 2 || q == 4 || q == 8; }
+
 bool fused_supported(const Geom& g, int prec, int model)
 {
 	(void)prec;
+	if (g.dim == 3) {
+		if (model == MMSPX_AC || model == MMSPX_AC_NOISE) return true;
+		if (model == MMSPX_GG) return march_q_compiled(g.fields);
+		if (model == MMSPX_ZENER) return march_q_compiled(g.fields) && prec != MMSPX_F32REF;
+	}
 	switch (model) {
 	case MMSPX_CH:
 	case MMSPX_CH_DW:
@@ -219,6 +244,23 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 {
 	if (!fused_supported(g, prec, model)) return TuneChoice{1, 0, 0, 0, 0};
 	const int64_t nodes = g.nodes;
+	// 3-D Allen-Cahn, model A, grain growth and Zener pinning: the plane march.
+	// 256^3 sweep (us per step): Allen-Cahn FP64 16 x 32 tiles with 32-plane
+	// chunks 44.5 (6.0 TB/s; 8 x 64 45.0, 32 x 64 46.5, one thread per node
+	// 93.5), FP32 32 x 64 with 16-plane chunks 24.6 (16 x 64 24.8; 32-plane
+	// chunks 31.7); grain growth 8 fields 8 x 64 with 64-plane chunks, FP32
+	// 252 (16 x 32 275, one thread per node 430), FP64 484 (935); Zener 4
+	// fields FP32 122 (211). Chunks are halved until the launch has 512
+	// workgroups (two per CU) or more.
+	if (g.dim == 3 && (model == MMSPX_AC || model == MMSPX_AC_NOISE || model == MMSPX_GG || model == MMSPX_ZENER)) {
+		const bool scalar = model == MMSPX_AC || model == MMSPX_AC_NOISE;
+		const int ty = scalar ? (prec == MMSPX_F64 ? 16 : 32) : 8;
+		const int tz = scalar ? (prec == MMSPX_F64 ? 32 : 64) : 64;
+		int chunk = scalar ? (prec == MMSPX_F64 ? 32 : 16) : 64;
+		const int64_t tiles = (int64_t)((g.n[1] + ty - 1) / ty) * ((g.n[2] + tz - 1) / tz);
+		while (chunk > 4 && tiles * ((g.n[0] + chunk - 1) / chunk) < 512) chunk /= 2;
+		return TuneChoice{2, ty, tz, chunk, 8};
+	}
 	// Allen-Cahn: periodic grids take temporal blocks of 48 columns and 8
 	// steps per pass (512^2: 2.05 us per step in FP64 against 4.58 us for one
 	// thread per node). Block heights from a sweep of 8, 16, 32 rows x 4, 8
@@ -304,7 +346,8 @@ bool tile_known(const Geom& g, const TuneChoice& t)
 	if (t.chunk < 2 || t.chunk > 4096) return false;
 	if (g.dim == 3)
 		return (t.ty == 8 && t.tz == 64) || (t.ty == 16 && t.tz == 64) || (t.ty == 32 && t.tz == 64) ||
-		       (t.ty == 8 && t.tz == 32) || (t.ty == 4 && t.tz == 128) || (t.ty == 32 && t.tz == 32);
+		       (t.ty == 8 && t.tz == 32) || (t.ty == 4 && t.tz == 128) || (t.ty == 32 && t.tz == 32) ||
+		       (t.ty == 16 && t.tz == 32);
 	if (g.dim == 2)
 		return (t.ty == 1 && (t.tz == 64 || t.tz == 128 || t.tz == 256 || t.tz == kChWaveOut)) ||
 		       ((t.ty == 8 || t.ty == 16 || t.ty == 32) && t.tz == kTbCols && t.chunk <= kTbMaxSteps) ||
@@ -334,6 +377,19 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		if (split)
 			for (int c : {8, 16, 32, 64})
 				for (int body = 4; body <= 5; body++) add(1, kWaveCols, c, body);
+		return n;
+	}
+	if (g.dim == 3 && !ch) {
+		// the plane march: Allen-Cahn on four tiles, the vector models on two
+		const bool scalar = model == MMSPX_AC || model == MMSPX_AC_NOISE;
+		for (int c : {16, 32, 64}) {
+			add(8, 64, c, 8);
+			add(16, 32, c, 8);
+			if (scalar) {
+				add(16, 64, c, 8);
+				add(32, 64, c, 8);
+			}
+		}
 		return n;
 	}
 	if (g.dim == 3) {
@@ -434,6 +490,56 @@ static int launch_ch(const Geom& g, const ModelParams& p, const void* a, void* b
 {
 	if (opt == 1) return launch_ch_body<PREC, MU, DIM, TY, TZ, NT, 1>(g, p, a, b, st, chunk);
 	return launch_ch_body<PREC, MU, DIM, TY, TZ, NT, 0>(g, p, a, b, st, chunk);
+}
+
+template <int PREC, int Q, int KIND, int TY, int TZ, int NT>
+static int launch_march_body(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	using S = typename Form<PREC>::S;
+	const int tiles_y = (g.n[1] + TY - 1) / TY;
+	const int tiles_z = (g.n[2] + TZ - 1) / TZ;
+	const int chunks = (g.n[0] + chunk - 1) / chunk;
+	const int ntiles = tiles_y * tiles_z * chunks;
+	const int blocks = (ntiles + 7) / 8 * 8;
+	hipLaunchKernelGGL((march3d_k<PREC, Q, KIND, TY, TZ, NT>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g,
+	                   p, chunk, tiles_y, tiles_z, ntiles);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+// tile body 8 on 3-D grids: Allen-Cahn on 8, 16 or 32 x 64 and 16 x 32 tiles;
+// the vector models (2, 4 or 8 fields) on 8 x 64 and 16 x 32 tiles, one node
+// per thread
+template <int PREC, int KIND>
+static int launch_march(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st,
+                        const TuneChoice& t)
+{
+	const int chunk = t.chunk > 0 ? t.chunk : 32;
+	if (t.opt != 8) return fail(MMSPX_EINVAL, "3-D Allen-Cahn, grain growth and Zener pinning run tile body 8");
+	if constexpr (KIND == 0 || KIND == 2) {
+		if (t.ty == 8 && t.tz == 64) return launch_march_body<PREC, 1, KIND, 8, 64, 256>(g, p, a, b, st, chunk);
+		if (t.ty == 16 && t.tz == 64) return launch_march_body<PREC, 1, KIND, 16, 64, 256>(g, p, a, b, st, chunk);
+		if (t.ty == 32 && t.tz == 64) return launch_march_body<PREC, 1, KIND, 32, 64, 512>(g, p, a, b, st, chunk);
+		if (t.ty == 16 && t.tz == 32) return launch_march_body<PREC, 1, KIND, 16, 32, 256>(g, p, a, b, st, chunk);
+	} else {
+		const bool t864 = t.ty == 8 && t.tz == 64, t1632 = t.ty == 16 && t.tz == 32;
+		switch (g.fields) {
+		case 2:
+			if (t864) return launch_march_body<PREC, 2, KIND, 8, 64, 512>(g, p, a, b, st, chunk);
+			if (t1632) return launch_march_body<PREC, 2, KIND, 16, 32, 512>(g, p, a, b, st, chunk);
+			break;
+		case 4:
+			if (t864) return launch_march_body<PREC, 4, KIND, 8, 64, 512>(g, p, a, b, st, chunk);
+			if (t1632) return launch_march_body<PREC, 4, KIND, 16, 32, 512>(g, p, a, b, st, chunk);
+			break;
+		case 8:
+			if (t864) return launch_march_body<PREC, 8, KIND, 8, 64, 512>(g, p, a, b, st, chunk);
+			if (t1632) return launch_march_body<PREC, 8, KIND, 16, 32, 512>(g, p, a, b, st, chunk);
+			break;
+		default: return fail(MMSPX_EUNSUPPORTED, "the 3-D march takes 2, 4 or 8 fields");
+		}
+	}
+	return fail(MMSPX_EINVAL, "no 3-D march compiled for this tile");
 }
 
 template <int PREC, int MU>
@@ -718,6 +824,7 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		}
 		return fail(MMSPX_EUNSUPPORTED, "Ostwald ripening runs the wave strips (1 x 60) or the generic kernels");
 	case MMSPX_ZENER:
+		if (g.dim == 3) return launch_march<PREC, 3>(g, p, a, b, st, t);
 		if ((t.opt == 4 || t.opt == 5) && t.ty == 1 && t.tz == kWaveCols)
 			return launch_split<PREC>(g, p, a, b, st, chunk, t.opt == 5 ? 4 : 2);
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
@@ -726,10 +833,12 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		return fail(MMSPX_EUNSUPPORTED, "multi-phase grain growth runs the wave strips (1 x 64, body 2) or the generic kernel");
 	case MMSPX_AC_NOISE:
+		if (g.dim == 3) return launch_march<PREC, 2>(g, p, a, b, st, t);
 		// model A: wave strips only (the noise is drawn per node and step)
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		return fail(MMSPX_EUNSUPPORTED, "Allen-Cahn with noise runs the wave strips (1 x 64, body 2) or the generic kernel");
 	case MMSPX_AC:
+		if (g.dim == 3) return launch_march<PREC, 0>(g, p, a, b, st, t);
 		if (t.tz == kTbCols) {
 			if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "temporal blocks need periodic boundaries");
 			if (nsteps < 1 || nsteps > kTbMaxSteps) return fail(MMSPX_EINVAL, "temporal blocks take 1..8 steps per pass");
@@ -741,6 +850,7 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		if (t.tz == 128) return launch_ac<PREC, 128>(g, p, a, b, st, chunk);
 		return launch_ac<PREC, 256>(g, p, a, b, st, chunk);
 	case MMSPX_GG:
+		if (g.dim == 3) return launch_march<PREC, 1>(g, p, a, b, st, t);
 		// ty > 1: 2-D tiles of ty (8 or 16) rows x 64 columns; body 2 with 64
 		// columns: wave strips; otherwise LDS-row marching strips
 		if ((t.opt == 4 || t.opt == 5) && t.ty == 1 && t.tz == kWaveCols)

@@ -566,6 +566,164 @@ __device__ __forceinline__ void ch_tile_rq2(const typename Form<PREC>::S* __rest
 	}
 }
 
+// ---- 3-D Allen-Cahn, model A, grain growth, Zener pinning: plane march -------
+// Any boundary condition. One (TY x TZ) tile of the (axis 1, axis 2) plane
+// marches over the axis-0 planes [xb, xe). Each thread owns fixed nodes of the
+// tile and keeps their Q fields at planes P-1, P and P+1 (P+2 in flight) in
+// registers; the in-plane neighbours come from plane P's tile in LDS, which
+// has a one-cell halo filled through check_boundary's mapping
+// (MMSP.utility.cpp:42-57): the halo cell at y = -1 holds the node the
+// reference's laplacian reads there, and the axis-0 planes are mapped the same
+// way. The LDS tile is double-buffered, so one barrier per plane orders its
+// writes and reads. The per-node arithmetic is the generic kernels' (the same
+// bits). KIND 0 Allen-Cahn (allen-cahn.cpp:68-71), 1 grain growth
+// (graingrowth.cpp:91-107), 2 model A (model_A.cpp:97-101), 3 Zener pinning
+// (zener.cpp:117-135: field 0 held).
+template <int PREC, int Q, int KIND, int TY, int TZ, int NT>
+__device__ __forceinline__ void march3d(const typename Form<PREC>::S* __restrict__ a,
+                                        typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                        int y0, int z0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	constexpr int PY = TY + 2, PZ = TZ + 2;
+	constexpr int PN = PY * PZ, ON = TY * TZ, RN = PN - ON;
+	constexpr int M = (ON + NT - 1) / NT;
+	constexpr int MR = (RN + NT - 1) / NT;
+	__shared__ S lt[2][Q][PN];
+
+	const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+	const int64_t plane = (int64_t)ny * nz;
+	const int64_t N = (int64_t)nx * plane;
+	// a coordinate one cell outside the grid maps as check_boundary maps it;
+	// tile positions further out are never read as neighbours (offset 0)
+	auto map1 = [&](int v, int n, int lo, int hi) { return (v >= -1 && v <= n) ? bc_map(v, n, lo, hi) : 0; };
+
+	int ooff[M], oq[M];
+	bool oout[M];
+#pragma unroll
+	for (int k = 0; k < M; k++) {
+		const int e = threadIdx.x + k * NT;
+		ooff[k] = oq[k] = 0;
+		oout[k] = false;
+		if (e < ON) {
+			const int r = e / TZ, c = e - (e / TZ) * TZ;
+			const int y = y0 + r, z = z0 + c;
+			ooff[k] = map1(y, ny, g.b0[1], g.b1[1]) * nz + map1(z, nz, g.b0[2], g.b1[2]);
+			oq[k] = (r + 1) * PZ + (c + 1);
+			oout[k] = y < ny && z < nz;
+		}
+	}
+	int roff[MR], rq[MR];
+#pragma unroll
+	for (int j = 0; j < MR; j++) {
+		const int t = threadIdx.x + j * NT;
+		roff[j] = rq[j] = 0;
+		if (t < RN) {
+			int pr, pc;
+			if (t < PZ) {
+				pr = 0;
+				pc = t;
+			} else if (t < 2 * PZ) {
+				pr = PY - 1;
+				pc = t - PZ;
+			} else {
+				const int u = t - 2 * PZ;
+				pr = 1 + u / 2;
+				pc = (u & 1) ? PZ - 1 : 0;
+			}
+			roff[j] = map1(y0 - 1 + pr, ny, g.b0[1], g.b1[1]) * nz + map1(z0 - 1 + pc, nz, g.b0[2], g.b1[2]);
+			rq[j] = pr * PZ + pc;
+		}
+	}
+	auto plane_of = [&](int x) { return (int64_t)map1(x, nx, g.b0[0], g.b1[0]) * plane; };
+	auto load_own = [&](int x, S (*v)[M]) {
+		const S* __restrict__ src = a + plane_of(x);
+#pragma unroll
+		for (int q = 0; q < Q; q++)
+#pragma unroll
+			for (int k = 0; k < M; k++) v[q][k] = src[q * N + ooff[k]];
+	};
+	auto load_ring = [&](int x, S (*v)[MR]) {
+		const S* __restrict__ src = a + plane_of(x);
+#pragma unroll
+		for (int q = 0; q < Q; q++)
+#pragma unroll
+			for (int j = 0; j < MR; j++) v[q][j] = src[q * N + roff[j]];
+	};
+
+	S f0[Q][M], f1[Q][M], f2[Q][M], f3[Q][M];
+	S r0[Q][MR], r1[Q][MR];
+	int nkx = -1;
+	float nkeep[KIND == 2 ? M : 1];
+	(void)nkx;
+	(void)nkeep;
+	// the previous tile of a persistent loop may still read the planes
+	__syncthreads();
+	load_own(xb - 1, f0);
+	load_own(xb, f1);
+	load_own(xb + 1, f2);
+	load_ring(xb, r0);
+
+	auto step = [&](int P, S (*fm)[M], S (*fc)[M], S (*fp)[M], S (*fn)[M], S (*rc)[MR], S (*rn)[MR]) {
+		load_own(P + 2, fn);
+		load_ring(P + 1, rn);
+		S(*lp)[PN] = lt[P & 1];
+#pragma unroll
+		for (int q = 0; q < Q; q++) {
+#pragma unroll
+			for (int k = 0; k < M; k++)
+				if (threadIdx.x + k * NT < ON) lp[q][oq[k]] = fc[q][k];
+#pragma unroll
+			for (int j = 0; j < MR; j++)
+				if (threadIdx.x + j * NT < RN) lp[q][rq[j]] = rc[q][j];
+		}
+		__syncthreads();
+		S* __restrict__ dst = b + (int64_t)P * plane;
+		const bool nhit = P == nkx;
+		(void)nhit;
+#pragma unroll
+		for (int k = 0; k < M; k++) {
+			if (!oout[k]) continue;
+			const int qq = oq[k];
+			typename F::Sum sum = 0;
+			if constexpr (KIND == 1 || KIND == 3) {
+#pragma unroll
+				for (int q = 0; q < Q; q++) sum += fc[q][k] * fc[q][k];
+			}
+			float gd = 0.0f;
+			if constexpr (KIND == 2) gd = normal_deviate_cached(p.seed, p.step, P, (uint32_t)ooff[k], nkx, nkeep[k]);
+#pragma unroll
+			for (int q = 0; q < Q; q++) {
+				const S y = fc[q][k];
+				if (KIND == 3 && q == 0) {
+					dst[ooff[k]] = y;  // the particles keep their values
+					continue;
+				}
+				typename F::L lap = F::lap_zero();
+				F::lap_add(lap, g.w[0], g.wf[0], fp[q][k], y, fm[q][k]);
+				F::lap_add(lap, g.w[1], g.wf[1], lp[q][qq + PZ], y, lp[q][qq - PZ]);
+				F::lap_add(lap, g.w[2], g.wf[2], lp[q][qq + 1], y, lp[q][qq - 1]);
+				S out;
+				if constexpr (KIND == 0) out = F::ac(y, lap, p);
+				else if constexpr (KIND == 2) out = F::ac_noise(y, lap, gd, p);
+				else out = F::gg(y, sum, lap, p);
+				dst[q * N + ooff[k]] = out;
+			}
+		}
+		if constexpr (KIND == 2) nkx = nhit ? -1 : (P ^ 1);
+	};
+	for (int P = xb; P < xe; P += 4) {
+		step(P, f0, f1, f2, f3, r0, r1);
+		if (P + 1 >= xe) break;
+		step(P + 1, f1, f2, f3, f0, r1, r0);
+		if (P + 2 >= xe) break;
+		step(P + 2, f2, f3, f0, f1, r0, r1);
+		if (P + 3 >= xe) break;
+		step(P + 3, f3, f0, f1, f2, r1, r0);
+	}
+}
+
 // ---- 2-D Cahn-Hilliard, wave strips (periodic): one wave per (strip, chunk) --
 // Lane l holds column c0 - 2 + l: mu is formed on lanes 1..62 and c' written
 // on lanes 2..61 (kChWaveOut columns per strip); column neighbours come from

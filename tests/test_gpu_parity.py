@@ -631,6 +631,78 @@ def test_multiphase_wave_strips(prec):
         assert np.array_equal(g.download(), O.gg_aniso(a, p, 2, form=prec))
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
+def test_march_3d(prec):
+    """The 3-D plane march (tile body 8): Allen-Cahn, model A, grain growth
+    and Zener pinning with 2, 4 and 8 fields, every compiled tile, chunks of 2,
+    3, 5, 7 and 16 planes (the four-plane unrolled march ends at each of its
+    positions), partial tiles, grids smaller than one tile, every boundary
+    condition on every axis; bitwise against the oracle."""
+    bcs = [None, [(N, M), (D, P), (M, N)], [(P, D), (N, N), (D, M)], [(M, M), (P, P), (N, D)]]
+    ac_tiles = [(8, 64), (16, 64), (32, 64), (16, 32)]
+    vec_tiles = [(8, 64), (16, 32)]
+    pa = [1.0, 1.0, 1.3, 1.0, 0.01]
+    pn = [1.0, 1.0, 1.0, 1.0, 0.01, 0.4, 17]
+    for shape, bc in [((13, 20, 70), bcs[1]), ((9, 40, 33), bcs[2]), ((6, 5, 7), bcs[3]), ((12, 18, 66), bcs[0])]:
+        say("[gpu] march 3-D ac %s %s %s" % (shape, bc, prec))
+        a = O.uniform(int(np.prod(shape)), 1300 + shape[2], -1.0, 1.0).reshape(shape)
+        if prec != "f64":
+            a = a.astype(np.float32)
+        exp = O.ac(a, pa, 7, form=prec, bc=bc)
+        expn = O.ac_noise(a, pn, 7, form=prec, bc=bc)
+        for ti, tile in enumerate(ac_tiles):
+            for chunk in ((2, 3, 5, 7, 16) if ti == 0 else (3, 16)):
+                with mx.Grid(shape, precision=prec, bc=bc) as g:
+                    g.set_variant(2)
+                    g.set_tile(tile[0], tile[1], chunk)
+                    g.set_tile_options(body=8)
+                    g.upload(a)
+                    g.step("ac", pa, 7)
+                    assert np.array_equal(g.download(), exp), (shape, tile, chunk)
+            with mx.Grid(shape, precision=prec, bc=bc) as g:
+                g.set_variant(2)
+                g.set_tile(tile[0], tile[1], 5)
+                g.set_tile_options(body=8)
+                g.upload(a)
+                g.step("ac_noise", pn, 3)
+                g.step("ac_noise", pn, 4)
+                assert np.array_equal(g.download(), expn), (shape, tile, "noise")
+    for shape, q, bc in [((11, 20, 70), 2, bcs[1]), ((9, 17, 33), 4, bcs[2]), ((10, 12, 40), 8, bcs[3]),
+                         ((5, 6, 7), 8, bcs[0])]:
+        say("[gpu] march 3-D gg/zener %s q=%d %s" % (shape, q, prec))
+        a = O.uniform(int(np.prod(shape)) * q, 1400 + q, 0.0, 1.0).reshape(shape + (q,))
+        if prec != "f64":
+            a = a.astype(np.float32)
+        exp = O.gg(a, 0.01, 7, form=prec, bc=bc)
+        expz = O.zener(a, 0.01, 7, form=prec, bc=bc) if prec != "f32ref" else None
+        for tile in vec_tiles:
+            for chunk in (2, 3, 5, 16):
+                with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+                    g.set_variant(2)
+                    g.set_tile(tile[0], tile[1], chunk)
+                    g.set_tile_options(body=8)
+                    g.upload(a)
+                    g.step("gg", [0.01], 7)
+                    assert np.array_equal(g.download(), exp), (shape, q, tile, chunk)
+                if expz is not None:
+                    with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+                        g.set_variant(2)
+                        g.set_tile(tile[0], tile[1], chunk)
+                        g.set_tile_options(body=8)
+                        g.upload(a)
+                        g.step("zener", [0.01], 7)
+                        assert np.array_equal(g.download(), expz), (shape, q, tile, chunk, "zener")
+    # the defaults take the march; other field counts stay on the generic kernel
+    a = O.uniform(10 * 12 * 14 * 3, 1500, 0.0, 1.0).reshape(10, 12, 14, 3)
+    if prec != "f64":
+        a = a.astype(np.float32)
+    with mx.Grid((10, 12, 14), fields=3, vector=True, precision=prec) as g:
+        g.upload(a)
+        g.step("gg", [0.01], 4)
+        assert g.get_tile()["variant"] == 1
+        assert np.array_equal(g.download(), O.gg(a, 0.01, 4, form=prec))
+
+
 def test_graph_replay_follows_new_parameters():
     """Runs of 100 steps replay a captured graph whose launches bake the
     model's constants in; a second run with other constants on the same grid
