@@ -206,6 +206,24 @@ static bool all_periodic(const Geom& g)
 
 static bool gg_q_compiled(int q) { return q == 2 || q == 4 || q == 8 || q == 16 || q == 32; }
 
+// every axis periodic on both sides or on neither (the fused Cahn-Hilliard
+// bodies map the non-periodic sides; an axis periodic on one side only takes
+// the two-pass kernels)
+static bool axes_uniform(const Geom& g)
+{
+	for (int k = 0; k < g.dim; k++)
+		if ((g.b0[k] == MMSPX_PERIODIC) != (g.b1[k] == MMSPX_PERIODIC)) return false;
+	return true;
+}
+
+// the Cahn-Hilliard bodies that map non-periodic sides: the register queues
+// (3-D and the 2-D row strips) and the 2-D wave strips
+static bool ch_body_maps_edges(const Geom& g, const TuneChoice& t)
+{
+	if (g.dim == 3) return t.opt == 1 || t.opt == 2 || t.opt == 3;
+	return t.tz == kChWaveOut || t.opt == 1;
+}
+
 // field counts of the 3-D march for grain growth and Zener pinning
 static bool march_q_compiled(int q) { return q == 2 || q == 4 || q == 8; }
 
@@ -220,7 +238,7 @@ bool fused_supported(const Geom& g, int prec, int model)
 	switch (model) {
 	case MMSPX_CH:
 	case MMSPX_CH_DW:
-	case MMSPX_CH_NOISE: return g.dim >= 2 && all_periodic(g);
+	case MMSPX_CH_NOISE: return g.dim >= 2 && axes_uniform(g);
 	case MMSPX_AC:
 	case MMSPX_AC_NOISE: return g.dim == 2;
 	case MMSPX_GG: return g.dim == 2 && gg_q_compiled(g.fields);
@@ -319,6 +337,11 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 		                                                                  : 16;
 		return TuneChoice{2, 1, kChWaveOut, c, 0};
 	}
+	// other boundaries than periodic: the register queues (8 x 32 tiles on
+	// 256 threads below 2^24 nodes, 32 x 64 on 768 above)
+	if (!all_periodic(g))
+		return nodes < (int64_t)16 * 64 * 32 * 512 ? TuneChoice{2, 8, 32, 16, 1}
+		                                            : TuneChoice{2, 32, 64, prec == MMSPX_F64 ? 32 : 16, 2};
 	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16, 0};
 	// 256^3: register queues on 32 x 64 tiles with 768 threads. FP64 with
 	// 32-plane chunks, one workgroup per CU (52.3 us per step, against 53.0
@@ -392,27 +415,28 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		}
 		return n;
 	}
+	const bool edges = !all_periodic(g);  // Cahn-Hilliard: the bodies that map non-periodic sides
 	if (g.dim == 3) {
 		const int tiles[4][2] = {{16, 64}, {8, 64}, {8, 32}, {4, 128}};
 		for (auto& t : tiles)
 			for (int c : {16, 32, 64})
-				for (int opt = 0; opt < 2; opt++) add(t[0], t[1], c, opt);
+				for (int opt = edges ? 1 : 0; opt < 2; opt++) add(t[0], t[1], c, opt);
 		// 32 x 64: register queues only (the LDS rings would leave one
 		// workgroup per CU), on 512 and 768 threads
 		for (int c : {12, 16, 22, 32})
 			for (int body = 1; body <= 3; body++) add(32, 64, c, body);
 		for (int c : {20, 22, 24}) add(16, 64, c, 1);
-		// two steps per pass (deterministic chemical potentials)
-		if (model == MMSPX_CH || model == MMSPX_CH_DW)
+		// two steps per pass (deterministic chemical potentials, periodic grids)
+		if ((model == MMSPX_CH || model == MMSPX_CH_DW) && !edges)
 			for (int c : {16, 32, 64}) {
 				add(32, 64, c, 6);
 				add(32, 32, c, 7);
 			}
 	} else if (g.dim == 2) {
-		// LDS-row strips (Cahn-Hilliard: both tile bodies)
+		// LDS-row strips (Cahn-Hilliard: both tile bodies; non-periodic: the register queues)
 		for (int tz : {64, 128, 256})
 			for (int c : {16, 32, 64, 128})
-				for (int opt = 0; opt < (ch ? 2 : 1); opt++) add(1, tz, c, opt);
+				for (int opt = (ch && edges) ? 1 : 0; opt < (ch ? 2 : 1); opt++) add(1, tz, c, opt);
 		// Cahn-Hilliard wave strips
 		if (ch)
 			for (int c : {8, 16, 32, 64}) add(1, kChWaveOut, c, 0);
@@ -561,6 +585,9 @@ static int launch_ch_prec(const Geom& g, const ModelParams& p, const void* a, vo
                           const TuneChoice& t, int nsteps)
 {
 	const int chunk = t.chunk > 0 ? t.chunk : 32;
+	if (!all_periodic(g) && !ch_body_maps_edges(g, t))
+		return fail(MMSPX_EUNSUPPORTED, "non-periodic boundaries run the register-queue bodies (1, 2, 3) or the "
+		                                "2-D wave strips");
 	if (g.dim == 3 && (t.opt == 6 || t.opt == 7)) {
 		// two steps per pass: body 6 on 32 x 64 tiles (896 threads; the c^n plane
 		// double-buffered in FP32, single in FP64 where the LDS would not hold

@@ -15,6 +15,31 @@ __device__ __forceinline__ int wrap(int v, int n)
 	return v < 0 ? v + n : v;
 }
 
+// The node a tile position v (any integer) reads along an axis of n nodes:
+// periodic sides wrap; on the other sides one or two cells outside map as
+// check_boundary maps them (MMSP.utility.cpp:42-57), clamped into the grid;
+// positions further out are never read and take node 0.
+__device__ __forceinline__ int tile_map(int v, int n, int lo, int hi)
+{
+	if (v >= 0 && v < n) return v;
+	if (v < 0 ? lo == MMSPX_PERIODIC : hi == MMSPX_PERIODIC) return wrap(v, n);
+	if (v < -2 || v > n + 1) return 0;
+	const int m = bc_map(v, n, lo, hi);
+	return m < 0 ? 0 : (m >= n ? n - 1 : m);
+}
+
+// Fused Cahn-Hilliard tiles on a non-periodic axis compute mu on the whole
+// tile, but a boundary node's c' reads mu at the node check_boundary maps its
+// outer neighbour to, as the reference's second pass does: the node itself
+// (Neumann, Dirichlet) or its inner neighbour (mirror).
+template <typename S>
+__device__ __forceinline__ void edge_mu(S& ml, S& mh, S m, int v, int n, int lo, int hi)
+{
+	const S l = ml, h = mh;
+	if (v == 0 && lo != MMSPX_PERIODIC) ml = lo == MMSPX_MIRROR ? h : m;
+	if (v == n - 1 && hi != MMSPX_PERIODIC) mh = hi == MMSPX_MIRROR ? l : m;
+}
+
 // ---- Cahn-Hilliard, one (TY x TZ) tile marching over planes [xb, xe) ---------
 // Periodic grids only; DIM 2 uses a single row (TY = 1, no axis-1 halo).
 // Two bodies with identical arithmetic (and so identical bits):
@@ -169,22 +194,26 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 
 	// own points: mu-region index e = tid + k*NT; (r, c) in the mu region is
 	// phi tile (r + HY/2, c + 1), i.e. node (y0 - HY/2 + r, z0 - 1 + c)
+	constexpr int AY = 1, AZ = DIM - 1;  // the BC slots of the in-plane axes
 	int ooff[MREG], oq[MREG];
 	bool oout[MREG];
+	int ozy[MREG];  // node (y, z) of an in-grid own point, for the boundary mapping of mu
 #pragma unroll
 	for (int k = 0; k < MREG; k++) {
 		const int e = threadIdx.x + k * NT;
 		ooff[k] = 0;
 		oq[k] = 0;
 		oout[k] = false;
+		ozy[k] = 0;
 		if (e < MN) {
 			const int r = e / MZ, c = e - (e / MZ) * MZ;
 			const int y = DIM == 3 ? y0 - HY / 2 + r : 0;
 			const int z = z0 - 1 + c;
-			ooff[k] = (DIM == 3 ? wrap(y, ny) : 0) * nz + wrap(z, nz);
+			ooff[k] = (DIM == 3 ? tile_map(y, ny, g.b0[AY], g.b1[AY]) : 0) * nz + tile_map(z, nz, g.b0[AZ], g.b1[AZ]);
 			oq[k] = (r + HY / 2) * PZ + (c + 1);
 			const bool inner = (DIM == 3 ? (r >= 1 && r <= TY) : true) && c >= 1 && c <= TZ;
 			oout[k] = inner && (DIM == 3 ? y < ny : true) && z < nz;
+			ozy[k] = (y << 16) | z;
 		}
 	}
 	// ring points: the phi tile's rows 0 and PY-1 and columns 0 and PZ-1
@@ -212,26 +241,30 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 				pr = 0;
 				pc = t ? PZ - 1 : 0;
 			}
-			const int y = DIM == 3 ? wrap(y0 - HY + pr, ny) : 0;
-			roff[j] = y * nz + wrap(z0 - 2 + pc, nz);
+			const int y = DIM == 3 ? tile_map(y0 - HY + pr, ny, g.b0[AY], g.b1[AY]) : 0;
+			roff[j] = y * nz + tile_map(z0 - 2 + pc, nz, g.b0[AZ], g.b1[AZ]);
 			rq[j] = pr * PZ + pc;
 		}
 	}
 	// Loads are unconditional: lanes past the region load node 0 of the plane
-	// (offset 0) and planes wrap, so every address is inside the grid. With a
+	// (offset 0) and planes are mapped, so every address is inside the grid. With a
 	// fixed count of loads per plane and no register copies of a plane in
 	// flight, the compiler waits for a plane's values only where they are
 	// first used, one plane after the load.
 	auto load_own = [&](int x, S* v) {
-		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
+		const S* __restrict__ src = a + (int64_t)tile_map(x, nx, g.b0[0], g.b1[0]) * plane;
 #pragma unroll
 		for (int k = 0; k < MREG; k++) v[k] = src[ooff[k]];
 	};
 	auto load_ring = [&](int x, S* v) {
-		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
+		const S* __restrict__ src = a + (int64_t)tile_map(x, nx, g.b0[0], g.b1[0]) * plane;
 #pragma unroll
 		for (int j = 0; j < RREG; j++) v[j] = src[roff[j]];
 	};
+
+	// some axis is not periodic: boundary nodes read mu through the mapping
+	bool edges = false;
+	for (int d = 0; d < DIM; d++) edges |= g.b0[d] != MMSPX_PERIODIC || g.b1[d] != MMSPX_PERIODIC;
 
 	// phi queue: four register planes whose roles (P-1, P, P+1, P+2 in flight)
 	// rotate with the unrolled steps; ring phi: two planes (P, P+1 in flight)
@@ -299,10 +332,18 @@ __device__ __forceinline__ void ch_tile_rq(const typename Form<PREC>::S* __restr
 				if (oout[k]) {
 					const int e = threadIdx.x + k * NT;
 					const S m = mc[k];
+					S xh = mp[k], xl = mm[k];
+					S yh = DIM == 3 ? pmu[e + MZ] : m, yl = DIM == 3 ? pmu[e - MZ] : m;
+					S zh = pmu[e + 1], zl = pmu[e - 1];
+					if (edges) {
+						edge_mu(xl, xh, m, P - 1, nx, g.b0[0], g.b1[0]);
+						if (DIM == 3) edge_mu(yl, yh, m, ozy[k] >> 16, ny, g.b0[AY], g.b1[AY]);
+						edge_mu(zl, zh, m, ozy[k] & 0xffff, nz, g.b0[AZ], g.b1[AZ]);
+					}
 					typename F::L lap = F::lap_zero();
-					F::lap_add(lap, g.w[0], g.wf[0], mp[k], m, mm[k]);
-					if (DIM == 3) F::lap_add(lap, g.w[1], g.wf[1], pmu[e + MZ], m, pmu[e - MZ]);
-					F::lap_add(lap, g.w[DIM - 1], g.wf[DIM - 1], pmu[e + 1], m, pmu[e - 1]);
+					F::lap_add(lap, g.w[0], g.wf[0], xh, m, xl);
+					if (DIM == 3) F::lap_add(lap, g.w[1], g.wf[1], yh, m, yl);
+					F::lap_add(lap, g.w[DIM - 1], g.wf[DIM - 1], zh, m, zl);
 					dst[ooff[k]] = F::ch(fm[k], lap, p);
 				}
 			}
@@ -744,9 +785,11 @@ __device__ __forceinline__ void ch_wave_dir(const typename Form<PREC>::S* __rest
 	const int lane = threadIdx.x & 63;
 	const int nx = g.n[0], ny = g.n[1];
 	const int c = c0 - 2 + lane;
-	const unsigned cm = (unsigned)wrap(c, ny);
+	const unsigned cm = (unsigned)tile_map(c, ny, g.b0[1], g.b1[1]);
 	const bool out = lane >= 2 && lane < 2 + kChWaveOut && c < ny;
-	auto ld = [&](int x) { return a[(size_t)wrap(x, nx) * ny + cm]; };
+	const bool edges = g.b0[0] != MMSPX_PERIODIC || g.b1[0] != MMSPX_PERIODIC || g.b0[1] != MMSPX_PERIODIC ||
+	                   g.b1[1] != MMSPX_PERIODIC;
+	auto ld = [&](int x) { return a[(size_t)tile_map(x, nx, g.b0[0], g.b1[0]) * ny + cm]; };
 	// mu at row r from phi at r-d (pb), r (pc), r+d (pa); rows come in
 	// marching order, so noise (MU 2) keeps the partner row's deviate
 	int nkx = -1;
@@ -791,9 +834,14 @@ __device__ __forceinline__ void ch_wave_dir(const typename Form<PREC>::S* __rest
 			if (i + 5 < n) f[(u + RING - 1) % RING] = ld(x + (RING - 1) * d);
 			const S f0 = f[u], f1 = f[(u + 1) % RING], f2 = f[(u + 2) % RING];
 			const S ma = mu(x + d, f0, f1, f2);  // mu(x+d)
-			const S zl = from_prev(mc, mc), zh = from_next(mc, mc);
+			S zl = from_prev(mc, mc), zh = from_next(mc, mc);
+			S xh = DOWN ? mb : ma, xl = DOWN ? ma : mb;
+			if (edges) {
+				edge_mu(xl, xh, mc, x, nx, g.b0[0], g.b1[0]);
+				edge_mu(zl, zh, mc, c, ny, g.b0[1], g.b1[1]);
+			}
 			typename F::L lap = F::lap_zero();
-			F::lap_add(lap, g.w[0], g.wf[0], DOWN ? mb : ma, mc, DOWN ? ma : mb);
+			F::lap_add(lap, g.w[0], g.wf[0], xh, mc, xl);
 			F::lap_add(lap, g.w[1], g.wf[1], zh, mc, zl);
 			if (out) b[(size_t)x * ny + c] = F::ch(f0, lap, p);
 			mb = mc;

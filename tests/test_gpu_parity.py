@@ -279,6 +279,83 @@ def test_fused_ch_options_and_directions(prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
+def test_fused_ch_non_periodic(prec):
+    """Fused Cahn-Hilliard with mirror, Neumann and Dirichlet sides (each axis
+    periodic on both sides or on neither): the 3-D register-queue bodies on
+    256, 768 and 1024 threads, the 2-D wave strips (chunks ending the
+    eight-row march at each position, both directions) and register-queue row
+    strips; the double-well and polynomial mu and model B's noise; bitwise
+    against the oracle's two passes. An axis periodic on one side only stays
+    on the two-pass kernels."""
+    p = [1.0, 1.0, 0.005]
+    pp = [1.0, 1.0, 1.0, 1.0, 0.005]
+    pn = [1.0, 1.0, 1.0, 1.0, 0.005, 0.3, 9]
+    cases3 = [((20, 36, 70), [(N, N), (M, M), (D, D)]), ((13, 9, 40), [(M, M), (P, P), (N, N)]),
+              ((6, 5, 7), [(D, D), (N, N), (M, M)])]
+    for shape, bc in cases3:
+        say("[gpu] fused CH non-periodic 3-D %s %s %s" % (shape, bc, prec))
+        a = O.uniform(int(np.prod(shape)), 1600 + shape[1], 0.45, 0.55).reshape(shape)
+        b0 = O.uniform(int(np.prod(shape)), 1610 + shape[1], -1.0, 1.0).reshape(shape)
+        if prec != "f64":
+            a, b0 = a.astype(np.float32), b0.astype(np.float32)
+        exp = O.ch(a, p, 9, mu=O.MU_DW, form=prec, bc=bc)
+        expp = O.ch(b0, pp, 9, mu=O.MU_POLY, form=prec, bc=bc)
+        expn = O.ch_noise(b0, pn, 9, form=prec, bc=bc)
+        runs = [((8, 32, 5), 1), ((16, 64, 7), 1), ((32, 64, 16), 2), ((32, 64, 3), 3), (None, None)]
+        for tile, body in runs:
+            for model, arr, params, ex in (("ch_dw", a, p, exp), ("ch", b0, pp, expp), ("ch_noise", b0, pn, expn)):
+                with mx.Grid(shape, precision=prec, bc=bc) as g:
+                    if tile:
+                        g.set_variant(2)
+                        g.set_tile(*tile)
+                        g.set_tile_options(body=body)
+                    g.upload(arr)
+                    g.step(model, params, 4)
+                    g.step(model, params, 5)
+                    assert g.get_tile()["variant"] == 2, (shape, tile, model)
+                    assert np.array_equal(g.download(), ex), (shape, bc, tile, body, model)
+    for shape, bc in [((40, 300), [(N, N), (M, M)]), ((17, 5), [(D, D), (N, N)]), ((33, 64), [(M, M), (P, P)]),
+                      ((30, 61), [(P, P), (D, D)])]:
+        say("[gpu] fused CH non-periodic 2-D %s %s %s" % (shape, bc, prec))
+        a = O.uniform(int(np.prod(shape)), 1700 + shape[1], 0.45, 0.55).reshape(shape)
+        if prec != "f64":
+            a = a.astype(np.float32)
+        exp = O.ch(a, p, 11, mu=O.MU_DW, form=prec, bc=bc)
+        expn = O.ch_noise(a, pn, 11, form=prec, bc=bc)
+        for tile, body in [((1, 60, c), 0) for c in (2, 3, 5, 7, 8, 13)] + [((1, 64, 8), 1), ((1, 128, 16), 1),
+                                                                        (None, None)]:
+            with mx.Grid(shape, precision=prec, bc=bc) as g:
+                if tile:
+                    g.set_variant(2)
+                    g.set_tile(*tile)
+                    if body:
+                        g.set_tile_options(body=body)
+                g.upload(a)
+                g.step("ch_dw", p, 11)
+                assert g.get_tile()["variant"] == 2
+                assert np.array_equal(g.download(), exp), (shape, bc, tile)
+            with mx.Grid(shape, precision=prec, bc=bc) as g:
+                if tile:
+                    g.set_variant(2)
+                    g.set_tile(*tile)
+                    if body:
+                        g.set_tile_options(body=body)
+                g.upload(a)
+                g.step("ch_noise", pn, 11)
+                assert np.array_equal(g.download(), expn), (shape, bc, tile, "noise")
+    # periodic on one side only: the two passes
+    shape, bc = (12, 14, 16), [(P, N), (P, P), (P, P)]
+    a = O.uniform(int(np.prod(shape)), 1800, 0.45, 0.55).reshape(shape)
+    if prec != "f64":
+        a = a.astype(np.float32)
+    with mx.Grid(shape, precision=prec, bc=bc) as g:
+        g.upload(a)
+        g.step("ch_dw", p, 3)
+        assert g.get_tile()["variant"] == 1
+        assert np.array_equal(g.download(), O.ch(a, p, 3, mu=O.MU_DW, form=prec, bc=bc))
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
 def test_grain_growth_2d_tiles(prec):
     """The 2-D grain-growth tiles (8x64 and 16x64 cells), partial tiles and
     every boundary condition, against the oracle bitwise."""
