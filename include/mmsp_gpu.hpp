@@ -242,7 +242,7 @@ inline void download_keep_digest(resident& r, void* host)
 // at least 100 steps between host synchronisations.
 inline void run_with_progress(mmspx_state* s, int model, const double* p, int np, int steps)
 {
-	const int kMinRun = 100;  // two of the library's captured 50-step graphs
+	const int kMinRun = 100;  // at least two of the library's captured graphs (48 or 50 steps)
 	const double t0 = now_s();
 	int done = 0;
 	std::vector<int> pending;

@@ -380,11 +380,14 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	if ((ch_family(model) || model == MMSPX_OSTWALD) && !fused && !s->aux) {
 		if (hipMalloc(&s->aux, s->bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");
 	}
-	// Small grids are launch-bound: replay an even number of steps from a
-	// captured graph (the buffers are back in place after each replay). Not
-	// the stochastic forms: each launch carries its step number.
+	// Small grids are launch-bound: replay an even number of launches from a
+	// captured graph (the buffers are back in place after each replay). The
+	// graph's step count is a multiple of two launches of spl steps (50 for
+	// one step per launch, 48 for two or eight), so no launch in it is split.
+	// Not the stochastic forms: each launch carries its step number.
 	const size_t small_bytes = (size_t)64 << 20;
-	if (steps >= 2 * kGraphSteps && s->bytes <= small_bytes && !noisy_model(model)) {
+	const int graph_steps = std::max(2 * spl, kGraphSteps / (2 * spl) * (2 * spl));
+	if (steps >= 2 * kGraphSteps && steps >= graph_steps && s->bytes <= small_bytes && !noisy_model(model)) {
 		const bool same_params = same_baked_params(s->graph_params, p);
 		const bool same_tile = s->graph_tile.variant == tc.variant && s->graph_tile.ty == tc.ty &&
 		                       s->graph_tile.tz == tc.tz && s->graph_tile.chunk == tc.chunk &&
@@ -398,7 +401,7 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 			MMSPX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
 			int64_t nl = 0;
 			void *a = s->cur, *b = s->nxt;
-			for (int n : plan_passes(kGraphSteps, spl, true)) {
+			for (int n : plan_passes(graph_steps, spl, true)) {
 				rc = fused ? launch_fused_step(g, s->desc.precision, p, a, b, s->stream, tc, n, &nl)
 				           : launch_generic_step(g, s->desc.precision, p, a, b, s->aux, s->stream, &nl);
 				if (rc) break;
@@ -421,11 +424,11 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 			s->graph_tile = tc;
 			s->graph_launches = (int)nl;
 		}
-		while (steps >= kGraphSteps) {
+		while (steps >= graph_steps) {
 			MMSPX_HIP(hipGraphLaunch(s->graph, s->stream));
 			s->launches += s->graph_launches;
-			s->steps_run += kGraphSteps;
-			steps -= kGraphSteps;
+			s->steps_run += graph_steps;
+			steps -= graph_steps;
 		}
 	}
 	while (steps > 0) {

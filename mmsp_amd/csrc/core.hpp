@@ -112,7 +112,7 @@ struct mmspx_state {
 	// captured multi-step graph for small grids (launch-bound), keyed by what
 	// its launches baked in
 	hipGraphExec_t graph = nullptr;
-	int graph_steps = 0, graph_model = -1, graph_launches = 0, graph_parity = 0;
+	int graph_model = -1, graph_launches = 0, graph_parity = 0;
 	bool graph_fused = false;
 	mmspx::TuneChoice graph_tile{};
 	mmspx::ModelParams graph_params{};

@@ -105,6 +105,11 @@ CASES = [
      (0.45, 0.55)),
     ("ch_poly_2d_f32", "ch", [1.0, 1.0, 1.0, 1.0, 0.01], "f32", [(N, M), (D, N)], 0, ((130, 170), (33, 40)), 9,
      (-1.0, 1.0)),
+    # 1-D Cahn-Hilliard and an axis periodic on one side only run the two
+    # passes, whose scalar plane lives in the library's fixed device array
+    ("ch_two_pass_1d", "ch_dw", [1.0, 1.0, 0.005], "f64", [(N, M)], 0, ((5000,), (900,)), 10, (0.45, 0.55)),
+    ("ch_two_pass_2d_f32", "ch", [1.0, 1.0, 1.0, 1.0, 0.01], "f32", [(P, N), (D, D)], 0, ((90, 120), (31, 17)), 9,
+     (-1.0, 1.0)),
     ("ostwald_3d", "ostwald", [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0], "f64", [(M, N), (P, P), (D, P)], 3,
      ((16, 18, 20), (8, 9, 10)), 12, (0.0, 1.0)),
     ("ch_periodic_3d", "ch_dw", [1.0, 1.0, 0.005], "f64", None, 0, ((40, 36, 70), (12, 10, 20)), 11, (0.45, 0.55)),
@@ -142,9 +147,10 @@ def test_noalloc_noise_continues_the_step_count():
 
 
 def test_noalloc_refuses_planes_beyond_the_array_and_bad_arguments():
-    # 256 x 256 x 65 FP64 two-pass Cahn-Hilliard: a 34 MB plane > 32 MiB
-    desc = mx.make_desc((256, 256, 65), precision="f64", bc=[(N, N), (P, P), (P, P)])
-    a, b = DevBuf(256 * 256 * 65 * 8), DevBuf(256 * 256 * 65 * 8)
+    # 1-D FP64 two-pass Cahn-Hilliard of 5 000 000 nodes: a 40 MB plane > 32 MiB
+    n1 = 5_000_000
+    desc = mx.make_desc((n1,), precision="f64", bc=[(N, N)])
+    a, b = DevBuf(n1 * 8), DevBuf(n1 * 8)
     try:
         with pytest.raises(mx.MmspxError) as e:
             mx.step_noalloc(desc, a.ptr, b.ptr, "ch_dw", [1.0, 1.0, 0.005], 1)
@@ -153,8 +159,8 @@ def test_noalloc_refuses_planes_beyond_the_array_and_bad_arguments():
             mx.step_noalloc(desc, a.ptr, a.ptr, "ch_dw", [1.0, 1.0, 0.005], 1)  # same buffer twice
         with pytest.raises(mx.MmspxError):
             mx.step_noalloc(desc, a.ptr, b.ptr, "gg", [0.05], 1)  # a vector model on a scalar grid
-        # the same grid periodic takes the fused kernel, which needs no plane
-        desc = mx.make_desc((256, 256, 65), precision="f64")
+        # a 3-D grid of the same size takes the fused kernel, which needs no plane
+        desc = mx.make_desc((200, 200, 125), precision="f64", bc=[(N, N), (P, P), (P, P)])
         assert mx.step_noalloc(desc, a.ptr, b.ptr, "ch_dw", [1.0, 1.0, 0.005], 0) == 0
         assert hip().hipDeviceSynchronize() == 0
     finally:
