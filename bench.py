@@ -533,6 +533,7 @@ def bench_serve(args):
     # every request alone on the FP64 single-grid path, one after another
     errs32, errs64, mass32, mass64 = [], [], [], []
     seq_s = 0.0
+    detail = []
     for k in done:
         r = trace[k]
         served = sv.fetch(ids[k])
@@ -543,11 +544,17 @@ def bench_serve(args):
             sec, _ = g.step_timed("ch_dw", params, r["steps"])
             r64 = g.download()
         seq_s += sec
+        detail.append({"shape": list(r["shape"]), "steps": r["steps"], "fp64_alone_s": sec,
+                       "precision": infos[k]["precision"], "t_submit": infos[k]["t_submit"],
+                       "t_admit": infos[k].get("t_admit"), "t_done": infos[k]["t_done"]})
         e = float(np.abs(served - r64).max() / np.abs(r64).max())
         m = abs(infos[k]["mass1"] - infos[k]["mass0"]) / abs(infos[k]["mass0"])
         (errs64 if f64 else errs32).append((r["steps"], e))
         (mass64 if f64 else mass32).append(m)
     sv.close()
+    if args.detail_out:
+        with open(args.detail_out, "w") as f:
+            json.dump(detail, f)
     all_err = [e for _, e in errs32 + errs64]
     return {
         "metric": "solves/s",
@@ -636,6 +643,7 @@ def main():
     ap.add_argument("--hbm-budget-gb", type=int, default=128, help="serve: device memory the server may hold")
     ap.add_argument("--fp64-above", type=int, default=1000,
                     help="serve: requests of more steps run in FP64 (0: every request in FP32)")
+    ap.add_argument("--detail-out", default=None, help="serve: write per-request timings (JSON) to this file")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph-steps", type=int, default=100, help="batch_ac: steps per captured graph (0: direct launches)")
     ap.add_argument("--no-sequential", action="store_true",

@@ -484,6 +484,17 @@ int mmspx_set_tile_options(mmspx_state* s, int body)
 		s->tuned = true;
 	}
 	s->tile.opt = body;
+	// 3-D: keep a tile the body runs (32 x 32 tiles run body 7 only, body 6
+	// runs 32 x 64 tiles)
+	if (s->geom.dim == 3 && s->tile.variant == 2) {
+		if (body == 7) {
+			s->tile.ty = 32;
+			s->tile.tz = 32;
+		} else if (body == 6 || (s->tile.ty == 32 && s->tile.tz == 32)) {
+			s->tile.ty = 32;
+			s->tile.tz = 64;
+		}
+	}
 	return 0;
 }
 

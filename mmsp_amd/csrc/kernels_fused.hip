@@ -256,6 +256,42 @@ bool fused_supported(const Geom& g, int prec, int model)
 	return false;
 }
 
+// Periodic 3-D Cahn-Hilliard with a deterministic chemical potential: two
+// steps per pass on 32 x 32 tiles (768 threads, one workgroup per CU) or, in
+// FP32, 32 x 64 tiles (896 threads), whichever the cost model below favours.
+// A workgroup's time grows with its planes (chunk + 6: three halo planes of
+// the two steps on each side are recomputed) times the R3 points each thread
+// owns; a launch takes as many such rounds as its workgroups fill the 256
+// CUs. The chunk with the least rounds x time wins (ties: the longer chunk).
+// Against a sweep of every chunk on cubes of side 48 to 256 (us per step):
+// the model picks the measured best in FP64 at every side (128^3: 8-plane
+// chunks, 10.8 against 22.9 for the former 8 x 32 LDS-ring default; 192^3:
+// 32, 25.8 against 41.5; 224^3: 48, 37.8 against 59.7; 256^3: 64, 50.5) and
+// in FP32 (128^3 32 x 32 tiles with 8-plane chunks 9.2 against 18.8; 192^3
+// 32 x 64 with 16, 17.0 against 34.0; 256^3 32 x 64 with 32, 31.3).
+static TuneChoice two_step_choice(const Geom& g, int prec)
+{
+	constexpr int kCUs = 256;
+	static const int chunks[] = {6, 8, 12, 16, 24, 32, 48, 64};
+	TuneChoice best{2, 32, 32, 64, 7};
+	double best_cost = 1e300;
+	for (int body = 7; body >= 6; body--) {
+		if (body == 6 && prec != MMSPX_F32) continue;  // FP64 32 x 64 tiles spill
+		const int ty = 32, tz = body == 7 ? 32 : 64, nt = body == 7 ? 768 : 896;
+		const double own = (double)(ty + 6) * (tz + 6) / nt;  // R3 points per thread
+		const int64_t tiles = (int64_t)((g.n[1] + ty - 1) / ty) * ((g.n[2] + tz - 1) / tz);
+		for (int c : chunks) {
+			const int64_t wgs = tiles * ((g.n[0] + c - 1) / c);
+			const double cost = (double)((wgs + kCUs - 1) / kCUs) * (c + 6) * own;
+			if (cost <= best_cost) {
+				best_cost = cost;
+				best = TuneChoice{2, ty, tz, c, body};
+			}
+		}
+	}
+	return best;
+}
+
 // Defaults come from sweeps of the candidates below on MI355X (the figures
 // quoted are microseconds per step from those sweeps).
 TuneChoice default_choice(const Geom& g, int prec, int model)
@@ -342,6 +378,7 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	if (!all_periodic(g))
 		return nodes < (int64_t)16 * 64 * 32 * 512 ? TuneChoice{2, 8, 32, 16, 1}
 		                                            : TuneChoice{2, 32, 64, prec == MMSPX_F64 ? 32 : 16, 2};
+	if (model != MMSPX_CH_NOISE && (prec == MMSPX_F64 || prec == MMSPX_F32)) return two_step_choice(g, prec);
 	if (nodes < (int64_t)16 * 64 * 32 * 512) return TuneChoice{2, 8, 32, 16, 0};
 	// 256^3: register queues on 32 x 64 tiles with 768 threads. FP64 with
 	// 32-plane chunks, one workgroup per CU (52.3 us per step, against 53.0
@@ -353,14 +390,6 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// 97.6 with 768 in one sweep, 92.6 against 91.6 in another: the two are
 	// within the noise; 16-plane chunks take 99-107 us)
 	if (model == MMSPX_CH_NOISE && prec == MMSPX_F64) return TuneChoice{2, 32, 64, 32, 3};
-	// Two steps per pass where the chemical potential is deterministic: 256^3
-	// FP64 on 32 x 32 tiles with 64-plane chunks (52.8 us per step against 59.9
-	// for the one-step register queues on the same box; 32-plane chunks 56.0,
-	// 16-plane 62.5; 32 x 64 tiles spill in FP64, 70.1), FP32 on 32 x 64 tiles
-	// with 32-plane chunks (32.9 against 38.0; 16-plane 37.1, 64-plane 50.4;
-	// 32 x 32 tiles 38.2)
-	if (model != MMSPX_CH_NOISE && prec == MMSPX_F64) return TuneChoice{2, 32, 32, 64, 7};
-	if (model != MMSPX_CH_NOISE && prec == MMSPX_F32) return TuneChoice{2, 32, 64, 32, 6};
 	return TuneChoice{2, 32, 64, prec == MMSPX_F64 ? 32 : 16, 2};
 }
 
@@ -428,7 +457,7 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		for (int c : {20, 22, 24}) add(16, 64, c, 1);
 		// two steps per pass (deterministic chemical potentials, periodic grids)
 		if ((model == MMSPX_CH || model == MMSPX_CH_DW) && !edges)
-			for (int c : {16, 32, 64}) {
+			for (int c : {6, 8, 12, 16, 24, 32, 48, 64}) {
 				add(32, 64, c, 6);
 				add(32, 32, c, 7);
 			}

@@ -20,15 +20,20 @@
 //     step to fill the GPU (plus any request left out for too long); it lasts
 //     until one of them finishes or `chunk_steps` steps, and is one
 //     persistent launch per step over every member's tiles (2-D and 3-D
-//     members in two launches). It is planned on the host and queued on the
+//     members in two launches). 3-D members of 2^21 nodes or more fill the
+//     GPU on their own: they run the single-grid kernels instead (for
+//     periodic Cahn-Hilliard two steps per pass), one launch per member and
+//     pass. A round is planned on the host and queued on the
 //     compute stream; up to two rounds are queued ahead, so the device does
 //     not wait for the host between rounds;
 //   * when a request's last round is queued, the downloader waits for that
 //     round, takes the FP64 mass of the final state, copies it out and frees
 //     its buffers; only then is its memory counted free for admission.
 // Uploads and downloads therefore overlap the compute of the other requests.
-// A member's state after s steps sits in buf[s & 1], so joining mid-stream or
-// sitting out rounds needs no copy. The tile bodies are the
+// A batched member's state after s steps sits in buf[s & 1], so joining
+// mid-stream or sitting out rounds needs no copy; a member on the single-grid
+// kernels keeps the index of the buffer holding its state (a two-step pass
+// swaps the buffers once). The tile bodies are the
 // single-grid kernels' (tiles.hpp): a request's result does not depend on what
 // else was resident.
 #include <algorithm>
@@ -49,6 +54,7 @@ constexpr int kSlots = 4;        // work-list buffers cycled by queued rounds
 constexpr int kRoundsAhead = 2;  // rounds queued on the compute stream at most
 constexpr int64_t kRoundNodes = (int64_t)1 << 25;  // nodes per step a round aims to hold
 constexpr int kMaxIdleRounds = 64;                 // a resident request advances at least this often
+constexpr int64_t kSoloNodes = (int64_t)1 << 21;   // 3-D members this large run the single-grid kernels
 
 struct Request {
 	int64_t id = 0;
@@ -66,6 +72,9 @@ struct Request {
 	size_t bytes = 0;     // one state buffer (request precision, padded to 256 B)
 	char* dev = nullptr;  // buf[0] at dev, buf[1] at dev + bytes
 	int idle_rounds = 0;             // rounds planned since it last advanced
+	bool solo = false;               // runs the single-grid kernels with `tile`
+	TuneChoice tile{};
+	int cur = 0;                     // solo: the buffer holding the state after `planned` steps
 	hipEvent_t ev_last = nullptr;    // recorded after its last round (or when dropped)
 	std::vector<char> init;          // initial state, until uploaded
 	std::vector<char> out;           // final state
@@ -205,7 +214,8 @@ static void uploader_main(mmspx_server* sv)
 static int download_one(mmspx_server* sv, Request* r, std::vector<double>& part_host)
 {
 	MMSPX_HIP(hipStreamWaitEvent(sv->dst, r->ev_last, 0));
-	const void* s = r->dev + ((r->steps & 1) ? r->bytes : 0);
+	const int last = r->solo ? r->cur : (r->steps & 1);
+	const void* s = r->dev + (last ? r->bytes : 0);
 	int rc = launch_reduce(r->g, r->prec, MMSPX_MASS, r->p, s, sv->dpart, sv->dst);
 	if (rc) return rc;
 	MMSPX_HIP(hipMemcpyAsync(part_host.data(), sv->dpart, part_host.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -322,6 +332,7 @@ static int enqueue_round(mmspx_server* sv)
 	// requests of an FP32 server; each group is one launch per step and kind
 	std::vector<BItem> l2[2], l3[2];
 	for (size_t m = 0; m < act.size(); m++) {
+		if (act[m]->solo) continue;
 		const int grp = act[m]->prec != sv->cfg.precision;
 		member_items((int)m, act[m]->g, sv->cfg.model, sv->tile, l2[grp], l3[grp]);
 	}
@@ -359,6 +370,20 @@ static int enqueue_round(mmspx_server* sv)
 			rc = launch_batch_step(sv->cfg.model, form[grp], 1, sv->tile, sl.d_mem, it, n2[grp], it + n2[grp], n3[grp],
 			                       (int)((sv->gstep + k) & 1), sv->cus, sv->st, &launches);
 			if (rc) return rc;
+		}
+	}
+	// the large 3-D members: their own single-grid launches
+	for (Request* r : act) {
+		if (!r->solo) continue;
+		const int spl = fused_steps_per_launch(sv->cfg.model, r->tile);
+		for (int left = steps; left > 0;) {
+			const int n = std::min(left, spl);
+			const char* a = r->dev + (r->cur ? r->bytes : 0);
+			char* b = r->dev + (r->cur ? 0 : r->bytes);
+			rc = launch_fused_step(r->g, r->prec, r->p, a, b, sv->st, r->tile, n, &launches);
+			if (rc) return rc;
+			r->cur ^= 1;
+			left -= n;
 		}
 	}
 	Round rd;
@@ -658,6 +683,13 @@ int mmspx_server_submit(mmspx_server* sv, const mmspx_desc* desc, const double* 
 		return rc;
 	}
 	r->steps = steps;
+	if (r->g.dim == 3 && r->g.nodes >= kSoloNodes && fused_supported(r->g, r->prec, sv->cfg.model)) {
+		const TuneChoice t = default_choice(r->g, r->prec, sv->cfg.model);
+		if (t.variant == 2) {
+			r->solo = true;
+			r->tile = t;
+		}
+	}
 	const size_t n = (size_t)r->g.nodes;
 	r->bytes = (n * elem_of(r) + 255) / 256 * 256;
 	r->init.resize(n * elem_of(r));

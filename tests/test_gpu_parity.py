@@ -181,10 +181,11 @@ def test_fused_ch_options_and_directions(prec):
                     g.step("ch_dw", p, 13)
                     assert np.array_equal(g.download(), ex), (arr.shape, body, chunk)
     # two steps per pass (body 6 on 32 x 64 tiles, 7 on 32 x 32): 13 steps are
-    # six two-step passes and one single step; chunks of 2, 5, 7, 16 and 32
-    # planes, partial tiles, a grid smaller than one tile
+    # six two-step passes and one single step; chunks of 2, 5 and 7 planes and
+    # every chunk the default picks (6 .. 48: the last covers the whole
+    # grid), partial tiles, a grid smaller than one tile
     for body, tile in ((6, (32, 64)), (7, (32, 32))):
-        for chunk in (2, 5, 7, 16, 32):
+        for chunk in (2, 5, 6, 7, 8, 12, 16, 24, 32, 48):
             for arr, ex in ((a, exp13), (a_s, exp_s)):
                 with mx.Grid(arr.shape, precision=prec) as g:
                     g.set_variant(2)
@@ -276,6 +277,27 @@ def test_fused_ch_options_and_directions(prec):
         g.set_tile_options(body=7)
         with pytest.raises(mx.MmspxError):
             g.step("ch_noise", [1.0, 1.0, 1.0, 1.0, 0.005, 0.3, 5], 2)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_two_step_defaults_on_mid_sizes(prec):
+    """Periodic 3-D Cahn-Hilliard below 256^3 takes the two-step bodies with
+    the chunk the cost model picks (odd sizes, non-cubic, 6- to 48-plane
+    chunks, both tile shapes in FP32); 31 steps (15 passes and one single
+    step), bitwise against the oracle."""
+    p = [1.0, 1.0, 0.005]
+    for shape in [(112, 112, 112), (100, 90, 130), (100, 224, 200), (30, 40, 50)]:
+        say("[gpu] two-step default %s %s" % (shape, prec))
+        a = O.uniform(int(np.prod(shape)), 83, 0.45, 0.55).reshape(shape)
+        if prec == "f32":
+            a = a.astype(np.float32)
+        with mx.Grid(shape, precision=prec) as g:
+            g.upload(a)
+            g.step("ch_dw", p, 31)
+            t = g.get_tile()
+            got = g.download()
+        assert t["variant"] == 2 and t["opt"] in (6, 7), t
+        assert np.array_equal(got, O.ch(a, p, 31, mu=O.MU_DW, form=prec)), (shape, t)
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])

@@ -92,6 +92,32 @@ def test_fp32_server_runs_long_requests_in_fp64():
         assert np.array_equal(outs[k], exp), (k, shape, steps)
 
 
+def test_large_3d_members_run_the_single_grid_kernels():
+    """3-D requests of 2^21 nodes or more advance on the single-grid kernels
+    (two steps per pass) in the same rounds as the batched ones: rounds of
+    odd lengths (members finish at odd steps, 16-step chunks), FP32 and the
+    FP64 form; each result bitwise the oracle's run of it, and the large FP64
+    one equal to the single-grid path's run."""
+    specs = [((130, 128, 128), 37, 51), ((128, 130, 129), 23, 52), ((20, 18, 30), 25, 53), ((40, 64), 41, 54),
+             ((129, 129, 128), 9, 55)]
+    say("[gpu] server with large 3-D members")
+    with mx.Server("ch_dw", "f32", max_slots=8, chunk_steps=16, fp64_above_steps=30) as sv:
+        ids = [sv.submit(req(shape, seed), P_DW, steps) for shape, steps, seed in specs]
+        infos = [sv.wait(i, 120) for i in ids]
+        outs = [sv.fetch(i) for i in ids]
+        st = sv.stats()
+    assert st["completed"] == len(specs) and st["worker_error"] == 0
+    for k, (shape, steps, seed) in enumerate(specs):
+        long = steps > 30
+        assert infos[k]["precision"] == ("f64" if long else "f32"), k
+        exp = O.ch(req(shape, seed), P_DW, steps, mu=O.MU_DW) if long else oracle_f32(req(shape, seed), P_DW, steps)
+        assert np.array_equal(outs[k], exp), (k, shape, steps)
+    with mx.Grid(specs[0][0], precision="f64") as g:
+        g.upload(req(specs[0][0], specs[0][2]))
+        g.step("ch_dw", P_DW, specs[0][1])
+        assert np.array_equal(g.download(), outs[0])
+
+
 def test_fp32_results_at_the_routing_threshold_hold_1e5():
     """At the benchmark stream's threshold (1000 steps), FP32-served results
     stay within 1e-5 normwise of the FP64 form from the same float-rounded
