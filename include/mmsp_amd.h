@@ -261,7 +261,7 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
  * per pass over HBM (6 on 32 x 64 tiles, 7 on 32 x 32 tiles; an odd last
  * step runs body 2). On a 3-D grid the tile follows the body: 7 takes 32 x
  * 32 tiles, 6 takes 32 x 64, and the other bodies leave 32 x 32 for 32 x 64.
- * 3-D Allen-Cahn, model A, grain growth and Zener pinning
+ * 3-D Allen-Cahn, model A, grain growth, Zener pinning and multi-phase grain growth
  * (2, 4 or 8 fields), any boundary: 8 marches the tile along axis 0 (8, 16,
  * 32 x 64 and 16 x 32 tiles for Allen-Cahn; 8 x 64 and 16 x 32 for the
  * vector models). */

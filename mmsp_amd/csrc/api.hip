@@ -485,9 +485,15 @@ int mmspx_set_tile_options(mmspx_state* s, int body)
 	}
 	s->tile.opt = body;
 	// 3-D: keep a tile the body runs (32 x 32 tiles run body 7 only, body 6
-	// runs 32 x 64 tiles)
+	// runs 32 x 64 tiles, the plane march 8 x 64 for every model)
 	if (s->geom.dim == 3 && s->tile.variant == 2) {
-		if (body == 7) {
+		const int ty = s->tile.ty, tz = s->tile.tz;
+		if (body == 8) {
+			if (!((ty == 8 && tz == 64) || (ty == 16 && tz == 32) || (ty == 16 && tz == 64) || (ty == 32 && tz == 64))) {
+				s->tile.ty = 8;
+				s->tile.tz = 64;
+			}
+		} else if (body == 7) {
 			s->tile.ty = 32;
 			s->tile.tz = 32;
 		} else if (body == 6 || (s->tile.ty == 32 && s->tile.tz == 32)) {

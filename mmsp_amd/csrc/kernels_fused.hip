@@ -233,7 +233,7 @@ bool fused_supported(const Geom& g, int prec, int model)
 	if (g.dim == 3) {
 		if (model == MMSPX_AC || model == MMSPX_AC_NOISE) return true;
 		if (model == MMSPX_GG) return march_q_compiled(g.fields);
-		if (model == MMSPX_ZENER) return march_q_compiled(g.fields) && prec != MMSPX_F32REF;
+		if (model == MMSPX_ZENER || model == MMSPX_GG_ANISO) return march_q_compiled(g.fields) && prec != MMSPX_F32REF;
 	}
 	switch (model) {
 	case MMSPX_CH:
@@ -298,7 +298,8 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 {
 	if (!fused_supported(g, prec, model)) return TuneChoice{1, 0, 0, 0, 0};
 	const int64_t nodes = g.nodes;
-	// 3-D Allen-Cahn, model A, grain growth and Zener pinning: the plane march.
+	// 3-D Allen-Cahn, model A, grain growth, Zener pinning and multi-phase
+	// grain growth: the plane march.
 	// 256^3 sweep (us per step): Allen-Cahn FP64 16 x 32 tiles with 32-plane
 	// chunks 44.5 (6.0 TB/s; 8 x 64 45.0, 32 x 64 46.5, one thread per node
 	// 93.5), FP32 32 x 64 with 16-plane chunks 24.6 (16 x 64 24.8; 32-plane
@@ -306,6 +307,20 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// 252 (16 x 32 275, one thread per node 430), FP64 484 (935); Zener 4
 	// fields FP32 122 (211). Chunks are halved until the launch has 512
 	// workgroups (two per CU) or more.
+	// Multi-phase grain growth is bound by its per-node arithmetic, not by
+	// bytes: one-hot grains with 8 fields, 128^3 FP64 200 us per step on 8 x
+	// 64 tiles with 16-plane chunks (256 workgroups; 206 with 8-plane chunks,
+	// 374 with 32) against 253 for one thread per node, FP32 87 against 155;
+	// at 64^3 one thread per node wins (41.5 against 54.1), so the march
+	// starts at 2^21 nodes, with 16-plane chunks halved to no less than 8
+	// until the launch has 256 workgroups.
+	if (g.dim == 3 && model == MMSPX_GG_ANISO) {
+		if (nodes < ((int64_t)1 << 21)) return TuneChoice{1, 0, 0, 0, 0};
+		const int64_t tiles = (int64_t)((g.n[1] + 7) / 8) * ((g.n[2] + 63) / 64);
+		int chunk = 16;
+		while (chunk > 8 && tiles * ((g.n[0] + chunk - 1) / chunk) < 256) chunk /= 2;
+		return TuneChoice{2, 8, 64, chunk, 8};
+	}
 	if (g.dim == 3 && (model == MMSPX_AC || model == MMSPX_AC_NOISE || model == MMSPX_GG || model == MMSPX_ZENER)) {
 		const bool scalar = model == MMSPX_AC || model == MMSPX_AC_NOISE;
 		const int ty = scalar ? (prec == MMSPX_F64 ? 16 : 32) : 8;
@@ -568,7 +583,7 @@ static int launch_march(const Geom& g, const ModelParams& p, const void* a, void
                         const TuneChoice& t)
 {
 	const int chunk = t.chunk > 0 ? t.chunk : 32;
-	if (t.opt != 8) return fail(MMSPX_EINVAL, "3-D Allen-Cahn, grain growth and Zener pinning run tile body 8");
+	if (t.opt != 8) return fail(MMSPX_EINVAL, "3-D Allen-Cahn, grain growth, Zener pinning and multi-phase grain growth run tile body 8");
 	if constexpr (KIND == 0 || KIND == 2) {
 		if (t.ty == 8 && t.tz == 64) return launch_march_body<PREC, 1, KIND, 8, 64, 256>(g, p, a, b, st, chunk);
 		if (t.ty == 16 && t.tz == 64) return launch_march_body<PREC, 1, KIND, 16, 64, 256>(g, p, a, b, st, chunk);
@@ -886,6 +901,10 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		return fail(MMSPX_EUNSUPPORTED, "Zener pinning runs the wave strips (1 x 64, body 2) or the generic kernel");
 	case MMSPX_GG_ANISO:
+		if (g.dim == 3) {
+			if constexpr (PREC == MMSPX_F32REF) return fail(MMSPX_EUNSUPPORTED, "no 3-D march for this form");
+			else return launch_march<PREC, 4>(g, p, a, b, st, t);
+		}
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		return fail(MMSPX_EUNSUPPORTED, "multi-phase grain growth runs the wave strips (1 x 64, body 2) or the generic kernel");
 	case MMSPX_AC_NOISE:

@@ -619,7 +619,10 @@ __device__ __forceinline__ void ch_tile_rq2(const typename Form<PREC>::S* __rest
 // writes and reads. The per-node arithmetic is the generic kernels' (the same
 // bits). KIND 0 Allen-Cahn (allen-cahn.cpp:68-71), 1 grain growth
 // (graingrowth.cpp:91-107), 2 model A (model_A.cpp:97-101), 3 Zener pinning
-// (zener.cpp:117-135: field 0 held).
+// (zener.cpp:117-135: field 0 held), 4 multi-phase grain growth
+// (grain_growth/anisotropic/phase_field/graingrowth.cpp:108-188: the fields
+// present among the node and its six neighbours, then gga_node; fewer than
+// two present, the node is copied).
 template <int PREC, int Q, int KIND, int TY, int TZ, int NT>
 __device__ __forceinline__ void march3d(const typename Form<PREC>::S* __restrict__ a,
                                         typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
@@ -734,6 +737,34 @@ __device__ __forceinline__ void march3d(const typename Form<PREC>::S* __restrict
 			}
 			float gd = 0.0f;
 			if constexpr (KIND == 2) gd = normal_deviate_cached(p.seed, p.step, P, (uint32_t)ooff[k], nkx, nkeep[k]);
+			if constexpr (KIND == 4) {
+				// the active set and the laplacians of its fields, axis by axis
+				// as the generic kernel forms them
+				S phi[Q], lap[Q];
+				uint32_t act = 0;
+				S cnt = 0;
+#pragma unroll
+				for (int q = 0; q < Q; q++) {
+					const S y = fc[q][k];
+					const S y1h = lp[q][qq + PZ], y1l = lp[q][qq - PZ], y2h = lp[q][qq + 1], y2l = lp[q][qq - 1];
+					phi[q] = y;
+					const bool on = y > S(0) || fp[q][k] > S(0) || fm[q][k] > S(0) || y1h > S(0) || y1l > S(0) ||
+					                y2h > S(0) || y2l > S(0);
+					typename F::L l = F::lap_zero();
+					if (on) {
+						act |= 1u << q;
+						cnt += S(1);
+						F::lap_add(l, g.w[0], g.wf[0], fp[q][k], y, fm[q][k]);
+						F::lap_add(l, g.w[1], g.wf[1], y1h, y, y1l);
+						F::lap_add(l, g.w[2], g.wf[2], y2h, y, y2l);
+					}
+					lap[q] = l;
+				}
+				if (cnt >= S(2)) gga_node<PREC, Q>(phi, lap, act, cnt, Q, p);
+#pragma unroll
+				for (int q = 0; q < Q; q++) dst[q * N + ooff[k]] = phi[q];
+				continue;
+			}
 #pragma unroll
 			for (int q = 0; q < Q; q++) {
 				const S y = fc[q][k];

@@ -732,8 +732,9 @@ def test_multiphase_wave_strips(prec):
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
 def test_march_3d(prec):
-    """The 3-D plane march (tile body 8): Allen-Cahn, model A, grain growth
-    and Zener pinning with 2, 4 and 8 fields, every compiled tile, chunks of 2,
+    """The 3-D plane march (tile body 8): Allen-Cahn, model A, grain growth,
+    Zener pinning and multi-phase grain growth with 2, 4 and 8 fields, every
+    compiled tile, chunks of 2,
     3, 5, 7 and 16 planes (the four-plane unrolled march ends at each of its
     positions), partial tiles, grids smaller than one tile, every boundary
     condition on every axis; bitwise against the oracle."""
@@ -791,6 +792,54 @@ def test_march_3d(prec):
                         g.upload(a)
                         g.step("zener", [0.01], 7)
                         assert np.array_equal(g.download(), expz), (shape, q, tile, chunk, "zener")
+    # multi-phase grain growth: one-hot grains and sparse random fields (every
+    # active-set size, both paths of the 8-field kernels)
+    if prec != "f32ref":
+        import importlib.util
+
+        spec = importlib.util.spec_from_file_location("make_golden", os.path.join(os.path.dirname(__file__), "golden",
+                                                                                  "make_golden.py"))
+        mg = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mg)
+        pg = [0.01, 8.0, 1.0, 1.0]
+        for shape, q, bc, kind in [((11, 20, 70), 2, bcs[1], "sparse"), ((9, 17, 33), 4, bcs[2], "grains"),
+                                   ((10, 12, 40), 8, bcs[3], "sparse"), ((12, 18, 66), 8, bcs[0], "grains"),
+                                   ((5, 6, 7), 4, bcs[1], "sparse")]:
+            say("[gpu] march 3-D gg_aniso %s q=%d %s %s" % (shape, q, kind, prec))
+            n = int(np.prod(shape))
+            if kind == "grains":
+                a = mg.one_hot_grains(shape, q, 1600 + q)
+            else:
+                v = O.uniform(n * q, 1610 + q, 0.0, 1.0).reshape(shape + (q,))
+                keep = O.uniform(n * q, 1620 + q, 0.0, 1.0).reshape(shape + (q,)) < 0.3
+                a = np.where(keep, v, 0.0)
+            if prec != "f64":
+                a = a.astype(np.float32)
+            exp = O.gg_aniso(a, pg, 7, form=prec, bc=bc)
+            for tile in vec_tiles:
+                for chunk in (2, 3, 5, 16):
+                    with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+                        g.set_variant(2)
+                        g.set_tile(tile[0], tile[1], chunk)
+                        g.set_tile_options(body=8)
+                        g.upload(a)
+                        g.step("gg_aniso", pg, 7)
+                        assert np.array_equal(g.download(), exp), (shape, q, tile, chunk, kind)
+            with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+                g.upload(a)
+                g.step("gg_aniso", pg, 3)
+                g.step("gg_aniso", pg, 4)
+                assert np.array_equal(g.download(), exp), (shape, q, "default", kind)
+        # the default takes the march from 2^21 nodes
+        shape = (130, 128, 128)
+        a = mg.one_hot_grains(shape, 4, 1630)
+        if prec != "f64":
+            a = a.astype(np.float32)
+        with mx.Grid(shape, fields=4, vector=True, precision=prec) as g:
+            g.upload(a)
+            g.step("gg_aniso", pg, 3)
+            assert g.get_tile()["opt"] == 8
+            assert np.array_equal(g.download(), O.gg_aniso(a, pg, 3, form=prec))
     # the defaults take the march; other field counts stay on the generic kernel
     a = O.uniform(10 * 12 * 14 * 3, 1500, 0.0, 1.0).reshape(10, 12, 14, 3)
     if prec != "f64":
