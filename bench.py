@@ -81,6 +81,15 @@ WORKLOADS = {
     # 3-D grain growth (graingrowth.cpp:84-105 with dt = 0.05), 256^3 with 8 fields, FP32
     "gg3d": dict(model="gg", shape=(256, 256, 256), prec="f32", params=[0.05], steps=200, seed=20260113,
                  lo=0.0, hi=0.01, fields=8, bytes_per_cell=2 * 8 * 4, name="grain-growth-3d-q8", cpu_steps=1),
+    # 3-D Ostwald ripening (ostwald.cpp constants as ostwald2d), 256^3 with C and one order parameter (the
+    # example's two fields), FP64: the two passes as plane marches
+    "ostwald3d": dict(model="ostwald", shape=(256, 256, 256), prec="f64",
+                      params=[0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0], steps=200, seed=20260114,
+                      lo=0.0, hi=1.0, fields=2, bytes_per_cell=2 * 2 * 8, name="ostwald-3d-q2", cpu_steps=1),
+    # 3-D multi-phase grain growth, 128^3 with 8 grains as one-hot blocks of 16 nodes, FP64
+    "gganiso3d": dict(model="gg_aniso", shape=(128, 128, 128), prec="f64", params=[0.01, 8.0, 1.0, 1.0], steps=200,
+                      seed=20260115, lo=0.0, hi=1.0, fields=8, bytes_per_cell=2 * 8 * 8, name="multiphase-gg-3d-q8",
+                      cpu_steps=1, init="grains"),
     "model_b3d": dict(model="ch_noise", shape=(256, 256, 256), prec="f64",
                       params=[1.0, 1.0, 1.0, 1.0, 0.01, math.sqrt(2.0 * 0.01 / (0.01 * 1.0)), 20260107], steps=2000,
                       seed=20260108, lo=-0.01, hi=0.01, fields=1, bytes_per_cell=16, name="model-b-3d", cpu_steps=2),

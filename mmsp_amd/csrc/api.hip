@@ -369,7 +369,9 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	if (s->variant != 1 && fused_supported(g, s->desc.precision, model)) {
 		tc = s->tuned ? s->tile : default_choice(g, s->desc.precision, model);
 		if (s->variant == 2 && tc.variant != 2) tc = default_choice(g, s->desc.precision, model);
-		if (s->variant == 2 && tc.variant != 2) tc = TuneChoice{2, 1, 128, 16, 0};  // strips where generic is the default
+		// where one thread per node is the default: the 3-D plane march, the 2-D strips
+		if (s->variant == 2 && tc.variant != 2)
+			tc = g.dim == 3 ? TuneChoice{2, 8, 64, 16, 8} : TuneChoice{2, 1, 128, 16, 0};
 		fused = s->variant == 2 || tc.variant == 2;
 	} else if (s->variant == 2) {
 		return fail(MMSPX_EUNSUPPORTED, "no tiled/fused kernel for this model, shape and boundary condition");
@@ -377,7 +379,8 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	s->last = fused ? tc : TuneChoice{1, 0, 0, 0, 0};
 	s->has_last = true;
 	const int spl = fused ? fused_steps_per_launch(model, tc) : 1;
-	if ((ch_family(model) || model == MMSPX_OSTWALD) && !fused && !s->aux) {
+	if ((((ch_family(model) || model == MMSPX_OSTWALD) && !fused) || (fused && fused_needs_plane(g, model))) &&
+	    !s->aux) {
 		if (hipMalloc(&s->aux, s->bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");
 	}
 	// Small grids are launch-bound: replay an even number of launches from a
@@ -402,7 +405,7 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 			int64_t nl = 0;
 			void *a = s->cur, *b = s->nxt;
 			for (int n : plan_passes(graph_steps, spl, true)) {
-				rc = fused ? launch_fused_step(g, s->desc.precision, p, a, b, s->stream, tc, n, &nl)
+				rc = fused ? launch_fused_step(g, s->desc.precision, p, a, b, s->stream, tc, n, &nl, s->aux)
 				           : launch_generic_step(g, s->desc.precision, p, a, b, s->aux, s->stream, &nl);
 				if (rc) break;
 				std::swap(a, b);
@@ -434,7 +437,7 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	while (steps > 0) {
 		const int n = std::min(steps, spl);
 		p.step = (uint64_t)s->steps_run;
-		rc = fused ? launch_fused_step(g, s->desc.precision, p, s->cur, s->nxt, s->stream, tc, n, &s->launches)
+		rc = fused ? launch_fused_step(g, s->desc.precision, p, s->cur, s->nxt, s->stream, tc, n, &s->launches, s->aux)
 		           : launch_generic_step(g, s->desc.precision, p, s->cur, s->nxt, s->aux, s->stream, &s->launches);
 		if (rc) return rc;
 		std::swap(s->cur, s->nxt);

@@ -141,7 +141,10 @@ bool fused_supported(const Geom& g, int prec, int model);
 // One launch of the tiled kernels; it advances `nsteps` steps, which must not
 // exceed fused_steps_per_launch (1 except for temporal blocks).
 int launch_fused_step(const Geom& g, int prec, const ModelParams& p, const void* a, void* b, hipStream_t st,
-                      const TuneChoice& t, int nsteps, int64_t* launches);
+                      const TuneChoice& t, int nsteps, int64_t* launches, void* aux = nullptr);
+// device scratch the fused kernels of `model` need beside the two state
+// buffers: one scalar plane for 3-D Ostwald ripening (w), none otherwise
+bool fused_needs_plane(const Geom& g, int model);
 int fused_steps_per_launch(int model, const TuneChoice& t);
 TuneChoice default_choice(const Geom& g, int prec, int model);
 int launch_reduce(const Geom& g, int prec, int what, const ModelParams& p, const void* a, double* part,

@@ -80,7 +80,8 @@ extern "C" int mmspx_step_noalloc(const mmspx_desc* desc, void* buf0, void* buf1
 		fused = tc.variant == 2;
 	}
 	const size_t elem = desc->precision == MMSPX_F64 ? 8 : 4;
-	const bool needs_plane = !fused && (ch_family(model) || model == MMSPX_OSTWALD);
+	const bool needs_plane =
+	    (!fused && (ch_family(model) || model == MMSPX_OSTWALD)) || (fused && fused_needs_plane(g, model));
 	if (needs_plane && (size_t)g.nodes * elem > (size_t)MMSPX_NOALLOC_SCRATCH_BYTES)
 		return fail(MMSPX_EINVAL, "this grid's two-pass kernels need a " + std::to_string((size_t)g.nodes * elem) +
 		                              "-byte scalar plane; mmspx_step_noalloc holds at most " +
@@ -94,7 +95,7 @@ extern "C" int mmspx_step_noalloc(const mmspx_desc* desc, void* buf0, void* buf1
 	while (steps > 0) {
 		const int n = std::min(steps, spl);
 		p.step = (uint64_t)(first_step + done);
-		rc = fused ? launch_fused_step(g, desc->precision, p, a, b, st, tc, n, &launches)
+		rc = fused ? launch_fused_step(g, desc->precision, p, a, b, st, tc, n, &launches, needs_plane ? sp : nullptr)
 		           : launch_generic_step(g, desc->precision, p, a, b, needs_plane ? sp : nullptr, st, &launches);
 		if (rc) return rc;
 		std::swap(a, b);

@@ -622,11 +622,17 @@ __device__ __forceinline__ void ch_tile_rq2(const typename Form<PREC>::S* __rest
 // (zener.cpp:117-135: field 0 held), 4 multi-phase grain growth
 // (grain_growth/anisotropic/phase_field/graingrowth.cpp:108-188: the fields
 // present among the node and its six neighbours, then gga_node; fewer than
-// two present, the node is copied).
+// two present, the node is copied). Ostwald ripening (ostwald.cpp:87-122) in
+// two marches, the generic kernels' two passes: KIND 5 forms w of field 0
+// into `wout` (one scalar plane per node) and writes the order parameters
+// (fields 1..Q-1) to b; KIND 6 marches over w (a = w, Q = 1) and writes
+// field 0 of b, C + dt*D*lap(w), reading C at the node from `cin`.
 template <int PREC, int Q, int KIND, int TY, int TZ, int NT>
 __device__ __forceinline__ void march3d(const typename Form<PREC>::S* __restrict__ a,
                                         typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
-                                        int y0, int z0, int xb, int xe)
+                                        int y0, int z0, int xb, int xe,
+                                        const typename Form<PREC>::S* __restrict__ cin = nullptr,
+                                        typename Form<PREC>::S* __restrict__ wout = nullptr)
 {
 	using F = Form<PREC>;
 	using S = typename F::S;
@@ -737,6 +743,35 @@ __device__ __forceinline__ void march3d(const typename Form<PREC>::S* __restrict
 			}
 			float gd = 0.0f;
 			if constexpr (KIND == 2) gd = normal_deviate_cached(p.seed, p.step, P, (uint32_t)ooff[k], nkx, nkeep[k]);
+			if constexpr (KIND == 5) {
+				// Ostwald pass 1: sum_j phi_j^2 over the order parameters, w
+				// from lap C, then each order parameter's local update
+				typename F::OSum osum = 0;
+#pragma unroll
+				for (int q = 1; q < Q; q++) osum += fc[q][k] * fc[q][k];
+				const S C = fc[0][k];
+#pragma unroll
+				for (int q = 0; q < Q; q++) {
+					const S y = fc[q][k];
+					typename F::L lap = F::lap_zero();
+					F::lap_add(lap, g.w[0], g.wf[0], fp[q][k], y, fm[q][k]);
+					F::lap_add(lap, g.w[1], g.wf[1], lp[q][qq + PZ], y, lp[q][qq - PZ]);
+					F::lap_add(lap, g.w[2], g.wf[2], lp[q][qq + 1], y, lp[q][qq - 1]);
+					if (q == 0) wout[(int64_t)P * plane + ooff[k]] = F::ow_w(C, osum, lap, p);
+					else dst[q * N + ooff[k]] = F::ow_phi(y, C, osum, lap, p);
+				}
+				continue;
+			}
+			if constexpr (KIND == 6) {
+				// Ostwald pass 2: C' = C + dt*D*lap(w)
+				const S y = fc[0][k];
+				typename F::L lap = F::lap_zero();
+				F::lap_add(lap, g.w[0], g.wf[0], fp[0][k], y, fm[0][k]);
+				F::lap_add(lap, g.w[1], g.wf[1], lp[0][qq + PZ], y, lp[0][qq - PZ]);
+				F::lap_add(lap, g.w[2], g.wf[2], lp[0][qq + 1], y, lp[0][qq - 1]);
+				dst[ooff[k]] = F::ow_c(cin[(int64_t)P * plane + ooff[k]], lap, p);
+				continue;
+			}
 			if constexpr (KIND == 4) {
 				// the active set and the laplacians of its fields, axis by axis
 				// as the generic kernel forms them

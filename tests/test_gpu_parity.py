@@ -733,8 +733,8 @@ def test_multiphase_wave_strips(prec):
 @pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
 def test_march_3d(prec):
     """The 3-D plane march (tile body 8): Allen-Cahn, model A, grain growth,
-    Zener pinning and multi-phase grain growth with 2, 4 and 8 fields, every
-    compiled tile, chunks of 2,
+    Zener pinning and multi-phase grain growth with 2, 4 and 8 fields, Ostwald
+    ripening with 2, 3 and 4 (two marches), every compiled tile, chunks of 2,
     3, 5, 7 and 16 planes (the four-plane unrolled march ends at each of its
     positions), partial tiles, grids smaller than one tile, every boundary
     condition on every axis; bitwise against the oracle."""
@@ -840,6 +840,40 @@ def test_march_3d(prec):
             g.step("gg_aniso", pg, 3)
             assert g.get_tile()["opt"] == 8
             assert np.array_equal(g.download(), O.gg_aniso(a, pg, 3, form=prec))
+    # Ostwald ripening (2, 3 and 4 fields): the two passes as marches
+    if prec != "f32ref":
+        po = [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]
+        for shape, q, bc in [((11, 20, 70), 2, bcs[1]), ((9, 17, 33), 3, bcs[2]), ((10, 12, 40), 4, bcs[3]),
+                             ((12, 18, 66), 4, bcs[0]), ((5, 6, 7), 2, bcs[0])]:
+            say("[gpu] march 3-D ostwald %s q=%d %s" % (shape, q, prec))
+            a = O.uniform(int(np.prod(shape)) * q, 1700 + q, 0.0, 1.0).reshape(shape + (q,))
+            if prec != "f64":
+                a = a.astype(np.float32)
+            exp = O.ostwald(a, po, 7, form=prec, bc=bc)
+            for tile in vec_tiles:
+                for chunk in (2, 3, 5, 16):
+                    with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+                        g.set_variant(2)
+                        g.set_tile(tile[0], tile[1], chunk)
+                        g.set_tile_options(body=8)
+                        g.upload(a)
+                        g.step("ostwald", po, 7)
+                        assert np.array_equal(g.download(), exp), (shape, q, tile, chunk, "ostwald")
+            with mx.Grid(shape, fields=q, vector=True, precision=prec, bc=bc) as g:
+                g.upload(a)
+                g.step("ostwald", po, 3)
+                g.step("ostwald", po, 4)
+                assert np.array_equal(g.download(), exp), (shape, q, "default", "ostwald")
+        # the default takes the marches from 2^20 nodes
+        shape = (104, 100, 101)
+        a = O.uniform(int(np.prod(shape)) * 2, 1750, 0.0, 1.0).reshape(shape + (2,))
+        if prec != "f64":
+            a = a.astype(np.float32)
+        with mx.Grid(shape, fields=2, vector=True, precision=prec) as g:
+            g.upload(a)
+            g.step("ostwald", po, 3)
+            assert g.get_tile()["opt"] == 8
+            assert np.array_equal(g.download(), O.ostwald(a, po, 3, form=prec)), "ostwald default"
     # the defaults take the march; other field counts stay on the generic kernel
     a = O.uniform(10 * 12 * 14 * 3, 1500, 0.0, 1.0).reshape(10, 12, 14, 3)
     if prec != "f64":
