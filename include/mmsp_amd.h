@@ -168,10 +168,12 @@ int mmspx_sync(mmspx_state* s);
  *   current device. On return *result_buf (0 or 1) names the buffer that will
  *   hold the state after `steps` steps; the other holds intermediate data.
  *   first_step numbers the first step (the stochastic forms' noise counter).
- * The two-pass Cahn-Hilliard and Ostwald kernels (non-periodic Cahn-Hilliard,
- * 1-D and 3-D Ostwald ripening ...) keep one scalar plane (nodes x element
- * size) in a fixed device array of MMSPX_NOALLOC_SCRATCH_BYTES that each GPU
- * holds until the process ends; larger planes are refused (MMSPX_EINVAL).
+ * The two-pass Cahn-Hilliard and Ostwald kernels (1-D Cahn-Hilliard, an axis
+ * periodic on one side only, 1-D and small 3-D Ostwald ripening) and the 3-D
+ * Ostwald marches (w between their two passes) keep one scalar plane (nodes x
+ * element size) in a fixed device array of MMSPX_NOALLOC_SCRATCH_BYTES that
+ * each GPU holds until the process ends; larger planes are refused
+ * (MMSPX_EINVAL).
  * Calls on one GPU share that array: calls whose work overlaps on one GPU
  * race. The first call on each device looks the array's address up (steps = 0
  * is enough); make that call before capturing later ones into a graph. */
