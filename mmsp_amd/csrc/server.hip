@@ -23,7 +23,9 @@
 //     members in two launches). 3-D members of 2^21 nodes or more fill the
 //     GPU on their own: they run the single-grid kernels instead (for
 //     periodic Cahn-Hilliard two steps per pass), one launch per member and
-//     pass. A round is planned on the host and queued on the
+//     pass, on a second compute stream, so they overlap the batch launches
+//     (both streams at the default priority). A round is planned on the host
+//     and queued on the
 //     compute stream; up to two rounds are queued ahead, so the device does
 //     not wait for the host between rounds;
 //   * when a request's last round is queued, the downloader waits for that
@@ -91,7 +93,8 @@ struct Slot {
 };
 
 struct Round {
-	hipEvent_t ev = nullptr;
+	hipEvent_t ev = nullptr;   // the batch launches (compute stream)
+	hipEvent_t ev2 = nullptr;  // the large members' launches (second compute stream), if any
 	int steps = 0;
 	std::vector<Request*> members;
 };
@@ -117,7 +120,8 @@ struct mmspx_server {
 	bool stop = false;
 	int64_t next_id = 1;
 	std::chrono::steady_clock::time_point t0;
-	hipStream_t st = nullptr;   // compute (worker)
+	hipStream_t st = nullptr;   // compute (worker): batch launches
+	hipStream_t st2 = nullptr;  // compute (worker): the large members' single-grid launches
 	hipStream_t ust = nullptr;  // uploads
 	hipStream_t dst = nullptr;  // downloads
 	hipMemPool_t pool = nullptr;
@@ -148,6 +152,9 @@ struct mmspx_server {
 };
 
 static size_t elem_of(const Request* r) { return r->prec == MMSPX_F64 ? 8 : 4; }
+
+// the compute stream that advances a request
+static hipStream_t stream_of(const mmspx_server* sv, const Request* r) { return r->solo ? sv->st2 : sv->st; }
 
 static void finish(mmspx_server* sv, Request* r, int status)
 {
@@ -380,7 +387,7 @@ static int enqueue_round(mmspx_server* sv)
 			const int n = std::min(left, spl);
 			const char* a = r->dev + (r->cur ? r->bytes : 0);
 			char* b = r->dev + (r->cur ? 0 : r->bytes);
-			rc = launch_fused_step(r->g, r->prec, r->p, a, b, sv->st, r->tile, n, &launches);
+			rc = launch_fused_step(r->g, r->prec, r->p, a, b, sv->st2, r->tile, n, &launches);
 			if (rc) return rc;
 			r->cur ^= 1;
 			left -= n;
@@ -390,6 +397,10 @@ static int enqueue_round(mmspx_server* sv)
 	rd.steps = steps;
 	MMSPX_HIP(hipEventCreateWithFlags(&rd.ev, hipEventDisableTiming));
 	MMSPX_HIP(hipEventRecord(rd.ev, sv->st));
+	if (std::any_of(act.begin(), act.end(), [](const Request* r) { return r->solo; })) {
+		MMSPX_HIP(hipEventCreateWithFlags(&rd.ev2, hipEventDisableTiming));
+		MMSPX_HIP(hipEventRecord(rd.ev2, sv->st2));
+	}
 	MMSPX_HIP(hipEventRecord(sl.done, sv->st));
 	sl.used = true;
 	sv->gstep += steps;
@@ -404,7 +415,7 @@ static int enqueue_round(mmspx_server* sv)
 			continue;
 		}
 		MMSPX_HIP(hipEventCreateWithFlags(&r->ev_last, hipEventDisableTiming));
-		MMSPX_HIP(hipEventRecord(r->ev_last, sv->st));
+		MMSPX_HIP(hipEventRecord(r->ev_last, stream_of(sv, r)));
 		fin.push_back({r, false});
 	}
 	// a finishing member may be fetched and deleted once downloaded: the round
@@ -425,24 +436,31 @@ static int enqueue_round(mmspx_server* sv)
 	return 0;
 }
 
-// Retires finished rounds from the front of the queue (the oldest one is waited for if `block`).
+// Retires finished rounds from the front of the queue (the oldest one is
+// waited for if `block`); a round is finished when both compute streams have
+// passed it.
 static int reap_rounds(mmspx_server* sv, bool block)
 {
 	while (!sv->rounds.empty()) {
 		Round& rd = sv->rounds.front();
 		if (block) {
 			MMSPX_HIP(hipEventSynchronize(rd.ev));
+			if (rd.ev2) MMSPX_HIP(hipEventSynchronize(rd.ev2));
 			block = false;
 		} else {
-			const hipError_t e = hipEventQuery(rd.ev);
-			if (e == hipErrorNotReady) break;
-			if (e != hipSuccess) return fail(MMSPX_EDEVICE, std::string("round: ") + hipGetErrorString(e));
+			for (hipEvent_t e : {rd.ev, rd.ev2}) {
+				if (!e) continue;
+				const hipError_t q = hipEventQuery(e);
+				if (q == hipErrorNotReady) return 0;
+				if (q != hipSuccess) return fail(MMSPX_EDEVICE, std::string("round: ") + hipGetErrorString(q));
+			}
 		}
 		{
 			std::lock_guard<std::mutex> lk(sv->mu);
 			for (Request* r : rd.members) r->done += rd.steps;
 		}
 		(void)hipEventDestroy(rd.ev);
+		if (rd.ev2) (void)hipEventDestroy(rd.ev2);
 		sv->rounds.pop_front();
 	}
 	return 0;
@@ -462,7 +480,7 @@ static int drop_cancelled(mmspx_server* sv)
 	}
 	for (IoJob& j : drop) {
 		MMSPX_HIP(hipEventCreateWithFlags(&j.r->ev_last, hipEventDisableTiming));
-		MMSPX_HIP(hipEventRecord(j.r->ev_last, sv->st));
+		MMSPX_HIP(hipEventRecord(j.r->ev_last, stream_of(sv, j.r)));
 	}
 	sv->resident.swap(keep);
 	if (!drop.empty()) {
@@ -542,6 +560,7 @@ static void worker_main(mmspx_server* sv)
 		const int rc = worker_pass(sv);
 		if (rc == 0) continue;
 		(void)hipStreamSynchronize(sv->st);
+		(void)hipStreamSynchronize(sv->st2);
 		if (rc == 1) break;
 		// a device error: fail every request not yet finished and stop serving
 		std::lock_guard<std::mutex> lk(sv->mu);
@@ -575,6 +594,7 @@ int mmspx_server_create(const mmspx_server_config* cfg, mmspx_server** out)
 	sv->t0 = std::chrono::steady_clock::now();
 	int rc = 0;
 	if (hipGetDevice(&sv->device) != hipSuccess || hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking) != hipSuccess ||
+	    hipStreamCreateWithFlags(&sv->st2, hipStreamNonBlocking) != hipSuccess ||
 	    hipStreamCreateWithFlags(&sv->ust, hipStreamNonBlocking) != hipSuccess ||
 	    hipStreamCreateWithFlags(&sv->dst, hipStreamNonBlocking) != hipSuccess)
 		rc = fail(MMSPX_EDEVICE, "HIP stream setup failed");
@@ -603,6 +623,7 @@ int mmspx_server_create(const mmspx_server_config* cfg, mmspx_server** out)
 		if (sv->dpart) (void)hipFree(sv->dpart);
 		if (sv->pool) (void)hipMemPoolDestroy(sv->pool);
 		if (sv->st) (void)hipStreamDestroy(sv->st);
+		if (sv->st2) (void)hipStreamDestroy(sv->st2);
 		if (sv->ust) (void)hipStreamDestroy(sv->ust);
 		if (sv->dst) (void)hipStreamDestroy(sv->dst);
 		delete sv;
@@ -630,10 +651,13 @@ void mmspx_server_destroy(mmspx_server* sv)
 	if (sv->uploader.joinable()) sv->uploader.join();
 	if (sv->downloader.joinable()) sv->downloader.join();
 	(void)hipStreamSynchronize(sv->st);
+	(void)hipStreamSynchronize(sv->st2);
 	(void)hipStreamSynchronize(sv->ust);
 	(void)hipStreamSynchronize(sv->dst);
-	for (auto& rd : sv->rounds)
+	for (auto& rd : sv->rounds) {
 		if (rd.ev) (void)hipEventDestroy(rd.ev);
+		if (rd.ev2) (void)hipEventDestroy(rd.ev2);
+	}
 	for (auto& kv : sv->all) {
 		Request* r = kv.second;
 		if (r->dev) (void)hipFreeAsync(r->dev, sv->dst);
@@ -652,6 +676,7 @@ void mmspx_server_destroy(mmspx_server* sv)
 	(void)hipFree(sv->dpart);
 	(void)hipMemPoolDestroy(sv->pool);
 	(void)hipStreamDestroy(sv->st);
+	(void)hipStreamDestroy(sv->st2);
 	(void)hipStreamDestroy(sv->ust);
 	(void)hipStreamDestroy(sv->dst);
 	delete sv;
