@@ -50,8 +50,11 @@ __global__ void __launch_bounds__(NT) ch_fused(const typename Form<PREC>::S* __r
 }
 
 // Two steps per pass (3-D, periodic, deterministic chemical potentials).
+// FP32 on 32 x 32 tiles asks for six waves per SIMD (80 VGPRs, no spill), so
+// two 768-thread workgroups share a CU.
 template <int PREC, int MU, int TY, int TZ, int NT, bool PHI2, bool UNIT>
-__global__ void __launch_bounds__(NT) ch_fused2(const typename Form<PREC>::S* __restrict__ a,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu((PREC != MMSPX_F64 && TZ == 32) ? 6 : 1)))
+ch_fused2(const typename Form<PREC>::S* __restrict__ a,
                                                 typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
                                                 int chunk_len, int tiles_y, int tiles_z, int ntiles)
 {
@@ -260,18 +263,23 @@ bool fused_supported(const Geom& g, int prec, int model)
 }
 
 // Periodic 3-D Cahn-Hilliard with a deterministic chemical potential: two
-// steps per pass on 32 x 32 tiles (768 threads, one workgroup per CU) or, in
-// FP32, 32 x 64 tiles (896 threads), whichever the cost model below favours.
-// A workgroup's time grows with its planes (chunk + 6: three halo planes of
-// the two steps on each side are recomputed) times the R3 points each thread
-// owns; a launch takes as many such rounds as its workgroups fill the 256
-// CUs. The chunk with the least rounds x time wins (ties: the longer chunk).
+// steps per pass on 32 x 32 tiles (768 threads) or, in FP32, 32 x 64 tiles
+// (896 threads), whichever the cost model below favours. A workgroup's time
+// grows with its planes (chunk + 6: three halo planes of the two steps on
+// each side are recomputed) times the R3 points each thread owns; a launch
+// takes as many such rounds as its workgroups fill the 256 CUs. One
+// workgroup fits a CU, except FP32 on 32 x 32 tiles (80 VGPRs, 44 KB of LDS:
+// two per CU), where a launch of more than 256 workgroups takes rounds of
+// 512, each 1.7 times as long (the two share the CU; fitted to the FP32
+// sweep, which the model then matches at every side). The chunk with the
+// least rounds x time wins (ties: the longer chunk).
 // Against a sweep of every chunk on cubes of side 48 to 256 (us per step):
 // the model picks the measured best in FP64 at every side (128^3: 8-plane
 // chunks, 10.8 against 22.9 for the former 8 x 32 LDS-ring default; 192^3:
 // 32, 25.8 against 41.5; 224^3: 48, 37.8 against 59.7; 256^3: 64, 50.5) and
-// in FP32 (128^3 32 x 32 tiles with 8-plane chunks 9.2 against 18.8; 192^3
-// 32 x 64 with 16, 17.0 against 34.0; 256^3 32 x 64 with 32, 31.3).
+// in FP32 (128^3 32 x 32 tiles with 8-plane chunks 8.8 against 18.8; 192^3
+// 32 x 64 with 16, 16.9 against 34.0; 224^3 32 x 32 with 24, 25.9 against
+// 47.1; 256^3 32 x 64 with 32, 29.6).
 static TuneChoice two_step_choice(const Geom& g, int prec)
 {
 	constexpr int kCUs = 256;
@@ -282,10 +290,13 @@ static TuneChoice two_step_choice(const Geom& g, int prec)
 		if (body == 6 && prec != MMSPX_F32) continue;  // FP64 32 x 64 tiles spill
 		const int ty = 32, tz = body == 7 ? 32 : 64, nt = body == 7 ? 768 : 896;
 		const double own = (double)(ty + 6) * (tz + 6) / nt;  // R3 points per thread
+		const bool two_per_cu = body == 7 && prec == MMSPX_F32;
 		const int64_t tiles = (int64_t)((g.n[1] + ty - 1) / ty) * ((g.n[2] + tz - 1) / tz);
 		for (int c : chunks) {
 			const int64_t wgs = tiles * ((g.n[0] + c - 1) / c);
-			const double cost = (double)((wgs + kCUs - 1) / kCUs) * (c + 6) * own;
+			const double rounds = (two_per_cu && wgs > kCUs) ? 1.7 * (double)((wgs + 2 * kCUs - 1) / (2 * kCUs))
+			                                                 : (double)((wgs + kCUs - 1) / kCUs);
+			const double cost = rounds * (c + 6) * own;
 			if (cost <= best_cost) {
 				best_cost = cost;
 				best = TuneChoice{2, ty, tz, c, body};
