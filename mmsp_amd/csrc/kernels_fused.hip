@@ -158,15 +158,15 @@ __global__ void __launch_bounds__(64 * W) wave_strips_split(const typename Form<
 
 // Temporal blocks (2-D periodic Allen-Cahn): one workgroup per TX x kTbCols
 // tile, `steps` (<= kTbMaxSteps) steps per launch.
-template <int PREC, int TX>
-__global__ void __launch_bounds__(256) ac_tblocks(const typename Form<PREC>::S* __restrict__ a,
-                                                  typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
-                                                  int tiles_y, int ntiles, int steps)
+template <int PREC, int TX, int NT>
+__global__ void __launch_bounds__(NT) ac_tblocks(const typename Form<PREC>::S* __restrict__ a,
+                                                 typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
+                                                 int tiles_y, int ntiles, int steps)
 {
 	const int tile = xcd_tile(ntiles);
 	if (tile < 0) return;
 	const int ty = tile % tiles_y, tx = tile / tiles_y;
-	ac_tblock<PREC, TX, kTbMaxSteps, 256>(a, b, g, p, tx * TX, ty * kTbCols, steps);
+	ac_tblock<PREC, TX, kTbMaxSteps, NT>(a, b, g, p, tx * TX, ty * kTbCols, steps);
 }
 
 // Cahn-Hilliard wave strips (2-D periodic): one wave per (strip, chunk) item,
@@ -359,17 +359,18 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	}
 	// Allen-Cahn: periodic grids take temporal blocks of 48 columns and 8
 	// steps per pass (512^2: 2.05 us per step in FP64 against 4.58 us for one
-	// thread per node). Block heights from a sweep of 8, 16, 32 rows x 4, 8
-	// steps: 8 rows up to 2^19 nodes, where more blocks fill more CUs (512^2
-	// FP64 1.71 us against 2.00 with 16 rows, FP32 1.35 against 1.64); 32
-	// rows in FP64 and 16 in FP32 below 2^21 (1024^2 3.33 and 2.26 us); 16
-	// above (2048^2 10.8 and 5.6 us).
+	// thread per node). Block heights from a sweep of 8, 16, 32 rows (on 512,
+	// 1024, 1024 threads) x 4, 8 steps, sides 256 to 4096: 16 rows up to 2^16
+	// nodes (256^2 FP64 1.10 us, FP32 0.97), 32 rows above (512^2 FP64 1.48
+	// against 1.57 with 16 rows and 1.69 with 8, FP32 1.26; 1024^2 3.09 and
+	// 2.22; 2048^2 10.6 against 13.6 with 16 rows, FP32 6.95 against 9.41;
+	// 4096^2 36.7 and 24.6 against 47.4 and 33.5).
 	// Other boundaries and model A: the wave strips, 4-row chunks below 2^21
 	// nodes, 8 below 2^23, 16 above (Neumann FP64: 1024^2 5.5 us per step
 	// against 8.6 for one thread per node, 2048^2 12.6 against 16.5, 4096^2
 	// 43.3 against 61.1; model A FP64 1024^2 7.7 us, 4096^2 60.8)
 	const int wave_chunk = nodes < ((int64_t)1 << 21) ? 4 : nodes < ((int64_t)1 << 23) ? 8 : 16;
-	const int tb_rows = nodes <= ((int64_t)1 << 19) ? 8 : nodes < ((int64_t)1 << 21) && prec == MMSPX_F64 ? 32 : 16;
+	const int tb_rows = nodes <= ((int64_t)1 << 16) ? 16 : 32;
 	if (model == MMSPX_AC)
 		return all_periodic(g) ? TuneChoice{2, tb_rows, kTbCols, 8, 3} : TuneChoice{2, 1, kWaveCols, wave_chunk, 2};
 	if (model == MMSPX_AC_NOISE) return TuneChoice{2, 1, kWaveCols, wave_chunk, 2};
@@ -804,8 +805,12 @@ static int launch_tb(const Geom& g, const ModelParams& p, const void* a, void* b
 	const int tiles_y = (g.n[1] + kTbCols - 1) / kTbCols;
 	const int ntiles = tiles_y * ((g.n[0] + TX - 1) / TX);
 	const int blocks = (ntiles + 7) / 8 * 8;
-	hipLaunchKernelGGL((ac_tblocks<PREC, TX>), dim3(blocks), dim3(256), 0, st, (const S*)a, (S*)b, g, p, tiles_y, ntiles,
-	                   steps);
+	// 8-row blocks on 8 waves, 16- and 32-row blocks on 16 (a wave takes 3
+	// of the 24, 32 or 48 region rows per step): 512^2 FP64 with 32 rows 1.48
+	// us per step on 1024 threads against 2.50 on 256
+	constexpr int NT = TX == 8 ? 512 : 1024;
+	hipLaunchKernelGGL((ac_tblocks<PREC, TX, NT>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g, p, tiles_y,
+	                   ntiles, steps);
 	MMSPX_HIP(hipGetLastError());
 	return 0;
 }
