@@ -45,6 +45,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <thread>
 #include <vector>
 
@@ -77,6 +78,8 @@ struct Request {
 	bool solo = false;               // runs the single-grid kernels with `tile`
 	TuneChoice tile{};
 	int cur = 0;                     // solo: the buffer holding the state after `planned` steps
+	int in_rounds = 0;               // queued rounds that list it (under the server's mutex)
+	bool fetched = false;            // fetched while rounds still listed it: the worker deletes it
 	hipEvent_t ev_last = nullptr;    // recorded after its last round (or when dropped)
 	std::vector<char> init;          // initial state, until uploaded
 	std::vector<char> out;           // final state
@@ -429,6 +432,7 @@ static int enqueue_round(mmspx_server* sv)
 	}
 	sv->rounds.push_back(std::move(rd));
 	std::lock_guard<std::mutex> lk(sv->mu);
+	for (Request* r : sv->rounds.back().members) r->in_rounds += 1;
 	sv->stats[3] += 1;
 	sv->stats[6] += launches;
 	for (const IoJob& j : fin) sv->down_jobs.push_back(j);
@@ -456,8 +460,14 @@ static int reap_rounds(mmspx_server* sv, bool block)
 			}
 		}
 		{
+			// a member fetched since this round was queued is deleted with its
+			// last listing round
 			std::lock_guard<std::mutex> lk(sv->mu);
-			for (Request* r : rd.members) r->done += rd.steps;
+			for (Request* r : rd.members) {
+				r->done += rd.steps;
+				r->in_rounds -= 1;
+				if (r->fetched && r->in_rounds == 0) delete r;
+			}
 		}
 		(void)hipEventDestroy(rd.ev);
 		if (rd.ev2) (void)hipEventDestroy(rd.ev2);
@@ -654,10 +664,14 @@ void mmspx_server_destroy(mmspx_server* sv)
 	(void)hipStreamSynchronize(sv->st2);
 	(void)hipStreamSynchronize(sv->ust);
 	(void)hipStreamSynchronize(sv->dst);
+	std::set<Request*> fetched;  // fetched, still listed by rounds never retired
 	for (auto& rd : sv->rounds) {
 		if (rd.ev) (void)hipEventDestroy(rd.ev);
 		if (rd.ev2) (void)hipEventDestroy(rd.ev2);
+		for (Request* r : rd.members)
+			if (r->fetched) fetched.insert(r);
 	}
+	for (Request* r : fetched) delete r;
 	for (auto& kv : sv->all) {
 		Request* r = kv.second;
 		if (r->dev) (void)hipFreeAsync(r->dev, sv->dst);
@@ -809,6 +823,14 @@ int mmspx_server_fetch(mmspx_server* sv, int64_t id, double* out, size_t count)
 	} else {
 		const float* f = (const float*)r->out.data();
 		for (size_t i = 0; i < count; i++) out[i] = f[i];
+	}
+	// a round the worker has not retired yet may still list the request (its
+	// last steps are done, the round's other work may not be): then the worker
+	// deletes it when it retires that round
+	{
+		std::lock_guard<std::mutex> lk(sv->mu);
+		r->fetched = true;
+		if (r->in_rounds > 0) return 0;
 	}
 	delete r;
 	return 0;

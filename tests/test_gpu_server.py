@@ -184,6 +184,35 @@ def test_cancel_queued_request():
             sv.fetch(i2)
 
 
+def test_cancel_running_large_member():
+    """A large 3-D member (the single-grid kernels on the second stream)
+    cancelled while it runs: it ends cancelled, a small request beside it
+    completes bitwise, and a large request submitted afterwards runs on the
+    freed slot and equals the single-grid path."""
+    say("[gpu] server cancel of a running large member")
+    big, small = (130, 128, 128), (24, 20, 28)
+    with mx.Server("ch_dw", "f64", max_slots=4, chunk_steps=50) as sv:
+        i0 = sv.submit(req(big, 61), P_DW, 40000)
+        i1 = sv.submit(req(small, 62), P_DW, 30)
+        t0 = time.time()
+        while sv.info(i0)["status"] == mx.QUEUED and time.time() - t0 < 60:
+            time.sleep(0.005)
+        assert sv.info(i0)["status"] == mx.RUNNING
+        sv.cancel(i0)
+        assert sv.wait(i0, 60)["status"] == mx.CANCELLED
+        assert sv.wait(i1, 60)["status"] == mx.DONE
+        assert np.array_equal(sv.fetch(i1), O.ch(req(small, 62), P_DW, 30, mu=O.MU_DW))
+        i2 = sv.submit(req(big, 63), P_DW, 21)
+        assert sv.wait(i2, 60)["status"] == mx.DONE
+        got = sv.fetch(i2)
+        st = sv.stats()
+    assert st["worker_error"] == 0
+    with mx.Grid(big, precision="f64") as g:
+        g.upload(req(big, 63))
+        g.step("ch_dw", P_DW, 21)
+        assert np.array_equal(g.download(), got)
+
+
 def test_submit_validation():
     with mx.Server("ch_dw", "f32") as sv:
         with pytest.raises(mx.MmspxError):
