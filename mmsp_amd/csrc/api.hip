@@ -479,14 +479,26 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 int mmspx_set_tile_options(mmspx_state* s, int body)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
-	if (body < 0 || body > 8 || (body == 3 && s->geom.dim != 3) || ((body == 4 || body == 5) && s->geom.dim != 2) ||
-	    (body >= 6 && s->geom.dim != 3))
+	if (body < 0 || body > 10 || (body == 3 && s->geom.dim != 3) ||
+	    ((body == 4 || body == 5 || body == 9 || body == 10) && s->geom.dim != 2) ||
+	    (body >= 6 && body <= 8 && s->geom.dim != 3))
 		return fail(MMSPX_EINVAL, "unknown tile body");
 	if (!s->tuned) {
 		s->tile = default_choice(s->geom, s->desc.precision, MMSPX_CH_DW);
 		s->tuned = true;
 	}
 	s->tile.opt = body;
+	// 2-D: the split strips run 1 x 64 strips (bodies 4, 5), two steps per
+	// pass 1 x kChWaveOut (bodies 9, 10)
+	if (s->geom.dim == 2 && s->tile.variant == 2) {
+		if (body == 9 || body == 10) {
+			s->tile.ty = 1;
+			s->tile.tz = kChWaveOut;
+		} else if (body == 4 || body == 5) {
+			s->tile.ty = 1;
+			s->tile.tz = kWaveCols;
+		}
+	}
 	// 3-D: keep a tile the body runs (32 x 32 tiles run body 7 only, body 6
 	// runs 32 x 64 tiles, the plane march 8 x 64 for every model)
 	if (s->geom.dim == 3 && s->tile.variant == 2) {

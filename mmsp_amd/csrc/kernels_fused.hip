@@ -156,6 +156,20 @@ __global__ void __launch_bounds__(64 * W) wave_strips_split(const typename Form<
 	strip_wave_split<PREC, Q, KIND, W>(a, b, g, p, s * kWaveCols, xb, min(xb + chunk_len, g.n[0]));
 }
 
+// Two steps per pass on the split strips (periodic): one workgroup of W
+// waves per (strip of kChWaveOut columns, chunk), strip-fastest.
+template <int PREC, int Q, int KIND, int W>
+__global__ void __launch_bounds__(64 * W) wave_strips_split2(const typename Form<PREC>::S* __restrict__ a,
+                                                             typename Form<PREC>::S* __restrict__ b, Geom g,
+                                                             ModelParams p, int chunk_len, int strips, int nitems)
+{
+	const int item = xcd_tile(nitems);
+	if (item < 0) return;  // the whole workgroup
+	const int s = item % strips, chunk = item / strips;
+	const int xb = chunk * chunk_len;
+	strip_wave_split2<PREC, Q, KIND, W>(a, b, g, p, s * kChWaveOut, xb, min(xb + chunk_len, g.n[0]));
+}
+
 // Temporal blocks (2-D periodic Allen-Cahn): one workgroup per TX x kTbCols
 // tile, `steps` (<= kTbMaxSteps) steps per launch.
 template <int PREC, int TX, int NT>
@@ -393,7 +407,15 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// FP32 115.4 against 118.3; 512^2 x 32 FP32 15.7 against 20.3 on two
 	// waves; 512^2 x 8 FP32 7.9 against 9.4 for one thread per node; 1024^2 x
 	// 16 FP64 47.1 against 82.0.
+	// Periodic grids take the split strips two steps per pass on four waves
+	// (body 10): 2048^2 x 32 FP32 149.6 us per step with 16-row chunks
+	// against 241.2 for the one-step split strips, FP64 275.0 with 32-row
+	// chunks against 465.6; 2048^2 x 16 Zener FP32 73.0 against 117.3;
+	// 1024^2 x 32 FP32 30.7 with 8-row chunks against 49.6; 512^2 x 8 FP32 5.0
+	// against 8.1; 1024^2 x 8 FP64 15.8 against 24.7 (two waves: 1-8 % slower).
 	if ((model == MMSPX_GG || model == MMSPX_ZENER) && (g.fields == 8 || g.fields == 16 || g.fields == 32)) {
+		if (all_periodic(g))
+			return TuneChoice{2, 1, kChWaveOut, nodes < ((int64_t)1 << 21) ? 8 : prec == MMSPX_F64 ? 32 : 16, 10};
 		if (nodes < ((int64_t)1 << 21)) return TuneChoice{2, 1, kWaveCols, prec == MMSPX_F64 ? 16 : 8, 5};
 		return TuneChoice{2, 1, kWaveCols, 64, 4};
 	}
@@ -472,6 +494,9 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		if (split)
 			for (int c : {8, 16, 32, 64})
 				for (int body = 4; body <= 5; body++) add(1, kWaveCols, c, body);
+		if (split && model == MMSPX_ZENER && all_periodic(g))
+			for (int c : {8, 16, 32, 64})
+				for (int body = 9; body <= 10; body++) add(1, kChWaveOut, c, body);
 		return n;
 	}
 	if (g.dim == 3 && !ch) {
@@ -524,6 +549,10 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		if (split)
 			for (int c : {8, 16, 32, 64})
 				for (int body = 4; body <= 5; body++) add(1, kWaveCols, c, body);
+		// periodic: the split strips two steps per pass
+		if (split && all_periodic(g))
+			for (int c : {8, 16, 32, 64})
+				for (int body = 9; body <= 10; body++) add(1, kChWaveOut, c, body);
 		// periodic Allen-Cahn: temporal blocks (chunk = steps per pass)
 		if (model == MMSPX_AC && all_periodic(g))
 			for (int tx : {8, 16, 32})
@@ -913,6 +942,50 @@ static int launch_split(const Geom& g, const ModelParams& p, const void* a, void
 	                  : launch_split_w<PREC, 1, 2>(g, p, a, b, st, chunk);
 }
 
+template <int PREC, int Q, int KIND, int W>
+static int launch_split2_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	using S = typename Form<PREC>::S;
+	const int strips = (g.n[1] + kChWaveOut - 1) / kChWaveOut;
+	const int nitems = strips * ((g.n[0] + chunk - 1) / chunk);
+	const int blocks = (nitems + 7) / 8 * 8;
+	hipLaunchKernelGGL((wave_strips_split2<PREC, Q, KIND, W>), dim3(blocks), dim3(64 * W), 0, st, (const S*)a, (S*)b,
+	                   g, p, chunk, strips, nitems);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+template <int PREC, int KIND, int W>
+static int launch_split2_w(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	switch (g.fields) {
+	case 8: return launch_split2_q<PREC, 8, KIND, W>(g, p, a, b, st, chunk);
+	case 16: return launch_split2_q<PREC, 16, KIND, W>(g, p, a, b, st, chunk);
+	case 32: return launch_split2_q<PREC, 32, KIND, W>(g, p, a, b, st, chunk);
+	}
+	return fail(MMSPX_EUNSUPPORTED, "split wave strips take 8, 16 or 32 fields");
+}
+
+// Grain growth and Zener pinning, periodic 2-D, 8, 16 or 32 fields: two steps
+// per pass on the split strips, W = 2 (body 9) or 4 (body 10) waves per
+// strip; a single step (an odd last one) runs the one-step split strips
+template <int PREC>
+static int launch_split2(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk,
+                         int waves, int nsteps)
+{
+	if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "the two-step split strips need periodic boundaries");
+	if (nsteps == 1) return launch_split<PREC>(g, p, a, b, st, chunk, waves);
+	if (nsteps != 2) return fail(MMSPX_EINVAL, "the two-step split strips take one or two steps per pass");
+	if (p.model == MMSPX_ZENER) {
+		if constexpr (PREC != MMSPX_F32REF)
+			return waves == 4 ? launch_split2_w<PREC, 3, 4>(g, p, a, b, st, chunk)
+			                  : launch_split2_w<PREC, 3, 2>(g, p, a, b, st, chunk);
+		return fail(MMSPX_EUNSUPPORTED, "Zener pinning runs the MMSPX_F64 and MMSPX_F32 forms");
+	}
+	return waves == 4 ? launch_split2_w<PREC, 1, 4>(g, p, a, b, st, chunk)
+	                  : launch_split2_w<PREC, 1, 2>(g, p, a, b, st, chunk);
+}
+
 template <int PREC, int Q>
 static int launch_ow_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
@@ -965,6 +1038,8 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		return fail(MMSPX_EUNSUPPORTED, "Ostwald ripening runs the wave strips (1 x 60) or the generic kernels");
 	case MMSPX_ZENER:
 		if (g.dim == 3) return launch_march<PREC, 3>(g, p, a, b, st, t);
+		if ((t.opt == 9 || t.opt == 10) && t.ty == 1 && t.tz == kChWaveOut)
+			return launch_split2<PREC>(g, p, a, b, st, chunk, t.opt == 10 ? 4 : 2, nsteps);
 		if ((t.opt == 4 || t.opt == 5) && t.ty == 1 && t.tz == kWaveCols)
 			return launch_split<PREC>(g, p, a, b, st, chunk, t.opt == 5 ? 4 : 2);
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
@@ -995,6 +1070,8 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		return launch_ac<PREC, 256>(g, p, a, b, st, chunk);
 	case MMSPX_GG:
 		if (g.dim == 3) return launch_march<PREC, 1>(g, p, a, b, st, t);
+		if ((t.opt == 9 || t.opt == 10) && t.ty == 1 && t.tz == kChWaveOut)
+			return launch_split2<PREC>(g, p, a, b, st, chunk, t.opt == 10 ? 4 : 2, nsteps);
 		// ty > 1: 2-D tiles of ty (8 or 16) rows x 64 columns; body 2 with 64
 		// columns: wave strips; otherwise LDS-row marching strips
 		if ((t.opt == 4 || t.opt == 5) && t.ty == 1 && t.tz == kWaveCols)
@@ -1014,6 +1091,7 @@ int fused_steps_per_launch(int model, const TuneChoice& t)
 {
 	if (model == MMSPX_AC && t.tz == kTbCols) return std::max(1, std::min(t.chunk, kTbMaxSteps));
 	if ((model == MMSPX_CH || model == MMSPX_CH_DW) && (t.opt == 6 || t.opt == 7)) return 2;
+	if ((model == MMSPX_GG || model == MMSPX_ZENER) && (t.opt == 9 || t.opt == 10)) return 2;
 	return 1;
 }
 

@@ -1568,4 +1568,142 @@ __device__ __forceinline__ void strip_wave_split(const typename Form<PREC>::S* _
 	}
 }
 
+// ---- 2-D grain growth, two steps per pass: split wave strips (periodic) ------
+// The split strips' field split (wave h of W holds fields h*Q/W ..), with the
+// Cahn-Hilliard wave strips' lane layout: lane l holds column c0 - 2 + l
+// (wrapped), phi^{n+1} is formed on lanes 1..62 and phi^{n+2} written on lanes
+// 2..61 (kChWaveOut columns per strip), column neighbours by DPP. Rows: phi^n
+// in a four-slot ring (x .. x+2, x+3 in flight) and phi^{n+1} in another
+// (x-1 .. x+1), both rotated over a four-row unrolled march; phi^{n+1} never
+// goes to memory. An iteration at row x first forms sum_j phi_j^2 of phi^n at
+// row x+1 and of phi^{n+1} at row x in one pass of the field-order chain
+// across the waves (W barriers, as the one-step strips take per row), then
+// phi^{n+1}(x+1) and phi^{n+2}(x). Every value is the one-step kernels' for
+// that node and step (graingrowth.cpp:91-107; KIND 3 holds field 0,
+// zener.cpp:117-135), so the result is bitwise two single steps. Periodic
+// grids only: a lane past the grid's edge holds the wrapped column, whose
+// neighbours are its lanes' neighbours.
+template <int PREC, int Q, int KIND, int W>
+__device__ __forceinline__ void strip_wave_split2(const typename Form<PREC>::S* __restrict__ a,
+                                                  typename Form<PREC>::S* __restrict__ b, const Geom& g,
+                                                  const ModelParams& p, int c0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	using Sum = typename F::Sum;
+	constexpr int H = Q / W;
+	static_assert(Q % W == 0 && W >= 2, "the two-step split strips hold Q/W fields per wave, W >= 2");
+	__shared__ Sum part[2][64], total[2][64];
+	const int lane = threadIdx.x & 63;
+	const int wv = (int)(threadIdx.x >> 6);
+	const int f0 = wv * H;
+	const int nx = g.n[0], ny = g.n[1];
+	const size_t N = (size_t)nx * ny;
+	const int c = c0 - 2 + lane;
+	const bool out = lane >= 2 && lane < 62 && c < ny;
+	const unsigned cm = (unsigned)wrap(c, ny);
+	const S* __restrict__ af = a + (size_t)f0 * N;
+	S* __restrict__ bf = b + (size_t)f0 * N;
+	// rows past the last one the chunk reads (xe + 1) load that row again
+	auto load_row = [&](int x, S* w) {
+		const unsigned r = (unsigned)wrap(min(x, xe + 1), nx) * (unsigned)ny;
+#pragma unroll
+		for (int j = 0; j < H; j++) w[j] = (af + j * N)[r + cm];
+	};
+	// sum_j u_j^2 and sum_j v_j^2 in field order over all Q fields: wave 0
+	// from zero over its fields, each next wave on from there
+	auto sums = [&](const S* u, const S* v, Sum& su, Sum& sv) {
+		Sum au = 0, av = 0;
+		if (wv == 0) {
+#pragma unroll
+			for (int j = 0; j < H; j++) {
+				au += u[j] * u[j];
+				av += v[j] * v[j];
+			}
+			part[0][lane] = au;
+			part[1][lane] = av;
+		}
+#pragma unroll
+		for (int k = 1; k < W; k++) {
+			__syncthreads();
+			if (wv == k) {
+				au = part[0][lane];
+				av = part[1][lane];
+#pragma unroll
+				for (int j = 0; j < H; j++) {
+					au += u[j] * u[j];
+					av += v[j] * v[j];
+				}
+				if (k + 1 < W) {
+					part[0][lane] = au;
+					part[1][lane] = av;
+				} else {
+					total[0][lane] = au;
+					total[1][lane] = av;
+				}
+			}
+		}
+		__syncthreads();
+		if (wv + 1 < W) {
+			au = total[0][lane];
+			av = total[1][lane];
+		}
+		su = au;
+		sv = av;
+	};
+	// the update of row x from rows x-1 (w0), x (w1), x+1 (w2), sum s
+	auto update = [&](const S* w0, const S* w1, const S* w2, Sum s, int j) -> S {
+		const S y = w1[j];
+		const S yl = from_prev(y, y), yh = from_next(y, y);
+		typename F::L lap = F::lap_zero();
+		F::lap_add(lap, g.w[0], g.wf[0], w2[j], y, w0[j]);
+		F::lap_add(lap, g.w[1], g.wf[1], yh, y, yl);
+		return (KIND == 3 && f0 + j == 0) ? y : F::gg(y, s, lap, p);
+	};
+	S n0[H], n1[H], n2[H], n3[H];  // phi^n ring
+	S q0[H], q1[H], q2[H], q3[H];  // phi^{n+1} ring
+#pragma unroll
+	for (int j = 0; j < H; j++) n3[j] = q2[j] = q3[j] = 0;
+	{
+		// phi^{n+1} of rows xb-1 (q0) and xb (q1) from phi^n rows xb-2 .. xb+1
+		S m2[H], m1[H];
+		load_row(xb - 2, m2);
+		load_row(xb - 1, m1);
+		load_row(xb, n0);
+		load_row(xb + 1, n1);
+		load_row(xb + 2, n2);
+		Sum sa, sb;
+		sums(m1, n0, sa, sb);
+#pragma unroll
+		for (int j = 0; j < H; j++) {
+			q0[j] = update(m2, m1, n0, sa, j);
+			q1[j] = update(m1, n0, n1, sb, j);
+		}
+	}
+	// row x: phi^n x (na), x+1 (nb), x+2 (nc), x+3 into nd; phi^{n+1} x-1
+	// (qa), x (qb), x+1 into qc
+	auto row = [&](int x, const S* na, const S* nb, const S* nc, S* nd, const S* qa, const S* qb, S* qc) {
+		load_row(x + 3, nd);
+		Sum s1, s2;
+		sums(nb, qb, s1, s2);
+#pragma unroll
+		for (int j = 0; j < H; j++) qc[j] = update(na, nb, nc, s1, j);
+		const unsigned o = (unsigned)x * (unsigned)ny + (unsigned)c;
+#pragma unroll
+		for (int j = 0; j < H; j++) {
+			const S v = update(qa, qb, qc, s2, j);
+			if (out) (bf + j * N)[o] = v;
+		}
+	};
+	for (int x = xb; x < xe; x += 4) {
+		row(x, n0, n1, n2, n3, q0, q1, q2);
+		if (x + 1 >= xe) break;
+		row(x + 1, n1, n2, n3, n0, q1, q2, q3);
+		if (x + 2 >= xe) break;
+		row(x + 2, n2, n3, n0, n1, q2, q3, q0);
+		if (x + 3 >= xe) break;
+		row(x + 3, n3, n0, n1, n2, q3, q0, q1);
+	}
+}
+
 }  // namespace mmspx

@@ -571,6 +571,67 @@ def test_c2_cahn_hilliard_full_size_mass():
     assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
+def test_split_strips_two_steps(prec):
+    """Grain growth and Zener pinning (periodic, 8, 16 and 32 fields) on the
+    split strips two steps per pass, on 2 and 4 waves (bodies 9 and 10):
+    partial strips, grids narrower than a strip and shorter than a chunk,
+    chunks of 2, 3, 5, 8, 13 and 64 rows (the four-row march ends at each
+    position), odd step counts (the last step on the one-step strips) and a
+    graph-replayed run; bitwise against the oracle."""
+    cases = [((41, 130), 8), ((33, 64), 16), ((9, 50), 8), ((5, 7), 16), ((70, 300), 32)]
+    for shape, q in cases:
+        if q == 32 and prec == "f32ref":
+            continue
+        a = O.uniform(int(np.prod(shape)) * q, 540 + q, 0.0, 1.0).reshape(shape + (q,))
+        if prec != "f64":
+            a = a.astype(np.float32)
+        for steps in (9, 10):
+            say("[gpu] two-step split strips %s q=%d %s steps %d" % (shape, q, prec, steps))
+            exp = O.gg(a, 0.01, steps, form=prec)
+            # Zener pinning: FP64 and FP32 forms, 32 fields in FP32 only (as its other fast paths)
+            expz = O.zener(a, 0.01, steps, form=prec) if prec != "f32ref" and (q < 32 or prec == "f32") else None
+            for body in (9, 10):
+                for chunk in ((2, 3, 5, 8, 13, 64) if steps == 9 else (4, 7)):
+                    with mx.Grid(shape, fields=q, vector=True, precision=prec) as g:
+                        g.set_variant(2)
+                        g.set_tile(1, 60, chunk)
+                        g.set_tile_options(body=body)
+                        g.upload(a)
+                        g.step("gg", [0.01], steps)
+                        assert g.get_tile()["opt"] == body
+                        assert np.array_equal(g.download(), exp), (shape, q, body, chunk, steps)
+                    if expz is not None and chunk in (3, 7, 64):
+                        with mx.Grid(shape, fields=q, vector=True, precision=prec) as g:
+                            g.set_variant(2)
+                            g.set_tile(1, 60, chunk)
+                            g.set_tile_options(body=body)
+                            g.upload(a)
+                            g.step("zener", [0.01], steps)
+                            assert np.array_equal(g.download(), expz), (shape, q, body, chunk, steps, "zener")
+    # a captured graph of two-step passes (100 steps on a small grid), from odd parity
+    shape, q = (33, 64), 8
+    a = O.uniform(int(np.prod(shape)) * q, 560, 0.0, 1.0).reshape(shape + (q,))
+    if prec != "f64":
+        a = a.astype(np.float32)
+    with mx.Grid(shape, fields=q, vector=True, precision=prec) as g:
+        g.set_variant(2)
+        g.set_tile(1, 60, 8)
+        g.set_tile_options(body=10)
+        g.upload(a)
+        g.step("gg", [0.01], 3)
+        g.step("gg", [0.01], 100)
+        assert np.array_equal(g.download(), O.gg(a, 0.01, 103, form=prec))
+    # non-periodic grids are refused
+    with mx.Grid((20, 70), fields=8, vector=True, precision=prec, bc=[(N, N), (P, P)]) as g:
+        g.set_variant(2)
+        g.set_tile(1, 60, 8)
+        g.set_tile_options(body=9)
+        g.upload(np.zeros((20, 70, 8)))
+        with pytest.raises(mx.MmspxError):
+            g.step("gg", [0.01], 2)
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_zener_and_ostwald(prec):
     """Zener pinning on the generic kernel, the wave strips and the split
