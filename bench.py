@@ -753,8 +753,10 @@ def main():
         kname = "ow_waves"
     elif tile["opt"] == 2:
         kname = "wave_strips"
-    elif tile["opt"] == 4:
+    elif tile["opt"] in (4, 5):
         kname = "wave_strips_split"
+    elif tile["opt"] in (9, 10):
+        kname = "wave_strips_split2"
     elif tile["opt"] == 3 and base == "ac":
         kname = "ac_tblocks"
     elif base == "gg" and tile["opt"] == 1:
