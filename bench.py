@@ -755,7 +755,7 @@ def main():
         kname = "wave_strips"
     elif tile["opt"] in (4, 5):
         kname = "wave_strips_split"
-    elif tile["opt"] in (9, 10):
+    elif tile["opt"] in (9, 10, 11):
         kname = "wave_strips_split2"
     elif tile["opt"] == 3 and base == "ac":
         kname = "ac_tblocks"

@@ -259,7 +259,8 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
  * neighbours across lanes by DPP, no LDS); 2-D grain growth and Zener pinning
  * with 8, 16 or 32 fields: 4 and 5 select the split strips (two or four
  * waves per strip, each holding its share of the fields), 9 and 10 the same
- * two steps per pass on periodic grids (1 x 60 strips). 3-D periodic
+ * two steps per pass on periodic grids (1 x 60 strips); 11 two steps per pass
+ * on one wave per strip for 2, 4 or 8 fields (periodic). 3-D periodic
  * Cahn-Hilliard (deterministic chemical potentials): 6 and 7 take two steps
  * per pass over HBM (6 on 32 x 64 tiles, 7 on 32 x 32 tiles; an odd last
  * step runs body 2). On a 3-D grid the tile follows the body: 7 takes 32 x

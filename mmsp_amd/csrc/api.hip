@@ -479,8 +479,8 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 int mmspx_set_tile_options(mmspx_state* s, int body)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
-	if (body < 0 || body > 10 || (body == 3 && s->geom.dim != 3) ||
-	    ((body == 4 || body == 5 || body == 9 || body == 10) && s->geom.dim != 2) ||
+	if (body < 0 || body > 11 || (body == 3 && s->geom.dim != 3) ||
+	    ((body == 4 || body == 5 || body >= 9) && s->geom.dim != 2) ||
 	    (body >= 6 && body <= 8 && s->geom.dim != 3))
 		return fail(MMSPX_EINVAL, "unknown tile body");
 	if (!s->tuned) {
@@ -491,7 +491,7 @@ int mmspx_set_tile_options(mmspx_state* s, int body)
 	// 2-D: the split strips run 1 x 64 strips (bodies 4, 5), two steps per
 	// pass 1 x kChWaveOut (bodies 9, 10)
 	if (s->geom.dim == 2 && s->tile.variant == 2) {
-		if (body == 9 || body == 10) {
+		if (body >= 9 && body <= 11) {
 			s->tile.ty = 1;
 			s->tile.tz = kChWaveOut;
 		} else if (body == 4 || body == 5) {

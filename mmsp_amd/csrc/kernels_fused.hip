@@ -423,6 +423,13 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// 4-row chunks below 2^21 nodes, 16 above (2 and 4 fields, 512^2 .. 4096^2,
 	// FP32 and FP64: 1.4 to 1.9 times faster than one thread per node; 1024^2
 	// x 4 FP32 8.0 us against 12.0, 4096^2 x 4 FP64 219.7 against 354.1)
+	// Periodic: two steps per pass on one wave per strip (body 11), 8-row
+	// chunks below 2^21 nodes and 16 above (2048^2 x 2 FP64 13.8 us per step
+	// against 22.2 for the wave strips, 2048^2 x 4 FP32 14.4 against 23.0,
+	// 1024^2 x 4 FP64 9.7 against 14.1, 512^2 x 2 FP64 3.2 against 4.6; Zener
+	// 2048^2 x 4 FP32 13.5 against 22.5)
+	if ((model == MMSPX_GG || model == MMSPX_ZENER) && all_periodic(g))
+		return TuneChoice{2, 1, kChWaveOut, nodes < ((int64_t)1 << 21) ? 8 : 16, 11};
 	if (model == MMSPX_GG || model == MMSPX_ZENER)
 		return TuneChoice{2, 1, kWaveCols, nodes < ((int64_t)1 << 21) ? 4 : 16, 2};
 	// 2-D Cahn-Hilliard: wave strips (1024^2 FP64 6.8 us per step against 13.5
@@ -497,6 +504,8 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		if (split && model == MMSPX_ZENER && all_periodic(g))
 			for (int c : {8, 16, 32, 64})
 				for (int body = 9; body <= 10; body++) add(1, kChWaveOut, c, body);
+		if (model == MMSPX_ZENER && (g.fields == 2 || g.fields == 4 || g.fields == 8) && all_periodic(g))
+			for (int c : {8, 16, 32, 64}) add(1, kChWaveOut, c, 11);
 		return n;
 	}
 	if (g.dim == 3 && !ch) {
@@ -549,10 +558,13 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		if (split)
 			for (int c : {8, 16, 32, 64})
 				for (int body = 4; body <= 5; body++) add(1, kWaveCols, c, body);
-		// periodic: the split strips two steps per pass
+		// periodic: the split strips two steps per pass (one wave for 2, 4, 8 fields)
 		if (split && all_periodic(g))
 			for (int c : {8, 16, 32, 64})
 				for (int body = 9; body <= 10; body++) add(1, kChWaveOut, c, body);
+		if ((model == MMSPX_GG || model == MMSPX_ZENER) && (g.fields == 2 || g.fields == 4 || g.fields == 8) &&
+		    all_periodic(g))
+			for (int c : {8, 16, 32, 64}) add(1, kChWaveOut, c, 11);
 		// periodic Allen-Cahn: temporal blocks (chunk = steps per pass)
 		if (model == MMSPX_AC && all_periodic(g))
 			for (int tx : {8, 16, 32})
@@ -958,32 +970,49 @@ static int launch_split2_q(const Geom& g, const ModelParams& p, const void* a, v
 template <int PREC, int KIND, int W>
 static int launch_split2_w(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
 {
-	switch (g.fields) {
-	case 8: return launch_split2_q<PREC, 8, KIND, W>(g, p, a, b, st, chunk);
-	case 16: return launch_split2_q<PREC, 16, KIND, W>(g, p, a, b, st, chunk);
-	case 32: return launch_split2_q<PREC, 32, KIND, W>(g, p, a, b, st, chunk);
+	if constexpr (W == 1) {
+		// one wave holds every field: 2, 4 or 8 of them
+		switch (g.fields) {
+		case 2: return launch_split2_q<PREC, 2, KIND, 1>(g, p, a, b, st, chunk);
+		case 4: return launch_split2_q<PREC, 4, KIND, 1>(g, p, a, b, st, chunk);
+		case 8: return launch_split2_q<PREC, 8, KIND, 1>(g, p, a, b, st, chunk);
+		}
+		return fail(MMSPX_EUNSUPPORTED, "the one-wave two-step strips take 2, 4 or 8 fields");
+	} else {
+		switch (g.fields) {
+		case 8: return launch_split2_q<PREC, 8, KIND, W>(g, p, a, b, st, chunk);
+		case 16: return launch_split2_q<PREC, 16, KIND, W>(g, p, a, b, st, chunk);
+		case 32: return launch_split2_q<PREC, 32, KIND, W>(g, p, a, b, st, chunk);
+		}
+		return fail(MMSPX_EUNSUPPORTED, "split wave strips take 8, 16 or 32 fields");
 	}
-	return fail(MMSPX_EUNSUPPORTED, "split wave strips take 8, 16 or 32 fields");
 }
 
-// Grain growth and Zener pinning, periodic 2-D, 8, 16 or 32 fields: two steps
-// per pass on the split strips, W = 2 (body 9) or 4 (body 10) waves per
-// strip; a single step (an odd last one) runs the one-step split strips
+// Grain growth and Zener pinning, periodic 2-D: two steps per pass on the
+// split strips, W = 2 (body 9) or 4 (body 10) waves per strip for 8, 16 or
+// 32 fields, one wave (body 11) for 2, 4 or 8; a single step (an odd last
+// one) runs the one-step split strips (W = 2, 4) or wave strips (W = 1)
 template <int PREC>
 static int launch_split2(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk,
                          int waves, int nsteps)
 {
 	if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "the two-step split strips need periodic boundaries");
-	if (nsteps == 1) return launch_split<PREC>(g, p, a, b, st, chunk, waves);
+	if (nsteps == 1)
+		return waves == 1 ? launch_wave<PREC>(g, p, a, b, st, chunk) : launch_split<PREC>(g, p, a, b, st, chunk, waves);
 	if (nsteps != 2) return fail(MMSPX_EINVAL, "the two-step split strips take one or two steps per pass");
-	if (p.model == MMSPX_ZENER) {
-		if constexpr (PREC != MMSPX_F32REF)
-			return waves == 4 ? launch_split2_w<PREC, 3, 4>(g, p, a, b, st, chunk)
-			                  : launch_split2_w<PREC, 3, 2>(g, p, a, b, st, chunk);
-		return fail(MMSPX_EUNSUPPORTED, "Zener pinning runs the MMSPX_F64 and MMSPX_F32 forms");
+	const bool zener = p.model == MMSPX_ZENER;
+	if constexpr (PREC == MMSPX_F32REF) {
+		if (zener) return fail(MMSPX_EUNSUPPORTED, "Zener pinning runs the MMSPX_F64 and MMSPX_F32 forms");
+		return waves == 1 ? launch_split2_w<PREC, 1, 1>(g, p, a, b, st, chunk)
+		     : waves == 4 ? launch_split2_w<PREC, 1, 4>(g, p, a, b, st, chunk)
+		                  : launch_split2_w<PREC, 1, 2>(g, p, a, b, st, chunk);
+	} else {
+		if (waves == 1)
+			return zener ? launch_split2_w<PREC, 3, 1>(g, p, a, b, st, chunk) : launch_split2_w<PREC, 1, 1>(g, p, a, b, st, chunk);
+		if (waves == 4)
+			return zener ? launch_split2_w<PREC, 3, 4>(g, p, a, b, st, chunk) : launch_split2_w<PREC, 1, 4>(g, p, a, b, st, chunk);
+		return zener ? launch_split2_w<PREC, 3, 2>(g, p, a, b, st, chunk) : launch_split2_w<PREC, 1, 2>(g, p, a, b, st, chunk);
 	}
-	return waves == 4 ? launch_split2_w<PREC, 1, 4>(g, p, a, b, st, chunk)
-	                  : launch_split2_w<PREC, 1, 2>(g, p, a, b, st, chunk);
 }
 
 template <int PREC, int Q>
@@ -1038,8 +1067,8 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		return fail(MMSPX_EUNSUPPORTED, "Ostwald ripening runs the wave strips (1 x 60) or the generic kernels");
 	case MMSPX_ZENER:
 		if (g.dim == 3) return launch_march<PREC, 3>(g, p, a, b, st, t);
-		if ((t.opt == 9 || t.opt == 10) && t.ty == 1 && t.tz == kChWaveOut)
-			return launch_split2<PREC>(g, p, a, b, st, chunk, t.opt == 10 ? 4 : 2, nsteps);
+		if (t.opt >= 9 && t.opt <= 11 && t.ty == 1 && t.tz == kChWaveOut)
+			return launch_split2<PREC>(g, p, a, b, st, chunk, t.opt == 11 ? 1 : t.opt == 10 ? 4 : 2, nsteps);
 		if ((t.opt == 4 || t.opt == 5) && t.ty == 1 && t.tz == kWaveCols)
 			return launch_split<PREC>(g, p, a, b, st, chunk, t.opt == 5 ? 4 : 2);
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
@@ -1070,8 +1099,8 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		return launch_ac<PREC, 256>(g, p, a, b, st, chunk);
 	case MMSPX_GG:
 		if (g.dim == 3) return launch_march<PREC, 1>(g, p, a, b, st, t);
-		if ((t.opt == 9 || t.opt == 10) && t.ty == 1 && t.tz == kChWaveOut)
-			return launch_split2<PREC>(g, p, a, b, st, chunk, t.opt == 10 ? 4 : 2, nsteps);
+		if (t.opt >= 9 && t.opt <= 11 && t.ty == 1 && t.tz == kChWaveOut)
+			return launch_split2<PREC>(g, p, a, b, st, chunk, t.opt == 11 ? 1 : t.opt == 10 ? 4 : 2, nsteps);
 		// ty > 1: 2-D tiles of ty (8 or 16) rows x 64 columns; body 2 with 64
 		// columns: wave strips; otherwise LDS-row marching strips
 		if ((t.opt == 4 || t.opt == 5) && t.ty == 1 && t.tz == kWaveCols)
@@ -1091,7 +1120,7 @@ int fused_steps_per_launch(int model, const TuneChoice& t)
 {
 	if (model == MMSPX_AC && t.tz == kTbCols) return std::max(1, std::min(t.chunk, kTbMaxSteps));
 	if ((model == MMSPX_CH || model == MMSPX_CH_DW) && (t.opt == 6 || t.opt == 7)) return 2;
-	if ((model == MMSPX_GG || model == MMSPX_ZENER) && (t.opt == 9 || t.opt == 10)) return 2;
+	if ((model == MMSPX_GG || model == MMSPX_ZENER) && t.opt >= 9 && t.opt <= 11) return 2;
 	return 1;
 }
 

@@ -1577,8 +1577,9 @@ __device__ __forceinline__ void strip_wave_split(const typename Form<PREC>::S* _
 // (x-1 .. x+1), both rotated over a four-row unrolled march; phi^{n+1} never
 // goes to memory. An iteration at row x first forms sum_j phi_j^2 of phi^n at
 // row x+1 and of phi^{n+1} at row x in one pass of the field-order chain
-// across the waves (W barriers, as the one-step strips take per row), then
-// phi^{n+1}(x+1) and phi^{n+2}(x). Every value is the one-step kernels' for
+// across the waves (W barriers, as the one-step strips take per row; none
+// when one wave holds all the fields, W = 1), then phi^{n+1}(x+1) and
+// phi^{n+2}(x). Every value is the one-step kernels' for
 // that node and step (graingrowth.cpp:91-107; KIND 3 holds field 0,
 // zener.cpp:117-135), so the result is bitwise two single steps. Periodic
 // grids only: a lane past the grid's edge holds the wrapped column, whose
@@ -1592,7 +1593,7 @@ __device__ __forceinline__ void strip_wave_split2(const typename Form<PREC>::S* 
 	using S = typename F::S;
 	using Sum = typename F::Sum;
 	constexpr int H = Q / W;
-	static_assert(Q % W == 0 && W >= 2, "the two-step split strips hold Q/W fields per wave, W >= 2");
+	static_assert(Q % W == 0, "the two-step split strips hold Q/W fields per wave");
 	__shared__ Sum part[2][64], total[2][64];
 	const int lane = threadIdx.x & 63;
 	const int wv = (int)(threadIdx.x >> 6);
@@ -1614,6 +1615,16 @@ __device__ __forceinline__ void strip_wave_split2(const typename Form<PREC>::S* 
 	// from zero over its fields, each next wave on from there
 	auto sums = [&](const S* u, const S* v, Sum& su, Sum& sv) {
 		Sum au = 0, av = 0;
+		if constexpr (W == 1) {  // one wave holds every field: no exchange
+#pragma unroll
+			for (int j = 0; j < H; j++) {
+				au += u[j] * u[j];
+				av += v[j] * v[j];
+			}
+			su = au;
+			sv = av;
+			return;
+		}
 		if (wv == 0) {
 #pragma unroll
 			for (int j = 0; j < H; j++) {

@@ -573,13 +573,15 @@ def test_c2_cahn_hilliard_full_size_mass():
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f32ref"])
 def test_split_strips_two_steps(prec):
-    """Grain growth and Zener pinning (periodic, 8, 16 and 32 fields) on the
-    split strips two steps per pass, on 2 and 4 waves (bodies 9 and 10):
+    """Grain growth and Zener pinning (periodic) on the split strips two steps
+    per pass: 8, 16 and 32 fields on 2 and 4 waves (bodies 9 and 10), 2, 4 and
+    8 fields on one wave (body 11):
     partial strips, grids narrower than a strip and shorter than a chunk,
     chunks of 2, 3, 5, 8, 13 and 64 rows (the four-row march ends at each
     position), odd step counts (the last step on the one-step strips) and a
     graph-replayed run; bitwise against the oracle."""
-    cases = [((41, 130), 8), ((33, 64), 16), ((9, 50), 8), ((5, 7), 16), ((70, 300), 32)]
+    cases = [((41, 130), 8), ((33, 64), 16), ((9, 50), 8), ((5, 7), 16), ((70, 300), 32), ((41, 130), 2),
+             ((9, 50), 4), ((5, 7), 2)]
     for shape, q in cases:
         if q == 32 and prec == "f32ref":
             continue
@@ -591,7 +593,8 @@ def test_split_strips_two_steps(prec):
             exp = O.gg(a, 0.01, steps, form=prec)
             # Zener pinning: FP64 and FP32 forms, 32 fields in FP32 only (as its other fast paths)
             expz = O.zener(a, 0.01, steps, form=prec) if prec != "f32ref" and (q < 32 or prec == "f32") else None
-            for body in (9, 10):
+            bodies = ((9, 10) if q > 4 else ()) + ((11,) if q <= 8 else ())
+            for body in bodies:
                 for chunk in ((2, 3, 5, 8, 13, 64) if steps == 9 else (4, 7)):
                     with mx.Grid(shape, fields=q, vector=True, precision=prec) as g:
                         g.set_variant(2)
