@@ -747,6 +747,8 @@ def main():
         kname = base + "_generic"
     elif len(shape) == 3 and tile["opt"] == 8:
         kname = "march3d"
+    elif len(shape) == 3 and tile["opt"] == 12:
+        kname = "march3d2"
     elif base == "ch":
         kname = "ch_fused"
     elif base == "ow":

@@ -268,7 +268,12 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
  * 3-D Allen-Cahn, model A, grain growth, Zener pinning and multi-phase grain growth
  * (2, 4 or 8 fields), any boundary: 8 marches the tile along axis 0 (8, 16,
  * 32 x 64 and 16 x 32 tiles for Allen-Cahn; 8 x 64 and 16 x 32 for the
- * vector models). */
+ * vector models). 12 is that march two steps per pass over HBM on periodic
+ * grids (Allen-Cahn on 16 x 32, 16 x 64 and 32 x 64 tiles; grain growth and
+ * Zener pinning with 2, 4 or 8 fields on 8 x 64, 16 x 32 and 8 x 32 tiles,
+ * the FP64 form with 8 fields on 8 x 32 only); an odd last step runs body 8;
+ * the tile follows the body (16 x 32 where the current tile is not one of
+ * those). */
 int mmspx_set_tile_options(mmspx_state* s, int body);
 /* Returns variant + 16 * tile body (1: register queues, or the 2-D
  * grain-growth tile; 2: wave strips, or 3-D register queues on 768 threads;

@@ -468,7 +468,7 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 	// 3-D two-step bodies: 32 x 32 tiles run body 7 only; body 6 runs 32 x 64
 	if (s->geom.dim == 3 && ty == 32 && tz == 32) opt = 7;
 	else if (s->geom.dim == 3 && s->tuned && s->tile.opt == 6 && ty == 32 && tz == 64) opt = 6;
-	else if (s->geom.dim == 3 && s->tuned && s->tile.opt == 8) opt = 8;
+	else if (s->geom.dim == 3 && s->tuned && (s->tile.opt == 8 || s->tile.opt == 12)) opt = s->tile.opt;
 	TuneChoice t{2, ty, tz, chunk, opt};
 	if (!tile_known(s->geom, t)) return fail(MMSPX_EINVAL, "no compiled tile of that shape for this grid");
 	s->tile = t;
@@ -479,9 +479,9 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk)
 int mmspx_set_tile_options(mmspx_state* s, int body)
 {
 	if (!s) return fail(MMSPX_EINVAL, "null state");
-	if (body < 0 || body > 11 || (body == 3 && s->geom.dim != 3) ||
-	    ((body == 4 || body == 5 || body >= 9) && s->geom.dim != 2) ||
-	    (body >= 6 && body <= 8 && s->geom.dim != 3))
+	if (body < 0 || body > 12 || (body == 3 && s->geom.dim != 3) ||
+	    ((body == 4 || body == 5 || (body >= 9 && body <= 11)) && s->geom.dim != 2) ||
+	    (((body >= 6 && body <= 8) || body == 12) && s->geom.dim != 3))
 		return fail(MMSPX_EINVAL, "unknown tile body");
 	if (!s->tuned) {
 		s->tile = default_choice(s->geom, s->desc.precision, MMSPX_CH_DW);
@@ -500,10 +500,17 @@ int mmspx_set_tile_options(mmspx_state* s, int body)
 		}
 	}
 	// 3-D: keep a tile the body runs (32 x 32 tiles run body 7 only, body 6
-	// runs 32 x 64 tiles, the plane march 8 x 64 for every model)
+	// runs 32 x 64 tiles, the plane march 8 x 64 for every model, the
+	// two-step march 16 x 32 for every model)
 	if (s->geom.dim == 3 && s->tile.variant == 2) {
 		const int ty = s->tile.ty, tz = s->tile.tz;
-		if (body == 8) {
+		if (body == 12) {
+			if (!((ty == 16 && tz == 32) || (ty == 16 && tz == 64) || (ty == 32 && tz == 64) || (ty == 8 && tz == 64) ||
+			      (ty == 8 && tz == 32))) {
+				s->tile.ty = 16;
+				s->tile.tz = 32;
+			}
+		} else if (body == 8) {
 			if (!((ty == 8 && tz == 64) || (ty == 16 && tz == 32) || (ty == 16 && tz == 64) || (ty == 32 && tz == 64))) {
 				s->tile.ty = 8;
 				s->tile.tz = 64;

@@ -88,6 +88,23 @@ __global__ void __launch_bounds__(NT) march3d_k(const typename Form<PREC>::S* __
 	march3d<PREC, Q, KIND, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, min(xb + chunk_len, g.n[0]), cin, wout);
 }
 
+// The plane march two steps per pass (periodic Allen-Cahn, grain growth,
+// Zener pinning): one workgroup per (tile, chunk), as march3d_k.
+template <int PREC, int Q, int KIND, int TY, int TZ, int NT>
+__global__ void __launch_bounds__(NT) march3d2_k(const typename Form<PREC>::S* __restrict__ a,
+                                                 typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
+                                                 int chunk_len, int tiles_y, int tiles_z, int ntiles)
+{
+	const int tile = xcd_tile(ntiles);
+	if (tile < 0) return;
+	const int tz = tile % tiles_z;
+	const int t2 = tile / tiles_z;
+	const int ty = t2 % tiles_y;
+	const int chunk = t2 / tiles_y;
+	const int xb = chunk * chunk_len;
+	march3d2<PREC, Q, KIND, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, min(xb + chunk_len, g.n[0]));
+}
+
 template <int PREC, int TZ>
 __global__ void __launch_bounds__(TZ) ac_tiled(const typename Form<PREC>::S* __restrict__ a,
                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
@@ -320,6 +337,65 @@ static TuneChoice two_step_choice(const Geom& g, int prec)
 	return best;
 }
 
+// Periodic 3-D Allen-Cahn, grain growth and Zener pinning: the plane march
+// two steps per pass (body 12) where its 64-node tile rows fit the last axis
+// (padding at most an eighth of it: 160^3 Allen-Cahn FP64 pads to 192 and
+// takes 15.5 us per step against 14.1 for the one-step march).
+// Allen-Cahn: 16 x 64 or 32 x 64 tiles and the chunk from the cost model of
+// two_step_choice: rounds x (chunk + 3 planes: the halo plane on each side
+// and the queue's first load) x owned points per thread (16 x 64: 1188 on
+// 640 threads, two workgroups per CU in rounds of 512 each 1.7 times as
+// long; 32 x 64: 2244 on 768, one per CU). It picks the measured best, or
+// one within 5 %, at every side swept (us per step, against the one-step
+// march): FP64 64^3 16 x 64 with 4-plane chunks 4.3 (5.7), 128^3 16 x 64
+// with 8 7.2 (10.3), 192^3 32 x 64 with 16 16.7 (21.9), 256^3 32 x 64 with
+// 32 30.0 (43.9; 16 x 64 29.8); FP32 64^3 3.6 (6.9), 128^3 5.6 (8.0; 32 x 64
+// with 4: 5.3), 192^3 11.2 (15.7), 256^3 19.0 (24.3).
+// Grain growth and Zener pinning (2 or 4 fields; 8 in FP32 from 2^22 nodes):
+// 8 x 64 tiles (one node and its halo per thread, 704 threads), 64-plane
+// chunks halved until a launch has 256 workgroups per CU that fit (two for
+// 2 fields, one otherwise by registers): 256^3 x 2 FP64 81.0 (117.0), x 4
+// FP64 178.6 (239.6), x 4 FP32 111.8 (125.2), x 8 FP32 222.0 (253.3); Zener
+// x 2 FP32 34.7 (54.8), x 4 107.0 (122.4); 192^3 x 8 FP32 109.1 (125.6);
+// 128^3 x 2 FP32 8.6 (12.2), x 4 FP64 24.6 (27.4); 64^3 x 4 FP32 5.6 (7.1).
+// 128^3 x 8 FP32 gains nothing (29.5 against 28.6), and 8 FP64 fields fit
+// the LDS on 8 x 32 tiles only (256^3: 488.5 against 481.0): those stay on
+// the one-step march.
+static bool march2_choice(const Geom& g, int prec, int model, TuneChoice* out)
+{
+	constexpr int kCUs = 256;
+	if (!all_periodic(g) || (prec != MMSPX_F64 && prec != MMSPX_F32)) return false;
+	if (model != MMSPX_AC && model != MMSPX_GG && model != MMSPX_ZENER) return false;
+	const int nz = g.n[2], pad = (nz + 63) / 64 * 64 - nz;
+	if (pad * 8 > nz) return false;
+	if (model == MMSPX_AC) {
+		double best_cost = 1e300;
+		for (int ty : {16, 32}) {
+			const double own = ty == 16 ? 1188.0 / 640 : 2244.0 / 768;
+			const int64_t tiles = (int64_t)((g.n[1] + ty - 1) / ty) * ((nz + 63) / 64);
+			for (int c : {4, 8, 16, 32, 64}) {
+				const int64_t wgs = tiles * ((g.n[0] + c - 1) / c);
+				const double rounds = (ty == 16 && wgs > kCUs) ? 1.7 * (double)((wgs + 2 * kCUs - 1) / (2 * kCUs))
+				                                               : (double)((wgs + kCUs - 1) / kCUs);
+				const double cost = rounds * (c + 3) * own;
+				if (cost < best_cost) {
+					best_cost = cost;
+					*out = TuneChoice{2, ty, 64, c, 12};
+				}
+			}
+		}
+		return true;
+	}
+	const int q = g.fields;
+	if (!(q == 2 || q == 4 || (q == 8 && prec == MMSPX_F32 && g.nodes >= ((int64_t)1 << 22)))) return false;
+	const int per_cu = q == 2 ? 2 : 1;
+	const int64_t tiles = (int64_t)((g.n[1] + 7) / 8) * ((nz + 63) / 64);
+	int chunk = 64;
+	while (chunk > 4 && tiles * ((g.n[0] + chunk - 1) / chunk) < (int64_t)kCUs * per_cu) chunk /= 2;
+	*out = TuneChoice{2, 8, 64, chunk, 12};
+	return true;
+}
+
 // Defaults come from sweeps of the candidates below on MI355X (the figures
 // quoted are microseconds per step from those sweeps).
 TuneChoice default_choice(const Geom& g, int prec, int model)
@@ -362,6 +438,8 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 		while (chunk > 4 && tiles * ((g.n[0] + chunk - 1) / chunk) < 1024) chunk /= 2;
 		return TuneChoice{2, 8, 64, chunk, 8};
 	}
+	TuneChoice two;
+	if (g.dim == 3 && march2_choice(g, prec, model, &two)) return two;
 	if (g.dim == 3 && (model == MMSPX_AC || model == MMSPX_AC_NOISE || model == MMSPX_GG || model == MMSPX_ZENER)) {
 		const bool scalar = model == MMSPX_AC || model == MMSPX_AC_NOISE;
 		const int ty = scalar ? (prec == MMSPX_F64 ? 16 : 32) : 8;
@@ -519,6 +597,22 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 				add(32, 64, c, 8);
 			}
 		}
+		// two steps per pass on periodic grids (Allen-Cahn, grain growth,
+		// Zener pinning; the 8-field FP64 planes fit the LDS on 8 x 32 only)
+		const bool two = all_periodic(g) && model != MMSPX_AC_NOISE && model != MMSPX_GG_ANISO &&
+		                 model != MMSPX_OSTWALD && (prec == MMSPX_F64 || prec == MMSPX_F32);
+		if (two)
+			for (int c : {8, 16, 32, 64}) {
+				if (scalar) {
+					add(16, 64, c, 12);
+					add(32, 64, c, 12);
+				} else if (g.fields == 8 && prec == MMSPX_F64) {
+					add(8, 32, c, 12);
+				} else {
+					add(8, 64, c, 12);
+					add(16, 32, c, 12);
+				}
+			}
 		return n;
 	}
 	const bool edges = !all_periodic(g);  // Cahn-Hilliard: the bodies that map non-periodic sides
@@ -710,6 +804,71 @@ static int launch_march(const Geom& g, const ModelParams& p, const void* a, void
 		}
 	}
 	return fail(MMSPX_EINVAL, "no 3-D march compiled for this tile");
+}
+
+// Tile body 12 on periodic 3-D grids: the plane march two steps per pass.
+// The threads own the tile and its one-cell halo, one point each for the
+// vector models (8 x 64 tiles on 704 threads, 16 x 32 on 640, 8 x 32 on
+// 384), two or three for Allen-Cahn (16 x 32 on 320, 16 x 64 on 640, 32 x 64
+// on 768). LDS: the phi^n plane with a two-cell halo and the phi^{n+1} plane
+// on the owned points, both double-buffered, Q fields each.
+template <int PREC, int Q, int KIND, int TY, int TZ, int NT>
+static int launch_march2_body(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	using S = typename Form<PREC>::S;
+	constexpr size_t lds = (size_t)2 * Q * ((TY + 4) * (TZ + 4) + (TY + 2) * (TZ + 2)) * sizeof(S);
+	if constexpr (lds > (size_t)160 * 1024) {
+		return fail(MMSPX_EINVAL, "the planes of this tile and field count do not fit the LDS");
+	} else {
+		const int tiles_y = (g.n[1] + TY - 1) / TY;
+		const int tiles_z = (g.n[2] + TZ - 1) / TZ;
+		const int chunks = (g.n[0] + chunk - 1) / chunk;
+		const int ntiles = tiles_y * tiles_z * chunks;
+		const int blocks = (ntiles + 7) / 8 * 8;
+		hipLaunchKernelGGL((march3d2_k<PREC, Q, KIND, TY, TZ, NT>), dim3(blocks), dim3(NT), 0, st, (const S*)a,
+		                   (S*)b, g, p, chunk, tiles_y, tiles_z, ntiles);
+		MMSPX_HIP(hipGetLastError());
+		return 0;
+	}
+}
+
+template <int PREC, int Q, int KIND>
+static int launch_march2_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st,
+                           const TuneChoice& t, int chunk)
+{
+	if (t.ty == 8 && t.tz == 64) return launch_march2_body<PREC, Q, KIND, 8, 64, 704>(g, p, a, b, st, chunk);
+	if (t.ty == 16 && t.tz == 32) return launch_march2_body<PREC, Q, KIND, 16, 32, 640>(g, p, a, b, st, chunk);
+	if (t.ty == 8 && t.tz == 32) return launch_march2_body<PREC, Q, KIND, 8, 32, 384>(g, p, a, b, st, chunk);
+	return fail(MMSPX_EINVAL, "no two-step 3-D march compiled for this tile");
+}
+
+template <int PREC, int KIND>
+static int launch_march2(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st,
+                         const TuneChoice& t, int nsteps)
+{
+	const int chunk = t.chunk > 0 ? t.chunk : 32;
+	if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "tile body 12 (two steps per pass) needs periodic boundaries");
+	if (nsteps == 1) {
+		// an odd last step: the one-step march
+		const TuneChoice t1{2, KIND == 0 ? 16 : 8, KIND == 0 ? 32 : 64, chunk, 8};
+		return launch_march<PREC, KIND>(g, p, a, b, st, t1);
+	}
+	if (nsteps != 2) return fail(MMSPX_EINVAL, "tile body 12 takes one or two steps per pass");
+	if constexpr (PREC == MMSPX_F32REF) {
+		return fail(MMSPX_EUNSUPPORTED, "tile body 12 runs the MMSPX_F64 and MMSPX_F32 forms");
+	} else if constexpr (KIND == 0) {
+		if (t.ty == 16 && t.tz == 32) return launch_march2_body<PREC, 1, 0, 16, 32, 320>(g, p, a, b, st, chunk);
+		if (t.ty == 16 && t.tz == 64) return launch_march2_body<PREC, 1, 0, 16, 64, 640>(g, p, a, b, st, chunk);
+		if (t.ty == 32 && t.tz == 64) return launch_march2_body<PREC, 1, 0, 32, 64, 768>(g, p, a, b, st, chunk);
+		return fail(MMSPX_EINVAL, "no two-step 3-D march compiled for this tile");
+	} else {
+		switch (g.fields) {
+		case 2: return launch_march2_q<PREC, 2, KIND>(g, p, a, b, st, t, chunk);
+		case 4: return launch_march2_q<PREC, 4, KIND>(g, p, a, b, st, t, chunk);
+		case 8: return launch_march2_q<PREC, 8, KIND>(g, p, a, b, st, t, chunk);
+		}
+		return fail(MMSPX_EUNSUPPORTED, "the two-step 3-D march takes 2, 4 or 8 fields");
+	}
 }
 
 template <int PREC, int MU>
@@ -1066,7 +1225,8 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		}
 		return fail(MMSPX_EUNSUPPORTED, "Ostwald ripening runs the wave strips (1 x 60) or the generic kernels");
 	case MMSPX_ZENER:
-		if (g.dim == 3) return launch_march<PREC, 3>(g, p, a, b, st, t);
+		if (g.dim == 3) return t.opt == 12 ? launch_march2<PREC, 3>(g, p, a, b, st, t, nsteps)
+		                                   : launch_march<PREC, 3>(g, p, a, b, st, t);
 		if (t.opt >= 9 && t.opt <= 11 && t.ty == 1 && t.tz == kChWaveOut)
 			return launch_split2<PREC>(g, p, a, b, st, chunk, t.opt == 11 ? 1 : t.opt == 10 ? 4 : 2, nsteps);
 		if ((t.opt == 4 || t.opt == 5) && t.ty == 1 && t.tz == kWaveCols)
@@ -1086,7 +1246,8 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		if (t.opt == 2 && t.ty == 1 && t.tz == kWaveCols) return launch_wave<PREC>(g, p, a, b, st, chunk);
 		return fail(MMSPX_EUNSUPPORTED, "Allen-Cahn with noise runs the wave strips (1 x 64, body 2) or the generic kernel");
 	case MMSPX_AC:
-		if (g.dim == 3) return launch_march<PREC, 0>(g, p, a, b, st, t);
+		if (g.dim == 3) return t.opt == 12 ? launch_march2<PREC, 0>(g, p, a, b, st, t, nsteps)
+		                                   : launch_march<PREC, 0>(g, p, a, b, st, t);
 		if (t.tz == kTbCols) {
 			if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "temporal blocks need periodic boundaries");
 			if (nsteps < 1 || nsteps > kTbMaxSteps) return fail(MMSPX_EINVAL, "temporal blocks take 1..8 steps per pass");
@@ -1098,7 +1259,8 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 		if (t.tz == 128) return launch_ac<PREC, 128>(g, p, a, b, st, chunk);
 		return launch_ac<PREC, 256>(g, p, a, b, st, chunk);
 	case MMSPX_GG:
-		if (g.dim == 3) return launch_march<PREC, 1>(g, p, a, b, st, t);
+		if (g.dim == 3) return t.opt == 12 ? launch_march2<PREC, 1>(g, p, a, b, st, t, nsteps)
+		                                   : launch_march<PREC, 1>(g, p, a, b, st, t);
 		if (t.opt >= 9 && t.opt <= 11 && t.ty == 1 && t.tz == kChWaveOut)
 			return launch_split2<PREC>(g, p, a, b, st, chunk, t.opt == 11 ? 1 : t.opt == 10 ? 4 : 2, nsteps);
 		// ty > 1: 2-D tiles of ty (8 or 16) rows x 64 columns; body 2 with 64
@@ -1121,6 +1283,7 @@ int fused_steps_per_launch(int model, const TuneChoice& t)
 	if (model == MMSPX_AC && t.tz == kTbCols) return std::max(1, std::min(t.chunk, kTbMaxSteps));
 	if ((model == MMSPX_CH || model == MMSPX_CH_DW) && (t.opt == 6 || t.opt == 7)) return 2;
 	if ((model == MMSPX_GG || model == MMSPX_ZENER) && t.opt >= 9 && t.opt <= 11) return 2;
+	if ((model == MMSPX_AC || model == MMSPX_GG || model == MMSPX_ZENER) && t.opt == 12) return 2;
 	return 1;
 }
 

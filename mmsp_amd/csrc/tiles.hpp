@@ -831,6 +831,194 @@ __device__ __forceinline__ void march3d(const typename Form<PREC>::S* __restrict
 	}
 }
 
+// ---- 3-D Allen-Cahn, grain growth, Zener pinning: two steps per pass -------
+// Periodic grids. The plane march of march3d advancing two steps per pass:
+// each thread owns fixed points of the tile and a one-cell in-plane halo
+// (region R1) and keeps, for each of them, the Q fields of phi^n at planes
+// P-1, P, P+1 (P+2 in flight) and of phi^{n+1} at P-2, P-1, P in registers.
+// The iteration at plane P publishes phi^n(P) on a plane tile with a two-cell
+// halo and forms phi^{n+1}(P) on R1 (published too), then phi^{n+2}(P-1) on
+// the tile from the phi^{n+1} plane the previous iteration published; both LDS
+// planes are double-buffered, so one barrier per iteration orders everything.
+// phi^{n+1} never goes to HBM. Every value is the one-step kernels' for that
+// node and step (the halo recomputes what the neighbouring tile computes), so
+// the result is bitwise two single steps. KIND 0 Allen-Cahn, 1 grain growth,
+// 3 Zener pinning (field 0 held).
+template <int PREC, int Q, int KIND, int TY, int TZ, int NT>
+__device__ __forceinline__ void march3d2(const typename Form<PREC>::S* __restrict__ a,
+                                         typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                         int y0, int z0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	static_assert(KIND == 0 || KIND == 1 || KIND == 3, "the two-step march takes the deterministic models");
+	constexpr int P2Y = TY + 4, P2Z = TZ + 4;  // phi^n plane tile (two-cell halo)
+	constexpr int R1Y = TY + 2, R1Z = TZ + 2;  // phi^{n+1}: the threads' own points
+	constexpr int PN = P2Y * P2Z, N1 = R1Y * R1Z, RN = PN - N1;
+	constexpr int M = (N1 + NT - 1) / NT;
+	constexpr int MR = (RN + NT - 1) / NT;
+	__shared__ S lphi[2][Q * PN];
+	__shared__ S lq[2][Q * N1];
+
+	const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+	const int64_t plane = (int64_t)ny * nz;
+	const int64_t N = (int64_t)nx * plane;
+
+	// own point k: R1 index e = tid + k*NT at (r, c), node (y0-1+r, z0-1+c);
+	// q2 its index in the phi^n tile; fl 1 in R1, 2 in the tile (written)
+	int ooff[M], q2[M], fl[M];
+#pragma unroll
+	for (int k = 0; k < M; k++) {
+		const int e = threadIdx.x + k * NT;
+		ooff[k] = q2[k] = fl[k] = 0;
+		if (e < N1) {
+			const int r = e / R1Z, c = e - (e / R1Z) * R1Z;
+			const int y = y0 - 1 + r, z = z0 - 1 + c;
+			ooff[k] = wrap(y, ny) * nz + wrap(z, nz);
+			q2[k] = (r + 1) * P2Z + (c + 1);
+			fl[k] = 1 | ((r >= 1 && r < TY + 1 && c >= 1 && c < TZ + 1 && y < ny && z < nz) ? 2 : 0);
+		}
+	}
+	// ring points: rows 0 and P2Y-1 and columns 0 and P2Z-1 of the phi^n tile
+	int roff[MR], rq[MR];
+#pragma unroll
+	for (int j = 0; j < MR; j++) {
+		const int t = threadIdx.x + j * NT;
+		roff[j] = rq[j] = 0;
+		if (t < RN) {
+			int pr, pc;
+			if (t < P2Z) {
+				pr = 0;
+				pc = t;
+			} else if (t < 2 * P2Z) {
+				pr = P2Y - 1;
+				pc = t - P2Z;
+			} else {
+				const int u = t - 2 * P2Z;
+				pr = 1 + u / 2;
+				pc = (u & 1) ? P2Z - 1 : 0;
+			}
+			roff[j] = wrap(y0 - 2 + pr, ny) * nz + wrap(z0 - 2 + pc, nz);
+			rq[j] = pr * P2Z + pc;
+		}
+	}
+	auto load_own = [&](int x, S (*v)[M]) {
+		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
+#pragma unroll
+		for (int q = 0; q < Q; q++)
+#pragma unroll
+			for (int k = 0; k < M; k++) v[q][k] = src[q * N + ooff[k]];
+	};
+	auto load_ring = [&](int x, S (*v)[MR]) {
+		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
+#pragma unroll
+		for (int q = 0; q < Q; q++)
+#pragma unroll
+			for (int j = 0; j < MR; j++) v[q][j] = src[q * N + roff[j]];
+	};
+	// the update of point k from its values at the planes before (fm), at
+	// (fc) and after (fp), its in-plane neighbours at index i of the LDS plane
+	// l (field stride FS, row stride RS); field q's new value
+	auto upd = [&](const S (*fm)[M], const S (*fc)[M], const S (*fp)[M], const S* l, int FS, int RS, int i, int k,
+	               typename F::Sum sum, int q) -> S {
+		const S y = fc[q][k];
+		if (KIND == 3 && q == 0) return y;  // the particles keep their values
+		typename F::L lap = F::lap_zero();
+		F::lap_add(lap, g.w[0], g.wf[0], fp[q][k], y, fm[q][k]);
+		F::lap_add(lap, g.w[1], g.wf[1], l[q * FS + i + RS], y, l[q * FS + i - RS]);
+		F::lap_add(lap, g.w[2], g.wf[2], l[q * FS + i + 1], y, l[q * FS + i - 1]);
+		if constexpr (KIND == 0) return F::ac(y, lap, p);
+		else return F::gg(y, sum, lap, p);
+	};
+	auto sumsq = [&](const S (*v)[M], int k) {
+		typename F::Sum sum = 0;
+		if constexpr (KIND != 0) {
+#pragma unroll
+			for (int q = 0; q < Q; q++) sum += v[q][k] * v[q][k];
+		}
+		return sum;
+	};
+
+	S f0[Q][M], f1[Q][M], f2[Q][M], f3[Q][M];  // phi^n queue: roles rotate over four unrolled iterations
+	S qm[Q][M], qc[Q][M], qp[Q][M];            // phi^{n+1} at P-2, P-1, P
+	S r0[Q][MR], r1[Q][MR];
+#pragma unroll
+	for (int q = 0; q < Q; q++) {
+#pragma unroll
+		for (int k = 0; k < M; k++) f3[q][k] = qm[q][k] = qc[q][k] = qp[q][k] = 0;
+#pragma unroll
+		for (int j = 0; j < MR; j++) r1[q][j] = 0;
+	}
+	// the previous tile of a persistent loop may still read the planes
+	__syncthreads();
+	const int x0 = xb - 1;  // the first iteration forms phi^{n+1}(xb-1)
+	load_own(x0 - 1, f0);
+	load_own(x0, f1);
+	load_own(x0 + 1, f2);
+	load_ring(x0, r0);
+
+	auto iter = [&](int P, const S (*fm)[M], const S (*fc)[M], const S (*fp)[M], S (*fn)[M], const S (*rc)[MR],
+	                S (*rn)[MR]) {
+		load_own(P + 2, fn);
+		load_ring(P + 1, rn);
+		S* __restrict__ lp = lphi[P & 1];
+#pragma unroll
+		for (int q = 0; q < Q; q++) {
+#pragma unroll
+			for (int k = 0; k < M; k++)
+				if (fl[k]) lp[q * PN + q2[k]] = fc[q][k];
+#pragma unroll
+			for (int j = 0; j < MR; j++)
+				if (threadIdx.x + j * NT < RN) lp[q * PN + rq[j]] = rc[q][j];
+		}
+		__syncthreads();
+		// phi^{n+1}(P) at every own point, published for the next iteration
+		S* __restrict__ lw = lq[P & 1];
+#pragma unroll
+		for (int k = 0; k < M; k++) {
+			if (fl[k]) {
+				const typename F::Sum sum = sumsq(fc, k);
+#pragma unroll
+				for (int q = 0; q < Q; q++) {
+					qp[q][k] = upd(fm, fc, fp, lp, PN, P2Z, q2[k], k, sum, q);
+					lw[q * N1 + threadIdx.x + k * NT] = qp[q][k];
+				}
+			}
+		}
+		// phi^{n+2}(P-1) at the own points of the tile: in-plane phi^{n+1}(P-1)
+		if (P - 1 >= xb) {
+			const S* __restrict__ lr = lq[(P - 1) & 1];
+			S* __restrict__ dst = b + (int64_t)(P - 1) * plane;
+#pragma unroll
+			for (int k = 0; k < M; k++) {
+				if (fl[k] & 2) {
+					const typename F::Sum sum = sumsq(qc, k);
+#pragma unroll
+					for (int q = 0; q < Q; q++)
+						dst[q * N + ooff[k]] = upd(qm, qc, qp, lr, N1, R1Z, threadIdx.x + k * NT, k, sum, q);
+				}
+			}
+		}
+#pragma unroll
+		for (int q = 0; q < Q; q++)
+#pragma unroll
+			for (int k = 0; k < M; k++) {
+				qm[q][k] = qc[q][k];
+				qc[q][k] = qp[q][k];
+			}
+	};
+	const int xlast = xe;  // phi^{n+2} at xe-1 is written in the iteration at xe
+	for (int P = x0; P <= xlast; P += 4) {
+		iter(P, f0, f1, f2, f3, r0, r1);
+		if (P + 1 > xlast) break;
+		iter(P + 1, f1, f2, f3, f0, r1, r0);
+		if (P + 2 > xlast) break;
+		iter(P + 2, f2, f3, f0, f1, r0, r1);
+		if (P + 3 > xlast) break;
+		iter(P + 3, f3, f0, f1, f2, r1, r0);
+	}
+}
+
 // ---- 2-D Cahn-Hilliard, wave strips (periodic): one wave per (strip, chunk) --
 // Lane l holds column c0 - 2 + l: mu is formed on lanes 1..62 and c' written
 // on lanes 2..61 (kChWaveOut columns per strip); column neighbours come from

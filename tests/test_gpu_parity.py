@@ -949,6 +949,107 @@ def test_march_3d(prec):
         assert np.array_equal(g.download(), O.gg(a, 0.01, 4, form=prec))
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_march_3d_two_steps(prec):
+    """The 3-D plane march two steps per pass (tile body 12, periodic grids):
+    Allen-Cahn on its three tiles, grain growth and Zener pinning with 2, 4
+    and 8 fields on theirs, chunks of 2, 3, 5 and 16 planes (the four-plane
+    unrolled march ends at each of its positions), partial tiles, grids
+    smaller than one tile and thinner than the halo, odd step counts (the last
+    step on the one-step march) and a graph-replayed run; bitwise against the
+    oracle. Tiles whose planes do not fit the LDS, non-periodic grids and the
+    F32REF form are refused."""
+    pa = [1.0, 1.0, 1.3, 1.0, 0.01]
+    for shape in [(13, 20, 70), (9, 40, 33), (6, 5, 7), (12, 18, 66), (3, 2, 130)]:
+        a = O.uniform(int(np.prod(shape)), 1600 + shape[2], -1.0, 1.0).reshape(shape)
+        if prec != "f64":
+            a = a.astype(np.float32)
+        for steps in (7, 8):
+            say("[gpu] two-step march ac %s %s steps %d" % (shape, prec, steps))
+            exp = O.ac(a, pa, steps, form=prec)
+            for ti, tile in enumerate([(16, 32), (16, 64), (32, 64)]):
+                for chunk in ((2, 3, 5, 16) if ti == 0 else (3, 16)):
+                    with mx.Grid(shape, precision=prec) as g:
+                        g.set_variant(2)
+                        g.set_tile(tile[0], tile[1], chunk)
+                        g.set_tile_options(body=12)
+                        g.upload(a)
+                        g.step("ac", pa, steps)
+                        assert g.get_tile()["opt"] == 12
+                        assert np.array_equal(g.download(), exp), (shape, tile, chunk, steps)
+    for shape, q in [((11, 20, 70), 2), ((9, 17, 33), 4), ((10, 12, 40), 8), ((5, 6, 7), 8), ((4, 33, 65), 4)]:
+        say("[gpu] two-step march gg/zener %s q=%d %s" % (shape, q, prec))
+        a = O.uniform(int(np.prod(shape)) * q, 1700 + q, 0.0, 1.0).reshape(shape + (q,))
+        if prec != "f64":
+            a = a.astype(np.float32)
+        exp = O.gg(a, 0.01, 7, form=prec)
+        expz = O.zener(a, 0.01, 7, form=prec)
+        for tile in [(8, 64), (16, 32), (8, 32)]:
+            fits = not (prec == "f64" and q == 8 and tile != (8, 32))
+            for chunk in (2, 3, 5, 16):
+                for model, want in (("gg", exp), ("zener", expz)):
+                    if model == "zener" and chunk not in (3, 16):
+                        continue
+                    with mx.Grid(shape, fields=q, vector=True, precision=prec) as g:
+                        g.set_variant(2)
+                        g.set_tile(tile[0], tile[1], chunk)
+                        g.set_tile_options(body=12)
+                        g.upload(a)
+                        if not fits:
+                            with pytest.raises(mx.MmspxError):
+                                g.step(model, [0.01], 2)
+                            continue
+                        g.step(model, [0.01], 7)
+                        assert np.array_equal(g.download(), want), (shape, q, tile, chunk, model)
+    # a captured graph of two-step passes (100 steps), from odd parity
+    shape, q = (10, 20, 36), 4
+    a = O.uniform(int(np.prod(shape)) * q, 1750, 0.0, 1.0).reshape(shape + (q,))
+    if prec != "f64":
+        a = a.astype(np.float32)
+    with mx.Grid(shape, fields=q, vector=True, precision=prec) as g:
+        g.set_variant(2)
+        g.set_tile(16, 32, 8)
+        g.set_tile_options(body=12)
+        g.upload(a)
+        g.step("gg", [0.01], 3)
+        g.step("gg", [0.01], 100)
+        assert np.array_equal(g.download(), O.gg(a, 0.01, 103, form=prec))
+    # the defaults: the two-step march where 64-node tile rows fit the last
+    # axis (Allen-Cahn; 2 and 4 fields), the one-step march elsewhere
+    for shape, q, model, body in [((70, 40, 64), 1, "ac", 12), ((40, 24, 128), 4, "gg", 12),
+                                  ((30, 20, 120), 2, "zener", 12), ((40, 24, 96), 1, "ac", 8),
+                                  ((30, 20, 64), 8, "gg", 8)]:
+        say("[gpu] two-step march default %s q=%d %s %s" % (shape, q, model, prec))
+        vec = q > 1
+        a = O.uniform(int(np.prod(shape)) * q, 1760 + q, 0.0 if vec else -1.0, 1.0)
+        a = a.reshape(shape + ((q,) if vec else ()))
+        if prec != "f64":
+            a = a.astype(np.float32)
+        params = [0.01] if vec else pa
+        exp = oracle_run(model, a, params, 5, prec)
+        with mx.Grid(shape, fields=q, vector=vec, precision=prec) as g:
+            g.upload(a)
+            g.step(model, params, 5)
+            t = g.get_tile()
+            assert (t["variant"], t["opt"]) == (2, body), (shape, t)
+            assert np.array_equal(g.download(), exp), (shape, q, model, "default")
+    # non-periodic grids and the F32REF form are refused
+    with mx.Grid((6, 20, 40), precision=prec, bc=[(N, N), (P, P), (P, P)]) as g:
+        g.set_variant(2)
+        g.set_tile(16, 32, 8)
+        g.set_tile_options(body=12)
+        g.upload(np.zeros((6, 20, 40)))
+        with pytest.raises(mx.MmspxError):
+            g.step("ac", pa, 2)
+    with mx.Grid((6, 20, 40), precision="f32ref") as g:
+        g.set_variant(2)
+        g.set_tile(16, 32, 8)
+        g.set_tile_options(body=12)
+        g.upload(np.zeros((6, 20, 40), dtype=np.float32))
+        with pytest.raises(mx.MmspxError):
+            g.step("ac", pa, 2)
+
+
 def test_graph_replay_follows_new_parameters():
     """Runs of 100 steps replay a captured graph whose launches bake the
     model's constants in; a second run with other constants on the same grid
