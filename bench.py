@@ -748,7 +748,7 @@ def main():
     elif len(shape) == 3 and tile["opt"] == 8:
         kname = "march3d"
     elif len(shape) == 3 and tile["opt"] == 12:
-        kname = "march3d2"
+        kname = "march3d_ow" if base == "ow" else "march3d2"
     elif base == "ch":
         kname = "ch_fused"
     elif base == "ow":

@@ -170,7 +170,8 @@ int mmspx_sync(mmspx_state* s);
  *   first_step numbers the first step (the stochastic forms' noise counter).
  * The two-pass Cahn-Hilliard and Ostwald kernels (1-D Cahn-Hilliard, an axis
  * periodic on one side only, 1-D and small 3-D Ostwald ripening) and the 3-D
- * Ostwald marches (w between their two passes) keep one scalar plane (nodes x
+ * Ostwald marches (w between their two passes; not the one-pass march of
+ * periodic grids with 2 fields) keep one scalar plane (nodes x
  * element size) in a fixed device array of MMSPX_NOALLOC_SCRATCH_BYTES that
  * each GPU holds until the process ends; larger planes are refused
  * (MMSPX_EINVAL).
@@ -271,9 +272,10 @@ int mmspx_set_tile(mmspx_state* s, int ty, int tz, int chunk);
  * vector models). 12 is that march two steps per pass over HBM on periodic
  * grids (Allen-Cahn on 16 x 32, 16 x 64 and 32 x 64 tiles; grain growth and
  * Zener pinning with 2, 4 or 8 fields on 8 x 64, 16 x 32 and 8 x 32 tiles,
- * the FP64 form with 8 fields on 8 x 32 only); an odd last step runs body 8;
- * the tile follows the body (16 x 32 where the current tile is not one of
- * those). */
+ * the FP64 form with 8 fields on 8 x 32 only; an odd last step runs body 8)
+ * and, for Ostwald ripening (2, 3 or 4 fields, 8 x 64 and 16 x 32 tiles),
+ * its two passes in one march with w kept in LDS; the tile follows the body
+ * (16 x 32 where the current tile is not one of those). */
 int mmspx_set_tile_options(mmspx_state* s, int body);
 /* Returns variant + 16 * tile body (1: register queues, or the 2-D
  * grain-growth tile; 2: wave strips, or 3-D register queues on 768 threads;

@@ -316,9 +316,13 @@ class Grid:
         check(lib.mmspx_set_tile(self._h, ty, tz, chunk))
 
     def set_tile_options(self, register_queues=False, body=None):
-        """Tile body: Cahn-Hilliard 0 LDS rings, 1 register queues, 2 register
-        queues on 768 threads (3-D 32 x 64 tiles); 2-D Allen-Cahn and grain
-        growth on 1 x 64 strips: 2 wave strips."""
+        """Tile body (mmspx_set_tile_options): Cahn-Hilliard 0 LDS rings, 1
+        register queues, 2 and 3 register queues on 768 and 1024 threads (3-D
+        32 x 64 tiles), 6 and 7 two steps per pass (3-D periodic); 2-D Allen-Cahn
+        and grain growth on 1 x 64 strips: 2 wave strips; 4 and 5 the split
+        strips, 9, 10 and 11 the split strips two steps per pass (2-D
+        periodic); 8 the 3-D plane march; 12 the 3-D plane march two steps per
+        pass (periodic; Ostwald ripening: its two passes in one march)."""
         check(lib.mmspx_set_tile_options(self._h, int(body) if body is not None else (1 if register_queues else 0)))
 
     def get_tile(self):

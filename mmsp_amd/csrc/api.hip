@@ -379,7 +379,7 @@ static int step_impl(mmspx_state* s, int model, const double* params, int np, in
 	s->last = fused ? tc : TuneChoice{1, 0, 0, 0, 0};
 	s->has_last = true;
 	const int spl = fused ? fused_steps_per_launch(model, tc) : 1;
-	if ((((ch_family(model) || model == MMSPX_OSTWALD) && !fused) || (fused && fused_needs_plane(g, model))) &&
+	if ((((ch_family(model) || model == MMSPX_OSTWALD) && !fused) || (fused && fused_needs_plane(g, model, tc))) &&
 	    !s->aux) {
 		if (hipMalloc(&s->aux, s->bytes) != hipSuccess) return fail(MMSPX_ENOMEM, "device allocation failed");
 	}

@@ -142,9 +142,10 @@ bool fused_supported(const Geom& g, int prec, int model);
 // exceed fused_steps_per_launch (1 except for temporal blocks).
 int launch_fused_step(const Geom& g, int prec, const ModelParams& p, const void* a, void* b, hipStream_t st,
                       const TuneChoice& t, int nsteps, int64_t* launches, void* aux = nullptr);
-// device scratch the fused kernels of `model` need beside the two state
-// buffers: one scalar plane for 3-D Ostwald ripening (w), none otherwise
-bool fused_needs_plane(const Geom& g, int model);
+// device scratch the fused kernels of `model` on tile choice t need beside
+// the two state buffers: one scalar plane for 3-D Ostwald ripening on the two
+// marches (w), none otherwise (the one-pass march, body 12, keeps w in LDS)
+bool fused_needs_plane(const Geom& g, int model, const TuneChoice& t);
 int fused_steps_per_launch(int model, const TuneChoice& t);
 TuneChoice default_choice(const Geom& g, int prec, int model);
 int launch_reduce(const Geom& g, int prec, int what, const ModelParams& p, const void* a, double* part,

@@ -105,6 +105,22 @@ __global__ void __launch_bounds__(NT) march3d2_k(const typename Form<PREC>::S* _
 	march3d2<PREC, Q, KIND, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, min(xb + chunk_len, g.n[0]));
 }
 
+// 3-D Ostwald ripening in one pass (periodic): one workgroup per (tile, chunk).
+template <int PREC, int Q, int TY, int TZ, int NT>
+__global__ void __launch_bounds__(NT) march3d_ow_k(const typename Form<PREC>::S* __restrict__ a,
+                                                   typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
+                                                   int chunk_len, int tiles_y, int tiles_z, int ntiles)
+{
+	const int tile = xcd_tile(ntiles);
+	if (tile < 0) return;
+	const int tz = tile % tiles_z;
+	const int t2 = tile / tiles_z;
+	const int ty = t2 % tiles_y;
+	const int chunk = t2 / tiles_y;
+	const int xb = chunk * chunk_len;
+	march3d_ow<PREC, Q, TY, TZ, NT>(a, b, g, p, ty * TY, tz * TZ, xb, min(xb + chunk_len, g.n[0]));
+}
+
 template <int PREC, int TZ>
 __global__ void __launch_bounds__(TZ) ac_tiled(const typename Form<PREC>::S* __restrict__ a,
                                                typename Form<PREC>::S* __restrict__ b, Geom g, ModelParams p,
@@ -431,9 +447,23 @@ TuneChoice default_choice(const Geom& g, int prec, int model)
 	// passes (FP32 121.5 against 172.6), 128^3 x 4 FP64 with 4-plane chunks
 	// 40.4 against 57.4; below 2^20 nodes the generic passes win (the
 	// example's 128 x 64 x 16 grid: 8.4 against 12.9)
+	// Periodic with 2 fields and 64-node tile rows that pad the last axis by at
+	// most an eighth: the one-pass march (body 12, w kept in LDS), 8 x 64
+	// tiles with 64-plane chunks halved until a launch has 512 workgroups
+	// (two per CU): 256^3 FP64 156.8 us per step against 213.7 for the two
+	// marches, FP32 75.1 against 121.6; 128^3 FP64 24.0 against 28.7. With 3
+	// or 4 fields its registers leave one workgroup per CU in FP64 (192^3 x 3
+	// FP64 134.0 against 125.6; 128^3 x 4 FP64 40.0 against 40.3), so those
+	// stay on the two marches.
 	if (g.dim == 3 && model == MMSPX_OSTWALD) {
 		if (nodes < ((int64_t)1 << 20)) return TuneChoice{1, 0, 0, 0, 0};
 		const int64_t tiles = (int64_t)((g.n[1] + 7) / 8) * ((g.n[2] + 63) / 64);
+		const int pad = (g.n[2] + 63) / 64 * 64 - g.n[2];
+		if (g.fields == 2 && all_periodic(g) && (prec == MMSPX_F64 || prec == MMSPX_F32) && pad * 8 <= g.n[2]) {
+			int c = 64;
+			while (c > 4 && tiles * ((g.n[0] + c - 1) / c) < 512) c /= 2;
+			return TuneChoice{2, 8, 64, c, 12};
+		}
 		int chunk = 32;
 		while (chunk > 4 && tiles * ((g.n[0] + chunk - 1) / chunk) < 1024) chunk /= 2;
 		return TuneChoice{2, 8, 64, chunk, 8};
@@ -599,8 +629,9 @@ int tile_candidates(const Geom& g, int prec, int model, TuneChoice* out, int max
 		}
 		// two steps per pass on periodic grids (Allen-Cahn, grain growth,
 		// Zener pinning; the 8-field FP64 planes fit the LDS on 8 x 32 only)
+		// (Ostwald ripening: the one-pass march)
 		const bool two = all_periodic(g) && model != MMSPX_AC_NOISE && model != MMSPX_GG_ANISO &&
-		                 model != MMSPX_OSTWALD && (prec == MMSPX_F64 || prec == MMSPX_F32);
+		                 (prec == MMSPX_F64 || prec == MMSPX_F32);
 		if (two)
 			for (int c : {8, 16, 32, 64}) {
 				if (scalar) {
@@ -769,6 +800,46 @@ static int launch_ow3d(const Geom& g, const ModelParams& p, const void* a, void*
 	if (rc) return rc;
 	return t864 ? launch_march_body<PREC, 1, 6, 8, 64, 256>(g, p, aux, b, st, chunk, a, nullptr)
 	            : launch_march_body<PREC, 1, 6, 16, 32, 256>(g, p, aux, b, st, chunk, a, nullptr);
+}
+
+// 3-D Ostwald ripening in one pass (tile body 12, periodic): 2, 3 or 4
+// fields on 8 x 64 tiles (704 threads) or 16 x 32 (640), one point of the
+// tile and its halo per thread
+template <int PREC, int Q, int TY, int TZ, int NT>
+static int launch_ow1_body(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, int chunk)
+{
+	using S = typename Form<PREC>::S;
+	const int tiles_y = (g.n[1] + TY - 1) / TY;
+	const int tiles_z = (g.n[2] + TZ - 1) / TZ;
+	const int chunks = (g.n[0] + chunk - 1) / chunk;
+	const int ntiles = tiles_y * tiles_z * chunks;
+	const int blocks = (ntiles + 7) / 8 * 8;
+	hipLaunchKernelGGL((march3d_ow_k<PREC, Q, TY, TZ, NT>), dim3(blocks), dim3(NT), 0, st, (const S*)a, (S*)b, g, p,
+	                   chunk, tiles_y, tiles_z, ntiles);
+	MMSPX_HIP(hipGetLastError());
+	return 0;
+}
+
+template <int PREC, int Q>
+static int launch_ow1_q(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st,
+                        const TuneChoice& t, int chunk)
+{
+	if (t.ty == 8 && t.tz == 64) return launch_ow1_body<PREC, Q, 8, 64, 704>(g, p, a, b, st, chunk);
+	if (t.ty == 16 && t.tz == 32) return launch_ow1_body<PREC, Q, 16, 32, 640>(g, p, a, b, st, chunk);
+	return fail(MMSPX_EINVAL, "no one-pass 3-D Ostwald march compiled for this tile");
+}
+
+template <int PREC>
+static int launch_ow1(const Geom& g, const ModelParams& p, const void* a, void* b, hipStream_t st, const TuneChoice& t)
+{
+	const int chunk = t.chunk > 0 ? t.chunk : 32;
+	if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "the one-pass 3-D Ostwald march needs periodic boundaries");
+	switch (g.fields) {
+	case 2: return launch_ow1_q<PREC, 2>(g, p, a, b, st, t, chunk);
+	case 3: return launch_ow1_q<PREC, 3>(g, p, a, b, st, t, chunk);
+	case 4: return launch_ow1_q<PREC, 4>(g, p, a, b, st, t, chunk);
+	}
+	return fail(MMSPX_EUNSUPPORTED, "the 3-D Ostwald march takes 2, 3 or 4 fields");
 }
 
 // tile body 8 on 3-D grids: Allen-Cahn on 8, 16 or 32 x 64 and 16 x 32 tiles;
@@ -1217,7 +1288,7 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 	case MMSPX_OSTWALD:
 		if (g.dim == 3) {
 			if constexpr (PREC == MMSPX_F32REF) return fail(MMSPX_EUNSUPPORTED, "no 3-D march for this form");
-			else return launch_ow3d<PREC>(g, p, a, b, aux, st, t);
+			else return t.opt == 12 ? launch_ow1<PREC>(g, p, a, b, st, t) : launch_ow3d<PREC>(g, p, a, b, aux, st, t);
 		}
 		if (g.dim == 2 && t.ty == 1 && t.tz == kChWaveOut) {
 			if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "Ostwald wave strips need periodic boundaries");
@@ -1276,7 +1347,10 @@ static int launch_prec(const Geom& g, const ModelParams& p, const void* a, void*
 	return fail(MMSPX_EINVAL, "unknown model");
 }
 
-bool fused_needs_plane(const Geom& g, int model) { return model == MMSPX_OSTWALD && g.dim == 3; }
+bool fused_needs_plane(const Geom& g, int model, const TuneChoice& t)
+{
+	return model == MMSPX_OSTWALD && g.dim == 3 && t.opt != 12;
+}
 
 int fused_steps_per_launch(int model, const TuneChoice& t)
 {

@@ -81,7 +81,7 @@ extern "C" int mmspx_step_noalloc(const mmspx_desc* desc, void* buf0, void* buf1
 	}
 	const size_t elem = desc->precision == MMSPX_F64 ? 8 : 4;
 	const bool needs_plane =
-	    (!fused && (ch_family(model) || model == MMSPX_OSTWALD)) || (fused && fused_needs_plane(g, model));
+	    (!fused && (ch_family(model) || model == MMSPX_OSTWALD)) || (fused && fused_needs_plane(g, model, tc));
 	if (needs_plane && (size_t)g.nodes * elem > (size_t)MMSPX_NOALLOC_SCRATCH_BYTES)
 		return fail(MMSPX_EINVAL, "this grid's two-pass kernels need a " + std::to_string((size_t)g.nodes * elem) +
 		                              "-byte scalar plane; mmspx_step_noalloc holds at most " +

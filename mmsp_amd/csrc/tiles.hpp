@@ -1019,6 +1019,193 @@ __device__ __forceinline__ void march3d2(const typename Form<PREC>::S* __restric
 	}
 }
 
+// ---- 3-D Ostwald ripening in one pass (periodic) ----------------------------
+// The two passes of the generic kernels (ostwald.cpp:87-122: w and the order
+// parameters, then C + dt*D*lap(w)) fused into one plane march. Each thread
+// owns fixed points of the tile and its one-cell in-plane halo (region R1)
+// with the Q fields at planes P-1, P, P+1 (P+2 in flight) in registers, and
+// w at planes P-2, P-1, P. The iteration at plane P publishes the fields of
+// plane P to LDS with a two-cell halo, forms w(P) on R1 and publishes it to a
+// second LDS plane, writes the order parameters of plane P on the tile, then C
+// of plane P-1 on the tile from w (planes P-2 and P from registers, in-plane
+// from the plane the previous iteration published). Both planes are
+// double-buffered: one barrier per iteration. w never goes to HBM; every value
+// is the two passes' for that node (the halo recomputes the w the
+// neighbouring tile computes), so the result is bitwise theirs.
+template <int PREC, int Q, int TY, int TZ, int NT>
+__device__ __forceinline__ void march3d_ow(const typename Form<PREC>::S* __restrict__ a,
+                                           typename Form<PREC>::S* __restrict__ b, const Geom& g, const ModelParams& p,
+                                           int y0, int z0, int xb, int xe)
+{
+	using F = Form<PREC>;
+	using S = typename F::S;
+	constexpr int P2Y = TY + 4, P2Z = TZ + 4;  // the fields' plane tile (two-cell halo)
+	constexpr int R1Y = TY + 2, R1Z = TZ + 2;  // w: the threads' own points
+	constexpr int PN = P2Y * P2Z, N1 = R1Y * R1Z, RN = PN - N1;
+	constexpr int M = (N1 + NT - 1) / NT;
+	constexpr int MR = (RN + NT - 1) / NT;
+	__shared__ S lphi[2][Q * PN];
+	__shared__ S lw[2][N1];
+
+	const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+	const int64_t plane = (int64_t)ny * nz;
+	const int64_t N = (int64_t)nx * plane;
+
+	// own point k: R1 index e = tid + k*NT at (r, c), node (y0-1+r, z0-1+c);
+	// q2 its index in the fields' tile; fl 1 in R1, 2 in the tile (written)
+	int ooff[M], q2[M], fl[M];
+#pragma unroll
+	for (int k = 0; k < M; k++) {
+		const int e = threadIdx.x + k * NT;
+		ooff[k] = q2[k] = fl[k] = 0;
+		if (e < N1) {
+			const int r = e / R1Z, c = e - (e / R1Z) * R1Z;
+			const int y = y0 - 1 + r, z = z0 - 1 + c;
+			ooff[k] = wrap(y, ny) * nz + wrap(z, nz);
+			q2[k] = (r + 1) * P2Z + (c + 1);
+			fl[k] = 1 | ((r >= 1 && r < TY + 1 && c >= 1 && c < TZ + 1 && y < ny && z < nz) ? 2 : 0);
+		}
+	}
+	// ring points: rows 0 and P2Y-1 and columns 0 and P2Z-1 of the fields' tile
+	int roff[MR], rq[MR];
+#pragma unroll
+	for (int j = 0; j < MR; j++) {
+		const int t = threadIdx.x + j * NT;
+		roff[j] = rq[j] = 0;
+		if (t < RN) {
+			int pr, pc;
+			if (t < P2Z) {
+				pr = 0;
+				pc = t;
+			} else if (t < 2 * P2Z) {
+				pr = P2Y - 1;
+				pc = t - P2Z;
+			} else {
+				const int u = t - 2 * P2Z;
+				pr = 1 + u / 2;
+				pc = (u & 1) ? P2Z - 1 : 0;
+			}
+			roff[j] = wrap(y0 - 2 + pr, ny) * nz + wrap(z0 - 2 + pc, nz);
+			rq[j] = pr * P2Z + pc;
+		}
+	}
+	auto load_own = [&](int x, S (*v)[M]) {
+		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
+#pragma unroll
+		for (int q = 0; q < Q; q++)
+#pragma unroll
+			for (int k = 0; k < M; k++) v[q][k] = src[q * N + ooff[k]];
+	};
+	auto load_ring = [&](int x, S (*v)[MR]) {
+		const S* __restrict__ src = a + (int64_t)wrap(x, nx) * plane;
+#pragma unroll
+		for (int q = 0; q < Q; q++)
+#pragma unroll
+			for (int j = 0; j < MR; j++) v[q][j] = src[q * N + roff[j]];
+	};
+
+	S f0[Q][M], f1[Q][M], f2[Q][M], f3[Q][M];  // the fields' queue: roles rotate over four unrolled iterations
+	S wm[M], wc[M], wp[M];                     // w at P-2, P-1, P
+	S r0[Q][MR], r1[Q][MR];
+#pragma unroll
+	for (int k = 0; k < M; k++) {
+		wm[k] = wc[k] = wp[k] = 0;
+#pragma unroll
+		for (int q = 0; q < Q; q++) f3[q][k] = 0;
+	}
+#pragma unroll
+	for (int q = 0; q < Q; q++)
+#pragma unroll
+		for (int j = 0; j < MR; j++) r1[q][j] = 0;
+	// the previous tile of a persistent loop may still read the planes
+	__syncthreads();
+	const int x0 = xb - 1;  // the first iteration forms w(xb-1)
+	load_own(x0 - 1, f0);
+	load_own(x0, f1);
+	load_own(x0 + 1, f2);
+	load_ring(x0, r0);
+
+	auto iter = [&](int P, const S (*fm)[M], const S (*fc)[M], const S (*fp)[M], S (*fn)[M], const S (*rc)[MR],
+	                S (*rn)[MR]) {
+		load_own(P + 2, fn);
+		load_ring(P + 1, rn);
+		S* __restrict__ lp = lphi[P & 1];
+#pragma unroll
+		for (int q = 0; q < Q; q++) {
+#pragma unroll
+			for (int k = 0; k < M; k++)
+				if (fl[k]) lp[q * PN + q2[k]] = fc[q][k];
+#pragma unroll
+			for (int j = 0; j < MR; j++)
+				if (threadIdx.x + j * NT < RN) lp[q * PN + rq[j]] = rc[q][j];
+		}
+		__syncthreads();
+		// w(P) at every own point (published), the order parameters of plane P
+		// on the tile
+		S* __restrict__ lwp = lw[P & 1];
+		const bool inside = P >= xb && P < xe;
+		S* __restrict__ dst = b + (int64_t)P * plane;
+#pragma unroll
+		for (int k = 0; k < M; k++) {
+			if (!fl[k]) continue;
+			const int i = q2[k];
+			typename F::OSum osum = 0;
+#pragma unroll
+			for (int q = 1; q < Q; q++) osum += fc[q][k] * fc[q][k];
+			const S C = fc[0][k];
+			{
+				typename F::L lap = F::lap_zero();
+				F::lap_add(lap, g.w[0], g.wf[0], fp[0][k], C, fm[0][k]);
+				F::lap_add(lap, g.w[1], g.wf[1], lp[i + P2Z], C, lp[i - P2Z]);
+				F::lap_add(lap, g.w[2], g.wf[2], lp[i + 1], C, lp[i - 1]);
+				wp[k] = F::ow_w(C, osum, lap, p);
+				lwp[threadIdx.x + k * NT] = wp[k];
+			}
+			if (inside && (fl[k] & 2)) {
+#pragma unroll
+				for (int q = 1; q < Q; q++) {
+					const S y = fc[q][k];
+					typename F::L lap = F::lap_zero();
+					F::lap_add(lap, g.w[0], g.wf[0], fp[q][k], y, fm[q][k]);
+					F::lap_add(lap, g.w[1], g.wf[1], lp[q * PN + i + P2Z], y, lp[q * PN + i - P2Z]);
+					F::lap_add(lap, g.w[2], g.wf[2], lp[q * PN + i + 1], y, lp[q * PN + i - 1]);
+					dst[q * N + ooff[k]] = F::ow_phi(y, C, osum, lap, p);
+				}
+			}
+		}
+		// C of plane P-1 on the tile: C + dt*D*lap(w)
+		if (P - 1 >= xb) {
+			const S* __restrict__ lr = lw[(P - 1) & 1];
+			S* __restrict__ dc = b + (int64_t)(P - 1) * plane;
+#pragma unroll
+			for (int k = 0; k < M; k++) {
+				if (!(fl[k] & 2)) continue;
+				const int e = threadIdx.x + k * NT;
+				const S y = wc[k];
+				typename F::L lap = F::lap_zero();
+				F::lap_add(lap, g.w[0], g.wf[0], wp[k], y, wm[k]);
+				F::lap_add(lap, g.w[1], g.wf[1], lr[e + R1Z], y, lr[e - R1Z]);
+				F::lap_add(lap, g.w[2], g.wf[2], lr[e + 1], y, lr[e - 1]);
+				dc[ooff[k]] = F::ow_c(fm[0][k], lap, p);
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < M; k++) {
+			wm[k] = wc[k];
+			wc[k] = wp[k];
+		}
+	};
+	for (int P = x0; P <= xe; P += 4) {
+		iter(P, f0, f1, f2, f3, r0, r1);
+		if (P + 1 > xe) break;
+		iter(P + 1, f1, f2, f3, f0, r1, r0);
+		if (P + 2 > xe) break;
+		iter(P + 2, f2, f3, f0, f1, r0, r1);
+		if (P + 3 > xe) break;
+		iter(P + 3, f3, f0, f1, f2, r1, r0);
+	}
+}
+
 // ---- 2-D Cahn-Hilliard, wave strips (periodic): one wave per (strip, chunk) --
 // Lane l holds column c0 - 2 + l: mu is formed on lanes 1..62 and c' written
 // on lanes 2..61 (kChWaveOut columns per strip); column neighbours come from

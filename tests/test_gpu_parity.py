@@ -956,9 +956,10 @@ def test_march_3d_two_steps(prec):
     and 8 fields on theirs, chunks of 2, 3, 5 and 16 planes (the four-plane
     unrolled march ends at each of its positions), partial tiles, grids
     smaller than one tile and thinner than the halo, odd step counts (the last
-    step on the one-step march) and a graph-replayed run; bitwise against the
-    oracle. Tiles whose planes do not fit the LDS, non-periodic grids and the
-    F32REF form are refused."""
+    step on the one-step march) and a graph-replayed run; Ostwald ripening
+    with 2, 3 and 4 fields on the one-pass march (w kept in LDS); the
+    defaults; bitwise against the oracle. Tiles whose planes do not fit the
+    LDS, non-periodic grids and the F32REF form are refused."""
     pa = [1.0, 1.0, 1.3, 1.0, 0.01]
     for shape in [(13, 20, 70), (9, 40, 33), (6, 5, 7), (12, 18, 66), (3, 2, 130)]:
         a = O.uniform(int(np.prod(shape)), 1600 + shape[2], -1.0, 1.0).reshape(shape)
@@ -1014,18 +1015,37 @@ def test_march_3d_two_steps(prec):
         g.step("gg", [0.01], 3)
         g.step("gg", [0.01], 100)
         assert np.array_equal(g.download(), O.gg(a, 0.01, 103, form=prec))
+    # Ostwald ripening in one pass (body 12: w kept in LDS; 2, 3 and 4 fields)
+    po = [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]
+    for shape, q in [((11, 20, 70), 2), ((9, 17, 33), 3), ((10, 12, 40), 4), ((5, 6, 7), 2), ((4, 33, 65), 3)]:
+        say("[gpu] one-pass march ostwald %s q=%d %s" % (shape, q, prec))
+        a = O.uniform(int(np.prod(shape)) * q, 1780 + q, 0.0, 1.0).reshape(shape + (q,))
+        if prec != "f64":
+            a = a.astype(np.float32)
+        exp = O.ostwald(a, po, 7, form=prec)
+        for tile in [(8, 64), (16, 32)]:
+            for chunk in (2, 3, 5, 16):
+                with mx.Grid(shape, fields=q, vector=True, precision=prec) as g:
+                    g.set_variant(2)
+                    g.set_tile(tile[0], tile[1], chunk)
+                    g.set_tile_options(body=12)
+                    g.upload(a)
+                    g.step("ostwald", po, 7)
+                    assert g.get_tile()["opt"] == 12
+                    assert np.array_equal(g.download(), exp), (shape, q, tile, chunk, "ostwald")
     # the defaults: the two-step march where 64-node tile rows fit the last
     # axis (Allen-Cahn; 2 and 4 fields), the one-step march elsewhere
+    # (Ostwald ripening with 2 fields from 2^20 nodes: the one-pass march)
     for shape, q, model, body in [((70, 40, 64), 1, "ac", 12), ((40, 24, 128), 4, "gg", 12),
                                   ((30, 20, 120), 2, "zener", 12), ((40, 24, 96), 1, "ac", 8),
-                                  ((30, 20, 64), 8, "gg", 8)]:
+                                  ((30, 20, 64), 8, "gg", 8), ((64, 128, 128), 2, "ostwald", 12)]:
         say("[gpu] two-step march default %s q=%d %s %s" % (shape, q, model, prec))
         vec = q > 1
         a = O.uniform(int(np.prod(shape)) * q, 1760 + q, 0.0 if vec else -1.0, 1.0)
         a = a.reshape(shape + ((q,) if vec else ()))
         if prec != "f64":
             a = a.astype(np.float32)
-        params = [0.01] if vec else pa
+        params = po if model == "ostwald" else [0.01] if vec else pa
         exp = oracle_run(model, a, params, 5, prec)
         with mx.Grid(shape, fields=q, vector=vec, precision=prec) as g:
             g.upload(a)
@@ -1048,6 +1068,13 @@ def test_march_3d_two_steps(prec):
         g.upload(np.zeros((6, 20, 40), dtype=np.float32))
         with pytest.raises(mx.MmspxError):
             g.step("ac", pa, 2)
+    with mx.Grid((6, 20, 40), fields=2, vector=True, precision=prec, bc=[(P, P), (N, N), (P, P)]) as g:
+        g.set_variant(2)
+        g.set_tile(8, 64, 8)
+        g.set_tile_options(body=12)
+        g.upload(np.zeros((6, 20, 40, 2)))
+        with pytest.raises(mx.MmspxError):
+            g.step("ostwald", po, 1)
 
 
 def test_graph_replay_follows_new_parameters():

@@ -115,6 +115,9 @@ CASES = [
     # from 2^20 nodes 3-D Ostwald ripening runs two marches with w in the fixed array
     ("ostwald_3d_march", "ostwald", [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0], "f64", [(N, P), (P, P), (M, D)],
      2, ((104, 100, 101), (8, 9, 10)), 5, (0.0, 1.0)),
+    # periodic with 2 fields: the one-pass march, w in LDS (no scalar plane)
+    ("ostwald_3d_one_pass", "ostwald", [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0], "f32", None, 2,
+     ((64, 128, 128), (8, 9, 10)), 5, (0.0, 1.0)),
     ("ch_periodic_3d", "ch_dw", [1.0, 1.0, 0.005], "f64", None, 0, ((40, 36, 70), (12, 10, 20)), 11, (0.45, 0.55)),
     ("ac_tblocks_2d", "ac", [1.0, 1.0, 1.0, 1.0, 0.01], "f64", None, 0, ((256, 256), (64, 48)), 13, (-0.01, 0.01)),
     ("gg_strips_2d_f32", "gg", [0.05], "f32", None, 8, ((70, 200), (20, 40)), 7, (0.0, 0.01)),
