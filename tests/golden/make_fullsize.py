@@ -22,11 +22,19 @@ Workloads (the seeded inputs bench.py and the tests draw; SplitMix64):
         and FP64 (from the float-rounded start) per member
   ac2d  2-D Allen-Cahn 512^2, phi0 = U(-0.01, 0.01) seed 20260101, 1000
         steps: FP64 with the correctly rounded cube and with libm pow
+  ac3d  3-D Allen-Cahn 256^3 periodic, phi0 = U(-0.01, 0.01) seed 20260112,
+        1000 steps: FP64 with the correctly rounded cube and with libm pow
+        (allen-cahn.cpp:64-73)
+  gg3d  3-D grain growth 256^3 x 8 fields periodic, phi0 = U(0, 0.01) seed
+        20260113 rounded to float, dt = 0.05, 200 steps: FP32 and FP64 (from
+        the float-rounded start, correctly rounded cube)
+  ostwald3d  3-D Ostwald ripening 256^3 x 2 fields periodic, U(0, 1) seed
+        20260114, the example's constants, 200 steps: FP64 (ostwald.cpp:87-122)
 
 Digest convention: SHA-256 of the state's bytes, node-major (the reference's
 node order; vector grids: each node's fields contiguous), little-endian.
 
-    python tests/golden/make_fullsize.py [--only ch3d,gg2d,batch_ac,ac2d]
+    python tests/golden/make_fullsize.py [--only ch3d,gg2d,batch_ac,ac2d,ac3d,gg3d,ostwald3d]
 
 Takes about an hour on 8 cores. Errors are normwise per field:
 max_i |x_i - ref_i| / max_i |ref_i|.
@@ -132,6 +140,40 @@ def ac2d():
     return dict(shape=list(shape), seed=20260101, lo=-0.01, hi=0.01, params=p, steps=steps,
                 f64=dict(sha256=digest(cr), err_vs_reference_pow=normwise(cr, pw, False)[0]),
                 f64_pow=dict(sha256=digest(pw)))
+
+
+def ac3d():
+    shape, steps, p = (256, 256, 256), 1000, [1.0, 1.0, 1.0, 1.0, 0.01]
+    a0 = O.uniform(int(np.prod(shape)), 20260112, -0.01, 0.01).reshape(shape)
+    log("ac3d f64 (correctly rounded cube)")
+    cr = O.ac(a0, p, steps, cube=O.CUBE_CR)
+    log("ac3d f64 (libm pow)")
+    pw = O.ac(a0, p, steps, cube=O.CUBE_POW)
+    return dict(shape=list(shape), seed=20260112, lo=-0.01, hi=0.01, params=p, steps=steps,
+                f64=dict(sha256=digest(cr), planes=plane_digests(cr), err_vs_reference_pow=normwise(cr, pw, False)[0]),
+                f64_pow=dict(sha256=digest(pw)))
+
+
+def gg3d():
+    shape, q, steps, dt = (256, 256, 256), 8, 200, 0.05
+    a0 = O.uniform(int(np.prod(shape)) * q, 20260113, 0.0, 0.01).astype(np.float32).reshape(shape + (q,))
+    log("gg3d f32")
+    f32 = O.gg(a0, dt, steps, form="f32")
+    log("gg3d f64 from the float start (correctly rounded cube)")
+    cr = O.gg(a0.astype(np.float64), dt, steps, cube=O.CUBE_CR)
+    return dict(shape=list(shape), fields=q, seed=20260113, lo=0.0, hi=0.01, dt=dt, steps=steps,
+                f64=dict(sha256=digest(cr), planes=plane_digests(cr)),
+                f32=dict(sha256=digest(f32), planes=plane_digests(f32), err_vs_f64=normwise(f32, cr, True)))
+
+
+def ostwald3d():
+    shape, q, steps = (256, 256, 256), 2, 200
+    p = [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]
+    a0 = O.uniform(int(np.prod(shape)) * q, 20260114, 0.0, 1.0).reshape(shape + (q,))
+    log("ostwald3d f64")
+    f64 = O.ostwald(a0, p, steps)
+    return dict(shape=list(shape), fields=q, seed=20260114, lo=0.0, hi=1.0, params=p, steps=steps,
+                f64=dict(sha256=digest(f64), planes=plane_digests(f64), maxabs=float(np.abs(f64).max())))
 
 
 def main():

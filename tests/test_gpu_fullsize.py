@@ -13,6 +13,10 @@ Bars (written next to each comparison):
     normwise error per field against the FP64 state from the same
     float-rounded start asserted against the bound stated beside it.
   * Cahn-Hilliard mass: relative change <= 1e-12 (FP64).
+Workloads: C1 512^2 Allen-Cahn, C2 256^3 Cahn-Hilliard (FP64 and FP32), C3
+2048^2 x 32 grain growth, C4 the 64 batched Allen-Cahn members, and the 3-D
+bench workloads on their default kernels: 256^3 Allen-Cahn, 256^3 x 8 grain
+growth, 256^3 x 2 Ostwald ripening.
 Normwise error: max_i |x_i - ref_i| / max_i |ref_i|, per field.
 """
 import hashlib
@@ -152,6 +156,68 @@ def test_c3_grain_growth_2048_q32_fp32_and_fp64():
     err = normwise(f32, f64, True)
     say("[gpu] gg2d fp32 err vs fp64 max over fields %.3e" % max(err))
     assert max(err) <= 1e-5  # the FP32 bar per field
+
+
+def test_ac3d_allen_cahn_256_fp64_digest():
+    """256^3 FP64 Allen-Cahn, 1000 steps on the shipped default (the plane
+    march two steps per pass): the oracle's digest; the correctly rounded cube
+    stays within 1e-12 of libm pow's (the reference's arithmetic)."""
+    w = need("ac3d")
+    say("[gpu] full size ac3d fp64 1000 steps")
+    shape = tuple(w["shape"])
+    a = O.uniform(int(np.prod(shape)), w["seed"], w["lo"], w["hi"]).reshape(shape)
+    with mx.Grid(shape) as g:
+        g.upload(a)
+        g.step("ac", w["params"], w["steps"])
+        tile = g.get_tile()
+        got = g.download()
+    assert (tile["variant"], tile["opt"]) == (2, 12), tile  # the kernel the bench times
+    assert digest(got) == w["f64"]["sha256"], ("planes differing", first_planes_differing(got, w["f64"]["planes"]))
+    assert w["f64"]["err_vs_reference_pow"] <= 1e-12  # GPU state vs the reference's arithmetic
+
+
+def test_gg3d_grain_growth_256_q8_fp32_and_fp64():
+    """256^3 x 8 fields, 200 steps, the shipped defaults (FP32: the plane
+    march two steps per pass; FP64: the one-step march): FP32 bitwise the
+    oracle's FP32 restatement, FP64 from the same float-rounded start bitwise
+    the oracle's (correctly rounded cube); FP32 within 1e-5 of FP64 per field."""
+    w = need("gg3d")
+    shape, q = tuple(w["shape"]), w["fields"]
+    a = O.uniform(int(np.prod(shape)) * q, w["seed"], w["lo"], w["hi"]).astype(np.float32).reshape(shape + (q,))
+    say("[gpu] full size gg3d fp32 200 steps")
+    with mx.Grid(shape, fields=q, vector=True, precision="f32") as g:
+        g.upload(a)
+        g.step("gg", [w["dt"]], w["steps"])
+        tile = g.get_tile()
+        f32 = g.download()
+    assert (tile["variant"], tile["opt"]) == (2, 12), tile
+    assert digest(f32) == w["f32"]["sha256"], ("planes differing", first_planes_differing(f32, w["f32"]["planes"]))
+    say("[gpu] full size gg3d fp64 200 steps")
+    with mx.Grid(shape, fields=q, vector=True, precision="f64") as g:
+        g.upload(a.astype(np.float64))
+        g.step("gg", [w["dt"]], w["steps"])
+        f64 = g.download()
+    assert digest(f64) == w["f64"]["sha256"], ("planes differing", first_planes_differing(f64, w["f64"]["planes"]))
+    err = normwise(f32, f64, True)
+    say("[gpu] gg3d fp32 err vs fp64 max over fields %.3e" % max(err))
+    assert max(err) <= 1e-5  # the FP32 bar per field
+
+
+def test_ostwald3d_256_q2_fp64_digest():
+    """256^3 x 2 fields FP64 Ostwald ripening, 200 steps on the shipped
+    default (the two passes in one plane march, w kept in LDS): bitwise the
+    oracle's state."""
+    w = need("ostwald3d")
+    shape, q = tuple(w["shape"]), w["fields"]
+    say("[gpu] full size ostwald3d fp64 200 steps")
+    a = O.uniform(int(np.prod(shape)) * q, w["seed"], w["lo"], w["hi"]).reshape(shape + (q,))
+    with mx.Grid(shape, fields=q, vector=True) as g:
+        g.upload(a)
+        g.step("ostwald", w["params"], w["steps"])
+        tile = g.get_tile()
+        got = g.download()
+    assert (tile["variant"], tile["opt"]) == (2, 12), tile
+    assert digest(got) == w["f64"]["sha256"], ("planes differing", first_planes_differing(got, w["f64"]["planes"]))
 
 
 def test_c4_batched_allen_cahn_64_members():
