@@ -961,7 +961,7 @@ def test_march_3d_two_steps(prec):
     defaults; bitwise against the oracle. Tiles whose planes do not fit the
     LDS, non-periodic grids and the F32REF form are refused."""
     pa = [1.0, 1.0, 1.3, 1.0, 0.01]
-    for shape in [(13, 20, 70), (9, 40, 33), (6, 5, 7), (12, 18, 66), (3, 2, 130)]:
+    for shape in [(13, 20, 70), (9, 40, 33), (6, 5, 7), (12, 18, 66), (3, 2, 130), (1, 9, 33), (2, 20, 70)]:
         a = O.uniform(int(np.prod(shape)), 1600 + shape[2], -1.0, 1.0).reshape(shape)
         if prec != "f64":
             a = a.astype(np.float32)
@@ -978,7 +978,8 @@ def test_march_3d_two_steps(prec):
                         g.step("ac", pa, steps)
                         assert g.get_tile()["opt"] == 12
                         assert np.array_equal(g.download(), exp), (shape, tile, chunk, steps)
-    for shape, q in [((11, 20, 70), 2), ((9, 17, 33), 4), ((10, 12, 40), 8), ((5, 6, 7), 8), ((4, 33, 65), 4)]:
+    for shape, q in [((11, 20, 70), 2), ((9, 17, 33), 4), ((10, 12, 40), 8), ((5, 6, 7), 8), ((4, 33, 65), 4),
+                     ((1, 20, 70), 2), ((2, 9, 33), 8)]:
         say("[gpu] two-step march gg/zener %s q=%d %s" % (shape, q, prec))
         a = O.uniform(int(np.prod(shape)) * q, 1700 + q, 0.0, 1.0).reshape(shape + (q,))
         if prec != "f64":
@@ -1017,7 +1018,8 @@ def test_march_3d_two_steps(prec):
         assert np.array_equal(g.download(), O.gg(a, 0.01, 103, form=prec))
     # Ostwald ripening in one pass (body 12: w kept in LDS; 2, 3 and 4 fields)
     po = [0.001, 1.0, 1.0, 0.05, 0.95, 2.0, 1.0, 3.0, 2.0]
-    for shape, q in [((11, 20, 70), 2), ((9, 17, 33), 3), ((10, 12, 40), 4), ((5, 6, 7), 2), ((4, 33, 65), 3)]:
+    for shape, q in [((11, 20, 70), 2), ((9, 17, 33), 3), ((10, 12, 40), 4), ((5, 6, 7), 2), ((4, 33, 65), 3),
+                     ((1, 12, 40), 2), ((2, 9, 70), 4)]:
         say("[gpu] one-pass march ostwald %s q=%d %s" % (shape, q, prec))
         a = O.uniform(int(np.prod(shape)) * q, 1780 + q, 0.0, 1.0).reshape(shape + (q,))
         if prec != "f64":
