@@ -864,18 +864,29 @@ __device__ __forceinline__ void march3d2(const typename Form<PREC>::S* __restric
 	const int64_t plane = (int64_t)ny * nz;
 	const int64_t N = (int64_t)nx * plane;
 
-	// own point k: R1 index e = tid + k*NT at (r, c), node (y0-1+r, z0-1+c);
-	// q2 its index in the phi^n tile; fl 1 in R1, 2 in the tile (written)
-	int ooff[M], q2[M], fl[M];
+	// own point k: e = tid + k*NT names R1 point (r, c), node (y0-1+r,
+	// z0-1+c): e < R1Y*TZ column e % TZ of row e / TZ, the next 2*R1Y the last
+	// two columns (TZ, TZ+1) of each row, so a wave's points lie in whole row
+	// segments (contiguous in LDS and in HBM); q2 the point's index in the
+	// phi^n tile, i1 in the R1 plane; fl 1 in R1, 2 in the tile (written)
+	int ooff[M], q2[M], i1[M], fl[M];
 #pragma unroll
 	for (int k = 0; k < M; k++) {
 		const int e = threadIdx.x + k * NT;
-		ooff[k] = q2[k] = fl[k] = 0;
+		ooff[k] = q2[k] = i1[k] = fl[k] = 0;
 		if (e < N1) {
-			const int r = e / R1Z, c = e - (e / R1Z) * R1Z;
+			int r, c;
+			if (e < R1Y * TZ) {
+				r = e / TZ;
+				c = e - r * TZ;
+			} else {
+				r = (e - R1Y * TZ) >> 1;
+				c = TZ + ((e - R1Y * TZ) & 1);
+			}
 			const int y = y0 - 1 + r, z = z0 - 1 + c;
 			ooff[k] = wrap(y, ny) * nz + wrap(z, nz);
 			q2[k] = (r + 1) * P2Z + (c + 1);
+			i1[k] = r * R1Z + c;
 			fl[k] = 1 | ((r >= 1 && r < TY + 1 && c >= 1 && c < TZ + 1 && y < ny && z < nz) ? 2 : 0);
 		}
 	}
@@ -981,7 +992,7 @@ __device__ __forceinline__ void march3d2(const typename Form<PREC>::S* __restric
 #pragma unroll
 				for (int q = 0; q < Q; q++) {
 					qp[q][k] = upd(fm, fc, fp, lp, PN, P2Z, q2[k], k, sum, q);
-					lw[q * N1 + threadIdx.x + k * NT] = qp[q][k];
+					lw[q * N1 + i1[k]] = qp[q][k];
 				}
 			}
 		}
@@ -995,7 +1006,7 @@ __device__ __forceinline__ void march3d2(const typename Form<PREC>::S* __restric
 					const typename F::Sum sum = sumsq(qc, k);
 #pragma unroll
 					for (int q = 0; q < Q; q++)
-						dst[q * N + ooff[k]] = upd(qm, qc, qp, lr, N1, R1Z, threadIdx.x + k * NT, k, sum, q);
+						dst[q * N + ooff[k]] = upd(qm, qc, qp, lr, N1, R1Z, i1[k], k, sum, q);
 				}
 			}
 		}
@@ -1051,18 +1062,29 @@ __device__ __forceinline__ void march3d_ow(const typename Form<PREC>::S* __restr
 	const int64_t plane = (int64_t)ny * nz;
 	const int64_t N = (int64_t)nx * plane;
 
-	// own point k: R1 index e = tid + k*NT at (r, c), node (y0-1+r, z0-1+c);
-	// q2 its index in the fields' tile; fl 1 in R1, 2 in the tile (written)
-	int ooff[M], q2[M], fl[M];
+	// own point k: e = tid + k*NT names R1 point (r, c), node (y0-1+r,
+	// z0-1+c): e < R1Y*TZ column e % TZ of row e / TZ, the next 2*R1Y the last
+	// two columns (TZ, TZ+1) of each row, so a wave's points lie in whole row
+	// segments (contiguous in LDS and in HBM); q2 the point's index in the
+	// fields' tile, i1 in the R1 plane; fl 1 in R1, 2 in the tile (written)
+	int ooff[M], q2[M], i1[M], fl[M];
 #pragma unroll
 	for (int k = 0; k < M; k++) {
 		const int e = threadIdx.x + k * NT;
-		ooff[k] = q2[k] = fl[k] = 0;
+		ooff[k] = q2[k] = i1[k] = fl[k] = 0;
 		if (e < N1) {
-			const int r = e / R1Z, c = e - (e / R1Z) * R1Z;
+			int r, c;
+			if (e < R1Y * TZ) {
+				r = e / TZ;
+				c = e - r * TZ;
+			} else {
+				r = (e - R1Y * TZ) >> 1;
+				c = TZ + ((e - R1Y * TZ) & 1);
+			}
 			const int y = y0 - 1 + r, z = z0 - 1 + c;
 			ooff[k] = wrap(y, ny) * nz + wrap(z, nz);
 			q2[k] = (r + 1) * P2Z + (c + 1);
+			i1[k] = r * R1Z + c;
 			fl[k] = 1 | ((r >= 1 && r < TY + 1 && c >= 1 && c < TZ + 1 && y < ny && z < nz) ? 2 : 0);
 		}
 	}
@@ -1159,7 +1181,7 @@ __device__ __forceinline__ void march3d_ow(const typename Form<PREC>::S* __restr
 				F::lap_add(lap, g.w[1], g.wf[1], lp[i + P2Z], C, lp[i - P2Z]);
 				F::lap_add(lap, g.w[2], g.wf[2], lp[i + 1], C, lp[i - 1]);
 				wp[k] = F::ow_w(C, osum, lap, p);
-				lwp[threadIdx.x + k * NT] = wp[k];
+				lwp[i1[k]] = wp[k];
 			}
 			if (inside && (fl[k] & 2)) {
 #pragma unroll
@@ -1180,7 +1202,7 @@ __device__ __forceinline__ void march3d_ow(const typename Form<PREC>::S* __restr
 #pragma unroll
 			for (int k = 0; k < M; k++) {
 				if (!(fl[k] & 2)) continue;
-				const int e = threadIdx.x + k * NT;
+				const int e = i1[k];
 				const S y = wc[k];
 				typename F::L lap = F::lap_zero();
 				F::lap_add(lap, g.w[0], g.wf[0], wp[k], y, wm[k]);
