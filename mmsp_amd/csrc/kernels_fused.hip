@@ -919,6 +919,7 @@ static int launch_march2(const Geom& g, const ModelParams& p, const void* a, voi
 {
 	const int chunk = t.chunk > 0 ? t.chunk : 32;
 	if (!all_periodic(g)) return fail(MMSPX_EUNSUPPORTED, "tile body 12 (two steps per pass) needs periodic boundaries");
+	if (PREC == MMSPX_F32REF) return fail(MMSPX_EUNSUPPORTED, "tile body 12 runs the MMSPX_F64 and MMSPX_F32 forms");
 	if (nsteps == 1) {
 		// an odd last step: the one-step march
 		const TuneChoice t1{2, KIND == 0 ? 16 : 8, KIND == 0 ? 32 : 64, chunk, 8};

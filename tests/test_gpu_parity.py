@@ -1063,13 +1063,14 @@ def test_march_3d_two_steps(prec):
         g.upload(np.zeros((6, 20, 40)))
         with pytest.raises(mx.MmspxError):
             g.step("ac", pa, 2)
-    with mx.Grid((6, 20, 40), precision="f32ref") as g:
-        g.set_variant(2)
-        g.set_tile(16, 32, 8)
-        g.set_tile_options(body=12)
-        g.upload(np.zeros((6, 20, 40), dtype=np.float32))
-        with pytest.raises(mx.MmspxError):
-            g.step("ac", pa, 2)
+    for steps in (1, 2):
+        with mx.Grid((6, 20, 40), precision="f32ref") as g:
+            g.set_variant(2)
+            g.set_tile(16, 32, 8)
+            g.set_tile_options(body=12)
+            g.upload(np.zeros((6, 20, 40), dtype=np.float32))
+            with pytest.raises(mx.MmspxError):
+                g.step("ac", pa, steps)
     with mx.Grid((6, 20, 40), fields=2, vector=True, precision=prec, bc=[(P, P), (N, N), (P, P)]) as g:
         g.set_variant(2)
         g.set_tile(8, 64, 8)
